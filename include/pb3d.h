@@ -1,0 +1,173 @@
+/*
+ * pb3d: MI355X (gfx950) kernels for the PolyBin3D binned-estimator path, C ABI.
+ *
+ * The reference has no C boundary: its hot path is the duck-typed `utils` module contract
+ * (IntegrationUtils / MapUtils / module functions, PolyBin3D/cython/utils.pyx:17-692) plus the FFT
+ * dispatch in PolyBin3D/base.py:176-229,256-302.  Each entry point below names the reference
+ * interface it replaces.  The Python host layer (polybin3d_amd/_capi.py) binds these with ctypes.
+ *
+ * Conventions
+ *   - All device pointers are owned by the caller (PyTorch tensors); the library never frees or
+ *     retains them past the call.  Plans own only their twiddle tables.
+ *   - Real grids are float32 [batch][n0][n1][n2]; half-complex grids are float2 (re,im)
+ *     [batch][n0][n1][n2/2+1]; both C-contiguous.
+ *   - Forward transforms are unnormalised (e^{-ik.x}); inverse transforms carry 1/(n0 n1 n2),
+ *     the numpy convention the reference uses (PolyBin3D/base.py:176-229).
+ *   - Every call is asynchronous on `stream` (a hipStream_t, NULL = default stream) and returns
+ *     0 on success or a negative PB3D_ERR_* code; pb3d_last_error() gives the message
+ *     (thread-local).  Argument checks happen on the host before any launch.
+ *   - Bin membership is given by a host-built uint8 bin-index table over the half grid
+ *     (255 = outside every bin), so it is bit-identical to the reference's FP64 test
+ *     `modk >= k_lo && modk < k_hi` (PolyBin3D/cython/utils.pyx:63).
+ */
+#ifndef PB3D_H
+#define PB3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PB3D_OK 0
+#define PB3D_ERR_ARG (-1)
+#define PB3D_ERR_UNSUPPORTED (-2)
+#define PB3D_ERR_HIP (-3)
+#define PB3D_ERR_ALLOC (-4)
+
+#define PB3D_NO_BIN 255
+
+typedef struct pb3d_plan pb3d_plan;
+
+/* Box geometry (PolyBin3D/base.py:31-116).  sight = unit line of sight for the global case
+ * (z axis); for local (Yamamoto) sightlines it is boxcenter/|boxcenter|, the distant-observer
+ * direction the reference uses for mu (base.py:111-116). */
+typedef struct {
+  int32_t n[3];
+  int32_t local;
+  double boxsize[3];
+  double boxcenter[3];
+  double sight[3];
+} pb3d_grid;
+
+/* Real-space voxel operation, used as the R2C prologue (input side) and the C2R epilogue
+ * (output side).  kind:
+ *   0 PLAIN   v = a
+ *   1 MUL     v = a * b                      (mask, pointing matrix; base.py:399, pspec.py:212)
+ *   2 DIFF    v = a*b - c*d                  (prod_real_diff, utils.pyx:371-383)
+ *   3 YLM     v = a * Y_lm(x_hat)            (apply_fourier_harmonics prologue, base.py:409)
+ *   4 MULYLM  v = a * b * Y_lm(x_hat)
+ *   5 SQUARE  v = a*a                        (C2R epilogue only: accumulate squares,
+ *                                             bspec.py:258)
+ * For C2R epilogues `a` is ignored (the transform output takes its place) and b..d are the
+ * extra operands; out = scale*v (accumulate=0) or out += scale*v (accumulate=1).
+ * Pointers a..d are advanced by `batch_stride` elements per batch entry (0 = broadcast). */
+typedef struct {
+  int32_t kind;
+  int32_t ell, m;
+  int32_t accumulate;
+  const float* a;
+  const float* b;
+  const float* c;
+  const float* d;
+  int64_t batch_stride_a, batch_stride_b, batch_stride_c, batch_stride_d;
+  float scale;
+  int32_t pad_;
+} pb3d_rop;
+
+/* Fourier-space mode operation, used as the C2R prologue (input side) and the R2C epilogue
+ * (output side).  kind:
+ *   0 PLAIN   v = x
+ *   1 FILTER  v = x * [bin(k) == bins[batch]] * L_ell(mu) * L_ell2(mu) * w(k)
+ *             (fourier_filter / fourier_filter_ll, utils.pyx:283-320; w optional)
+ *   2 YLM     v = x * [bin(k) == bins[batch] or bins==NULL] * Y_lm(k_hat) * w(k)
+ *             (prod_fourier_filter with a Y_lm grid, utils.pyx:325-338; base.py:398,410)
+ *   3 WEIGHT  v = x * w(k)                   (prod_fourier with invPk0, utils.pyx:213-225)
+ * For R2C epilogues x is the transform output; out = scale*v or out += scale*v.
+ * `x_batch_stride` (complex elements) advances the C2R input per batch entry (0 = broadcast:
+ * one input map filtered into many bins). */
+typedef struct {
+  int32_t kind;
+  int32_t ell, ell2;
+  int32_t m;
+  int32_t accumulate;
+  int32_t pad_;
+  const uint8_t* bin_table;
+  const int32_t* bins;
+  const float* w;
+  int64_t x_batch_stride;
+  float scale;
+  int32_t pad2_;
+} pb3d_kop;
+
+/* ---------------------------------------------------------------- library */
+int pb3d_version(void);
+const char* pb3d_last_error(void);
+/* FFT counters kept like PolyBin3D.n_FFTs_forward / n_FFTs_reverse (base.py:134-135,178,204):
+ * each batch entry counts one transform.  dir: 0 = forward, 1 = reverse, -1 = reset both. */
+int64_t pb3d_fft_counter(int dir);
+
+/* ---------------------------------------------------------------- FFT plans
+ * Replace _fft_setup (base.py:256-302).  Supported: n0, n1 powers of two in [2, 512], n2 a power
+ * of two in [4, 512] (real FFT layout). */
+int pb3d_plan_create(pb3d_plan** plan, const pb3d_grid* grid);
+int pb3d_plan_destroy(pb3d_plan* plan);
+
+/* 3D real-to-complex transform, unnormalised (to_fourier, base.py:176-200).
+ * in: real [batch][n0][n1][n2] read through `rop` (NULL = plain); out: half complex written
+ * through `kop` (NULL = plain).  work: half-complex scratch of batch*n0*n1*(n2/2+1) float2, or
+ * NULL to run the intermediate passes in `out` (not allowed when kop accumulates). */
+int pb3d_fft_r2c(const pb3d_plan* plan, const float* in, void* out, void* work, int batch,
+                 const pb3d_rop* rop, const pb3d_kop* kop, void* stream);
+
+/* 3D complex-to-real transform with 1/N (to_real, base.py:202-229); numpy semantics on the
+ * self-conjugate kz=0 and kz=n2/2 planes (imaginary parts ignored).
+ * in: half complex read through `kop` (NULL = plain; never modified); work: half-complex
+ * scratch of batch*n0*n1*(n2/2+1) float2; out: real written through `rop` (NULL = plain). */
+int pb3d_fft_c2r(const pb3d_plan* plan, const void* in, float* out, void* work, int batch,
+                 const pb3d_kop* kop, const pb3d_rop* rop, void* stream);
+
+/* ---------------------------------------------------------------- mode-space maps */
+/* out[batch] = kop(x) over the half grid, without a transform (fourier_filter,
+ * prod_fourier, prod_fourier_filter; utils.pyx:213-338). */
+int pb3d_kmap(const pb3d_grid* grid, const void* x, void* out, int batch, const pb3d_kop* kop,
+              void* stream);
+
+/* Gaussian random field amplitudes (generate_data, base.py:305-353): given the full-grid
+ * draws re/im (float64 [n0][n1][n2], the reference's numpy stream, uploaded), write the half
+ * grid 0.5*(r(k)+conj(r(-k))) * sqrt(P(k)) * N/sqrt(V) * W(k), DC set to zero before the
+ * symmetrisation.  sqrtP_w: float32 half-grid of sqrt(P)*W (NULL = 1). */
+int pb3d_grf_hermitian(const pb3d_grid* grid, const double* re, const double* im,
+                       const float* sqrtP_w, float norm, void* out, void* stream);
+
+/* ---------------------------------------------------------------- bin reductions
+ * One pass over the half grid for a batch of maps (cross_integrate / integrate /
+ * real_integrate_double, utils.pyx:51-134):
+ *   out[q][o][b] = sum_{k in bin b} degk * Re[x_q(k) conj(y_q(k))] * L_{ells[o]}(mu)
+ *                  * L_{ells2[o]}(mu)
+ * x: nq maps (x_stride complex elements apart); y: maps for each output o: y + (q*y_qstride +
+ * o*y_ostride) (y == NULL: y = x).  mode 1 = x is real (float32, stride in floats) and the
+ * summand is degk*x^2*L*L.  Sums: FP32 inside a wave segment, FP64 across segments; out FP64.
+ * work: FP64 scratch of pb3d_bin_reduce_work_size() bytes. */
+size_t pb3d_bin_reduce_work_size(const pb3d_grid* grid, int nq, int nout, int nbins);
+int pb3d_bin_reduce(const pb3d_grid* grid, const void* x, int64_t x_stride, const void* y,
+                    int64_t y_qstride, int64_t y_ostride, int nq, int nout, const int32_t* ells,
+                    const int32_t* ells2, int mode, const uint8_t* bin_table, int nbins,
+                    double* out, void* work, void* stream);
+
+/* ---------------------------------------------------------------- triplet contraction
+ * T[t] = sum_x g[i1[t]](x) g[i2[t]](x) g[i3[t]](x) for ntrip index triples into the
+ * float32 map stack g[nmaps][nvox] (assemble_b3 / sum_triplet, utils.pyx:357-366,614-636).
+ * precision 0 = FP32 products with FP64 accumulation.  Index triples are device int32 arrays;
+ * an index outside [0, nmaps) yields NaN for that triplet.  out: FP64[ntrip] (device).
+ * work: scratch of pb3d_triplet_work_size() bytes. */
+size_t pb3d_triplet_work_size(int64_t nvox, int ntrip, int precision);
+int pb3d_triplet_sum(const float* g, int nmaps, int64_t nvox, const int32_t* i1, const int32_t* i2,
+                     const int32_t* i3, int ntrip, int precision, double* out, void* work,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PB3D_H */

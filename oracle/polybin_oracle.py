@@ -1,0 +1,1018 @@
+"""FP64 CPU oracle for the PolyBin3D binned-estimator path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: it may be imported by ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` and by nothing else.  The
+product package ``polybin3d_amd`` never imports it and has no CPU fallback.
+
+It restates, in plain numpy/scipy FP64, the algorithm of the reference estimators so that the HIP
+path can be compared against it on identical seeded inputs.  Every function cites the reference
+file:line it follows (paths relative to the reference repository root).  The FFTs are
+``scipy.fft.rfftn/irfftn`` (numpy conventions: forward unnormalised, inverse x 1/N), standing in for
+the reference's FFTW/MKL calls; FFT counters are kept exactly as the reference keeps them
+(PolyBin3D/base.py:176-229) because the published notebook counters pin the call graph.
+
+How each part is pinned (tests/test_oracle_*.py):
+  * grids, bin membership, pixel windows: closed forms and the notebook's printed kF/kNy
+    (Tutorial 1 cell 8) plus mode counting by brute force;
+  * FFT wrappers: brute-force DFT on small grids, Parseval, plane wave <-> delta;
+  * utils kernels (integrate, cross_integrate, real_integrate_double, integrate_row, filters,
+    sum_pair/sum_triplet, assemble_b3/b1): explicit per-bin loops on tiny grids, and the compiled
+    reference kernels in oracle/_ref when that build is present;
+  * real spherical harmonics: the addition theorem sum_m Y_lm(a) Y_lm(b) = L_l(a.b);
+  * Pk_ideal: plane-wave known answer P = A^2 V / (2 N_modes);
+  * Bk numerator: exact Fourier-space triple sum by brute force on 8^3 grids;
+  * FFT counts of the whole call graphs: the notebook counters 1/0, 15/1, 2/96 and 452/682;
+  * MC Fisher (global, constant mask): per-seed closed form from the GRF amplitudes; local and
+    masked branches: dense-operator restatement on 6^3-8^3 grids via the addition theorem;
+    expectation E[F_MC] -> F_ideal statistically.
+Functions with no check beyond restatement are listed as "parity unpinned" in DESIGN.md:
+BSpec._compute_fisher's per-seed values for local sightline with l>0 (checked only in expectation
+against the ideal Fisher).
+"""
+import numpy as np
+import scipy.fft
+import scipy.interpolate
+import os
+
+
+# ----------------------------------------------------------------------------------------------
+# Small helpers (PolyBin3D/cython/utils.pyx)
+# ----------------------------------------------------------------------------------------------
+
+def legendre(l, mu):
+    """Explicit Legendre polynomials L_0..L_6 (PolyBin3D/cython/utils.pyx:385-400)."""
+    mu = np.asarray(mu, dtype=np.float64)
+    if l == 0:
+        return np.ones_like(mu)
+    if l == 1:
+        return mu
+    if l == 2:
+        return 0.5 * (3 * mu ** 2 - 1)
+    if l == 3:
+        return (-3 * mu + 5 * mu ** 3) / 2.
+    if l == 4:
+        return (3 - 30 * mu ** 2 + 35 * mu ** 4) / 8.
+    if l == 5:
+        return (15 * mu - 70 * mu ** 3 + 63 * mu ** 5) / 8.
+    if l == 6:
+        return (-5 + 105 * mu ** 2 - 315 * mu ** 4 + 231 * mu ** 6) / 16.
+    raise ValueError("Legendre polynomial only defined up to l=6")
+
+
+def real_harmonics(coords, lmax, odd_l):
+    """Real spherical harmonics Y_lm(n) for l=1..lmax, normalised so that
+    sum_m Y_lm(a) Y_lm(b) = L_l(a.b) (the convention of compute_real_harmonics,
+    PolyBin3D/cython/utils.pyx:518-609).  Odd l only when odd_l.  Zero vectors map to Y=0 (l>0
+    members vanish at the origin there too, utils.pyx:534-542).
+
+    Returns {l: array[2l+1, ...]}.  The basis is built from the standard tesseral harmonics
+    (Condon-Shortley phases dropped) times sqrt(4 pi / (2l+1)); any orthonormal real basis gives
+    identical estimator outputs because only sum_m Y_lm Y_lm enters them.
+    """
+    c = np.asarray(coords, dtype=np.float64)
+    r = np.sqrt(c[0] ** 2 + c[1] ** 2 + c[2] ** 2)
+    safe = np.where(r == 0, 1.0, r)
+    x = np.where(r == 0, 0.0, c[0] / safe)
+    y = np.where(r == 0, 0.0, c[1] / safe)
+    z = np.where(r == 0, 0.0, c[2] / safe)
+    out = {}
+    if odd_l and lmax >= 1:
+        out[1] = np.stack([x, y, z])
+    if lmax >= 2:
+        s3 = np.sqrt(3.)
+        out[2] = np.stack([s3 * x * y, s3 * y * z, z * z - 0.5 * (x * x + y * y), s3 * x * z,
+                           0.5 * s3 * (x * x - y * y)])
+    if odd_l and lmax >= 3:
+        out[3] = np.stack([
+            np.sqrt(5. / 8.) * y * (3 * x * x - y * y),
+            np.sqrt(15.) * x * y * z,
+            np.sqrt(3. / 8.) * y * (5 * z * z - 1),
+            0.5 * z * (5 * z * z - 3),
+            np.sqrt(3. / 8.) * x * (5 * z * z - 1),
+            0.5 * np.sqrt(15.) * z * (x * x - y * y),
+            np.sqrt(5. / 8.) * x * (x * x - 3 * y * y)])
+    if lmax >= 4:
+        out[4] = np.stack([
+            0.5 * np.sqrt(35.) * x * y * (x * x - y * y),
+            np.sqrt(35. / 8.) * y * z * (3 * x * x - y * y),
+            0.5 * np.sqrt(5.) * x * y * (7 * z * z - 1),
+            np.sqrt(5. / 8.) * y * z * (7 * z * z - 3),
+            (35 * z ** 4 - 30 * z * z + 3) / 8.,
+            np.sqrt(5. / 8.) * x * z * (7 * z * z - 3),
+            0.25 * np.sqrt(5.) * (x * x - y * y) * (7 * z * z - 1),
+            np.sqrt(35. / 8.) * x * z * (x * x - 3 * y * y),
+            np.sqrt(35.) / 8. * (x ** 4 - 6 * x * x * y * y + y ** 4)])
+    return out
+
+
+def bin_index(modk, edges):
+    """Index b with edges[b] <= modk < edges[b+1], or -1 (half-open bins as in
+    PolyBin3D/cython/utils.pyx:63,92,128,173,292,313,336).  Requires increasing edges."""
+    edges = np.asarray(edges, dtype=np.float64)
+    assert np.all(np.diff(edges) > 0), "bin edges must be strictly increasing"
+    idx = np.searchsorted(edges, modk, side='right') - 1
+    idx[(idx < 0) | (idx >= len(edges) - 1)] = -1
+    return idx
+
+
+# ----------------------------------------------------------------------------------------------
+# Integration / map utilities (utils contract, PolyBin3D/cython/utils.pyx:17-383)
+# ----------------------------------------------------------------------------------------------
+
+class Integrator:
+    """IntegrationUtils restated (PolyBin3D/cython/utils.pyx:17-179) for real FFT grids."""
+
+    def __init__(self, modk, degk, muk):
+        self.modk, self.degk, self.muk = modk, degk, muk
+        self._cache = {}
+
+    def _idx(self, edges):
+        key = tuple(np.asarray(edges, dtype=np.float64).tolist())
+        if key not in self._cache:
+            self._cache[key] = bin_index(self.modk, edges)
+        return self._cache[key]
+
+    def _binsum(self, w, edges):
+        idx = self._idx(edges)
+        nb = len(edges) - 1
+        sel = idx >= 0
+        return np.bincount(idx[sel], weights=w[sel], minlength=nb)[:nb]
+
+    def cross_integrate(self, s1, s2, edges, l):
+        """sum_{k in b} degk Re[s1 conj(s2)] L_l(mu) (utils.pyx:51-75)."""
+        w = self.degk * np.real(s1 * np.conj(s2)) * (legendre(l, self.muk) if l != 0 else 1.)
+        return self._binsum(w, edges)
+
+    def integrate(self, s, edges, l):
+        """sum_{k in b} degk |s|^2 L_l(mu) (utils.pyx:80-104)."""
+        w = self.degk * np.real(s * np.conj(s)) * (legendre(l, self.muk) if l != 0 else 1.)
+        return self._binsum(w, edges)
+
+    def real_integrate_double(self, s, l1, l2, edges):
+        """sum_{k in b} degk s^2 L_l1 L_l2 for real s (utils.pyx:109-134)."""
+        leg = 1. if (l1 == 0 and l2 == 0) else legendre(l1, self.muk) * legendre(l2, self.muk)
+        return self._binsum(self.degk * s ** 2 * leg, edges)
+
+    def integrate_row(self, m, arr, kmin, kmax):
+        """out[j] = sum_{kmin<=k<kmax} degk Re[conj(m) arr_j] (utils.pyx:162-179)."""
+        sel = (self.modk >= kmin) & (self.modk < kmax)
+        w = (self.degk[sel] * np.conj(m[sel]))[None, :]
+        return np.real(np.sum(w * arr[:, sel], axis=1))
+
+
+class MapUtils:
+    """MapUtils restated (PolyBin3D/cython/utils.pyx:184-383)."""
+
+    def __init__(self, modk, muk):
+        self.modk, self.muk = modk, muk
+
+    def fourier_filter(self, kmap, l, kmin, kmax):
+        """Theta_[kmin,kmax)(|k|) L_l(mu) kmap (utils.pyx:283-299)."""
+        sel = (self.modk >= kmin) & (self.modk < kmax)
+        leg = legendre(l, self.muk) if l != 0 else 1.
+        return np.where(sel, kmap * leg, 0.).astype(np.complex128)
+
+    def fourier_filter_ll(self, kmap, l1, l2, kmin, kmax):
+        """Theta L_l1 L_l2 kmap (utils.pyx:304-320)."""
+        sel = (self.modk >= kmin) & (self.modk < kmax)
+        leg = 1. if (l1 == 0 and l2 == 0) else legendre(l1, self.muk) * legendre(l2, self.muk)
+        return np.where(sel, kmap * leg, 0.).astype(np.complex128)
+
+    def prod_fourier_filter(self, m1, m2, kmin, kmax):
+        """Theta m1 m2 (utils.pyx:325-338)."""
+        sel = (self.modk >= kmin) & (self.modk < kmax)
+        return np.where(sel, m1 * m2, 0.).astype(np.complex128)
+
+    @staticmethod
+    def div_fourier(m1, m2):
+        """m1/m2 with 0 where m2==0 (utils.pyx:230-243)."""
+        out = np.zeros(m1.shape, dtype=np.complex128)
+        nz = m2 != 0
+        out[nz] = m1[nz] / m2[nz]
+        return out
+
+
+def sum_pair(a, b):
+    """sum_x a b (utils.pyx:343-352)."""
+    return float(np.sum(a * b))
+
+
+def sum_triplet(a, b, c):
+    """sum_x a b c (utils.pyx:357-366)."""
+    return float(np.sum(a * b * c))
+
+
+def assemble_b3(g, all_bins):
+    """B3[i] = sum_x g[0][b1] g[0][b2] g[l/2][b3] (utils.pyx:614-636)."""
+    out = np.zeros(len(all_bins))
+    for i, (b1, b2, b3, l) in enumerate(all_bins):
+        out[i] = np.sum(g[0][b1] * g[0][b2] * g[l // 2][b3])
+    return out
+
+
+def assemble_b1(g, s, all_bins):
+    """Linear-term triple sums with simulation maps s (utils.pyx:668-692)."""
+    out = np.zeros(len(all_bins))
+    for i, (b1, b2, b3, l) in enumerate(all_bins):
+        out[i] = np.sum(g[0][b1] * s[0][b2] * s[l // 2][b3] + s[0][b1] * g[0][b2] * s[l // 2][b3]
+                        + s[0][b1] * s[0][b2] * g[l // 2][b3])
+    return out
+
+
+# ----------------------------------------------------------------------------------------------
+# Base class (PolyBin3D/base.py)
+# ----------------------------------------------------------------------------------------------
+
+PIXEL_WINDOWS = ['none', 'cic', 'tsc', 'pcs', 'interlaced-cic', 'interlaced-tsc', 'interlaced-pcs']
+
+
+def pixel_window_1d(kind, x):
+    """1D voxel windows at x = pi m / n (PolyBin3D/base.py:231-254)."""
+    if kind == 'cic':
+        s = np.sin(x) ** 2
+        return np.sqrt(1. - 2. / 3. * s)
+    if kind == 'tsc':
+        s = np.sin(x) ** 2
+        return np.sqrt(1. - s + 2. / 15 * s ** 2)
+    if kind == 'pcs':
+        s = np.sin(x) ** 2
+        return np.sqrt(1. - 4. / 3. * s + 2. / 5. * s ** 2 - 4. / 315. * s ** 3)
+    if kind == 'interlaced-cic':
+        s = np.sinc(x / np.pi)
+        v = np.ones_like(s)
+        v[s != 0.] = s[s != 0.] ** 2
+        return v
+    if kind == 'interlaced-tsc':
+        return np.sinc(x / np.pi) ** 3
+    if kind == 'interlaced-pcs':
+        return np.sinc(x / np.pi) ** 4
+    raise ValueError("Unknown window function '%s' supplied!" % kind)
+
+
+class PolyBin3D:
+    """FP64 restatement of PolyBin3D.PolyBin3D (PolyBin3D/base.py:6-437), real_fft=True only."""
+
+    def __init__(self, boxsize, gridsize, Pk=None, boxcenter=None, pixel_window='none',
+                 sightline='global', nthreads=None):
+        # base.py:31-60
+        self.boxsize = np.asarray([boxsize] * 3 if np.ndim(boxsize) == 0 else boxsize, dtype=np.float64)
+        self.boxcenter = (np.zeros(3) if boxcenter is None else
+                          np.asarray([boxcenter] * 3 if np.ndim(boxcenter) == 0 else boxcenter, dtype=np.float64))
+        self.gridsize = np.asarray([gridsize] * 3 if np.ndim(gridsize) == 0 else gridsize, dtype=int)
+        assert len(self.boxsize) == 3 and len(self.gridsize) == 3 and len(self.boxcenter) == 3
+        if self.gridsize[2] % 2 != 0:
+            raise Exception("Last dimension of grid must be even for real-to-complex FFTs.")
+        assert sightline in ['local', 'global']
+        if sightline == 'global':
+            assert np.abs(self.boxcenter).sum() == 0.
+        else:
+            assert np.abs(self.boxcenter).sum() != 0.
+        self.sightline = sightline
+        self.pixel_window = pixel_window
+        self.nthreads = nthreads or os.cpu_count()
+        # base.py:63-74
+        self.volume = self.boxsize.prod()
+        self.kF = 2. * np.pi / self.boxsize
+        self.kNy = self.gridsize * self.kF * 0.5
+        if Pk is None:
+            k_tmp = np.arange(0., np.max(self.kNy) * 2, np.min(self.kF) / 10.)
+            self.Pfid = np.asarray([k_tmp, 1. + 0. * k_tmp])
+        else:
+            assert len(Pk) == 2
+            self.Pfid = np.asarray(Pk, dtype=np.float64)
+        # base.py:77-80 (no half-cell offset)
+        if sightline == 'local':
+            r_arrs = [np.fft.fftshift(np.arange(-self.gridsize[i] // 2, self.gridsize[i] // 2))
+                      * self.boxsize[i] / self.gridsize[i] + self.boxcenter[i] for i in range(3)]
+            self.r_grids = np.meshgrid(*r_arrs, indexing='ij')
+        # base.py:86-94
+        k_arrs = [np.fft.fftshift(np.arange(-self.gridsize[i] // 2, self.gridsize[i] // 2)) * self.kF[i]
+                  for i in range(2)]
+        k_arrs += [np.arange(0, self.gridsize[2] // 2 + 1) * self.kF[2]]
+        self.k_arrs = k_arrs
+        self.k_grids = np.meshgrid(*k_arrs, indexing='ij')
+        self.modk_grid = np.sqrt(self.k_grids[0] ** 2 + self.k_grids[1] ** 2 + self.k_grids[2] ** 2)
+        self.degeneracy_factor = np.asarray(1. + (self.k_grids[2] > 0), dtype=np.float64)
+        # base.py:103-108
+        assert pixel_window in PIXEL_WINDOWS
+        if pixel_window != 'none':
+            w1 = [pixel_window_1d(pixel_window, np.pi / self.gridsize[i] * k_arrs[i] / self.kF[i])
+                  for i in range(3)]
+            self.pixel_window_grid = np.asarray(np.meshgrid(*w1, indexing='ij')).prod(axis=0)
+        # base.py:111-116
+        sv = np.asarray([0., 0., 1.]) if sightline == 'global' else self.boxcenter / np.sqrt(np.sum(self.boxcenter ** 2))
+        self.sight_vector = sv
+        self.muk_grid = np.zeros(self.modk_grid.shape)
+        nz = self.modk_grid != 0
+        self.muk_grid[nz] = np.sum(sv[:, None] * np.asarray(self.k_grids)[:, nz], axis=0) / self.modk_grid[nz]
+        # base.py:122-131
+        self.Pk0_grid = scipy.interpolate.interp1d(self.Pfid[0], self.Pfid[1], bounds_error=False,
+                                                   fill_value=0.)(self.modk_grid)
+        self.invPk0_grid = np.zeros(self.modk_grid.shape)
+        good = (self.Pk0_grid != 0) & np.isfinite(self.Pk0_grid)
+        self.invPk0_grid[good] = 1. / self.Pk0_grid[good]
+        self.Pk0_grid[~np.isfinite(self.Pk0_grid)] = 0.
+        self.invPk0_grid[self.modk_grid == 0] = 1.
+        self.Pk0_grid[self.modk_grid == 0] = 1.
+        self.n_FFTs_forward = 0
+        self.n_FFTs_reverse = 0
+        self.integrator = Integrator(self.modk_grid, self.degeneracy_factor, self.muk_grid)
+        self.map_utils = MapUtils(self.modk_grid, self.muk_grid)
+
+    def real_zeros(self, size=None):
+        shape = list(self.gridsize) if size is None else (
+            [size] if np.ndim(size) == 0 else list(size)) + list(self.gridsize)
+        return np.zeros(shape)
+
+    def to_fourier(self, m):
+        """Unnormalised R2C (base.py:176-200)."""
+        self.n_FFTs_forward += 1
+        return scipy.fft.rfftn(np.asarray(m, dtype=np.float64), axes=(0, 1, 2), workers=self.nthreads)
+
+    def to_real(self, m):
+        """C2R with 1/N (base.py:202-229)."""
+        self.n_FFTs_reverse += 1
+        return scipy.fft.irfftn(m, s=tuple(self.gridsize), axes=(0, 1, 2), workers=self.nthreads)
+
+    def generate_data(self, seed=None, Pk_input=(), output_type='real', include_pixel_window=True):
+        """Gaussian random field with the reference's numpy stream (base.py:305-353)."""
+        rng = np.random.RandomState(seed) if seed is not None else np.random
+        if len(Pk_input) == 0:
+            Pk_grid = self.Pk0_grid
+        else:
+            Pk_grid = scipy.interpolate.interp1d(Pk_input[0], Pk_input[1], bounds_error=False,
+                                                 fill_value=0.)(self.modk_grid)
+            if len(Pk_input) > 2:
+                Pk_grid = Pk_grid + legendre(2, self.muk_grid) * scipy.interpolate.interp1d(
+                    Pk_input[0], Pk_input[2], bounds_error=False, fill_value=0.)(self.modk_grid)
+            if len(Pk_input) > 3:
+                Pk_grid = Pk_grid + legendre(4, self.muk_grid) * scipy.interpolate.interp1d(
+                    Pk_input[0], Pk_input[3], bounds_error=False, fill_value=0.)(self.modk_grid)
+        n = self.gridsize
+        r = rng.randn(*n) + 1.0j * rng.randn(*n)
+        r[0, 0, 0] = 0.
+        ii = (-np.arange(n[0])) % n[0]
+        jj = (-np.arange(n[1])) % n[1]
+        kk = (-np.arange(n[2])) % n[2]
+        r = 0.5 * (r + np.conj(r[ii][:, jj][:, :, kk]))
+        r = np.ascontiguousarray(r[:, :, :n[2] // 2 + 1])
+        r = r * np.sqrt(Pk_grid) * (n.prod() / np.sqrt(self.volume))
+        if self.pixel_window != 'none' and include_pixel_window:
+            r = r * self.pixel_window_grid
+        return r if output_type == 'fourier' else self.to_real(r)
+
+    def applySinv_trivial(self, d, input_type='real', output_type='real'):
+        """base.py:355-362."""
+        if input_type == output_type:
+            return d
+        if input_type == 'fourier':
+            return self.to_real(d)
+        return self.to_fourier(d)
+
+    def applyAinv(self, d, invPk0_grid=None, input_type='real', output_type='real'):
+        """base.py:364-389."""
+        invPk0 = self.invPk0_grid if invPk0_grid is None else invPk0_grid
+        f = self.to_fourier(d) if input_type == 'real' else d.copy()
+        s = f * invPk0
+        if self.pixel_window != 'none':
+            s = MapUtils.div_fourier(MapUtils.div_fourier(s, self.pixel_window_grid), self.pixel_window_grid)
+        return self.to_real(s) if output_type == 'real' else s
+
+    def apply_real_harmonics(self, a_fourier, Ylm_real, Ylm_fourier):
+        """sum_m IFFT[a Y_lm(k)] Y_lm(x) (base.py:391-400)."""
+        out = self.real_zeros()
+        for m in range(len(Ylm_real)):
+            out += self.to_real(a_fourier * Ylm_fourier[m]) * Ylm_real[m]
+        return out
+
+    def apply_fourier_harmonics(self, a_real, Ylm_real, Ylm_fourier):
+        """sum_m FFT[a Y_lm(x)] Y_lm(k) (base.py:402-411)."""
+        out = np.zeros(self.modk_grid.shape, dtype=np.complex128)
+        for m in range(len(Ylm_real)):
+            out += self.to_fourier(a_real * Ylm_real[m]) * Ylm_fourier[m]
+        return out
+
+
+# ----------------------------------------------------------------------------------------------
+# Power spectrum (PolyBin3D/pspec.py)
+# ----------------------------------------------------------------------------------------------
+
+class PSpec:
+    """FP64 restatement of PolyBin3D.PSpec (PolyBin3D/pspec.py:10-784), hot-path members only
+    (no covariance/theory matrices, no integral constraints)."""
+
+    def __init__(self, base, k_bins, lmax=2, mask=None, applySinv=None, odd_l=False,
+                 applySinv_transpose=None, mask_shot=None):
+        # pspec.py:27-120
+        self.base = base
+        self.k_bins = np.asarray(k_bins, dtype=np.float64)
+        self.lmax = lmax
+        self.Nk = len(k_bins) - 1
+        self.odd_l = odd_l
+        self.Nl = lmax + 1 if odd_l else lmax // 2 + 1
+        self.Nl_even = lmax // 2 + 1
+        self.N_bins = self.Nk * self.Nl
+        assert np.max(self.k_bins) < np.min(base.kNy), "k_max must be less than k_Nyquist!"
+        assert np.min(self.k_bins) >= np.max(base.kF), "k_min must be at least the k_fundamental!"
+        assert np.max(self.k_bins) <= base.Pfid[0][-1]
+        assert np.min(self.k_bins) >= base.Pfid[0][0]
+        if odd_l:
+            assert lmax > 0 and base.sightline == 'local'
+        assert lmax <= 4
+        if mask is None:
+            self.const_mask, self.mask_mean, self.sq_mask_mean = True, 1., 1.
+        else:
+            self.mask = np.asarray(mask, dtype=np.float64)
+            if np.std(self.mask) < 1e-12:
+                self.const_mask, self.mask_mean = True, np.mean(self.mask)
+            else:
+                self.const_mask = False
+            self.sq_mask_mean = np.mean(self.mask ** 2)
+        if mask_shot is not None:
+            self.mask_shot = np.asarray(mask_shot, dtype=np.float64)
+        if applySinv is None:
+            self.applySinv = base.applySinv_trivial
+            self.applySinv_transpose = base.applySinv_trivial if applySinv_transpose is None else applySinv_transpose
+        else:
+            self.applySinv = applySinv
+            self.applySinv_transpose = applySinv if applySinv_transpose is None else applySinv_transpose
+        if base.sightline == 'local' and lmax > 0:
+            self.Ylm_real = real_harmonics(np.asarray(base.r_grids), lmax, odd_l)
+            self.Ylm_fourier = real_harmonics(np.asarray(base.k_grids), lmax, odd_l)
+        else:
+            self.Ylm_real = self.Ylm_fourier = None
+
+    def get_ks(self):
+        """pspec.py:122-128."""
+        return 0.5 * (self.k_bins[1:] + self.k_bins[:-1])
+
+    def apply_pointing(self, m, transpose=False):
+        """mask multiply (pspec.py:199-213, no integral constraints)."""
+        return m * self.mask
+
+    # --- numerator ------------------------------------------------------------------------
+    def _compute_pk_numerator(self, data, filtering='ideal'):
+        """pspec.py:613-686."""
+        b = self.base
+        data = np.asarray(data, dtype=np.float64)
+        if b.pixel_window != 'none':
+            df = MapUtils.div_fourier(b.to_fourier(data), b.pixel_window_grid)
+            if filtering == 'ideal':
+                S = df * b.invPk0_grid / np.sqrt(self.sq_mask_mean)
+            else:
+                S = self.applySinv(df, input_type='fourier', output_type='fourier')
+        else:
+            if filtering == 'ideal':
+                S = b.to_fourier(data) * b.invPk0_grid / np.sqrt(self.sq_mask_mean)
+            else:
+                S = self.applySinv(data, input_type='real', output_type='fourier')
+        if b.sightline == 'local' and self.lmax > 0:
+            S_real = b.to_real(S)
+        Nl, lf = (self.Nl, 2 - self.odd_l) if filtering == 'Sinv' else (self.Nl_even, 2)
+        out = np.zeros(Nl * self.Nk)
+        out[:self.Nk] = 0.5 * b.integrator.integrate(S, self.k_bins, 0)
+        for li in range(1, Nl if self.lmax > 0 else 1):
+            if b.sightline == 'global':
+                out[li * self.Nk:(li + 1) * self.Nk] = 0.5 * b.integrator.integrate(S, self.k_bins, 2 * li)
+            else:
+                lm = b.apply_fourier_harmonics(S_real, self.Ylm_real[lf * li], self.Ylm_fourier[lf * li])
+                if filtering == 'Sinv' and self.odd_l and li % 2 == 1:
+                    lm = lm * (-1.0j)
+                out[li * self.Nk:(li + 1) * self.Nk] = 0.5 * b.integrator.cross_integrate(S, lm, self.k_bins, 0)
+        return out * b.volume / b.gridsize.prod() ** 2
+
+    def Pk_numerator_ideal(self, data):
+        return self._compute_pk_numerator(data, 'ideal')
+
+    def Pk_numerator(self, data):
+        return self._compute_pk_numerator(data, 'Sinv')
+
+    # --- ideal Fisher ---------------------------------------------------------------------
+    def compute_fisher_ideal(self, discreteness_correction=True):
+        """pspec.py:702-752 (Pk_fid='default')."""
+        b = self.base
+        Nk, Nl = self.Nk, self.Nl_even
+        fish = np.zeros((Nl * Nk, Nl * Nk))
+        inv = b.invPk0_grid
+        if discreteness_correction:
+            for la in range(Nl):
+                for lb in range(Nl):
+                    fish[la * Nk:(la + 1) * Nk, lb * Nk:(lb + 1) * Nk] = np.diag(
+                        0.5 * b.integrator.real_integrate_double(inv, 2 * la, 2 * lb, self.k_bins))
+        else:
+            d = 0.5 * b.integrator.real_integrate_double(inv, 0, 0, self.k_bins)
+            for la in range(Nl):
+                fish[la * Nk:(la + 1) * Nk, la * Nk:(la + 1) * Nk] = np.diag(d) / (4. * la + 1.)
+        self.fish_ideal = fish
+        self.inv_fish_ideal = np.linalg.inv(fish)
+        return fish
+
+    def Pk_ideal(self, data, fish_ideal=(), discreteness_correction=True):
+        """pspec.py:754-784."""
+        if len(fish_ideal) != 0:
+            self.fish_ideal = np.asarray(fish_ideal)
+            self.inv_fish_ideal = np.linalg.inv(self.fish_ideal)
+        if not hasattr(self, 'inv_fish_ideal'):
+            self.compute_fisher_ideal(discreteness_correction)
+        out = self.inv_fish_ideal @ self.Pk_numerator_ideal(data)
+        return {'p%d' % l: out[i * self.Nk:(i + 1) * self.Nk] for i, l in enumerate(range(0, self.lmax + 1, 2))}
+
+    # --- MC Fisher ------------------------------------------------------------------------
+    def compute_fisher_contribution(self, seed, verb=False):
+        """pspec.py:141-147 -> _compute_fisher (pspec.py:215-493, compute_cov=False, compute_theory=False)."""
+        b = self.base
+        Nk = self.Nk
+        F = np.zeros((self.N_bins, self.N_bins))
+        lf = 2 - self.odd_l
+        a = b.generate_data(seed=seed + int(1e7), output_type='fourier')            # 266
+        if not self.const_mask:
+            a_real = b.to_real(a)                                                   # 268
+
+        def apply_filter(m):                                                        # 270-276
+            if self.const_mask:
+                return self.mask_mean * self.applySinv(m, input_type='fourier', output_type='fourier')
+            return self.applySinv(self.apply_pointing(m), input_type='real', output_type='fourier')
+
+        def apply_filter_dagger(m, real=False):                                     # 291-304
+            it = 'real' if real else 'fourier'
+            if self.const_mask:
+                return self.mask_mean * self.applySinv_transpose(m, input_type=it, output_type='fourier')
+            return b.to_fourier(self.apply_pointing(self.applySinv_transpose(m, input_type=it, output_type='real'),
+                                                    transpose=True))
+
+        Ainv = b.applyAinv(a, input_type='fourier', output_type='fourier')          # 328
+        SinvP = apply_filter(a if self.const_mask else a_real)                     # 329-332
+        if b.sightline == 'local' and self.lmax > 0:
+            SinvP_real = b.to_real(SinvP)                                           # 336
+            Ainv_real = b.to_real(Ainv)                                             # 343
+        leg_maps = {}
+        if b.sightline == 'local':                                                  # 349-352
+            for li in range(1, self.Nl):
+                leg_maps[li * lf] = b.apply_fourier_harmonics(Ainv_real, self.Ylm_real[lf * li], self.Ylm_fourier[lf * li])
+        scaling = 0.5 * b.volume / b.gridsize.prod() ** 2                          # 390
+
+        def add_to_matrix(row, Q):                                                  # 380-413
+            F[row, :Nk] += b.integrator.cross_integrate(Ainv, Q, self.k_bins, 0) * scaling
+            for li in range(1, self.Nl):
+                if b.sightline == 'global':
+                    F[row, li * Nk:(li + 1) * Nk] += b.integrator.cross_integrate(Ainv, Q, self.k_bins, 2 * li) * scaling
+                else:
+                    Qx = 1.0j * Q if (self.odd_l and li % 2 == 1) else Q
+                    F[row, li * Nk:(li + 1) * Nk] += b.integrator.cross_integrate(leg_maps[li * lf], Qx, self.k_bins, 0) * scaling
+
+        for ki in range(Nk):                                                        # 444-446
+            km = b.map_utils.fourier_filter(SinvP, 0, self.k_bins[ki], self.k_bins[ki + 1])
+            add_to_matrix(ki, apply_filter_dagger(km))
+        for li in range(1, self.Nl):                                                # 449-488
+            if b.sightline == 'global':
+                for ki in range(Nk):
+                    km = b.map_utils.fourier_filter(SinvP, li * 2, self.k_bins[ki], self.k_bins[ki + 1])
+                    add_to_matrix(li * Nk + ki, apply_filter_dagger(km))
+            else:
+                Yr, Yf = self.Ylm_real[lf * li], self.Ylm_fourier[lf * li]
+                leg_map = b.apply_fourier_harmonics(SinvP_real, Yr, Yf)
+                for ki in range(Nk):
+                    km = b.map_utils.fourier_filter(leg_map, 0, self.k_bins[ki], self.k_bins[ki + 1])
+                    if li % 2 == 1 and self.odd_l:
+                        km = km * (-1.0j)
+                    Q = 0.5 * apply_filter_dagger(km)
+                    real_map = b.real_zeros()
+                    for m in range(len(Yr)):
+                        k_map = b.map_utils.prod_fourier_filter(SinvP, Yf[m], self.k_bins[ki], self.k_bins[ki + 1])
+                        if li % 2 == 1 and self.odd_l:
+                            k_map = k_map * (-1.0j)
+                        real_map += b.to_real(k_map) * Yr[m]
+                    Q = Q + 0.5 * apply_filter_dagger(real_map, real=True)
+                    add_to_matrix(li * Nk + ki, Q)
+        return F
+
+    def compute_fisher(self, N_it, verb=False):
+        """pspec.py:553-569."""
+        fish = np.zeros((self.N_bins, self.N_bins))
+        for seed in range(N_it):
+            fish += self.compute_fisher_contribution(seed) / N_it
+        self.fish = fish
+        self.inv_fish = np.linalg.inv(fish)
+        return fish
+
+    def compute_shot_contribution(self, seed):
+        """pspec.py:495-551."""
+        b = self.base
+        shot = np.zeros(self.N_bins)
+        PkA = [b.Pfid[0], 1. / b.Pfid[1]]
+        PkA_grid = scipy.interpolate.interp1d(PkA[0], PkA[1], bounds_error=False,
+                                              fill_value=(PkA[1][0], PkA[1][-1]))(b.modk_grid)
+        invPkA = np.zeros_like(PkA_grid)
+        invPkA[b.modk_grid != 0] = 1. / PkA_grid[b.modk_grid != 0]
+        a = b.generate_data(seed=seed + int(1e7), Pk_input=PkA, output_type='fourier')
+        if self.const_mask:
+            AiA = b.applyAinv(a, invPk0_grid=invPkA, input_type='fourier', output_type='fourier')
+            SNA = self.mask_mean * self.applySinv(AiA, input_type='fourier', output_type='fourier')
+        else:
+            AiA = b.applyAinv(a, invPk0_grid=invPkA, input_type='fourier', output_type='real')
+            SNA = self.applySinv(AiA * self.mask_shot, input_type='real', output_type='fourier')
+        Sa = self.applySinv(a, input_type='fourier', output_type='fourier')
+        if self.lmax > 0 and b.sightline == 'local':
+            Sa_real = b.to_real(Sa)
+        shot[:self.Nk] = 0.5 * b.integrator.cross_integrate(Sa, SNA, self.k_bins, 0)
+        lf = 2 - self.odd_l
+        for li in range(1, self.Nl):
+            if b.sightline == 'global':
+                shot[li * self.Nk:(li + 1) * self.Nk] = 0.5 * b.integrator.cross_integrate(Sa, SNA, self.k_bins, 2 * li)
+            else:
+                lm = b.apply_fourier_harmonics(Sa_real, self.Ylm_real[lf * li], self.Ylm_fourier[lf * li])
+                if self.odd_l and li % 2 == 1:
+                    lm = lm * (-1.0j)
+                shot[li * self.Nk:(li + 1) * self.Nk] = 0.5 * b.integrator.cross_integrate(lm, SNA, self.k_bins, 0)
+        return shot * b.volume / b.gridsize.prod() ** 2
+
+    def Pk_unwindowed(self, data, fish=(), shot_num=(), subtract_shotnoise=False):
+        """pspec.py:571-611."""
+        if len(fish) != 0:
+            self.fish = np.asarray(fish)
+            self.inv_fish = np.linalg.inv(self.fish)
+        if len(shot_num) != 0 or subtract_shotnoise:
+            self.shot_num = np.asarray(shot_num)
+            self.shot = self.inv_fish @ self.shot_num
+        if not hasattr(self, 'inv_fish'):
+            raise Exception("Need to compute Fisher matrix first!")
+        out = self.inv_fish @ self.Pk_numerator(data)
+        if subtract_shotnoise:
+            out = out - self.shot
+        res, idx = {}, 0
+        for l in range(0, self.lmax + 1, 2 - self.odd_l):
+            res['p%d' % l] = out[idx:idx + self.Nk]
+            idx += self.Nk
+        return res
+
+
+# ----------------------------------------------------------------------------------------------
+# Bispectrum (PolyBin3D/bspec.py)
+# ----------------------------------------------------------------------------------------------
+
+def triangle_bins(k_bins, k_bins_squeeze, lmax, include_partial_triangles=False):
+    """all_bins = [(b1<=b2<=b3, l)], l outer (PolyBin3D/bspec.py:67-75, _check_bin 171-199)."""
+    kb, ks = np.asarray(k_bins, dtype=np.float64), np.asarray(k_bins_squeeze, dtype=np.float64)
+    Nk, Nks = len(kb) - 1, len(ks) - 1
+
+    def check(b1, b2, b3):
+        if include_partial_triangles:
+            hi = ks[b1 + 1] + ks[b2 + 1]
+            if b1 > b2:
+                lo = ks[b1] - ks[b2 + 1]
+            elif b1 == b2:
+                lo = 0.
+            else:
+                lo = ks[b2] - ks[b1 + 1]
+            return not (ks[b3] >= hi - 1e-10 or ks[b3 + 1] <= lo + 1e-10)
+        k1 = 0.5 * (kb[b1] + kb[b1 + 1])
+        k2 = 0.5 * (ks[b2] + ks[b2 + 1])
+        k3 = 0.5 * (ks[b3] + ks[b3 + 1])
+        return not (k3 < abs(k1 - k2) or k3 > k1 + k2)
+
+    out = []
+    for l in range(0, lmax + 1, 2):
+        for b1 in range(Nk):
+            for b2 in range(b1, Nks):
+                for b3 in range(b2, Nks):
+                    if check(b1, b2, b3):
+                        out.append((b1, b2, b3, l))
+    return np.asarray(out, dtype=np.int64).reshape(-1, 4)
+
+
+class BSpec:
+    """FP64 restatement of PolyBin3D.BSpec (PolyBin3D/bspec.py:12-1119), hot-path members only."""
+
+    def __init__(self, base, k_bins, lmax=0, mask=None, applySinv=None, k_bins_squeeze=None,
+                 include_partial_triangles=False, Pk_fid=None, applySinv_transpose=None):
+        b = self.base = base
+        self.k_bins = np.asarray(k_bins, dtype=np.float64)
+        self.lmax = lmax
+        self.Nk = len(k_bins) - 1
+        self.Nl = lmax // 2 + 1
+        self.include_partial_triangles = include_partial_triangles
+        assert lmax % 2 == 0 and lmax <= 4
+        assert np.max(self.k_bins) < np.min(b.kNy) and np.min(self.k_bins) >= np.max(b.kF)
+        if k_bins_squeeze is not None:
+            self.k_bins_squeeze = np.asarray(k_bins_squeeze, dtype=np.float64)
+            assert (self.k_bins_squeeze[:self.Nk + 1] == self.k_bins).all()
+            assert np.max(self.k_bins_squeeze) < np.min(b.kNy)
+        else:
+            self.k_bins_squeeze = self.k_bins.copy()
+        self.Nk_squeeze = len(self.k_bins_squeeze) - 1
+        self.all_bins = triangle_bins(self.k_bins, self.k_bins_squeeze, lmax, include_partial_triangles)
+        self.N_bins = len(self.all_bins)
+        self.N3 = self.N_bins // self.Nl
+        if mask is None:
+            self.const_mask, self.mask_mean, self.cube_mask_mean = True, 1., 1.
+        else:
+            self.mask = np.asarray(mask, dtype=np.float64)
+            if np.std(self.mask) < 1e-12:
+                self.const_mask, self.mask_mean = True, np.mean(self.mask)
+            else:
+                self.const_mask = False
+            self.cube_mask_mean = np.mean(self.mask ** 3)
+        if applySinv is None:
+            self.applySinv = b.applySinv_trivial
+            self.applySinv_transpose = b.applySinv_trivial if applySinv_transpose is None else applySinv_transpose
+        else:
+            self.applySinv = applySinv
+            self.applySinv_transpose = applySinv if applySinv_transpose is None else applySinv_transpose
+        # bspec.py:138-156
+        if isinstance(Pk_fid, str) and Pk_fid == 'default':
+            self.invPk0 = b.invPk0_grid
+        else:
+            if Pk_fid is None:
+                k_tmp = np.arange(np.min(b.kF) / 10., np.max(b.kNy) * 2, np.min(b.kF) / 10.)
+                Pfid = np.asarray([k_tmp, 1. + 0. * k_tmp])
+            else:
+                Pfid = np.asarray(Pk_fid, dtype=np.float64)
+            P = scipy.interpolate.interp1d(Pfid[0], Pfid[1], bounds_error=False, fill_value=0.)(b.modk_grid)
+            self.invPk0 = np.zeros(P.shape)
+            self.invPk0[P != 0] = 1. / P[P != 0]
+        # bspec.py:158-169
+        self.Ylm_real = real_harmonics(np.asarray(b.r_grids), lmax, False) if (b.sightline == 'local' and lmax > 0) else None
+        self.Ylm_fourier = real_harmonics(np.asarray(b.k_grids), lmax, False) if lmax > 0 else None
+
+    def get_ks(self):
+        """bspec.py:201-220."""
+        kc = 0.5 * (self.k_bins[1:] + self.k_bins[:-1])
+        ksc = 0.5 * (self.k_bins_squeeze[1:] + self.k_bins_squeeze[:-1])
+        t = self.all_bins[:self.N3]
+        return np.asarray([kc[t[:, 0]], ksc[t[:, 1]], ksc[t[:, 2]]])
+
+    def _filt(self, m, l, b):
+        ks = self.k_bins_squeeze
+        return self.base.map_utils.fourier_filter(m, l, ks[b], ks[b + 1])
+
+    def _compute_symmetry_factor(self):
+        """bspec.py:222-279."""
+        b = self.base
+        t = self.all_bins[:self.N3]
+        sym = []
+        for (b1, b2, b3, _) in t:
+            sym.append(6. if (b1 == b2 == b3) else (2. if (b1 == b2 or b1 == b3 or b2 == b3) else 1.))
+        if self.lmax > 0:
+            inv = self.invPk0.astype(np.complex128)
+            bins = np.asarray([b.to_real(self._filt(inv, 0, q)) for q in range(self.Nk_squeeze)])
+            for l in range(2, self.lmax + 1, 2):
+                b23 = b.real_zeros(self.Nk_squeeze)
+                ks = self.k_bins_squeeze
+                for q in range(self.Nk_squeeze):
+                    for m in range(len(self.Ylm_fourier[l])):
+                        b23[q] += b.to_real(b.map_utils.prod_fourier_filter(inv, self.Ylm_fourier[l][m], ks[q], ks[q + 1])) ** 2
+                for (b1, b2, b3, _) in t:
+                    if b1 != b2 and b2 != b3:
+                        sym.append(1.)
+                    elif b1 == b2 and b2 != b3:
+                        sym.append(2.)
+                    elif b1 != b2 and b2 == b3:
+                        sym.append(1. + sum_pair(bins[b1], b23[b3]) / sum_triplet(bins[b1], bins[b3], bins[b3]))
+                    else:
+                        sym.append(2. * (1. + 2. * sum_pair(bins[b3], b23[b3]) / sum_triplet(bins[b3], bins[b3], bins[b3])))
+        self.sym_factor = np.asarray(sym)
+        assert len(self.sym_factor) == self.N_bins
+
+    def _g_maps(self, S_fourier):
+        """g_{l,b}(x) maps (bspec.py:931-956)."""
+        b = self.base
+        g = b.real_zeros((self.Nl, self.Nk_squeeze))
+        if b.sightline == 'local' and self.lmax > 0:
+            S_real = b.to_real(S_fourier)
+        for l in range(0, self.lmax + 1, 2):
+            if l == 0:
+                for q in range(self.Nk_squeeze):
+                    g[0, q] = b.to_real(self._filt(S_fourier, 0, q))
+            elif b.sightline == 'global':
+                for q in range(self.Nk_squeeze):
+                    g[l // 2, q] = b.to_real(self._filt(S_fourier, l, q))
+            else:
+                leg = b.apply_fourier_harmonics(S_real, self.Ylm_real[l], self.Ylm_fourier[l])
+                for q in range(self.Nk_squeeze):
+                    g[l // 2, q] = b.to_real(self._filt(leg, 0, q))
+        return g
+
+    def _compute_bk_numerator(self, data, include_linear_term=True, filtering='ideal'):
+        """bspec.py:895-983."""
+        b = self.base
+        if include_linear_term and not hasattr(self, 'preload'):
+            raise Exception("Need to generate or specify bias simulations!")
+        data = np.asarray(data, dtype=np.float64)
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        if b.pixel_window != 'none':
+            df = MapUtils.div_fourier(b.to_fourier(data), b.pixel_window_grid)
+            S = (df * self.invPk0 / self.cube_mask_mean ** (1. / 3.) if filtering == 'ideal'
+                 else self.applySinv(df, input_type='fourier', output_type='fourier'))
+        else:
+            S = (b.to_fourier(data) * self.invPk0 / self.cube_mask_mean ** (1. / 3.) if filtering == 'ideal'
+                 else self.applySinv(data, input_type='real', output_type='fourier'))
+        g = self._g_maps(S)
+        B3 = assemble_b3(g, self.all_bins)
+        B1 = np.zeros(self.N_bins)
+        if include_linear_term:
+            for ii in range(self.N_it):
+                B1 += -1. / self.N_it * assemble_b1(g, self.g_bl_maps[ii], self.all_bins)
+        return (B3 + B1) * b.volume / b.gridsize.prod() / self.sym_factor
+
+    def Bk_numerator_ideal(self, data):
+        return self._compute_bk_numerator(data, include_linear_term=False, filtering='ideal')
+
+    def Bk_numerator(self, data, include_linear_term=True):
+        return self._compute_bk_numerator(data, include_linear_term=include_linear_term, filtering='Sinv')
+
+    def compute_fisher_ideal(self, discreteness_correction=True):
+        """bspec.py:999-1083."""
+        b = self.base
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        inv = self.invPk0.astype(np.complex128)
+        ks = self.k_bins_squeeze
+        N3 = self.N3
+        if discreteness_correction:
+            fish = np.zeros((self.N_bins, self.N_bins))
+            bins_l = np.asarray([[b.to_real(self._filt(inv, l, q)) for q in range(self.Nk_squeeze)]
+                                 for l in range(0, self.lmax + 1, 2)])
+            for l in range(0, self.lmax + 1, 2):
+                for lp in range(l, self.lmax + 1, 2):
+                    bllp = np.asarray([b.to_real(b.map_utils.fourier_filter_ll(inv, l, lp, ks[q], ks[q + 1]))
+                                       for q in range(self.Nk_squeeze)])
+                    for i in range(N3):
+                        b1, b2, b3 = self.all_bins[i, :3]
+                        r, c = l // 2 * N3 + i, lp // 2 * N3 + i
+                        if b1 != b2 and b2 != b3:
+                            fish[r, c] = sum_triplet(bins_l[0][b1], bins_l[0][b2], bllp[b3])
+                        elif b1 == b2 and b2 != b3:
+                            fish[r, c] = 2. * sum_triplet(bins_l[0][b1], bins_l[0][b2], bllp[b3])
+                        elif b1 != b2 and b2 == b3:
+                            fish[r, c] = (sum_triplet(bins_l[0][b1], bins_l[l // 2][b2], bins_l[lp // 2][b3])
+                                          + sum_triplet(bins_l[0][b1], bins_l[0][b2], bllp[b3]))
+                        else:
+                            fish[r, c] = (4. * sum_triplet(bins_l[0][b1], bins_l[l // 2][b2], bins_l[lp // 2][b3])
+                                          + 2. * sum_triplet(bins_l[0][b1], bins_l[0][b2], bllp[b3]))
+            fish = fish + fish.T - np.diag(np.diag(fish))
+            fish = fish / np.outer(self.sym_factor, self.sym_factor) * b.gridsize.prod() ** 2 / b.volume
+        else:
+            bins = np.asarray([b.to_real(self._filt(inv, 0, q)) for q in range(self.Nk_squeeze)])
+            Ntri = np.asarray([sum_triplet(bins[b1], bins[b2], bins[b3]) for (b1, b2, b3, _) in self.all_bins[:N3]])
+            d = np.concatenate([Ntri / (2. * l + 1.) for l in range(0, self.lmax + 1, 2)])
+            fish = np.diag(d / self.sym_factor) * b.gridsize.prod() ** 2 / b.volume
+        self.fish_ideal = fish
+        self.inv_fish_ideal = np.linalg.inv(fish)
+        self.discreteness_correction = discreteness_correction
+        return fish
+
+    def Bk_ideal(self, data, fish_ideal=(), discreteness_correction=True):
+        """bspec.py:1085-1119."""
+        if len(fish_ideal) != 0:
+            self.fish_ideal = np.asarray(fish_ideal)
+            self.inv_fish_ideal = np.linalg.inv(self.fish_ideal)
+            self.discreteness_correction = discreteness_correction
+        if not hasattr(self, 'inv_fish_ideal') or self.discreteness_correction != discreteness_correction:
+            self.compute_fisher_ideal(discreteness_correction)
+        out = self.inv_fish_ideal @ self.Bk_numerator_ideal(data)
+        return {'b%d' % l: out[i * self.N3:(i + 1) * self.N3] for i, l in enumerate(range(0, self.lmax + 1, 2))}
+
+    # --- linear term simulations ------------------------------------------------------------
+    def apply_pointing(self, m, transpose=False):
+        return m * self.mask
+
+    def _process_sim(self, sim, input_type='real'):
+        """bspec.py:281-326."""
+        b = self.base
+        if b.pixel_window != 'none':
+            sf = b.to_fourier(sim) if input_type == 'real' else sim.copy()
+            sf = sf / b.pixel_window_grid
+            S = self.applySinv(sf, input_type='fourier', output_type='fourier')
+        else:
+            S = self.applySinv(sim, input_type=input_type, output_type='fourier')
+        return self._g_maps(S)
+
+    def generate_sims(self, N_sim, Pk_input=()):
+        """bspec.py:362-410 (preload=True)."""
+        b = self.base
+        self.N_it = N_sim
+        self.preload = True
+        maps = []
+        for ii in range(N_sim):
+            if self.const_mask:
+                sf = self.mask_mean * b.generate_data(int(1e5) + ii, Pk_input=Pk_input, output_type='fourier')
+                maps.append(self._process_sim(sf, input_type='fourier'))
+            else:
+                sr = self.apply_pointing(b.generate_data(int(1e5) + ii, Pk_input=Pk_input, output_type='real',
+                                                         include_pixel_window=False))
+                if b.pixel_window != 'none':
+                    maps.append(self._process_sim(b.to_fourier(sr) * b.pixel_window_grid, input_type='fourier'))
+                else:
+                    maps.append(self._process_sim(sr, input_type='real'))
+        self.g_bl_maps = np.asarray(maps)
+
+    # --- MC Fisher ------------------------------------------------------------------------
+    def compute_fisher_contribution(self, seed, verb=False):
+        """bspec.py:441-445 -> _compute_fisher (bspec.py:473-676, compute_cov=False)."""
+        b = self.base
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        fish = np.zeros((self.N_bins, self.N_bins))
+        ks = self.k_bins_squeeze
+        ab = self.all_bins
+
+        def apply_filter(m):                                                       # 499-505
+            if self.const_mask:
+                return self.mask_mean * self.applySinv(m, input_type='fourier', output_type='fourier')
+            return self.applySinv(self.apply_pointing(m), input_type='real', output_type='fourier')
+
+        Q_Ainv = np.zeros((self.N_bins,) + b.modk_grid.shape, dtype=np.complex128)   # 643
+
+        def compute_Q(weighting, af1, ar1, af2, ar2):                              # 521-640
+            if weighting == 'Sinv':
+                w1 = apply_filter(af1 if self.const_mask else ar1)
+                w2 = apply_filter(af2 if self.const_mask else ar2)
+            else:
+                w1 = b.applyAinv(af1, input_type='fourier', output_type='fourier')
+                w2 = b.applyAinv(af2, input_type='fourier', output_type='fourier')
+            local = b.sightline == 'local' and self.lmax > 0
+            if local:
+                wr1, wr2 = b.to_real(w1), b.to_real(w2)
+            g1 = np.asarray([b.to_real(self._filt(w1, 0, q)) for q in range(self.Nk_squeeze)])
+            g2 = np.asarray([b.to_real(self._filt(w2, 0, q)) for q in range(self.Nk_squeeze)])
+            if local:
+                leg1 = [b.apply_fourier_harmonics(wr1, self.Ylm_real[l], self.Ylm_fourier[l]) for l in range(2, self.lmax + 1, 2)]
+                leg2 = [b.apply_fourier_harmonics(wr2, self.Ylm_real[l], self.Ylm_fourier[l]) for l in range(2, self.lmax + 1, 2)]
+            for B in range(self.Nk_squeeze):
+                if self.lmax > 0:
+                    gB1, gB2 = [], []
+                    for l in range(2, self.lmax + 1, 2):
+                        if b.sightline == 'global':
+                            gB1.append(b.to_real(self._filt(w1, l, B)))
+                            gB2.append(b.to_real(self._filt(w2, l, B)))
+                        else:
+                            gB1.append(b.to_real(self._filt(leg1[l // 2 - 1], 0, B)))
+                            gB2.append(b.to_real(self._filt(leg2[l // 2 - 1], 0, B)))
+                for A in range(B + 1):
+                    if weighting == 'Sinv':
+                        ind = np.where((ab[:, 2] == B) & (ab[:, 0] == A))[0]
+                        if len(ind) == 0:
+                            continue
+                        for l in range(0, self.lmax + 1, 2):
+                            if l == 0:
+                                ft = b.to_fourier(g1[A] * g1[B] - g2[A] * g2[B])
+                            else:
+                                ft = b.to_fourier(g1[A] * gB1[l // 2 - 1] - g2[A] * gB2[l // 2 - 1])
+                            for ii in ind[ab[ind, 3] == l]:
+                                q2 = ab[ii, 1]
+                                fish[ii] = b.integrator.integrate_row(ft, Q_Ainv, ks[q2], ks[q2 + 1])
+                    else:
+                        i1 = np.where((ab[:, 0] == A) & (ab[:, 1] == B))[0]
+                        i2 = np.where((ab[:, 1] == A) & (ab[:, 2] == B))[0]
+                        i3 = np.where((ab[:, 2] == B) & (ab[:, 0] == A))[0]
+                        if len(i1) + len(i2) + len(i3) == 0:
+                            continue
+                        ft = [b.to_fourier(g1[A] * g1[B] - g2[A] * g2[B])]
+                        for l in range(2, self.lmax + 1, 2):
+                            ft.append(b.to_fourier(g1[A] * gB1[l // 2 - 1] - g2[A] * gB2[l // 2 - 1]))
+
+                        def add_Q(ci, inds, largest):
+                            for ii in inds:
+                                C = ab[ii, ci]
+                                l = ab[ii, 3]
+                                if l == 0 or not largest:
+                                    Q_Ainv[ii] += self._filt(ft[l // 2], 0, C)
+                                elif b.sightline == 'global':
+                                    Q_Ainv[ii] += self._filt(ft[0], l, C)
+                                else:
+                                    fC = self._filt(ft[0], 0, C)
+                                    Q_Ainv[ii] += b.to_fourier(b.apply_real_harmonics(fC, self.Ylm_real[l], self.Ylm_fourier[l]))
+                        add_Q(2, i1, True)
+                        add_Q(0, i2, False)
+                        add_Q(1, i3, False)
+
+        a1 = b.generate_data(seed=seed + int(1e7), output_type='fourier')            # 648-649
+        a2 = b.generate_data(seed=seed + int(2e7), output_type='fourier')
+        ar1 = ar2 = None
+        if not self.const_mask:
+            ar1, ar2 = b.to_real(a1), b.to_real(a2)
+        compute_Q('Ainv', a1, ar1, a2, ar2)                                         # 659
+        for i in range(self.N_bins):                                                # 662-666
+            Q_Ainv[i] = apply_filter(Q_Ainv[i]) if self.const_mask else apply_filter(b.to_real(Q_Ainv[i]))
+        compute_Q('Sinv', a1, ar1, a2, ar2)                                         # 671
+        return 0.5 * fish * b.volume / b.gridsize.prod() ** 2 / np.outer(self.sym_factor, self.sym_factor)
+
+    def compute_fisher(self, N_it, verb=False):
+        """bspec.py:678-693."""
+        fish = np.zeros((self.N_bins, self.N_bins))
+        for seed in range(N_it):
+            fish += self.compute_fisher_contribution(seed) / N_it
+        self.fish = fish
+        self.inv_fish = np.linalg.inv(fish)
+        return fish
+
+    def Bk_unwindowed(self, data, fish=(), include_linear_term=True):
+        """bspec.py:863-893."""
+        if len(fish) != 0:
+            self.fish = np.asarray(fish)
+            self.inv_fish = np.linalg.inv(self.fish)
+        if not hasattr(self, 'inv_fish'):
+            raise Exception("Need to compute Fisher matrix first!")
+        out = self.inv_fish @ self.Bk_numerator(data, include_linear_term=include_linear_term)
+        return {'b%d' % l: out[i * self.N3:(i + 1) * self.N3] for i, l in enumerate(range(0, self.lmax + 1, 2))}

@@ -1,0 +1,492 @@
+"""PolyBin3D base class, backend='hip' (MI355X).
+
+Mirrors PolyBin3D.PolyBin3D (PolyBin3D/base.py:6-437): the same constructor arguments, grids,
+FFT counters, GRF generator and filtering helpers.  Maps live on the GPU as torch tensors
+(float32 real grids [n0,n1,n2], complex64 half grids [n0,n1,n2/2+1]); every transform and map
+operation is a hand-written HIP kernel in libpb3d.so reached through ``_capi``.  Host numpy is used
+only for set-up tables that must be bit-identical to the reference (FP64 |k| bin membership,
+interpolated fiducial spectra) and for the reference's numpy random stream.
+"""
+import os
+
+import numpy as np
+import scipy.interpolate
+import torch
+
+from . import _capi as C
+
+PIXEL_WINDOWS = ['none', 'cic', 'tsc', 'pcs', 'interlaced-cic', 'interlaced-tsc', 'interlaced-pcs']
+
+
+def _pixel_window_1d(kind, x):
+    """Voxel windows with first-order alias corrections at x = pi m / n (PolyBin3D/base.py:231-254)."""
+    if kind == 'cic':
+        s = np.sin(x) ** 2
+        return (1. - 2. / 3. * s) ** 0.5
+    if kind == 'tsc':
+        s = np.sin(x) ** 2
+        return (1. - s + 2. / 15 * s ** 2.) ** 0.5
+    if kind == 'pcs':
+        s = np.sin(x) ** 2
+        return (1. - 4. / 3. * s + 2. / 5. * s ** 2 - 4. / 315. * s ** 3) ** 0.5
+    if kind == 'interlaced-cic':
+        s = np.sinc(x / np.pi)
+        v = np.ones_like(s)
+        v[s != 0.] = s[s != 0.] ** 2.
+        return v
+    if kind == 'interlaced-tsc':
+        return np.sinc(x / np.pi) ** 3.
+    if kind == 'interlaced-pcs':
+        return np.sinc(x / np.pi) ** 4.
+    raise Exception("Unknown window function '%s' supplied!" % kind)
+
+
+class YlmTag:
+    """Stands for the (2l+1) real spherical harmonics of order l on the real ('real') or Fourier
+    ('fourier') grid.  The reference materialises these as arrays (compute_real_harmonics,
+    PolyBin3D/cython/utils.pyx:518-609); here the kernels evaluate Y_lm in place, so the tables
+    are only labels.  len() gives 2l+1 as the reference arrays do."""
+
+    def __init__(self, ell, space):
+        self.ell, self.space = int(ell), space
+
+    def __len__(self):
+        return 2 * self.ell + 1
+
+
+def harmonics_tags(lmax, odd_l, space):
+    """Counterpart of compute_real_harmonics(coords, lmax, odd_l) (utils.pyx:518): {l: tag}."""
+    return {l: YlmTag(l, space) for l in range(1, lmax + 1) if (l % 2 == 0 or odd_l)}
+
+
+class PolyBin3D:
+    """Base class (PolyBin3D/base.py:6-144) with backend='hip'.
+
+    Inputs as in the reference: boxsize, gridsize, Pk=[k, P0(k)] (None = unit power), boxcenter,
+    pixel_window, backend (only 'hip'), nthreads (unused on the GPU), sightline, real_fft (must be
+    True).  Extra: device (default 'cuda'), rng ('numpy' reproduces the reference's legacy
+    RandomState stream; 'device' draws on the GPU with the same distribution).
+    """
+
+    def __init__(self, boxsize, gridsize, Pk=None, boxcenter=None, pixel_window='none', backend='hip',
+                 nthreads=None, sightline='global', real_fft=True, device=None, rng='numpy', verb=False):
+        if backend != 'hip':
+            raise Exception("polybin3d_amd implements backend='hip' only")
+        if not real_fft:
+            raise Exception("backend='hip' supports real-to-complex FFTs only (real_fft=True)")
+        self.backend = backend
+        self.nthreads = nthreads if nthreads is not None else os.cpu_count()
+        self.sightline = sightline
+        self.pixel_window = pixel_window
+        self.real_fft = True
+        self.rng = rng
+        self.device = torch.device(device if device is not None else 'cuda')
+        if self.device.type != 'cuda':
+            raise Exception("backend='hip' needs a GPU device")
+        # base.py:31-60
+        self.boxsize = np.asarray([boxsize] * 3 if np.ndim(boxsize) == 0 else boxsize, dtype=np.float64)
+        assert len(self.boxsize) == 3, "Need to specify x, y, z boxsize"
+        if boxcenter is None:
+            self.boxcenter = np.zeros(3)
+        else:
+            self.boxcenter = np.asarray([boxcenter] * 3 if np.ndim(boxcenter) == 0 else boxcenter, dtype=np.float64)
+            assert len(self.boxcenter) == 3, "Need to specify x, y, z boxcenter"
+        self.gridsize = np.asarray([gridsize] * 3 if np.ndim(gridsize) == 0 else gridsize, dtype=int)
+        assert len(self.gridsize) == 3, "Need to specify n_x, n_y, n_z gridsize"
+        if self.gridsize[2] % 2 != 0:
+            raise Exception("Last dimension of grid must be even for real-to-complex FFTs.")
+        assert self.sightline in ['local', 'global'], "Sight-line must be 'local' or 'global'"
+        if self.sightline == 'global':
+            assert np.abs(self.boxcenter).sum() == 0., "Global sightlines should only be used for simulation data!"
+        else:
+            assert np.abs(self.boxcenter).sum() != 0., "Local sightlines should only be used for lightcone data!"
+        # base.py:63-74
+        self.volume = self.boxsize.prod()
+        self.kF = 2. * np.pi / self.boxsize
+        self.kNy = self.gridsize * self.kF * 0.5
+        if Pk is None:
+            k_tmp = np.arange(0., np.max(self.kNy) * 2, np.min(self.kF) / 10.)
+            self.Pfid = np.asarray([k_tmp, 1. + 0. * k_tmp])
+        else:
+            assert len(Pk) == 2, "Pk should contain k and P_0(k) columns"
+            self.Pfid = np.asarray(Pk, dtype=np.float64)
+        self.kmin, self.kmax = np.min(self.Pfid[0, 0]), np.max(self.Pfid[0, -1])
+        if verb:
+            print("# Dimensions: [%.2e, %.2e, %.2e] Mpc/h" % tuple(self.boxsize))
+        # Fourier grid (base.py:86-94): fftshift(arange(-n//2, n//2))*kF on axes 0,1, arange on axis 2
+        n = self.gridsize
+        self.k_arrs = [np.fft.fftshift(np.arange(-n[i] // 2, n[i] // 2)) * self.kF[i] for i in range(2)]
+        self.k_arrs += [np.arange(0, n[2] // 2 + 1) * self.kF[2]]
+        self.half_shape = (int(n[0]), int(n[1]), int(n[2] // 2 + 1))
+        self.H = int(np.prod(self.half_shape))
+        self.N = int(np.prod(n))
+        self._modk = None
+        self._muk = None
+        # line of sight (base.py:111-116)
+        if self.sightline == 'global':
+            self.sight_vector = np.asarray([0., 0., 1.])
+        else:
+            self.sight_vector = self.boxcenter / np.sqrt(np.sum(self.boxcenter ** 2))
+        assert self.pixel_window in PIXEL_WINDOWS, "Unknown pixel window '%s' supplied!" % pixel_window
+        self.cgrid = C.make_grid(n, self.boxsize, self.boxcenter, self.sight_vector, self.sightline == 'local')
+        plan = C.c_void_p()
+        C.check(C.lib().pb3d_plan_create(C.ctypes.byref(plan), C.ctypes.byref(self.cgrid)))
+        self._plan = plan
+        # fiducial spectrum on the grid (base.py:122-131), FP64 on host then FP32 on device
+        modk = self.modk_grid
+        Pk0 = scipy.interpolate.interp1d(self.Pfid[0], self.Pfid[1], bounds_error=False, fill_value=0.)(modk)
+        invPk0 = np.zeros(modk.shape)
+        good = (Pk0 != 0) & np.isfinite(Pk0)
+        invPk0[good] = 1. / Pk0[good]
+        Pk0[~np.isfinite(Pk0)] = 0.
+        invPk0[modk == 0] = 1.
+        Pk0[modk == 0] = 1.
+        self.Pk0_grid_host = Pk0
+        self.invPk0_grid_host = invPk0
+        self.invPk0_is_one = bool(np.all(invPk0 == 1.))
+        self.invPk0_grid = torch.from_numpy(invPk0.astype(np.float32)).to(self.device)
+        if self.pixel_window != 'none':
+            w1 = [_pixel_window_1d(self.pixel_window, np.pi / n[i] * self.k_arrs[i] / self.kF[i]) for i in range(3)]
+            pw = np.asarray(np.meshgrid(*w1, indexing='ij')).prod(axis=0)
+            self.pixel_window_grid_host = pw
+            self.pixel_window_grid = torch.from_numpy(pw.astype(np.float32)).to(self.device)
+        self.n_FFTs_forward = 0
+        self.n_FFTs_reverse = 0
+        self._bin_tables = {}
+        self._bufs = {}
+        self.integrator = IntegrationUtils(self)
+        self.map_utils = MapUtils(self)
+
+    def __del__(self):
+        try:
+            if getattr(self, '_plan', None) is not None and C._lib is not None:
+                C.lib().pb3d_plan_destroy(self._plan)
+                self._plan = None
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ host-side grids
+    @property
+    def k_grids(self):
+        return np.meshgrid(*self.k_arrs, indexing='ij')
+
+    @property
+    def modk_grid(self):
+        """|k| in FP64 exactly as the reference computes it (base.py:91-92)."""
+        if self._modk is None:
+            kg = self.k_grids
+            self._modk = np.sqrt(kg[0] ** 2 + kg[1] ** 2 + kg[2] ** 2)
+        return self._modk
+
+    @property
+    def muk_grid(self):
+        if self._muk is None:
+            kg = self.k_grids
+            modk = self.modk_grid
+            mu = np.zeros(modk.shape)
+            nz = modk != 0
+            mu[nz] = np.sum(self.sight_vector[:, None] * np.asarray(kg)[:, nz], axis=0) / modk[nz]
+            self._muk = mu
+        return self._muk
+
+    @property
+    def r_grids(self):
+        n = self.gridsize
+        r = [np.fft.fftshift(np.arange(-n[i] // 2, n[i] // 2)) * self.boxsize[i] / n[i] + self.boxcenter[i]
+             for i in range(3)]
+        return np.meshgrid(*r, indexing='ij')
+
+    def bin_table(self, edges):
+        """uint8 device table: index b with edges[b] <= |k| < edges[b+1] (FP64, as
+        utils.pyx:63), or 255.  Cached per edge list."""
+        edges = np.asarray(edges, dtype=np.float64)
+        key = tuple(edges.tolist())
+        if key not in self._bin_tables:
+            nb = len(edges) - 1
+            if nb < 1 or nb > 254:
+                raise ValueError("between 1 and 254 bins supported")
+            if not np.all(np.diff(edges) > 0):
+                raise ValueError("bin edges must be strictly increasing")
+            idx = np.searchsorted(edges, self.modk_grid, side='right') - 1
+            idx[(idx < 0) | (idx >= nb)] = C.PB3D_NO_BIN
+            self._bin_tables[key] = torch.from_numpy(idx.astype(np.uint8)).to(self.device)
+        return self._bin_tables[key]
+
+    # ------------------------------------------------------------------ buffers
+    def real_zeros(self, size=None, numpy=False):
+        shape = self._shape(size, tuple(int(v) for v in self.gridsize))
+        return np.zeros(shape) if numpy else torch.zeros(shape, dtype=torch.float32, device=self.device)
+
+    def complex_zeros(self, size=None, numpy=False):
+        shape = self._shape(size, self.half_shape)
+        return np.zeros(shape, dtype=np.complex128) if numpy else torch.zeros(shape, dtype=torch.complex64,
+                                                                                 device=self.device)
+
+    @staticmethod
+    def _shape(size, base):
+        if size is None:
+            return base
+        if np.ndim(size) == 0:
+            return (int(size),) + base
+        return tuple(int(s) for s in size) + base
+
+    def work(self, name, nbytes):
+        """Reusable device scratch (stream-ordered use only)."""
+        b = self._bufs.get(name)
+        if b is None or b.numel() < nbytes:
+            b = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=self.device)
+            self._bufs[name] = b
+        return b
+
+    def stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    # ------------------------------------------------------------------ data movement
+    def as_real(self, x):
+        """float32 contiguous device tensor of shape [...,n0,n1,n2]."""
+        if isinstance(x, np.ndarray):
+            x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32))
+        if not torch.is_tensor(x):
+            raise TypeError("expected a numpy array or torch tensor")
+        x = x.to(device=self.device, dtype=torch.float32).contiguous()
+        if tuple(x.shape[-3:]) != tuple(int(v) for v in self.gridsize):
+            raise ValueError("real map has shape %s, grid is %s" % (tuple(x.shape), tuple(self.gridsize)))
+        return x
+
+    def as_fourier(self, x):
+        """complex64 contiguous device tensor of shape [...,n0,n1,n2/2+1]."""
+        if isinstance(x, np.ndarray):
+            x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex64))
+        if not torch.is_tensor(x):
+            raise TypeError("expected a numpy array or torch tensor")
+        x = x.to(device=self.device, dtype=torch.complex64).contiguous()
+        if tuple(x.shape[-3:]) != self.half_shape:
+            raise ValueError("Fourier map has shape %s, half grid is %s" % (tuple(x.shape), self.half_shape))
+        return x
+
+    # ------------------------------------------------------------------ FFTs
+    def r2c(self, x, out=None, rop=None, kop=None, batch=None, accumulate=False):
+        """Batched fused R2C.  x: real tensor (or None when rop.a is set)."""
+        if batch is None:
+            batch = 1 if x is None or x.dim() == 3 else int(np.prod(x.shape[:-3]))
+        if out is None:
+            out = torch.empty((batch,) + self.half_shape if batch > 1 else self.half_shape,
+                              dtype=torch.complex64, device=self.device)
+        needs_work = kop is not None and bool(kop.accumulate)
+        work = self.work('fft_r2c', batch * self.H * 8) if needs_work else None
+        C.check(C.lib().pb3d_fft_r2c(self._plan, C._ptr(x), out.data_ptr(), C._ptr(work), batch,
+                                     C.byref_or_none(rop), C.byref_or_none(kop), self.stream()))
+        self.n_FFTs_forward += batch
+        return out
+
+    def c2r(self, k, out=None, kop=None, rop=None, batch=None):
+        """Batched fused C2R with 1/N."""
+        if batch is None:
+            batch = 1 if k.dim() == 3 else int(np.prod(k.shape[:-3]))
+        if out is None:
+            g = tuple(int(v) for v in self.gridsize)
+            out = torch.empty((batch,) + g if batch > 1 else g, dtype=torch.float32, device=self.device)
+        work = self.work('fft_c2r', batch * self.H * 8)
+        C.check(C.lib().pb3d_fft_c2r(self._plan, k.data_ptr(), out.data_ptr(), work.data_ptr(), batch,
+                                     C.byref_or_none(kop), C.byref_or_none(rop), self.stream()))
+        self.n_FFTs_reverse += batch
+        return out
+
+    def to_fourier(self, _input_rmap):
+        """Unnormalised R2C (base.py:176-200); float32 maps on the GPU."""
+        x = self.as_real(_input_rmap)
+        return self.r2c(x)
+
+    def to_real(self, _input_kmap):
+        """C2R with 1/N (base.py:202-229)."""
+        k = self.as_fourier(_input_kmap)
+        return self.c2r(k)
+
+    def kmap(self, x, kop, out=None, batch=1):
+        if out is None:
+            out = torch.empty((batch,) + self.half_shape if batch > 1 else self.half_shape,
+                              dtype=torch.complex64, device=self.device)
+        C.check(C.lib().pb3d_kmap(C.ctypes.byref(self.cgrid), x.data_ptr(), out.data_ptr(), batch,
+                                  C.ctypes.byref(kop), self.stream()))
+        return out
+
+    # ------------------------------------------------------------------ GRFs
+    def _Pk_grid_host(self, Pk_input):
+        if len(Pk_input) == 0:
+            return self.Pk0_grid_host
+        assert len(np.asarray(Pk_input).shape) == 2, "Pk should contain k and P_0, (and optionally P_2, P_4) columns"
+        assert len(Pk_input) in [2, 3, 4], "Pk should contain k and P_0, (and optionally P_2, P_4) columns"
+        modk = self.modk_grid
+        P = scipy.interpolate.interp1d(Pk_input[0], Pk_input[1], bounds_error=False, fill_value=0.)(modk)
+        mu = self.muk_grid
+        if len(Pk_input) > 2:
+            P = P + 0.5 * (3 * mu ** 2 - 1) * scipy.interpolate.interp1d(
+                Pk_input[0], Pk_input[2], bounds_error=False, fill_value=0.)(modk)
+        if len(Pk_input) > 3:
+            P = P + (3 - 30 * mu ** 2 + 35 * mu ** 4) / 8. * scipy.interpolate.interp1d(
+                Pk_input[0], Pk_input[3], bounds_error=False, fill_value=0.)(modk)
+        return P
+
+    def generate_data(self, seed=None, Pk_input=[], output_type='real', include_pixel_window=True):
+        """Gaussian random field (base.py:305-353).  rng='numpy': the reference's legacy
+        RandomState(seed).randn stream, drawn on the host and symmetrised on the GPU."""
+        assert output_type in ['fourier', 'real'], "Valid output types are 'fourier' and 'real' only!"
+        key = ('sqrtPw', tuple(map(tuple, np.asarray(Pk_input))) if len(Pk_input) else None,
+               bool(include_pixel_window and self.pixel_window != 'none'))
+        if key not in self._bufs:
+            P = self._Pk_grid_host(Pk_input)
+            with np.errstate(invalid='ignore'):
+                s = np.sqrt(P)
+            if self.pixel_window != 'none' and include_pixel_window:
+                s = s * self.pixel_window_grid_host
+            self._bufs[key] = torch.from_numpy(s.astype(np.float32)).to(self.device)
+        sqrtPw = self._bufs[key]
+        n = tuple(int(v) for v in self.gridsize)
+        rs = np.random.RandomState(seed) if seed is not None else np.random.mtrand._rand
+        re = torch.from_numpy(rs.randn(*n)).to(self.device)
+        im = torch.from_numpy(rs.randn(*n)).to(self.device)
+        out = torch.empty(self.half_shape, dtype=torch.complex64, device=self.device)
+        norm = float(self.N / np.sqrt(self.volume))
+        C.check(C.lib().pb3d_grf_hermitian(C.ctypes.byref(self.cgrid), re.data_ptr(), im.data_ptr(),
+                                           sqrtPw.data_ptr(), norm, out.data_ptr(), self.stream()))
+        if output_type == 'fourier':
+            return out
+        return self.c2r(out)
+
+    # ------------------------------------------------------------------ filters
+    def applySinv_trivial(self, input_data, input_type='real', output_type='real'):
+        """base.py:355-362."""
+        if input_type == output_type:
+            return input_data
+        if input_type == 'fourier':
+            return self.to_real(input_data)
+        return self.to_fourier(input_data)
+
+    def ainv_weight(self, invPk0_grid=None):
+        """invPk0 / W^2 as a device FP32 half grid, or None when it is identically one
+        (applyAinv, base.py:364-389)."""
+        if invPk0_grid is None and self.pixel_window == 'none':
+            return None if self.invPk0_is_one else self.invPk0_grid
+        key = ('ainv', id(invPk0_grid))
+        if key not in self._bufs:
+            inv = self.invPk0_grid_host if invPk0_grid is None else np.asarray(invPk0_grid)
+            w = inv.copy()
+            if self.pixel_window != 'none':
+                pw = self.pixel_window_grid_host
+                nz = pw != 0
+                w[nz] = w[nz] / pw[nz] ** 2
+                w[~nz] = 0.
+            self._bufs[key] = torch.from_numpy(w.astype(np.float32)).to(self.device)
+        return self._bufs[key]
+
+    def applyAinv(self, input_data, invPk0_grid=None, input_type='real', output_type='real'):
+        """base.py:364-389: multiply by invPk0 and divide by the pixel window twice."""
+        w = self.ainv_weight(invPk0_grid)
+        op = C.kop(C.KOP_WEIGHT, w=w) if w is not None else None
+        if input_type == 'real':
+            f = self.r2c(self.as_real(input_data), kop=op)
+        else:
+            f = self.as_fourier(input_data)
+            f = self.kmap(f, op) if op is not None else f.clone()
+        return self.c2r(f) if output_type == 'real' else f
+
+    def apply_fourier_harmonics(self, a_map_real, Ylm_real, Ylm_fourier, out=None):
+        """sum_m FFT[a(x) Y_lm(x)] Y_lm(k) (base.py:402-411), one fused R2C per m."""
+        ell = Ylm_real.ell if isinstance(Ylm_real, YlmTag) else int(Ylm_real)
+        a = self.as_real(a_map_real)
+        if out is None:
+            out = torch.empty(self.half_shape, dtype=torch.complex64, device=self.device)
+        for m in range(2 * ell + 1):
+            self.r2c(a, out=out, rop=C.rop(C.ROP_YLM, ell=ell, m=m),
+                     kop=C.kop(C.KOP_YLM, ell=ell, m=m, accumulate=m > 0))
+        return out
+
+    def apply_real_harmonics(self, a_map_fourier, Ylm_real, Ylm_fourier, out=None):
+        """sum_m IFFT[a(k) Y_lm(k)] Y_lm(x) (base.py:391-400), one fused C2R per m."""
+        ell = Ylm_real.ell if isinstance(Ylm_real, YlmTag) else int(Ylm_real)
+        a = self.as_fourier(a_map_fourier)
+        if out is None:
+            out = torch.empty(tuple(int(v) for v in self.gridsize), dtype=torch.float32, device=self.device)
+        for m in range(2 * ell + 1):
+            self.c2r(a, out=out, kop=C.kop(C.KOP_YLM, ell=ell, m=m, x_batch_stride=0),
+                     rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=m > 0))
+        return out
+
+    # ------------------------------------------------------------------ reductions
+    def bin_reduce(self, x, edges, ells, y=None, ells2=None, real=False, y_ostride=None, y_qstride=0,
+                   nq=None):
+        """out[q][o][b] = sum_{k in b} degk Re[x_q conj(y_{q,o})] L_ells[o] L_ells2[o]  (FP64 tensor)."""
+        table = self.bin_table(edges)
+        nb = len(edges) - 1
+        nout = len(ells)
+        if nq is None:
+            nq = 1 if x.dim() == 3 else int(np.prod(x.shape[:-3]))
+        xs = self.H
+        ws = C.lib().pb3d_bin_reduce_work_size(C.ctypes.byref(self.cgrid), nq, nout, nb)
+        work = self.work('bins', ws)
+        out = torch.empty((nq, nout, nb), dtype=torch.float64, device=self.device)
+        l1 = C.int_array(ells)
+        l2 = C.int_array(ells2 if ells2 is not None else [0] * nout)
+        if y_ostride is None:
+            y_ostride = self.H
+        C.check(C.lib().pb3d_bin_reduce(C.ctypes.byref(self.cgrid), x.data_ptr(), xs, C._ptr(y), y_qstride,
+                                        y_ostride, nq, nout, l1, l2, 1 if real else 0, table.data_ptr(), nb,
+                                        out.data_ptr(), work.data_ptr(), self.stream()))
+        return out
+
+
+class IntegrationUtils:
+    """The reference's IntegrationUtils contract (PolyBin3D/cython/utils.pyx:17-179) on GPU maps:
+    one fused pass over the half grid per call instead of one sweep per bin."""
+
+    def __init__(self, base):
+        self.base = base
+
+    def cross_integrate(self, spec1, spec2, k_edges, l):
+        b = self.base
+        return b.bin_reduce(b.as_fourier(spec1), k_edges, [l], y=b.as_fourier(spec2))[0, 0].cpu().numpy()
+
+    def integrate(self, spec, k_edges, l):
+        b = self.base
+        return b.bin_reduce(b.as_fourier(spec), k_edges, [l])[0, 0].cpu().numpy()
+
+    def real_integrate_double(self, spec, l1, l2, k_edges):
+        b = self.base
+        s = spec if torch.is_tensor(spec) else torch.from_numpy(np.asarray(spec, dtype=np.float32))
+        s = s.to(device=b.device, dtype=torch.float32).contiguous()
+        return b.bin_reduce(s, k_edges, [l1], ells2=[l2], real=True)[0, 0].cpu().numpy()
+
+    def integrate_row(self, input_map, input_array, kmin, kmax):
+        """out[j] = sum_{kmin<=k<kmax} degk Re[conj(m) Q_j] for every map Q_j (utils.pyx:162-179)."""
+        b = self.base
+        m = b.as_fourier(input_map)
+        Q = b.as_fourier(input_array)
+        nq = Q.shape[0]
+        r = b.bin_reduce(Q, [kmin, kmax], [0], y=m, nq=nq, y_qstride=0)
+        return r[:, 0, 0].cpu().numpy()
+
+
+class MapUtils:
+    """The reference's MapUtils contract (PolyBin3D/cython/utils.pyx:184-383) on GPU maps."""
+
+    def __init__(self, base):
+        self.base = base
+
+    def fourier_filter(self, kmap, l, kmin, kmax):
+        b = self.base
+        t = b.bin_table([kmin, kmax])
+        return b.kmap(b.as_fourier(kmap), C.kop(C.KOP_FILTER, bin_table=t, ell=l))
+
+    def fourier_filter_ll(self, kmap, l1, l2, kmin, kmax):
+        b = self.base
+        t = b.bin_table([kmin, kmax])
+        return b.kmap(b.as_fourier(kmap), C.kop(C.KOP_FILTER, bin_table=t, ell=l1, ell2=l2))
+
+    def prod_fourier(self, map1, map2):
+        b = self.base
+        w = map2 if torch.is_tensor(map2) else torch.from_numpy(np.asarray(map2, dtype=np.float32))
+        w = w.to(device=b.device, dtype=torch.float32).contiguous()
+        return b.kmap(b.as_fourier(map1), C.kop(C.KOP_WEIGHT, w=w))
+
+    def prod_real(self, map1, map2):
+        return self.base.as_real(map1) * self.base.as_real(map2)

@@ -1,0 +1,543 @@
+// 3D real FFTs for gfx950: Stockham radix-16/8/4/2 butterflies in registers, LDS exchange
+// between stages, one pass per axis.  R2C = z-rows (real->half complex) -> y-columns ->
+// x-columns; C2R = x-columns -> y-columns -> z-rows.  The voxel/mode operations of pb3d_ops.h are
+// fused into the first load and last store (prologue/epilogue), so filters, masks, Y_lm weights
+// and products never take a separate pass over HBM.
+//
+// Replaces the reference's FFT dispatch to_fourier / to_real / _fft_setup
+// (PolyBin3D/base.py:176-229, 256-302); numpy conventions (forward unnormalised, inverse 1/N).
+#include "pb3d_ops.h"
+#include <vector>
+#include <cmath>
+#include <atomic>
+
+namespace pb3d {
+
+// ------------------------------------------------------------------------------------------
+// Register DFTs (decimation in frequency, natural-order output).  DIR=-1 forward, +1 inverse.
+// ------------------------------------------------------------------------------------------
+// w16(j) = exp(DIR * 2*pi*i*j/16) for compile-time j after unrolling.
+template <int DIR>
+__device__ __forceinline__ float2 w16(int j) {
+  constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f, h = 0.70710678118654752f;
+  float c, s;
+  switch (j & 15) {
+    case 0: c = 1.f; s = 0.f; break;
+    case 1: c = c1; s = s1; break;
+    case 2: c = h; s = h; break;
+    case 3: c = s1; s = c1; break;
+    case 4: c = 0.f; s = 1.f; break;
+    case 5: c = -s1; s = c1; break;
+    case 6: c = -h; s = h; break;
+    case 7: c = -c1; s = s1; break;
+    default: c = -1.f; s = 0.f; break;
+  }
+  return make_float2(c, DIR * s);
+}
+
+template <int R, int DIR>
+struct DFT {
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr int H = R / 2;
+    float2 b[H], c[H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      b[k] = cadd(v[k], v[k + H]);
+      float2 d = csub(v[k], v[k + H]);
+      const int j = k * (16 / R);  // exp(DIR 2 pi i k / R)
+      if (j == 0) {
+        c[k] = d;
+      } else if (j == 4) {  // DIR * i
+        c[k] = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+      } else {
+        c[k] = cmul(d, w16<DIR>(j));
+      }
+    }
+    DFT<H, DIR>::run(b);
+    DFT<H, DIR>::run(c);
+#pragma unroll
+    for (int m = 0; m < H; ++m) {
+      v[2 * m] = b[m];
+      v[2 * m + 1] = c[m];
+    }
+  }
+};
+template <int DIR>
+struct DFT<1, DIR> {
+  __device__ __forceinline__ static void run(float2*) {}
+};
+
+// ------------------------------------------------------------------------------------------
+// Compile-time decomposition: radices R0,R1,R2 (1 = unused) and threads per pencil P.
+// ------------------------------------------------------------------------------------------
+template <int L> struct Decomp;
+template <> struct Decomp<2>    { static constexpr int R0 = 2,  R1 = 1,  R2 = 1, P = 1; };
+template <> struct Decomp<4>    { static constexpr int R0 = 4,  R1 = 1,  R2 = 1, P = 1; };
+template <> struct Decomp<8>    { static constexpr int R0 = 8,  R1 = 1,  R2 = 1, P = 1; };
+template <> struct Decomp<16>   { static constexpr int R0 = 16, R1 = 1,  R2 = 1, P = 1; };
+template <> struct Decomp<32>   { static constexpr int R0 = 8,  R1 = 4,  R2 = 1, P = 4; };
+template <> struct Decomp<64>   { static constexpr int R0 = 8,  R1 = 8,  R2 = 1, P = 8; };
+template <> struct Decomp<128>  { static constexpr int R0 = 16, R1 = 8,  R2 = 1, P = 8; };
+template <> struct Decomp<256>  { static constexpr int R0 = 16, R1 = 16, R2 = 1, P = 16; };
+template <> struct Decomp<512>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8, P = 64; };
+
+// One Stockham stage: butterflies j = t + b*P, inputs src(j + k*L/R), twiddles from the length-L
+// table tw[m] = exp(-2 pi i m / L) (conjugated for DIR=+1), outputs dst((j/NS)*NS*R + j%NS + k*NS).
+// If `barrier` the block synchronises between the reads and the writes (src/dst share LDS).
+template <int L, int R, int NS, int P, int DIR, typename Src, typename Dst>
+__device__ __forceinline__ void stage(int t, Src src, Dst dst, const float2* __restrict__ tw,
+                                      bool barrier) {
+  constexpr int NB = L / R;
+  constexpr int B = NB / P;
+  static_assert(B >= 1 && NB % P == 0, "bad decomposition");
+  float2 v[B][R];
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + b * P;
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[b][k] = src(j + k * NB);
+    if (NS > 1) {
+      const int jm = j % NS;
+#pragma unroll
+      for (int k = 1; k < R; ++k) {
+        float2 w = tw[jm * k * (L / (NS * R))];
+        if (DIR > 0) w.y = -w.y;
+        v[b][k] = cmul(v[b][k], w);
+      }
+    }
+    DFT<R, DIR>::run(v[b]);
+  }
+  if (barrier) __syncthreads();
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + b * P;
+    const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+    for (int k = 0; k < R; ++k) dst(base + k * NS, v[b][k]);
+  }
+}
+
+// Full pencil FFT: first stage reads `src0`, last stage writes `dstN`, intermediate data lives in
+// the LDS accessor `lds`.  All threads of the block must call this (it contains barriers).
+template <int L, int DIR, typename Src0, typename Lds, typename DstN>
+__device__ __forceinline__ void pencil_fft(int t, Src0 src0, Lds lds, DstN dstN,
+                                           const float2* __restrict__ tw, bool src_is_lds) {
+  using D = Decomp<L>;
+  constexpr int P = D::P;
+  if constexpr (D::R1 == 1) {
+    stage<L, D::R0, 1, P, DIR>(t, src0, dstN, tw, src_is_lds);
+  } else if constexpr (D::R2 == 1) {
+    stage<L, D::R0, 1, P, DIR>(t, src0, lds, tw, src_is_lds);
+    __syncthreads();
+    stage<L, D::R1, D::R0, P, DIR>(t, lds, dstN, tw, true);
+  } else {
+    stage<L, D::R0, 1, P, DIR>(t, src0, lds, tw, src_is_lds);
+    __syncthreads();
+    stage<L, D::R1, D::R0, P, DIR>(t, lds, lds, tw, true);
+    __syncthreads();
+    stage<L, D::R2, D::R0 * D::R1, P, DIR>(t, lds, dstN, tw, true);
+  }
+}
+
+template <int L> struct ColCfg { static constexpr int T = (256 / Decomp<L>::P) > 8 ? 256 / Decomp<L>::P : 8; };
+template <> struct ColCfg<2>    { static constexpr int T = 256; };
+template <> struct ColCfg<4>    { static constexpr int T = 256; };
+template <> struct ColCfg<8>    { static constexpr int T = 128; };
+template <> struct ColCfg<16>   { static constexpr int T = 128; };
+
+// ------------------------------------------------------------------------------------------
+// Column pass along axis 0 or 1 of the half-complex grid.
+//   column c in [0, ncols): base(c) = (c / cpp) * plane + (c % cpp); element p at base + p*S.
+//   MODE 0: in-place/out-of-place plain.  MODE 1: first C2R pass, input through KOp (broadcast
+//   by x_stride).  MODE 2: last R2C pass, output through KOp (may accumulate).
+// ------------------------------------------------------------------------------------------
+template <int L, int DIR, int MODE>
+__global__ void __launch_bounds__(Decomp<L>::P * ColCfg<L>::T)
+fft_cols(const float2* in, float2* out, int ncols, int cpp, long long plane,
+         long long S, long long in_bstride, long long out_bstride, const float2* __restrict__ tw, Geo geo,
+         KOp kop) {
+  constexpr int P = Decomp<L>::P;
+  constexpr int T = ColCfg<L>::T;
+  constexpr int LD = T + 1;
+  __shared__ float2 lds[(Decomp<L>::R1 == 1) ? 1 : L * LD];
+  const int tid = threadIdx.x;
+  const int col = tid % T;
+  const int t = tid / T;
+  const long long bt = blockIdx.y;
+  const int c = blockIdx.x * T + col;
+  const bool valid = c < ncols;
+  const long long base = valid ? (long long)(c / cpp) * plane + (c % cpp) : 0;
+  const float2* src = in + bt * in_bstride;
+  float2* dst = out + bt * out_bstride;
+  // mode coordinates for KOp (only the x-column pass uses them: axis 0, cpp = n1*n2h)
+  int mj = 0, ml = 0;
+  if (MODE != 0) { mj = (int)(base / geo.n2h); ml = (int)(base % geo.n2h); }
+
+  auto load = [&](int p) -> float2 {
+    if (!valid) return make_float2(0.f, 0.f);
+    float2 v = src[base + p * S];
+    if (MODE == 1) v = kop_value(kop, geo, bt, base + p * S, p, mj, ml, v);
+    return v;
+  };
+  auto store = [&](int p, float2 v) {
+    if (!valid) return;
+    if (MODE == 2) {
+      long long md = base + p * S;
+      v = kop_value(kop, geo, bt, md, p, mj, ml, v);
+      v = cscale(v, kop.scale);
+      if (kop.accumulate) v = cadd(v, dst[md]);
+      dst[md] = v;
+    } else {
+      dst[base + p * S] = v;
+    }
+  };
+  struct LdsRW {
+    float2* l; int LDv; int colv;
+    __device__ float2 operator()(int p) const { return l[p * LDv + colv]; }
+    __device__ void operator()(int p, float2 v) const { l[p * LDv + colv] = v; }
+  } lrw{lds, LD, col};
+  pencil_fft<L, DIR>(t, load, lrw, store, tw, false);
+}
+
+// ------------------------------------------------------------------------------------------
+// Row pass (axis 2).  R2C: real rows of n2 = 2M floats packed as M complex, FFT_M, then split
+// into the n2/2+1 half spectrum.  C2R: the inverse (numpy/pocketfft semantics: Im of the kz=0
+// and kz=M entries ignored), with 1/N and the real-space epilogue fused into the store.
+// ------------------------------------------------------------------------------------------
+template <int M> struct RowCfg { static constexpr int T = (256 / Decomp<M>::P) > 8 ? 256 / Decomp<M>::P : 8; };
+template <> struct RowCfg<2>  { static constexpr int T = 128; };
+template <> struct RowCfg<4>  { static constexpr int T = 128; };
+template <> struct RowCfg<8>  { static constexpr int T = 64; };
+template <> struct RowCfg<16> { static constexpr int T = 64; };
+
+template <int M>
+__global__ void __launch_bounds__(Decomp<M>::P * RowCfg<M>::T)
+r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows, int rows_per_batch,
+         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop) {
+  constexpr int P = Decomp<M>::P;
+  constexpr int T = RowCfg<M>::T;
+  constexpr int NT = P * T;
+  constexpr int LD = M + 1;
+  __shared__ float2 buf[T * LD];
+  const int tid = threadIdx.x;
+  const long long row0 = (long long)blockIdx.x * T;
+  // 1. coalesced load of T rows (T*M float2) with the voxel prologue
+  for (int f = tid; f < T * M; f += NT) {
+    const int r = f / M, m = f % M;
+    const long long row = row0 + r;
+    float2 v = make_float2(0.f, 0.f);
+    if (row < nrows) {
+      const long long bt = row / rows_per_batch;
+      const long long rr = row % rows_per_batch;
+      const long long vox = rr * (2 * M) + 2 * m;
+      const float* a = rop.a ? rop.a + bt * rop.sa : in + bt * (long long)rows_per_batch * 2 * M;
+      float x0 = a[vox], x1 = a[vox + 1];
+      if (rop.kind != 0) {
+        const int i0 = (int)(rr / geo.n1), i1 = (int)(rr % geo.n1);
+        x0 = rop_value(rop, geo, bt, vox, i0, i1, 2 * m, x0);
+        x1 = rop_value(rop, geo, bt, vox + 1, i0, i1, 2 * m + 1, x1);
+      }
+      v = make_float2(x0 * rop.scale, x1 * rop.scale);
+    }
+    buf[r * LD + m] = v;
+  }
+  __syncthreads();
+  // 2. complex FFT of each row in LDS
+  const int r = tid / P, t = tid % P;
+  struct RowRW {
+    float2* l;
+    __device__ float2 operator()(int p) const { return l[p]; }
+    __device__ void operator()(int p, float2 v) const { l[p] = v; }
+  } rw{buf + r * LD};
+  pencil_fft<M, -1>(t, rw, rw, rw, tw, true);
+  __syncthreads();
+  // 3. split: X[k] = E[k] + W^k O[k], E = (Z[k]+conj Z[M-k])/2, O = (Z[k]-conj Z[M-k])/(2i)
+  float2* o = out + row0 * (M + 1);
+  for (int f = tid; f < T * (M + 1); f += NT) {
+    const int rr = f / (M + 1), k = f % (M + 1);
+    if (row0 + rr >= nrows) continue;
+    const float2 zk = buf[rr * LD + (k == M ? 0 : k)];
+    const float2 zm = cconj(buf[rr * LD + (k == 0 ? 0 : M - k)]);
+    const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+    const float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y));
+    const float2 od = make_float2(d.y, -d.x);  // d / i
+    o[f] = cadd(e, cmul(twN[k], od));
+  }
+}
+
+template <int M>
+__global__ void __launch_bounds__(Decomp<M>::P * RowCfg<M>::T)
+c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows, int rows_per_batch,
+         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop, float norm) {
+  constexpr int P = Decomp<M>::P;
+  constexpr int T = RowCfg<M>::T;
+  constexpr int NT = P * T;
+  constexpr int LD = M + 1;
+  __shared__ float2 buf[T * LD];
+  const int tid = threadIdx.x;
+  const long long row0 = (long long)blockIdx.x * T;
+  const float2* src = in + row0 * (M + 1);
+  for (int f = tid; f < T * (M + 1); f += NT) {
+    const int rr = f / (M + 1), k = f % (M + 1);
+    float2 v = make_float2(0.f, 0.f);
+    if (row0 + rr < nrows) v = src[f];
+    if (k == 0 || k == M) v.y = 0.f;
+    buf[rr * LD + k] = v;
+  }
+  __syncthreads();
+  // Z'[k] = (X[k] + conj X[M-k]) + i W^{-k} (X[k] - conj X[M-k]),  k = 0..M-1; pairs (k, M-k)
+  for (int f = tid; f < T * (M / 2 + 1); f += NT) {
+    const int rr = f / (M / 2 + 1), k = f % (M / 2 + 1);
+    float2* b = buf + rr * LD;
+    const float2 xk = b[k], xm = b[M - k];
+    {
+      const float2 s = cadd(xk, cconj(xm));
+      const float2 d = csub(xk, cconj(xm));
+      const float2 w = cconj(twN[k]);
+      const float2 wd = cmul(w, d);
+      b[k] = make_float2(s.x - wd.y, s.y + wd.x);
+    }
+    if (k != 0 && 2 * k != M) {
+      const float2 s = cadd(xm, cconj(xk));
+      const float2 d = csub(xm, cconj(xk));
+      const float2 w = cconj(twN[M - k]);
+      const float2 wd = cmul(w, d);
+      b[M - k] = make_float2(s.x - wd.y, s.y + wd.x);
+    }
+  }
+  __syncthreads();
+  const int r = tid / P, t = tid % P;
+  struct RowRW {
+    float2* l;
+    __device__ float2 operator()(int p) const { return l[p]; }
+    __device__ void operator()(int p, float2 v) const { l[p] = v; }
+  } rw{buf + r * LD};
+  pencil_fft<M, +1>(t, rw, rw, rw, tw, true);
+  __syncthreads();
+  for (int f = tid; f < T * M; f += NT) {
+    const int rr = f / M, m = f % M;
+    const long long row = row0 + rr;
+    if (row >= nrows) continue;
+    const long long bt = row / rows_per_batch;
+    const long long rw_ = row % rows_per_batch;
+    const long long vox = rw_ * (2 * M) + 2 * m;
+    const float2 z = buf[rr * LD + m];
+    float x0 = z.x * norm, x1 = z.y * norm;
+    float* o = out + bt * (long long)rows_per_batch * 2 * M;
+    if (rop.kind != 0) {
+      const int i0 = (int)(rw_ / geo.n1), i1 = (int)(rw_ % geo.n1);
+      x0 = rop_value(rop, geo, bt, vox, i0, i1, 2 * m, x0);
+      x1 = rop_value(rop, geo, bt, vox + 1, i0, i1, 2 * m + 1, x1);
+    }
+    x0 *= rop.scale; x1 *= rop.scale;
+    if (rop.accumulate) {
+      float2 prev = *reinterpret_cast<const float2*>(o + vox);
+      x0 += prev.x; x1 += prev.y;
+    }
+    *reinterpret_cast<float2*>(o + vox) = make_float2(x0, x1);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------
+static std::atomic<long long> g_fwd{0}, g_rev{0};
+
+static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+}  // namespace pb3d
+
+struct pb3d_plan {
+  pb3d_grid grid;
+  pb3d::Geo geo;
+  int L[3];         // n0, n1, n2/2
+  float2* tw[3];    // device twiddles exp(-2 pi i m / L), m < L
+  float2* twN;      // exp(-2 pi i k / n2), k = 0..n2/2
+};
+
+namespace pb3d {
+
+template <int DIR, int MODE>
+static hipError_t launch_cols(int L, const float2* in, float2* out, int ncols, int cpp, long long plane,
+                              long long S, long long in_bs, long long out_bs, int batch, const float2* tw,
+                              const Geo& geo, const KOp& kop, hipStream_t s) {
+#define PB3D_COLS(LL)                                                                          \
+  case LL: {                                                                                   \
+    constexpr int T = ColCfg<LL>::T;                                                           \
+    dim3 grid((ncols + T - 1) / T, batch);                                                     \
+    fft_cols<LL, DIR, MODE><<<grid, Decomp<LL>::P * T, 0, s>>>(in, out, ncols, cpp, plane, S,  \
+                                                             in_bs, out_bs, tw, geo, kop);     \
+    return hipGetLastError();                                                                  \
+  }
+  switch (L) {
+    PB3D_COLS(2) PB3D_COLS(4) PB3D_COLS(8) PB3D_COLS(16) PB3D_COLS(32) PB3D_COLS(64)
+    PB3D_COLS(128) PB3D_COLS(256) PB3D_COLS(512)
+    default: return hipErrorInvalidValue;
+  }
+#undef PB3D_COLS
+}
+
+static hipError_t launch_r2c_rows(int M, const float* in, float2* out, long long nrows, int rpb,
+                                  const float2* tw, const float2* twN, const Geo& geo, const ROp& rop,
+                                  hipStream_t s) {
+#define PB3D_ROWS(MM)                                                                            \
+  case MM: {                                                                                     \
+    constexpr int T = RowCfg<MM>::T;                                                             \
+    r2c_rows<MM><<<(unsigned)((nrows + T - 1) / T), Decomp<MM>::P * T, 0, s>>>(in, out, nrows, rpb, tw, \
+                                                                              twN, geo, rop);    \
+    return hipGetLastError();                                                                    \
+  }
+  switch (M) {
+    PB3D_ROWS(2) PB3D_ROWS(4) PB3D_ROWS(8) PB3D_ROWS(16) PB3D_ROWS(32) PB3D_ROWS(64)
+    PB3D_ROWS(128) PB3D_ROWS(256)
+    default: return hipErrorInvalidValue;
+  }
+#undef PB3D_ROWS
+}
+
+static hipError_t launch_c2r_rows(int M, const float2* in, float* out, long long nrows, int rpb,
+                                  const float2* tw, const float2* twN, const Geo& geo, const ROp& rop,
+                                  float norm, hipStream_t s) {
+#define PB3D_ROWS(MM)                                                                            \
+  case MM: {                                                                                     \
+    constexpr int T = RowCfg<MM>::T;                                                             \
+    c2r_rows<MM><<<(unsigned)((nrows + T - 1) / T), Decomp<MM>::P * T, 0, s>>>(in, out, nrows, rpb, tw, \
+                                                                              twN, geo, rop, norm); \
+    return hipGetLastError();                                                                    \
+  }
+  switch (M) {
+    PB3D_ROWS(2) PB3D_ROWS(4) PB3D_ROWS(8) PB3D_ROWS(16) PB3D_ROWS(32) PB3D_ROWS(64)
+    PB3D_ROWS(128) PB3D_ROWS(256)
+    default: return hipErrorInvalidValue;
+  }
+#undef PB3D_ROWS
+}
+
+static float2* upload_twiddles(int L, int count) {
+  std::vector<float2> h(count);
+  const double twopi = 6.283185307179586476925286766559;
+  for (int m = 0; m < count; ++m) {
+    double a = -twopi * (double)m / (double)L;
+    h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  float2* d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * count) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(float2) * count, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  return d;
+}
+
+int check_rop(const pb3d_rop* r, bool input_side) {
+  if (!r) return 0;
+  if (r->kind < 0 || r->kind > 5) return fail(PB3D_ERR_ARG, "rop.kind out of range");
+  if (input_side && r->kind == 5) return fail(PB3D_ERR_ARG, "rop SQUARE is an output-side op");
+  if ((r->kind == 1 || r->kind == 2 || r->kind == 4) && !r->b) return fail(PB3D_ERR_ARG, "rop.b required");
+  if (r->kind == 2 && (!r->c || !r->d)) return fail(PB3D_ERR_ARG, "rop DIFF needs c and d");
+  if ((r->kind == 3 || r->kind == 4) && (r->ell < 0 || r->ell > 4 || r->m < 0 || r->m > 2 * r->ell))
+    return fail(PB3D_ERR_ARG, "rop Y_lm index out of range (l<=4)");
+  return 0;
+}
+
+int check_kop(const pb3d_kop* k) {
+  if (!k) return 0;
+  if (k->kind < 0 || k->kind > 3) return fail(PB3D_ERR_ARG, "kop.kind out of range");
+  if (k->kind == 1 && (!k->bin_table || k->ell < 0 || k->ell > 6 || k->ell2 < 0 || k->ell2 > 6))
+    return fail(PB3D_ERR_ARG, "kop FILTER needs bin_table and 0<=ell,ell2<=6");
+  if (k->kind == 2 && (k->ell < 0 || k->ell > 4 || k->m < 0 || k->m > 2 * k->ell))
+    return fail(PB3D_ERR_ARG, "kop Y_lm index out of range (l<=4)");
+  if (k->kind == 3 && !k->w) return fail(PB3D_ERR_ARG, "kop WEIGHT needs w");
+  return 0;
+}
+
+}  // namespace pb3d
+
+using namespace pb3d;
+
+extern "C" int64_t pb3d_fft_counter(int dir) {
+  if (dir < 0) { g_fwd = 0; g_rev = 0; return 0; }
+  return dir == 0 ? (int64_t)g_fwd.load() : (int64_t)g_rev.load();
+}
+
+extern "C" int pb3d_plan_create(pb3d_plan** plan, const pb3d_grid* grid) {
+  if (!plan || !grid) return fail(PB3D_ERR_ARG, "null plan/grid pointer");
+  *plan = nullptr;
+  const int n0 = grid->n[0], n1 = grid->n[1], n2 = grid->n[2];
+  if (!pow2_in(n0, 2, 512) || !pow2_in(n1, 2, 512) || !pow2_in(n2, 4, 512))
+    return fail(PB3D_ERR_UNSUPPORTED,
+                "grid sizes must be powers of two: n0,n1 in [2,512], n2 in [4,512]");
+  for (int i = 0; i < 3; ++i)
+    if (!(grid->boxsize[i] > 0)) return fail(PB3D_ERR_ARG, "boxsize must be positive");
+  pb3d_plan* p = new pb3d_plan();
+  p->grid = *grid;
+  p->geo = make_geo(*grid);
+  p->L[0] = n0; p->L[1] = n1; p->L[2] = n2 / 2;
+  for (int a = 0; a < 3; ++a) {
+    p->tw[a] = upload_twiddles(p->L[a], p->L[a]);
+    if (!p->tw[a]) { pb3d_plan_destroy(p); return fail(PB3D_ERR_ALLOC, "twiddle upload failed"); }
+  }
+  p->twN = upload_twiddles(n2, n2 / 2 + 1);
+  if (!p->twN) { pb3d_plan_destroy(p); return fail(PB3D_ERR_ALLOC, "twiddle upload failed"); }
+  *plan = p;
+  return 0;
+}
+
+extern "C" int pb3d_plan_destroy(pb3d_plan* p) {
+  if (!p) return 0;
+  for (int a = 0; a < 3; ++a) if (p->tw[a]) (void)hipFree(p->tw[a]);
+  if (p->twN) (void)hipFree(p->twN);
+  delete p;
+  return 0;
+}
+
+extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, void* work_, int batch,
+                            const pb3d_rop* rop_, const pb3d_kop* kop_, void* stream) {
+  if (!p || !out_ || batch < 1) return fail(PB3D_ERR_ARG, "r2c: null plan/output or batch < 1");
+  if (!in && !(rop_ && rop_->a)) return fail(PB3D_ERR_ARG, "r2c: no input");
+  if (int e = check_rop(rop_, true)) return e;
+  if (int e = check_kop(kop_)) return e;
+  if (kop_ && kop_->accumulate && !work_) return fail(PB3D_ERR_ARG, "r2c: accumulating output needs work");
+  hipStream_t s = (hipStream_t)stream;
+  const Geo& g = p->geo;
+  float2* out = (float2*)out_;
+  float2* work = work_ ? (float2*)work_ : out;
+  ROp rop = to_rop(rop_);
+  KOp kop = to_kop(kop_);
+  KOp plain = to_kop(nullptr);
+  const long long rows = (long long)batch * g.n0 * g.n1;
+  PB3D_CHECK_HIP(launch_r2c_rows(p->L[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, s));
+  // y columns: cols = n0*n2h, cpp = n2h, plane = n1*n2h, stride n2h (in place in work)
+  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, g.n0 * g.n2h, g.n2h, (long long)g.n1 * g.n2h,
+                                     g.n2h, g.H, g.H, batch, p->tw[1], g, plain, s)));
+  // x columns: cols = n1*n2h, stride n1*n2h, epilogue kop into out
+  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, out, g.n1 * g.n2h, g.n1 * g.n2h, 0,
+                                     (long long)g.n1 * g.n2h, g.H, g.H, batch, p->tw[0], g, kop, s)));
+  g_fwd += batch;
+  return 0;
+}
+
+extern "C" int pb3d_fft_c2r(const pb3d_plan* p, const void* in_, float* out, void* work_, int batch,
+                            const pb3d_kop* kop_, const pb3d_rop* rop_, void* stream) {
+  if (!p || !in_ || !out || !work_ || batch < 1) return fail(PB3D_ERR_ARG, "c2r: null pointer or batch < 1");
+  if (in_ == work_) return fail(PB3D_ERR_ARG, "c2r: work must not alias the input");
+  if (int e = check_rop(rop_, false)) return e;
+  if (int e = check_kop(kop_)) return e;
+  hipStream_t s = (hipStream_t)stream;
+  const Geo& g = p->geo;
+  const float2* in = (const float2*)in_;
+  float2* work = (float2*)work_;
+  ROp rop = to_rop(rop_);
+  KOp kop = to_kop(kop_);
+  KOp plain = to_kop(nullptr);
+  const long long in_bs = kop_ ? kop_->x_batch_stride : g.H;
+  PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], in, work, g.n1 * g.n2h, g.n1 * g.n2h, 0,
+                                     (long long)g.n1 * g.n2h, in_bs, g.H, batch, p->tw[0], g, kop, s)));
+  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, g.n0 * g.n2h, g.n2h, (long long)g.n1 * g.n2h,
+                                     g.n2h, g.H, g.H, batch, p->tw[1], g, plain, s)));
+  const long long rows = (long long)batch * g.n0 * g.n1;
+  PB3D_CHECK_HIP(launch_c2r_rows(p->L[2], work, out, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop,
+                                 (float)(1.0 / (double)g.N), s));
+  g_rev += batch;
+  return 0;
+}

@@ -1,0 +1,415 @@
+"""Power-spectrum estimators, backend='hip' (PolyBin3D/pspec.py).
+
+Same API as PolyBin3D.PSpec: Pk_ideal, Pk_numerator_ideal, compute_fisher_ideal, Pk_numerator,
+compute_fisher_contribution, compute_fisher, compute_shot_contribution, Pk_unwindowed, get_ks.
+Maps stay on the GPU; each step is a fused HIP kernel:
+  * numerators: one R2C with the 1/P_fid weight in its epilogue, then one segmented bin reduction
+    for all multipoles (global) or R2C-with-Y_lm prologue/epilogue harmonics sums (local);
+  * MC Fisher: all Q maps of one multipole are produced by one batched filtered C2R (Theta_b L_l in
+    the prologue, mask in the epilogue) followed by one batched R2C; the Fisher rows of all of them
+    come from one bin-reduction launch.  Fisher rows accumulate in FP64.
+Results are returned as numpy FP64 arrays like the reference's.
+"""
+import numpy as np
+import scipy.interpolate
+import torch
+
+from . import _capi as C
+from .base import harmonics_tags
+
+
+class PSpec:
+    """Power spectrum estimators (PolyBin3D/pspec.py:10-784)."""
+
+    def __init__(self, base, k_bins, lmax=2, mask=None, applySinv=None, odd_l=False, add_GIC=False,
+                 mask_IC=None, add_RIC=False, radial_bins_RIC=[], mask_shot=None, applySinv_transpose=None,
+                 verb=False):
+        # pspec.py:27-120
+        self.base = base
+        self.k_bins = np.asarray(k_bins, dtype=np.float64)
+        self.lmax = lmax
+        self.Nk = len(k_bins) - 1
+        self.odd_l = odd_l
+        self.Nl = lmax + 1 if odd_l else lmax // 2 + 1
+        self.Nl_even = lmax // 2 + 1
+        self.N_bins = self.Nk * self.Nl
+        self.add_GIC, self.add_RIC = add_GIC, add_RIC
+        assert np.max(self.k_bins) < np.min(base.kNy), "k_max must be less than k_Nyquist!"
+        assert np.min(self.k_bins) >= np.max(base.kF), "k_min must be at least the k_fundamental!"
+        assert np.max(self.k_bins) <= base.Pfid[0][-1], "Fiducial power spectrum should extend up to k_max!"
+        assert np.min(self.k_bins) >= base.Pfid[0][0], "Fiducial power spectrum should extend down to k_min!"
+        if odd_l:
+            assert self.lmax > 0, "lmax must be greater than 0 to use odd harmonics!"
+            assert base.sightline == 'local', "Odd spherical harmonics should only be used for local line-of-sight!"
+            raise NotImplementedError("odd_l (wide-angle) multipoles are not implemented by backend='hip'")
+        assert type(self.lmax) == int, "l-max must be an integer!"
+        assert self.lmax <= 4, "Only l-max<=4 is currently supported!"
+        if add_GIC or add_RIC:
+            raise NotImplementedError("integral constraints are not implemented by backend='hip'")
+        if mask is None:
+            self.const_mask, self.mask_mean, self.sq_mask_mean = True, 1., 1.
+        else:
+            m = mask.detach().cpu().numpy() if torch.is_tensor(mask) else np.asarray(mask)
+            m = m.astype(np.float64)
+            if np.std(m) < 1e-12:
+                self.const_mask, self.mask_mean = True, float(np.mean(m))
+            else:
+                self.const_mask = False
+            self.sq_mask_mean = float(np.mean(m ** 2))
+            self.mask = base.as_real(m)
+        if mask_shot is not None:
+            ms = mask_shot.detach().cpu().numpy() if torch.is_tensor(mask_shot) else np.asarray(mask_shot)
+            self.mask_shot = base.as_real(ms.astype(np.float64))
+        # pspec.py:92-99
+        self.user_Sinv = applySinv is not None
+        if applySinv is None:
+            self.applySinv = base.applySinv_trivial
+        else:
+            self.applySinv = applySinv
+            if not callable(applySinv_transpose):
+                applySinv_transpose = applySinv
+        self.applySinv_transpose = base.applySinv_trivial if applySinv_transpose is None else applySinv_transpose
+        if base.sightline == 'local' and self.lmax > 0:
+            self.Ylm_real = harmonics_tags(lmax, odd_l, 'real')
+            self.Ylm_fourier = harmonics_tags(lmax, odd_l, 'fourier')
+        else:
+            self.Ylm_real = self.Ylm_fourier = None
+        self.table = base.bin_table(self.k_bins)
+
+    def get_ks(self):
+        """Bin centres (pspec.py:122-128)."""
+        return np.asarray([0.5 * (self.k_bins[b] + self.k_bins[b + 1]) for b in range(self.Nk)])
+
+    # ------------------------------------------------------------------ numerators
+    def _filtered_data(self, data, filtering):
+        """S^-1 d in Fourier space (pspec.py:617-640)."""
+        b = self.base
+        if filtering == 'ideal':
+            # invPk0 / sqrt(<mask^2>), and / W when a pixel window is deconvolved (pspec.py:627-638)
+            w = b.invPk0_grid
+            if b.pixel_window != 'none':
+                w = self._ideal_pw_weight()
+            elif b.invPk0_is_one:
+                w = None
+            op = C.kop(C.KOP_WEIGHT, w=w, scale=1. / np.sqrt(self.sq_mask_mean)) if w is not None else \
+                C.kop(C.KOP_PLAIN, scale=1. / np.sqrt(self.sq_mask_mean))
+            return b.r2c(b.as_real(data), kop=op)
+        if b.pixel_window != 'none':
+            df = b.r2c(b.as_real(data), kop=C.kop(C.KOP_WEIGHT, w=self._inv_pw()))
+            return self.applySinv(df, input_type='fourier', output_type='fourier')
+        return b.as_fourier(self.applySinv(b.as_real(data), input_type='real', output_type='fourier'))
+
+    def _inv_pw(self):
+        b = self.base
+        key = 'inv_pw'
+        if key not in b._bufs:
+            pw = b.pixel_window_grid_host
+            w = np.zeros(pw.shape)
+            w[pw != 0] = 1. / pw[pw != 0]
+            b._bufs[key] = torch.from_numpy(w.astype(np.float32)).to(b.device)
+        return b._bufs[key]
+
+    def _ideal_pw_weight(self):
+        b = self.base
+        key = 'ideal_pw'
+        if key not in b._bufs:
+            pw = b.pixel_window_grid_host
+            w = np.zeros(pw.shape)
+            w[pw != 0] = b.invPk0_grid_host[pw != 0] / pw[pw != 0]
+            b._bufs[key] = torch.from_numpy(w.astype(np.float32)).to(b.device)
+        return b._bufs[key]
+
+    def _compute_pk_numerator(self, data, filtering='ideal'):
+        """pspec.py:613-686.  Global: one bin pass for all multipoles.  Local: S_real, then for each
+        l>0 the harmonics sum (2l+1 fused R2Cs) and one cross-integral pass."""
+        assert filtering in ['ideal', 'Sinv'], "Unknown filtering option supplied!"
+        b = self.base
+        Nl = self.Nl if filtering == 'Sinv' else self.Nl_even
+        ells = [2 * li for li in range(Nl)]
+        stack = torch.empty((Nl,) + b.half_shape, dtype=torch.complex64, device=b.device)
+        S = self._filtered_data(data, filtering)
+        stack[0].copy_(S)
+        if b.sightline == 'global' or self.lmax == 0:
+            out = b.bin_reduce(stack[0], self.k_bins, ells)[0]
+        else:
+            S_real = b.c2r(stack[0])
+            for li in range(1, Nl):
+                b.apply_fourier_harmonics(S_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=stack[li])
+            out = b.bin_reduce(stack[0], self.k_bins, [0] * Nl, y=stack, y_ostride=b.H)[0]
+        Pk = 0.5 * out.reshape(-1).cpu().numpy()
+        return Pk * b.volume / b.gridsize.prod() ** 2
+
+    def Pk_numerator_ideal(self, data):
+        """pspec.py:689-700."""
+        if not self.const_mask:
+            assert tuple(np.shape(data)) == tuple(self.mask.shape), "Data must have same shape as the mask!"
+        return self._compute_pk_numerator(data, filtering='ideal')
+
+    def Pk_numerator(self, data):
+        """pspec.py:131-139."""
+        if not self.const_mask:
+            assert tuple(np.shape(data)) == tuple(self.mask.shape), "Data must have same shape as the mask!"
+        return self._compute_pk_numerator(data, filtering='Sinv')
+
+    # ------------------------------------------------------------------ ideal Fisher
+    def compute_fisher_ideal(self, discreteness_correction=True, Pk_fid='default'):
+        """pspec.py:702-752: block-diagonal 0.5 sum degk invPk0^2 L_la L_lb, all (la, lb) in one pass."""
+        b = self.base
+        Nk, Nl = self.Nk, self.Nl_even
+        if not (isinstance(Pk_fid, str) and Pk_fid == 'default'):
+            if Pk_fid is None or (isinstance(Pk_fid, str) and Pk_fid == 'none'):
+                k_tmp = np.arange(np.min(b.kF) / 10., np.max(b.kNy) * 2, np.min(b.kF) / 10.)
+                Pfid = np.asarray([k_tmp, 1. + 0. * k_tmp])
+            else:
+                assert len(Pk_fid) == 2, "Pk should contain k and P_0(k) columns"
+                Pfid = np.asarray(Pk_fid)
+            P = scipy.interpolate.interp1d(Pfid[0], Pfid[1], bounds_error=False, fill_value=0.)(b.modk_grid)
+            inv = np.zeros(P.shape)
+            good = (P != 0) & np.isfinite(P)
+            inv[good] = 1. / P[good]
+            inv_d = torch.from_numpy(inv.astype(np.float32)).to(b.device)
+        else:
+            inv_d = b.invPk0_grid
+        fish = np.zeros((Nl * Nk, Nl * Nk))
+        if discreteness_correction:
+            pairs = [(la, lb) for la in range(Nl) for lb in range(Nl)]
+            r = b.bin_reduce(inv_d, self.k_bins, [2 * la for la, _ in pairs], ells2=[2 * lb for _, lb in pairs],
+                             real=True)[0].cpu().numpy()
+            for o, (la, lb) in enumerate(pairs):
+                fish[la * Nk:(la + 1) * Nk, lb * Nk:(lb + 1) * Nk] = np.diag(0.5 * r[o])
+        else:
+            d = 0.5 * b.bin_reduce(inv_d, self.k_bins, [0], real=True)[0, 0].cpu().numpy()
+            for la in range(Nl):
+                fish[la * Nk:(la + 1) * Nk, la * Nk:(la + 1) * Nk] = np.diag(d) / (4. * la + 1.)
+        self.fish_ideal = fish
+        self.inv_fish_ideal = np.linalg.inv(fish)
+        return fish
+
+    def Pk_ideal(self, data, fish_ideal=[], discreteness_correction=True):
+        """pspec.py:754-784 (keeps the reference's caching: an existing inverse Fisher is reused)."""
+        if len(fish_ideal) != 0:
+            self.fish_ideal = np.asarray(fish_ideal)
+            self.inv_fish_ideal = np.linalg.inv(self.fish_ideal)
+        if not hasattr(self, 'inv_fish_ideal'):
+            self.compute_fisher_ideal(discreteness_correction=discreteness_correction)
+        out = np.matmul(self.inv_fish_ideal, self.Pk_numerator_ideal(data))
+        res, idx = {}, 0
+        for l in range(0, self.lmax + 1, 2):
+            res['p%s' % l] = out[idx:idx + self.Nk]
+            idx += self.Nk
+        return res
+
+    # ------------------------------------------------------------------ MC Fisher
+    def _pointing_rop(self, scale=1.0):
+        """Real-space epilogue applying the pointing matrix P (mask) (pspec.py:199-213)."""
+        if self.const_mask:
+            return C.rop(C.ROP_PLAIN, scale=scale * self.mask_mean)
+        return C.rop(C.ROP_MUL, b=self.mask, scale=scale)
+
+    def _q_maps(self, src, li, Q, R):
+        """Q_{l,b} = (S^-1 P)^dag Theta_b L_l S^-1 P a for all bins b of multipole index li, written into
+        Q[Nk, H] (pspec.py:444-488) with trivial S^-1.  R is real scratch [Nk, N].
+          global / l=0 : Q = R2C(P * C2R(Theta_b L_l SinvP))
+          local  l>0   : Q = 0.5 R2C(P * [C2R(Theta_b leg) + sum_m Y_m(x) C2R(Theta_b Y_m(k) SinvP)])
+        (the reference applies (S^-1 P)^dag to both local half-terms separately; the sum is linear, so one
+        R2C of the summed real map is the same operator)."""
+        b = self.base
+        Nk = self.Nk
+        bins = self._bins_dev
+        SinvP, leg = src
+        ell = 2 * li
+        if b.sightline == 'global' or li == 0:
+            b.c2r(SinvP, out=R, batch=Nk,
+                  kop=C.kop(C.KOP_FILTER, bin_table=self.table, bins=bins, ell=ell, x_batch_stride=0))
+            b.r2c(None, out=Q, batch=Nk, rop=self._fisher_rop(R, 1.0))
+        else:
+            b.c2r(leg[li], out=R, batch=Nk,
+                  kop=C.kop(C.KOP_FILTER, bin_table=self.table, bins=bins, ell=0, x_batch_stride=0))
+            for m in range(2 * ell + 1):
+                b.c2r(SinvP, out=R, batch=Nk,
+                      kop=C.kop(C.KOP_YLM, bin_table=self.table, bins=bins, ell=ell, m=m, x_batch_stride=0),
+                      rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=1))
+            b.r2c(None, out=Q, batch=Nk, rop=self._fisher_rop(R, 0.5))
+
+    def _fisher_rop(self, R, scale):
+        b = self.base
+        if self.const_mask:
+            return C.rop(C.ROP_PLAIN, a=R, sa=b.N, scale=scale * self.mask_mean)
+        return C.rop(C.ROP_MUL, a=R, sa=b.N, b=self.mask, sb=0, scale=scale)
+
+    def compute_fisher_contribution(self, seed, verb=False):
+        """One MC realization of the Fisher matrix (pspec.py:141-147, 215-493; compute_cov=False).
+        Returns an FP64 numpy array [N_bins, N_bins]; row = Q-map (l, b), as in the reference."""
+        return self._fisher_contribution_dev(seed).cpu().numpy()
+
+    def _fisher_contribution_dev(self, seed, a_map_fourier=None):
+        b = self.base
+        if self.user_Sinv:
+            return torch.from_numpy(self._fisher_contribution_generic(seed))
+        Nk, Nl = self.Nk, self.Nl
+        if not hasattr(self, '_bins_dev'):
+            self._bins_dev = torch.arange(Nk, dtype=torch.int32, device=b.device)
+        # pspec.py:266-268
+        a = b.generate_data(seed=seed + int(1e7), output_type='fourier') if a_map_fourier is None else a_map_fourier
+        # A^-1 a (pspec.py:328)
+        w = b.ainv_weight()
+        Ainv = b.kmap(a, C.kop(C.KOP_WEIGHT, w=w)) if w is not None else a
+        # S^-1 P a (pspec.py:329-332), trivial S^-1
+        if self.const_mask:
+            SinvP = b.kmap(a, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
+        else:
+            a_real = b.c2r(a)
+            SinvP = b.r2c(None, rop=C.rop(C.ROP_MUL, a=a_real, b=self.mask))
+        local = b.sightline == 'local' and self.lmax > 0
+        lefts = torch.empty((Nl,) + b.half_shape, dtype=torch.complex64, device=b.device)
+        lefts[0].copy_(Ainv)
+        leg = {}
+        if local:
+            SinvP_real = b.c2r(SinvP)
+            Ainv_real = b.c2r(Ainv)
+            for li in range(1, Nl):
+                b.apply_fourier_harmonics(Ainv_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=lefts[li])
+                leg[li] = b.apply_fourier_harmonics(SinvP_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li])
+        else:
+            for li in range(1, Nl):
+                lefts[li].copy_(Ainv)
+        ells = [0] * Nl if local else [2 * li for li in range(Nl)]
+        Q = torch.empty((Nk,) + b.half_shape, dtype=torch.complex64, device=b.device)
+        R = torch.empty((Nk,) + tuple(int(v) for v in b.gridsize), dtype=torch.float32, device=b.device)
+        F = torch.zeros((self.N_bins, self.N_bins), dtype=torch.float64, device=b.device)
+        scaling = 0.5 * b.volume / b.gridsize.prod() ** 2
+        for li in range(Nl):
+            self._q_maps((SinvP, leg), li, Q, R)
+            r = b.bin_reduce(Q, self.k_bins, ells, y=lefts, y_qstride=0, y_ostride=b.H, nq=Nk)  # [Nk, Nl, Nk]
+            F[li * Nk:(li + 1) * Nk, :] = r.reshape(Nk, Nl * Nk) * scaling
+        return F
+
+    def _fisher_contribution_generic(self, seed):
+        """Reference call sequence with a user-supplied S^-1 (pspec.py:215-493), maps on the GPU."""
+        b = self.base
+        Nk, Nl = self.Nk, self.Nl
+        F = np.zeros((self.N_bins, self.N_bins))
+        a = b.generate_data(seed=seed + int(1e7), output_type='fourier')
+        if not self.const_mask:
+            a_real = b.c2r(a)
+
+        def apply_filter(m):
+            if self.const_mask:
+                return self.mask_mean * b.as_fourier(self.applySinv(m, input_type='fourier', output_type='fourier'))
+            return b.as_fourier(self.applySinv(b.as_real(m) * self.mask, input_type='real', output_type='fourier'))
+
+        def apply_filter_dagger(m, real=False):
+            it = 'real' if real else 'fourier'
+            if self.const_mask:
+                return self.mask_mean * b.as_fourier(self.applySinv_transpose(m, input_type=it, output_type='fourier'))
+            r = b.as_real(self.applySinv_transpose(m, input_type=it, output_type='real'))
+            return b.r2c(None, rop=C.rop(C.ROP_MUL, a=r, b=self.mask))
+
+        w = b.ainv_weight()
+        Ainv = b.kmap(a, C.kop(C.KOP_WEIGHT, w=w)) if w is not None else a
+        SinvP = apply_filter(a if self.const_mask else a_real)
+        local = b.sightline == 'local' and self.lmax > 0
+        lefts = torch.empty((Nl,) + b.half_shape, dtype=torch.complex64, device=b.device)
+        lefts[0].copy_(Ainv)
+        if local:
+            SinvP_real = b.c2r(SinvP)
+            Ainv_real = b.c2r(Ainv)
+            for li in range(1, Nl):
+                b.apply_fourier_harmonics(Ainv_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=lefts[li])
+        else:
+            for li in range(1, Nl):
+                lefts[li].copy_(Ainv)
+        ells = [0] * Nl if local else [2 * li for li in range(Nl)]
+        scaling = 0.5 * b.volume / b.gridsize.prod() ** 2
+
+        def add_row(row, Qm):
+            r = b.bin_reduce(Qm, self.k_bins, ells, y=lefts, y_qstride=0, y_ostride=b.H)[0]
+            F[row] = r.reshape(-1).cpu().numpy() * scaling
+
+        for li in range(Nl):
+            ell = 2 * li
+            if not local or li == 0:
+                for ki in range(Nk):
+                    km = b.map_utils.fourier_filter(SinvP, ell, self.k_bins[ki], self.k_bins[ki + 1])
+                    add_row(li * Nk + ki, apply_filter_dagger(km))
+            else:
+                legm = b.apply_fourier_harmonics(SinvP_real, self.Ylm_real[ell], self.Ylm_fourier[ell])
+                for ki in range(Nk):
+                    t = b.bin_table([self.k_bins[ki], self.k_bins[ki + 1]])
+                    km = b.kmap(legm, C.kop(C.KOP_FILTER, bin_table=t))
+                    Qm = 0.5 * apply_filter_dagger(km)
+                    real_map = torch.zeros(tuple(int(v) for v in b.gridsize), dtype=torch.float32, device=b.device)
+                    for m in range(2 * ell + 1):
+                        b.c2r(SinvP, out=real_map, kop=C.kop(C.KOP_YLM, bin_table=t, ell=ell, m=m),
+                              rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=1))
+                    Qm = Qm + 0.5 * apply_filter_dagger(real_map, real=True)
+                    add_row(li * Nk + ki, Qm)
+        return F
+
+    def compute_fisher(self, N_it, verb=False):
+        """Mean of N_it realizations, accumulated in FP64 on the GPU, then inverted (pspec.py:553-569)."""
+        fish = None
+        for seed in range(N_it):
+            if verb and seed % 5 == 0:
+                print("Computing Fisher contribution %d of %d" % (seed + 1, N_it))
+            f = self._fisher_contribution_dev(seed)
+            fish = f / N_it if fish is None else fish + f / N_it
+        self.fish = fish.cpu().numpy()
+        self.inv_fish = torch.linalg.inv(fish).cpu().numpy()
+        return self.fish
+
+    def compute_shot_contribution(self, seed):
+        """Shot-noise numerator from one GRF drawn with 1/P_fid (pspec.py:495-551), trivial S^-1."""
+        b = self.base
+        if not self.const_mask:
+            assert hasattr(self, 'mask_shot'), "Must supply mask_shot to compute shot-noise contribution"
+        if self.user_Sinv:
+            raise NotImplementedError("compute_shot_contribution with a user S^-1 is not implemented by backend='hip'")
+        PkA = [b.Pfid[0], 1. / b.Pfid[1]]
+        modk = b.modk_grid
+        PkA_grid = scipy.interpolate.interp1d(PkA[0], PkA[1], bounds_error=False,
+                                              fill_value=(PkA[1][0], PkA[1][-1]))(modk)
+        invPkA = np.zeros_like(PkA_grid)
+        invPkA[modk != 0] = 1. / PkA_grid[modk != 0]
+        a = b.generate_data(seed=seed + int(1e7), Pk_input=PkA, output_type='fourier')
+        wA = b.ainv_weight(invPkA)
+        AiA = b.kmap(a, C.kop(C.KOP_WEIGHT, w=wA))
+        if self.const_mask:
+            SNA = b.kmap(AiA, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
+        else:
+            AiA_r = b.c2r(AiA)
+            SNA = b.r2c(None, rop=C.rop(C.ROP_MUL, a=AiA_r, b=self.mask_shot))
+        Nl = self.Nl
+        local = self.lmax > 0 and b.sightline == 'local'
+        stack = torch.empty((Nl,) + b.half_shape, dtype=torch.complex64, device=b.device)
+        stack[0].copy_(a)
+        if local:
+            Sa_real = b.c2r(a)
+            for li in range(1, Nl):
+                b.apply_fourier_harmonics(Sa_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=stack[li])
+            r = b.bin_reduce(SNA, self.k_bins, [0] * Nl, y=stack, y_ostride=b.H)[0]
+        else:
+            r = b.bin_reduce(SNA, self.k_bins, [2 * li for li in range(Nl)], y=a)[0]
+        shot = 0.5 * r.reshape(-1).cpu().numpy()
+        return shot * b.volume / b.gridsize.prod() ** 2
+
+    def Pk_unwindowed(self, data, fish=[], shot_num=[], subtract_shotnoise=False):
+        """pspec.py:571-611."""
+        if len(fish) != 0:
+            self.fish = np.asarray(fish)
+            self.inv_fish = np.linalg.inv(self.fish)
+        if len(shot_num) != 0 or subtract_shotnoise:
+            self.shot_num = np.asarray(shot_num)
+            self.shot = np.matmul(self.inv_fish, self.shot_num)
+        if not hasattr(self, 'inv_fish'):
+            raise Exception("Need to compute Fisher matrix first!")
+        if not hasattr(self, 'shot') and subtract_shotnoise:
+            raise Exception("Need to compute shot-noise first!")
+        out = np.matmul(self.inv_fish, self.Pk_numerator(data))
+        if subtract_shotnoise:
+            out = out - self.shot
+        res, idx = {}, 0
+        for l in range(0, self.lmax + 1, 2 - self.odd_l):
+            res['p%s' % l] = out[idx:idx + self.Nk]
+            idx += self.Nk
+        return res

@@ -1,0 +1,30 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def hip():
+    """The built libpb3d.so on a GPU; a missing library or GPU is an error for GPU tests."""
+    import torch
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    from polybin3d_amd import _capi
+    _capi.lib()
+    return _capi

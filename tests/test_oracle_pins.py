@@ -1,0 +1,370 @@
+"""Pins for the FP64 oracle (CPU only): every check here is independent of the oracle's own code
+path (notebook-printed counters and sizes, closed forms, brute force, the addition theorem), so a
+dropped term, a wrong sign or index, or a transposed operand in oracle/polybin_oracle.py fails one.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import polybin_oracle as O
+
+
+def field(n, seed=1):
+    return np.random.RandomState(seed).randn(*n)
+
+
+# ---------------------------------------------------------------- printed metadata / counters
+def test_grid_metadata_tutorial1():
+    """T1 cell 8 prints kF = 0.0063 and kNy = 0.402 for 128^3, L = 1000."""
+    b = O.PolyBin3D(1000., [16, 16, 16])
+    assert "%.4f" % (2 * np.pi / 1000.) == "0.0063"
+    assert "%.3f" % (128 * 2 * np.pi / 1000. * 0.5) == "0.402"
+    assert np.allclose(b.kF, 2 * np.pi / 1000.)
+
+
+def test_fft_counter_pk_ideal_global():
+    """T1 cell 10: Pk_ideal, global, lmax=4 -> 1 forward, 0 reverse."""
+    b = O.PolyBin3D(1000., [32, 32, 32])
+    ps = O.PSpec(b, np.arange(0.01, 0.1, 0.01), lmax=4)
+    ps.Pk_ideal(field((32, 32, 32)))
+    assert (b.n_FFTs_forward, b.n_FFTs_reverse) == (1, 0)
+
+
+def test_fft_counter_pk_ideal_local():
+    """T1 cell 21: Pk_ideal, local, lmax=4 -> 15 forward, 1 reverse."""
+    b = O.PolyBin3D(1000., [32, 32, 32], boxcenter=[0., 1170., 826.], sightline='local')
+    ps = O.PSpec(b, np.arange(0.01, 0.1, 0.01), lmax=4)
+    ps.Pk_ideal(field((32, 32, 32)))
+    assert (b.n_FFTs_forward, b.n_FFTs_reverse) == (15, 1)
+
+
+def test_fft_counter_bk_ideal_and_nbins():
+    """T4 cells 10/12: 4 bins + 6 squeezed bins, lmax=2 -> N_bins 78; Bk_ideal with and without
+    the discreteness correction -> 2 forward, 96 reverse."""
+    b = O.PolyBin3D(1000., [48, 48, 48])
+    bs = O.BSpec(b, np.arange(0.01, 0.10, 0.02), lmax=2, k_bins_squeeze=np.arange(0.01, 0.15, 0.02),
+                 Pk_fid='default')
+    assert bs.N_bins == 78
+    d = field((48, 48, 48))
+    bs.Bk_ideal(d, discreteness_correction=True)
+    bs.Bk_ideal(d, discreteness_correction=False)
+    assert (b.n_FFTs_forward, b.n_FFTs_reverse) == (2, 96)
+
+
+def test_fft_counter_pk_mc_fisher_local_masked():
+    """T3 cells 16-26: one Pk_unwindowed, 10 Fisher + 10 shot realizations, one Pk_unwindowed
+    (local, lmax=2, 9 bins, spatially varying mask) -> 452 forward, 682 reverse."""
+    n = (16, 16, 16)
+    b = O.PolyBin3D(1000., list(n), boxcenter=[700., 10., 200.], sightline='local')
+    mask = 1 + 0.1 * np.random.RandomState(2).rand(*n)
+    ps = O.PSpec(b, np.linspace(0.0063, 0.049, 10), lmax=2, mask=mask, mask_shot=mask)
+    d = field(n)
+    ps.Pk_unwindowed(d, fish=np.eye(ps.N_bins))
+    F = [ps.compute_fisher_contribution(s) for s in range(10)]
+    S = [ps.compute_shot_contribution(100 + s) for s in range(10)]
+    ps.Pk_unwindowed(d, fish=np.mean(F, 0), shot_num=np.mean(S, 0))
+    assert (b.n_FFTs_forward, b.n_FFTs_reverse) == (452, 682)
+
+
+def test_nbins_and_allocation_tutorials():
+    """T1 cell 10 '38 bins'; T5 cell 12 'N_bins: 56'; Q-map allocation prints 1.24 GB (T4 cell 25,
+    128^3, 78 bins) and 0.40 GB (T5 cell 19, 78x146x82, 56 bins) = N_bins * n0 n1 (n2/2+1) * 16 B."""
+    assert len(np.arange(0.01, 0.4, 0.01)) - 1 == 38
+    e5 = np.linspace(0.02, 0.10, 6)
+    t5 = O.triangle_bins(e5, e5, 2)
+    assert len(t5) == 56
+    t4 = O.triangle_bins(np.arange(0.01, 0.10, 0.02), np.arange(0.01, 0.15, 0.02), 2)
+    assert "%.2f" % (len(t4) * 128 * 128 * 65 * 16 / 1024. ** 3) == "1.24"
+    assert "%.2f" % (len(t5) * 78 * 146 * 42 * 16 / 1024. ** 3) == "0.40"
+
+
+# ---------------------------------------------------------------- harmonics, FFT conventions
+def test_real_harmonics_addition_theorem():
+    rng = np.random.default_rng(0)
+    a, c = rng.normal(size=(3, 500)), rng.normal(size=(3, 500))
+    an, cn = a / np.linalg.norm(a, axis=0), c / np.linalg.norm(c, axis=0)
+    Ya, Yc = O.real_harmonics(a, 4, True), O.real_harmonics(c, 4, True)
+    for l in (1, 2, 3, 4):
+        assert Ya[l].shape[0] == 2 * l + 1
+        np.testing.assert_allclose((Ya[l] * Yc[l]).sum(0), O.legendre(l, (an * cn).sum(0)), atol=1e-13)
+        # orthogonality on the sphere: <Y_m Y_m'> = delta / (2l+1)
+        v = rng.normal(size=(3, 200000))
+        Y = O.real_harmonics(v, 4, True)[l]
+        np.testing.assert_allclose(Y @ Y.T / v.shape[1], np.eye(2 * l + 1) / (2 * l + 1), atol=1e-2)
+
+
+def test_legendre_closed_forms():
+    mu = np.linspace(-1, 1, 41)
+    for l in range(7):
+        ref = np.polynomial.legendre.legval(mu, [0] * l + [1])
+        np.testing.assert_allclose(O.legendre(l, mu), ref, atol=1e-13)
+
+
+def test_fft_bruteforce_dft():
+    n = (4, 6, 8)
+    b = O.PolyBin3D(100., list(n))
+    x = field(n, 3)
+    X = b.to_fourier(x)
+    i, j, l = np.meshgrid(*[np.arange(m) for m in n], indexing='ij')
+    ref = np.zeros(b.modk_grid.shape, dtype=complex)
+    for p in range(n[0]):
+        for q in range(n[1]):
+            for r in range(n[2] // 2 + 1):
+                ph = np.exp(-2j * np.pi * (p * i / n[0] + q * j / n[1] + r * l / n[2]))
+                ref[p, q, r] = np.sum(x * ph)
+    np.testing.assert_allclose(X, ref, atol=1e-10)
+    np.testing.assert_allclose(b.to_real(X), x, atol=1e-12)
+    # Parseval with the degeneracy weights: sum_x x^2 = (1/N) sum_k degk |X|^2 (no Nyquist double count)
+    deg = np.ones(X.shape)
+    deg[:, :, 1:n[2] // 2] = 2
+    assert np.isclose(np.sum(x ** 2), np.sum(deg * np.abs(X) ** 2) / np.prod(n))
+
+
+def test_grid_conventions():
+    """k ordering fftshift(arange(-n//2, n//2)) (also for odd n), half axis, degk, mu."""
+    b = O.PolyBin3D(10., [5, 4, 6])
+    assert np.allclose(b.k_arrs[0] / b.kF[0], [0, 1, -3, -2, -1])
+    assert np.allclose(b.k_arrs[1] / b.kF[1], [0, 1, -2, -1])
+    assert np.allclose(b.k_arrs[2] / b.kF[2], [0, 1, 2, 3])
+    assert np.all(b.degeneracy_factor[:, :, 0] == 1) and np.all(b.degeneracy_factor[:, :, 1:] == 2)
+    kz = b.k_grids[2]
+    nz = b.modk_grid > 0
+    np.testing.assert_allclose(b.muk_grid[nz], kz[nz] / b.modk_grid[nz])
+    assert b.muk_grid[0, 0, 0] == 0
+
+
+# ---------------------------------------------------------------- bin integrals
+def test_integrators_bruteforce_loops():
+    """Per-bin full sweeps, written as the reference's explicit loop nest (utils.pyx:51-134)."""
+    n = (6, 6, 8)
+    b = O.PolyBin3D(100., list(n))
+    rng = np.random.RandomState(4)
+    s1 = rng.randn(*b.modk_grid.shape) + 1j * rng.randn(*b.modk_grid.shape)
+    s2 = rng.randn(*b.modk_grid.shape) + 1j * rng.randn(*b.modk_grid.shape)
+    r = rng.rand(*b.modk_grid.shape)
+    edges = np.array([0.07, 0.13, 0.2, 0.25])
+    for l in (0, 2, 4):
+        ci = np.zeros(3)
+        it = np.zeros(3)
+        rd = np.zeros(3)
+        for bb in range(3):
+            for idx in itertools.product(*[range(m) for m in b.modk_grid.shape]):
+                k = b.modk_grid[idx]
+                if edges[bb] <= k < edges[bb + 1]:
+                    L = O.legendre(l, b.muk_grid[idx])
+                    g = b.degeneracy_factor[idx]
+                    ci[bb] += g * (s1[idx] * np.conj(s2[idx])).real * L
+                    it[bb] += g * abs(s1[idx]) ** 2 * L
+                    rd[bb] += g * r[idx] ** 2 * L * O.legendre(2, b.muk_grid[idx])
+        np.testing.assert_allclose(b.integrator.cross_integrate(s1, s2, edges, l), ci, rtol=1e-12)
+        np.testing.assert_allclose(b.integrator.integrate(s1, edges, l), it, rtol=1e-12)
+        np.testing.assert_allclose(b.integrator.real_integrate_double(r, l, 2, edges), rd, rtol=1e-12)
+
+
+def test_mode_count_invariant():
+    """real_integrate_double(1, 0, 0)[b] = number of full-grid lattice modes in shell b."""
+    n = 12
+    b = O.PolyBin3D(1000., [n, n, n])
+    edges = np.array([1.0, 2.0, 3.5, 5.5]) * b.kF[0]
+    got = b.integrator.real_integrate_double(np.ones(b.modk_grid.shape), 0, 0, edges)
+    cnt = np.zeros(3)
+    for v in itertools.product(range(-n // 2 + 1, n // 2), repeat=3):
+        k = np.sqrt(sum((c * b.kF[0]) ** 2 for c in v))
+        for bb in range(3):
+            if edges[bb] <= k < edges[bb + 1]:
+                cnt[bb] += 1
+    np.testing.assert_array_equal(got, cnt)
+
+
+# ---------------------------------------------------------------- estimators: known answers
+def test_pk_ideal_plane_wave():
+    """delta = A cos(k0.x), unit P_fid, global -> P0[b] = A^2 V / (2 N_modes(b)), other bins 0."""
+    n, L, A = 16, 400., 2.5
+    b = O.PolyBin3D(L, [n, n, n])
+    kF = b.kF[0]
+    edges = np.array([1.0, 2.0, 3.0, 4.0]) * kF + 1e-9
+    ps = O.PSpec(b, edges, lmax=0)
+    x = np.meshgrid(*[np.arange(n) * L / n] * 3, indexing='ij')
+    k0 = np.array([1, 2, 0]) * kF  # |k0| = sqrt(5) kF in bin 1
+    d = A * np.cos(k0[0] * x[0] + k0[1] * x[1] + k0[2] * x[2])
+    p0 = ps.Pk_ideal(d)['p0']
+    nmodes = sum(1 for v in itertools.product(range(-n // 2 + 1, n // 2), repeat=3)
+                 if edges[1] <= np.sqrt(sum(c * c for c in v)) * kF < edges[2])
+    np.testing.assert_allclose(p0, [0., A ** 2 * b.volume / (2 * nmodes), 0.], atol=1e-9)
+
+
+def test_bk_numerator_fourier_bruteforce():
+    """sum_x g_a g_b g_c = N^-2 sum_{k1+k2+k3=0} g_a(k1) g_b(k2) g_c(k3) on an 8^3 grid: the oracle's
+    g-maps + assemble_b3 + V/N/sym normalisation against a direct Fourier-space triangle sum."""
+    n = 8
+    b = O.PolyBin3D(100., [n, n, n])
+    kF = b.kF[0]
+    edges = np.array([1.0, 1.5, 2.5, 3.5]) * kF
+    bs = O.BSpec(b, edges, lmax=0, Pk_fid='default')
+    d = field((n, n, n), 5)
+    d = d + 0.3 * (d ** 2 - np.mean(d ** 2))
+    num = bs.Bk_numerator_ideal(d)
+    dk = np.fft.fftn(d)
+    f = np.fft.fftfreq(n, 1. / n)
+    kk = np.sqrt(f[:, None, None] ** 2 + f[None, :, None] ** 2 + f[None, None, :] ** 2) * kF
+    masks = [(kk >= edges[i]) & (kk < edges[i + 1]) for i in range(len(edges) - 1)]
+    idx = np.arange(n)
+    ref = []
+    for (b1, b2, b3, _), sym in zip(bs.all_bins, bs.sym_factor):
+        ga, gb, gc = dk * masks[b1], dk * masks[b2], dk * masks[b3]
+        tot = 0.
+        for p in itertools.product(range(n), repeat=3):
+            if ga[p] == 0:
+                continue
+            # sum over k2 of gb(k2) gc(-k1-k2)
+            mi = (-p[0] - idx[:, None, None]) % n
+            mj = (-p[1] - idx[None, :, None]) % n
+            mk = (-p[2] - idx[None, None, :]) % n
+            tot += ga[p] * np.sum(gb * gc[mi, mj, mk])
+        ref.append((tot / n ** 6).real * b.volume / n ** 3 / sym)
+    np.testing.assert_allclose(num, ref, rtol=1e-9, atol=1e-12 * np.max(np.abs(ref)))
+
+
+def test_bk_ideal_fisher_triangle_count():
+    """compute_fisher_ideal(discreteness_correction=False), l=0, unit weights: diagonal entries are
+    #{k1+k2+k3=0, ki in bin bi} / sym / V (closed form by lattice enumeration)."""
+    n = 8
+    b = O.PolyBin3D(100., [n, n, n])
+    kF = b.kF[0]
+    edges = np.array([1.0, 1.5, 2.5, 3.5]) * kF
+    bs = O.BSpec(b, edges, lmax=0, Pk_fid='default')
+    F = bs.compute_fisher_ideal(discreteness_correction=False)
+    f = np.fft.fftfreq(n, 1. / n)
+    kk = np.sqrt(f[:, None, None] ** 2 + f[None, :, None] ** 2 + f[None, None, :] ** 2) * kF
+    binof = np.full(kk.shape, -1)
+    for i in range(len(edges) - 1):
+        binof[(kk >= edges[i]) & (kk < edges[i + 1])] = i
+    idx = np.arange(n)
+    for t, (b1, b2, b3, _) in enumerate(bs.all_bins):
+        cnt = 0
+        for p in itertools.product(range(n), repeat=3):
+            if binof[p] != b1:
+                continue
+            mi = (-p[0] - idx[:, None, None]) % n
+            mj = (-p[1] - idx[None, :, None]) % n
+            mk = (-p[2] - idx[None, None, :]) % n
+            cnt += int(np.sum((binof == b2) & (binof[mi, mj, mk] == b3)))
+        sym = 6. if b1 == b2 == b3 else (2. if (b1 == b2 or b2 == b3) else 1.)
+        assert bs.sym_factor[t] == sym
+        np.testing.assert_allclose(F[t, t], cnt / sym / b.volume, rtol=1e-10)
+
+
+def test_grf_statistics_and_hermiticity():
+    """generate_data: E|a_k|^2 = P N^2 / V; real output; reproducible per seed."""
+    n = 16
+    b = O.PolyBin3D(500., [n, n, n])
+    a = [b.generate_data(seed=s, output_type='fourier') for s in range(40)]
+    p = np.mean([np.abs(x[b.modk_grid > 0]) ** 2 for x in a])
+    assert abs(p / (n ** 6 / b.volume) - 1) < 0.02
+    np.testing.assert_array_equal(a[3], b.generate_data(seed=3, output_type='fourier'))
+    r = b.to_real(a[0])
+    np.testing.assert_allclose(b.to_fourier(r), a[0], atol=1e-9)
+
+
+def test_pk_mc_fisher_global_closed_form():
+    """Global sightline, constant mask, trivial S^-1: per seed
+    F[(l,b),(l',b')] = delta_bb' m^2 V/(2N^2) sum_{k in b} degk invPk0 |a_k|^2 L_l L_l'."""
+    n = 16
+    Pk = [np.linspace(0, 2, 400), 1e3 * (1 + np.linspace(0, 2, 400))]
+    b = O.PolyBin3D(600., [n, n, n], Pk=Pk)
+    edges = np.linspace(0.012, 0.07, 5)
+    m = 0.7
+    ps = O.PSpec(b, edges, lmax=4, mask=m * np.ones((n, n, n)))
+    F = ps.compute_fisher_contribution(3)
+    a = b.generate_data(seed=3 + int(1e7), output_type='fourier')
+    ref = np.zeros_like(F)
+    Nk = ps.Nk
+    idx = O.bin_index(b.modk_grid, edges)
+    for l1 in range(3):
+        for l2 in range(3):
+            for k in range(Nk):
+                sel = idx == k
+                w = b.degeneracy_factor[sel] * b.invPk0_grid[sel] * np.abs(a[sel]) ** 2
+                w = w * O.legendre(2 * l1, b.muk_grid[sel]) * O.legendre(2 * l2, b.muk_grid[sel])
+                ref[l1 * Nk + k, l2 * Nk + k] = m ** 2 * b.volume / (2 * n ** 6) * w.sum()
+    np.testing.assert_allclose(F, ref, atol=1e-12 * np.abs(ref).max())
+
+
+def _dense_ops(b):
+    """Full-grid DFT matrix and the addition-theorem kernels L_l(k_hat . x_hat) e^{-ik.x}."""
+    n = b.gridsize
+    N = int(np.prod(n))
+    idx = np.stack(np.meshgrid(*[np.arange(m) for m in n], indexing='ij'), -1).reshape(N, 3)
+    fr = np.stack(np.meshgrid(*[np.fft.fftfreq(m, 1. / m) for m in n], indexing='ij'), -1).reshape(N, 3)
+    ph = np.exp(-2j * np.pi * (fr / n) @ idx.T)                      # [k, x]
+    kv = fr * b.kF
+    xs = np.stack(np.meshgrid(*[np.fft.fftshift(np.arange(-m // 2, m // 2)) * b.boxsize[i] / m + b.boxcenter[i]
+                                for i, m in enumerate(n)], indexing='ij'), -1).reshape(N, 3)
+    kh = kv / np.maximum(np.linalg.norm(kv, axis=1), 1e-300)[:, None]
+    xh = xs / np.maximum(np.linalg.norm(xs, axis=1), 1e-300)[:, None]
+    return ph, kv, kh @ xh.T, fr
+
+
+def test_pk_mc_fisher_local_masked_dense():
+    """Local sightline, varying mask, l<=2: the per-seed MC Fisher against a dense-matrix restatement
+    (explicit DFT matrices and addition-theorem kernels instead of Y_lm sums and FFTs)."""
+    n = 8
+    b = O.PolyBin3D(200., [n, n, n], boxcenter=[300., 140., 90.], sightline='local')
+    mask = 0.5 + np.random.RandomState(6).rand(n, n, n)
+    edges = np.array([1.0, 2.0, 3.0]) * b.kF[0] + 1e-9
+    ps = O.PSpec(b, edges, lmax=2, mask=mask)
+    F = ps.compute_fisher_contribution(2)
+    N = n ** 3
+    ph, kv, cosang, fr = _dense_ops(b)
+    # full-grid Fourier amplitudes of the same GRF
+    ah = b.generate_data(seed=2 + int(1e7), output_type='fourier')
+    a = np.fft.fftn(b.to_real(ah)).reshape(N)
+    ar = np.fft.ifftn(a.reshape(n, n, n)).real.reshape(N)
+    modk = np.linalg.norm(kv, axis=1)
+    Pk0 = np.ones(N)
+    inv = 1. / Pk0
+    mk = mask.reshape(N)
+    SinvP = ph @ (mk * ar)
+    Ainv = inv * a
+    Ainv_r = (ph.conj().T @ Ainv).real / N
+    Lk = {0: np.ones((N, N)), 2: O.legendre(2, cosang)}
+    lefts = {0: Ainv, 2: (ph * Lk[2]) @ Ainv_r}
+    mu = np.where(modk > 0, kv[:, 2] * 0, 0)  # unused for local
+    Nk = 2
+    ref = np.zeros((2 * Nk, 2 * Nk))
+    SinvP_r = (ph.conj().T @ SinvP).real / N
+    for li, ell in enumerate((0, 2)):
+        for kb in range(Nk):
+            th = (modk >= edges[kb]) & (modk < edges[kb + 1])
+            if ell == 0:
+                km = th * SinvP
+                Q = ph @ (mk * (ph.conj().T @ km).real / N)
+            else:
+                leg = (ph * Lk[2]) @ SinvP_r
+                t1 = ph @ (mk * (ph.conj().T @ (th * leg)).real / N)
+                real_map = ((ph.conj() * Lk[2]).T @ (th * SinvP)).real / N
+                t2 = ph @ (mk * real_map)
+                Q = 0.5 * t1 + 0.5 * t2
+            for lj, ell2 in enumerate((0, 2)):
+                for kb2 in range(Nk):
+                    th2 = (modk >= edges[kb2]) & (modk < edges[kb2 + 1])
+                    ref[li * Nk + kb, lj * Nk + kb2] = 0.5 * b.volume / N ** 2 * np.sum(
+                        th2 * (lefts[ell2] * np.conj(Q)).real)
+    np.testing.assert_allclose(F, ref, atol=1e-10 * np.abs(ref).max(), rtol=1e-8)
+
+
+def test_pk_unwindowed_equals_ideal_with_ideal_fisher():
+    """Pk_unwindowed with F = F_ideal, S^-1 = invPk0 and no mask reproduces Pk_ideal
+    (pspec.py:631-640 apply the same filter)."""
+    n = 16
+    Pk = [np.linspace(0, 2, 300), 2e3 / (1 + 10 * np.linspace(0, 2, 300))]
+    b = O.PolyBin3D(800., [n, n, n], Pk=Pk)
+    edges = np.linspace(0.01, 0.05, 6)
+    ps = O.PSpec(b, edges, lmax=2, applySinv=lambda m, input_type='real', output_type='fourier':
+                 b.to_fourier(m) * b.invPk0_grid)
+    d = field((n, n, n), 9)
+    ideal = ps.Pk_ideal(d)
+    unw = ps.Pk_unwindowed(d, fish=ps.fish_ideal)
+    for k in ideal:
+        np.testing.assert_allclose(unw[k], ideal[k], rtol=1e-10)
