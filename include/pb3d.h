@@ -84,6 +84,13 @@ typedef struct {
  *   2 YLM     v = x * [bin(k) == bins[batch] or bins==NULL] * Y_lm(k_hat) * w(k)
  *             (prod_fourier_filter with a Y_lm grid, utils.pyx:325-338; base.py:398,410)
  *   3 WEIGHT  v = x * w(k)                   (prod_fourier with invPk0, utils.pyx:213-225)
+ *   4 BKQ     C2R prologue of the bispectrum MC Fisher: the Q map of triplet trip[trip0+batch] =
+ *             (b1,b2,b3,l) assembled from packed FT[g_A g_B] rows (add_Q_element,
+ *             PolyBin3D/bspec.py:615-640): terms bit0 = Theta_b3 [L_l(mu)] FT_0(b1,b2);
+ *             bit1 = Theta_b1 FT_l(b2,b3) + Theta_b2 FT_l(b1,b3); bit2 = Theta_b3 Y_lm(k_hat)
+ *             FT_0(b1,b2).  FT_l(A,B) = ft + (l/2)*ft_lstride + pair_index[A*nk+B]*packed_len.
+ *   5 PACK    R2C epilogue: out[batch*packed_len + pack_index[k]] = v for modes with
+ *             pack_index >= 0 (the in-bin modes, stored bin-major).
  * For R2C epilogues x is the transform output; out = scale*v or out += scale*v.
  * `x_batch_stride` (complex elements) advances the C2R input per batch entry (0 = broadcast:
  * one input map filtered into many bins). */
@@ -99,6 +106,17 @@ typedef struct {
   int64_t x_batch_stride;
   float scale;
   int32_t pad2_;
+  /* PACK / BKQ */
+  const int32_t* pack_index;
+  int64_t packed_len;
+  const int32_t* trip;
+  const int32_t* pair_index;
+  const void* ft;
+  int64_t ft_lstride;
+  int32_t nk;
+  int32_t terms;
+  int32_t trip0;
+  int32_t leg3_legendre;
 } pb3d_kop;
 
 /* ---------------------------------------------------------------- library */
@@ -107,6 +125,16 @@ const char* pb3d_last_error(void);
 /* FFT counters kept like PolyBin3D.n_FFTs_forward / n_FFTs_reverse (base.py:134-135,178,204):
  * each batch entry counts one transform.  dir: 0 = forward, 1 = reverse, -1 = reset both. */
 int64_t pb3d_fft_counter(int dir);
+
+/* Sampled launch timing for roofline reports (no counterpart in the reference): with period P > 0
+ * every P-th launch of each FFT pass kind is bracketed by hipEvents on its stream.  kinds:
+ * 0 R2C z-rows, 1 C2R z-rows, 2 plain column pass, 3 C2R first column pass (mode prologue),
+ * 4 R2C last column pass (mode epilogue).  pb3d_prof_read synchronises the sampled events and
+ * returns the number of sampled launches, their summed duration (ms) and their summed
+ * algorithmic bytes (column pass: 16 B per complex mode; row pass: 4 B per voxel + 8 B per
+ * mode).  pb3d_prof_enable resets the totals; period 0 switches sampling off. */
+int pb3d_prof_enable(int period);
+int pb3d_prof_read(int kind, int64_t* sampled, double* ms, double* bytes);
 
 /* ---------------------------------------------------------------- FFT plans
  * Replace _fft_setup (base.py:256-302).  Supported: n0, n1 powers of two in [2, 512], n2 a power
@@ -123,7 +151,7 @@ int pb3d_fft_r2c(const pb3d_plan* plan, const float* in, void* out, void* work, 
 
 /* 3D complex-to-real transform with 1/N (to_real, base.py:202-229); numpy semantics on the
  * self-conjugate kz=0 and kz=n2/2 planes (imaginary parts ignored).
- * in: half complex read through `kop` (NULL = plain; never modified); work: half-complex
+ * in: half complex read through `kop` (NULL = plain; never modified; may be NULL for BKQ); work: half-complex
  * scratch of batch*n0*n1*(n2/2+1) float2; out: real written through `rop` (NULL = plain). */
 int pb3d_fft_c2r(const pb3d_plan* plan, const void* in, float* out, void* work, int batch,
                  const pb3d_kop* kop, const pb3d_rop* rop, void* stream);
@@ -140,6 +168,23 @@ int pb3d_kmap(const pb3d_grid* grid, const void* x, void* out, int batch, const 
  * symmetrisation.  sqrtP_w: float32 half-grid of sqrt(P)*W (NULL = 1). */
 int pb3d_grf_hermitian(const pb3d_grid* grid, const double* re, const double* im,
                        const float* sqrtP_w, float norm, void* out, void* stream);
+
+/* Gaussian random field drawn on the GPU: same recipe and distribution as pb3d_grf_hermitian, with
+ * the full-grid normals (re, im) generated from a counter-based hash of (seed, full-grid index)
+ * instead of the numpy stream.  For throughput runs; parity runs use pb3d_grf_hermitian. */
+int pb3d_grf_device(const pb3d_grid* grid, uint64_t seed, const float* sqrtP_w, float norm, void* out,
+                    void* stream);
+
+/* Packed-mode assembly without a transform: out[batch][p] (+)= scale * kop(mode = modes[p]) for
+ * p < packed_len (kop kinds FILTER, YLM, WEIGHT, BKQ; for BKQ the input x is ignored).  Used
+ * for constant masks, where the Fisher Q maps need no real-space step (bspec.py:662-664). */
+int pb3d_kmap_packed(const pb3d_grid* grid, const void* x, const int32_t* modes, int64_t packed_len,
+                     void* out, int batch, const pb3d_kop* kop, void* stream);
+
+/* Row gather for the Fisher contraction: out[r][q] = src[rows[r]*src_stride + offset + q] *
+ * w[offset + q] for q < len (complex src/out, real w; rows device int32 in [0, nsrc)). */
+int pb3d_gather_rows(const void* src, int64_t src_stride, int nsrc, const int32_t* rows, int nrows,
+                     int64_t offset, int64_t len, const float* w, void* out, void* stream);
 
 /* ---------------------------------------------------------------- bin reductions
  * One pass over the half grid for a batch of maps (cross_integrate / integrate /

@@ -21,11 +21,19 @@ EXPORTS = {
     "pb3d_version": (ctypes.c_int, []),
     "pb3d_last_error": (ctypes.c_char_p, []),
     "pb3d_fft_counter": (c_int64, [ctypes.c_int]),
+    "pb3d_prof_enable": (ctypes.c_int, [ctypes.c_int]),
+    "pb3d_prof_read": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_int64), ctypes.POINTER(c_double),
+                                      ctypes.POINTER(c_double)]),
     "pb3d_plan_create": (ctypes.c_int, [ctypes.POINTER(c_void_p), c_void_p]),
     "pb3d_plan_destroy": (ctypes.c_int, [c_void_p]),
     "pb3d_fft_r2c": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
     "pb3d_fft_c2r": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
     "pb3d_kmap": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p]),
+    "pb3d_kmap_packed": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, ctypes.c_int, c_void_p,
+                                        c_void_p]),
+    "pb3d_gather_rows": (ctypes.c_int, [c_void_p, c_int64, ctypes.c_int, c_void_p, ctypes.c_int, c_int64, c_int64,
+                                        c_void_p, c_void_p, c_void_p]),
+    "pb3d_grf_device": (ctypes.c_int, [c_void_p, ctypes.c_uint64, c_void_p, c_float, c_void_p, c_void_p]),
     "pb3d_grf_hermitian": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
     "pb3d_bin_reduce_work_size": (c_size_t, [c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pb3d_bin_reduce": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, ctypes.c_int,
@@ -55,11 +63,14 @@ class KOp(ctypes.Structure):
     """pb3d_kop: Fourier-space mode op (C2R prologue / R2C epilogue)."""
     _fields_ = [("kind", c_int32), ("ell", c_int32), ("ell2", c_int32), ("m", c_int32),
                 ("accumulate", c_int32), ("pad_", c_int32), ("bin_table", c_void_p), ("bins", c_void_p),
-                ("w", c_void_p), ("x_batch_stride", c_int64), ("scale", c_float), ("pad2_", c_int32)]
+                ("w", c_void_p), ("x_batch_stride", c_int64), ("scale", c_float), ("pad2_", c_int32),
+                ("pack_index", c_void_p), ("packed_len", c_int64), ("trip", c_void_p), ("pair_index", c_void_p),
+                ("ft", c_void_p), ("ft_lstride", c_int64), ("nk", c_int32), ("terms", c_int32),
+                ("trip0", c_int32), ("leg3_legendre", c_int32)]
 
 
 ROP_PLAIN, ROP_MUL, ROP_DIFF, ROP_YLM, ROP_MULYLM, ROP_SQUARE = range(6)
-KOP_PLAIN, KOP_FILTER, KOP_YLM, KOP_WEIGHT = range(4)
+KOP_PLAIN, KOP_FILTER, KOP_YLM, KOP_WEIGHT, KOP_BKQ, KOP_PACK = range(6)
 
 _lib = None
 
@@ -99,16 +110,24 @@ def rop(kind=ROP_PLAIN, a=None, b=None, c=None, d=None, ell=0, m=0, accumulate=0
     o.a, o.b, o.c, o.d = (_ptr(a), _ptr(b), _ptr(c), _ptr(d))
     o.batch_stride_a, o.batch_stride_b, o.batch_stride_c, o.batch_stride_d = sa, sb, sc, sd
     o.scale = float(scale)
+    o._t = {k: v for k, v in (('a', a), ('b', b), ('c', c), ('d', d)) if v is not None and not isinstance(v, int)}
     return o
 
 
 def kop(kind=KOP_PLAIN, bin_table=None, bins=None, w=None, ell=0, ell2=0, m=0, accumulate=0, scale=1.0,
-        x_batch_stride=0):
+        x_batch_stride=0, pack_index=None, packed_len=0, trip=None, pair_index=None, ft=None, ft_lstride=0,
+        nk=0, terms=0, trip0=0, leg3_legendre=0):
     o = KOp()
     o.kind, o.ell, o.ell2, o.m, o.accumulate = kind, ell, ell2, m, int(accumulate)
     o.bin_table, o.bins, o.w = _ptr(bin_table), _ptr(bins), _ptr(w)
     o.x_batch_stride = int(x_batch_stride)
     o.scale = float(scale)
+    o.pack_index, o.packed_len = _ptr(pack_index), int(packed_len)
+    o.trip, o.pair_index, o.ft, o.ft_lstride = _ptr(trip), _ptr(pair_index), _ptr(ft), int(ft_lstride)
+    o.nk, o.terms, o.trip0, o.leg3_legendre = int(nk), int(terms), int(trip0), int(leg3_legendre)
+    o._t = {k: v for k, v in (('bin_table', bin_table), ('bins', bins), ('w', w), ('pack_index', pack_index),
+                              ('trip', trip), ('pair_index', pair_index), ('ft', ft))
+            if v is not None and not isinstance(v, int)}
     return o
 
 

@@ -83,6 +83,8 @@ class PolyBin3D:
         self.device = torch.device(device if device is not None else 'cuda')
         if self.device.type != 'cuda':
             raise Exception("backend='hip' needs a GPU device")
+        if self.device.index is None:
+            self.device = torch.device('cuda', torch.cuda.current_device())
         # base.py:31-60
         self.boxsize = np.asarray([boxsize] * 3 if np.ndim(boxsize) == 0 else boxsize, dtype=np.float64)
         assert len(self.boxsize) == 3, "Need to specify x, y, z boxsize"
@@ -196,6 +198,16 @@ class PolyBin3D:
              for i in range(3)]
         return np.meshgrid(*r, indexing='ij')
 
+    def bin_index_host(self, edges):
+        """int64 host array: b with edges[b] <= |k| < edges[b+1] in FP64 (utils.pyx:63), else -1."""
+        edges = np.asarray(edges, dtype=np.float64)
+        nb = len(edges) - 1
+        if not np.all(np.diff(edges) > 0):
+            raise ValueError("bin edges must be strictly increasing")
+        idx = np.searchsorted(edges, self.modk_grid, side='right') - 1
+        idx[(idx < 0) | (idx >= nb)] = -1
+        return idx
+
     def bin_table(self, edges):
         """uint8 device table: index b with edges[b] <= |k| < edges[b+1] (FP64, as
         utils.pyx:63), or 255.  Cached per edge list."""
@@ -241,6 +253,40 @@ class PolyBin3D:
     def stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
 
+    def _need(self, t, n, what):
+        """Raise unless tensor t is a contiguous device tensor holding at least n elements of its
+        kind (complex64 modes or float32 voxels), so no kernel reads or writes past it."""
+        if t is None:
+            raise ValueError("%s: missing tensor" % what)
+        if not t.is_contiguous() or t.device != self.device:
+            raise ValueError("%s: tensor must be contiguous on %s" % (what, self.device))
+        if t.numel() < n:
+            raise ValueError("%s: tensor has %d elements, needs %d" % (what, t.numel(), n))
+
+    def _check_rop(self, rop, batch, what):
+        """Operands of a voxel op must hold (batch-1)*stride + N voxels."""
+        if rop is None:
+            return
+        for name in ('a', 'b', 'c', 'd'):
+            t = getattr(rop, '_t', {}).get(name)
+            if t is not None:
+                self._need(t, (batch - 1) * getattr(rop, 'batch_stride_' + name) + self.N, what + ' rop.' + name)
+
+    def _check_kop(self, kop, batch, what):
+        """Tables of a mode op must cover the half grid (bins: one entry per batch entry)."""
+        if kop is None:
+            return
+        t = getattr(kop, '_t', {})
+        for name in ('bin_table', 'w', 'pack_index'):
+            if name in t:
+                self._need(t[name], self.H, what + ' kop.' + name)
+        if 'bins' in t:
+            self._need(t['bins'], batch, what + ' kop.bins')
+        if kop.kind == C.KOP_BKQ:
+            self._need(t.get('trip'), 4 * (kop.trip0 + batch), what + ' kop.trip')
+            self._need(t.get('pair_index'), kop.nk * kop.nk, what + ' kop.pair_index')
+            self._need(t.get('ft'), kop.packed_len, what + ' kop.ft')
+
     # ------------------------------------------------------------------ data movement
     def as_real(self, x):
         """float32 contiguous device tensor of shape [...,n0,n1,n2]."""
@@ -272,22 +318,38 @@ class PolyBin3D:
         if out is None:
             out = torch.empty((batch,) + self.half_shape if batch > 1 else self.half_shape,
                               dtype=torch.complex64, device=self.device)
-        needs_work = kop is not None and bool(kop.accumulate)
+        needs_work = kop is not None and (bool(kop.accumulate) or kop.kind == C.KOP_PACK)
         work = self.work('fft_r2c', batch * self.H * 8) if needs_work else None
+        if x is not None:
+            self._need(x, batch * self.N, 'r2c input')
+        elif rop is None or not rop.a:
+            raise ValueError("r2c: no input")
+        self._check_rop(rop, batch, 'r2c')
+        self._check_kop(kop, batch, 'r2c')
+        packed = kop is not None and kop.kind == C.KOP_PACK
+        self._need(out, batch * (kop.packed_len if packed else self.H), 'r2c output')
         C.check(C.lib().pb3d_fft_r2c(self._plan, C._ptr(x), out.data_ptr(), C._ptr(work), batch,
                                      C.byref_or_none(rop), C.byref_or_none(kop), self.stream()))
         self.n_FFTs_forward += batch
         return out
 
     def c2r(self, k, out=None, kop=None, rop=None, batch=None):
-        """Batched fused C2R with 1/N."""
+        """Batched fused C2R with 1/N (k may be None for the BKQ prologue)."""
         if batch is None:
             batch = 1 if k.dim() == 3 else int(np.prod(k.shape[:-3]))
         if out is None:
             g = tuple(int(v) for v in self.gridsize)
             out = torch.empty((batch,) + g if batch > 1 else g, dtype=torch.float32, device=self.device)
         work = self.work('fft_c2r', batch * self.H * 8)
-        C.check(C.lib().pb3d_fft_c2r(self._plan, k.data_ptr(), out.data_ptr(), work.data_ptr(), batch,
+        xs = self.H if kop is None else kop.x_batch_stride
+        if k is not None:
+            self._need(k, (batch - 1) * xs + self.H, 'c2r input')
+        elif kop is None or kop.kind != C.KOP_BKQ:
+            raise ValueError("c2r: no input")
+        self._check_kop(kop, batch, 'c2r')
+        self._check_rop(rop, batch, 'c2r')
+        self._need(out, batch * self.N, 'c2r output')
+        C.check(C.lib().pb3d_fft_c2r(self._plan, C._ptr(k), out.data_ptr(), work.data_ptr(), batch,
                                      C.byref_or_none(kop), C.byref_or_none(rop), self.stream()))
         self.n_FFTs_reverse += batch
         return out
@@ -306,6 +368,9 @@ class PolyBin3D:
         if out is None:
             out = torch.empty((batch,) + self.half_shape if batch > 1 else self.half_shape,
                               dtype=torch.complex64, device=self.device)
+        self._need(x, (batch - 1) * kop.x_batch_stride + self.H, 'kmap input')
+        self._need(out, batch * self.H, 'kmap output')
+        self._check_kop(kop, batch, 'kmap')
         C.check(C.lib().pb3d_kmap(C.ctypes.byref(self.cgrid), x.data_ptr(), out.data_ptr(), batch,
                                   C.ctypes.byref(kop), self.stream()))
         return out
@@ -342,13 +407,18 @@ class PolyBin3D:
             self._bufs[key] = torch.from_numpy(s.astype(np.float32)).to(self.device)
         sqrtPw = self._bufs[key]
         n = tuple(int(v) for v in self.gridsize)
-        rs = np.random.RandomState(seed) if seed is not None else np.random.mtrand._rand
-        re = torch.from_numpy(rs.randn(*n)).to(self.device)
-        im = torch.from_numpy(rs.randn(*n)).to(self.device)
         out = torch.empty(self.half_shape, dtype=torch.complex64, device=self.device)
         norm = float(self.N / np.sqrt(self.volume))
-        C.check(C.lib().pb3d_grf_hermitian(C.ctypes.byref(self.cgrid), re.data_ptr(), im.data_ptr(),
-                                           sqrtPw.data_ptr(), norm, out.data_ptr(), self.stream()))
+        if self.rng == 'device':
+            s = int(seed) if seed is not None else int(np.random.randint(0, 2 ** 62))
+            C.check(C.lib().pb3d_grf_device(C.ctypes.byref(self.cgrid), s, sqrtPw.data_ptr(), norm,
+                                            out.data_ptr(), self.stream()))
+        else:
+            rs = np.random.RandomState(seed) if seed is not None else np.random.mtrand._rand
+            re = torch.from_numpy(rs.randn(*n)).to(self.device)
+            im = torch.from_numpy(rs.randn(*n)).to(self.device)
+            C.check(C.lib().pb3d_grf_hermitian(C.ctypes.byref(self.cgrid), re.data_ptr(), im.data_ptr(),
+                                               sqrtPw.data_ptr(), norm, out.data_ptr(), self.stream()))
         if output_type == 'fourier':
             return out
         return self.c2r(out)
@@ -429,6 +499,10 @@ class PolyBin3D:
         l2 = C.int_array(ells2 if ells2 is not None else [0] * nout)
         if y_ostride is None:
             y_ostride = self.H
+        # every read stays inside the tensors handed in
+        self._need(x, nq * self.H if not real else nq * self.H, 'bin_reduce x')
+        if y is not None:
+            self._need(y, (nq - 1) * y_qstride + (nout - 1) * y_ostride + self.H, 'bin_reduce y')
         C.check(C.lib().pb3d_bin_reduce(C.ctypes.byref(self.cgrid), x.data_ptr(), xs, C._ptr(y), y_qstride,
                                         y_ostride, nq, nout, l1, l2, 1 if real else 0, table.data_ptr(), nb,
                                         out.data_ptr(), work.data_ptr(), self.stream()))

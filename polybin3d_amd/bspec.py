@@ -1,0 +1,597 @@
+"""Bispectrum estimators, backend='hip' (PolyBin3D/bspec.py).
+
+Same API as PolyBin3D.BSpec: Bk_ideal, Bk_numerator_ideal, compute_fisher_ideal, Bk_numerator,
+compute_fisher_contribution, compute_fisher, Bk_unwindowed, generate_sims, load_sims, get_ks.
+  * g_{l,b}(x) maps: one batched C2R per multipole with Theta_b L_l in the prologue (all bins at once);
+  * triplet sums sum_x g_a g_b g_c: pb3d_triplet_sum over the map stack;
+  * MC Fisher: Q maps stored packed over the in-bin modes only, assembled inside the C2R prologue
+    from packed FT[g_A g_B] maps, masked in the C2R epilogue / R2C prologue, and contracted with the
+    S^-1-weighted maps as per-bin GEMMs; rows accumulate in FP64.
+"""
+import numpy as np
+import scipy.interpolate
+import torch
+
+from . import _capi as C
+from .base import harmonics_tags
+
+
+def triangle_bins(k_bins, k_bins_squeeze, lmax, include_partial_triangles):
+    """all_bins (PolyBin3D/bspec.py:67-75) with the triangle test of _check_bin (bspec.py:171-199)."""
+    kb = np.asarray(k_bins, dtype=np.float64)
+    ks = np.asarray(k_bins_squeeze, dtype=np.float64)
+    Nk, Nks = len(kb) - 1, len(ks) - 1
+
+    def check(b1, b2, b3):
+        if include_partial_triangles:
+            hi = ks[b1 + 1] + ks[b2 + 1]
+            if b1 > b2:
+                lo = ks[b1] - ks[b2 + 1]
+            elif b1 == b2:
+                lo = 0.
+            else:
+                lo = ks[b2] - ks[b1 + 1]
+            return not (ks[b3] >= hi - 1e-10 or ks[b3 + 1] <= lo + 1e-10)
+        k1 = 0.5 * (kb[b1] + kb[b1 + 1])
+        k2 = 0.5 * (ks[b2] + ks[b2 + 1])
+        k3 = 0.5 * (ks[b3] + ks[b3 + 1])
+        return not (k3 < abs(k1 - k2) or k3 > k1 + k2)
+
+    rows = [(b1, b2, b3, l) for l in range(0, lmax + 1, 2) for b1 in range(Nk) for b2 in range(b1, Nks)
+            for b3 in range(b2, Nks) if check(b1, b2, b3)]
+    return np.asarray(rows, dtype=np.int64).reshape(-1, 4)
+
+
+class BSpec:
+    """Bispectrum estimators (PolyBin3D/bspec.py:12-1119)."""
+
+    def __init__(self, base, k_bins, lmax=0, mask=None, applySinv=None, k_bins_squeeze=None,
+                 include_partial_triangles=False, add_GIC=False, mask_IC=None, add_RIC=False, radial_bins_RIC=[],
+                 Pk_fid=None, mask_shot=None, applySinv_transpose=None, verb=False, precision='fp32'):
+        b = self.base = base
+        self.k_bins = np.asarray(k_bins, dtype=np.float64)
+        self.lmax = lmax
+        self.Nk = len(k_bins) - 1
+        self.Nl = lmax // 2 + 1
+        self.include_partial_triangles = include_partial_triangles
+        self.Pk_fid = Pk_fid
+        self.precision = precision
+        assert self.lmax % 2 == 0, "l-max must be even!"
+        assert np.max(self.k_bins) < np.min(b.kNy), "k_max must be less than k_Nyquist!"
+        assert np.min(self.k_bins) >= np.max(b.kF), "k_min must be at least the k_fundamental!"
+        assert np.max(self.k_bins) <= b.Pfid[0][-1], "Fiducial power spectrum should extend up to k_max!"
+        assert np.min(self.k_bins) >= b.Pfid[0][0], "Fiducial power spectrum should extend down to k_min!"
+        if k_bins_squeeze is not None:
+            self.k_bins_squeeze = np.asarray(k_bins_squeeze, dtype=np.float64)
+            self.Nk_squeeze = len(k_bins_squeeze) - 1
+            assert (self.k_bins_squeeze[:self.Nk + 1] == self.k_bins).all(), "k_bins_squeeze must contain all non-squeezed bins!"
+            assert np.max(self.k_bins_squeeze) < np.min(b.kNy), "k_max must be less than k_Nyquist!"
+            assert np.min(self.k_bins_squeeze) >= np.max(b.kF), "k_min must be at least the k_fundamental!"
+            assert np.max(self.k_bins_squeeze) <= b.Pfid[0][-1], "Fiducial power spectrum should extend up to k_max!"
+        else:
+            self.k_bins_squeeze = self.k_bins.copy()
+            self.Nk_squeeze = self.Nk
+        self.all_bins = triangle_bins(self.k_bins, self.k_bins_squeeze, lmax, include_partial_triangles)
+        self.N_bins = len(self.all_bins)
+        self.N3 = self.N_bins // self.Nl
+        if verb:
+            print("N_bins: %d" % self.N_bins)
+        assert type(self.lmax) == int, "l-max must be an integer!"
+        assert self.lmax <= 4, "Only l-max<=4 is currently supported!"
+        if add_GIC or add_RIC:
+            raise NotImplementedError("integral constraints are not implemented by backend='hip'")
+        if mask is None:
+            self.const_mask, self.mask_mean, self.cube_mask_mean = True, 1., 1.
+        else:
+            m = mask.detach().cpu().numpy() if torch.is_tensor(mask) else np.asarray(mask)
+            m = np.broadcast_to(np.asarray(m, dtype=np.float64), tuple(int(v) for v in b.gridsize))
+            if np.std(m) < 1e-12:
+                self.const_mask, self.mask_mean = True, float(np.mean(m))
+            else:
+                self.const_mask = False
+            self.cube_mask_mean = float(np.mean(m ** 3))
+            self.mask = b.as_real(np.ascontiguousarray(m))
+        if mask_shot is not None:
+            ms = mask_shot.detach().cpu().numpy() if torch.is_tensor(mask_shot) else np.asarray(mask_shot)
+            self.mask_shot = b.as_real(ms.astype(np.float64))
+        self.user_Sinv = applySinv is not None
+        if applySinv is None:
+            self.applySinv = b.applySinv_trivial
+        else:
+            self.applySinv = applySinv
+            if not callable(applySinv_transpose):
+                applySinv_transpose = applySinv
+        self.applySinv_transpose = b.applySinv_trivial if applySinv_transpose is None else applySinv_transpose
+        # fiducial weighting (bspec.py:138-156); Pk_fid=None means unit weights with invPk0(0)=0
+        if isinstance(Pk_fid, str) and Pk_fid == 'default':
+            inv = b.invPk0_grid_host
+        else:
+            if Pk_fid is None:
+                k_tmp = np.arange(np.min(b.kF) / 10., np.max(b.kNy) * 2, np.min(b.kF) / 10.)
+                Pfid = np.asarray([k_tmp, 1. + 0. * k_tmp])
+            else:
+                assert len(Pk_fid) == 2, "Pk should contain k and P_0(k) columns"
+                Pfid = np.asarray(Pk_fid)
+            P = scipy.interpolate.interp1d(Pfid[0], Pfid[1], bounds_error=False, fill_value=0.)(b.modk_grid)
+            inv = np.zeros(P.shape)
+            inv[P != 0] = 1. / P[P != 0]
+        self.invPk0_host = inv
+        self.invPk0 = torch.from_numpy(inv.astype(np.float32)).to(b.device)
+        self.invPk0_c = torch.complex(self.invPk0, torch.zeros_like(self.invPk0))
+        self.Ylm_real = harmonics_tags(lmax, False, 'real') if (b.sightline == 'local' and lmax > 0) else None
+        self.Ylm_fourier = harmonics_tags(lmax, False, 'fourier') if lmax > 0 else None
+        self.table = b.bin_table(self.k_bins_squeeze)
+        self._bins_dev = torch.arange(self.Nk_squeeze, dtype=torch.int32, device=b.device)
+
+    def get_ks(self):
+        """Central (k1, k2, k3) of every triangle bin (bspec.py:201-220)."""
+        kc = 0.5 * (self.k_bins[1:] + self.k_bins[:-1])
+        ksc = 0.5 * (self.k_bins_squeeze[1:] + self.k_bins_squeeze[:-1])
+        t = self.all_bins[:self.N3]
+        return np.asarray([kc[t[:, 0]], ksc[t[:, 1]], ksc[t[:, 2]]])
+
+    # ------------------------------------------------------------------ building blocks
+    def _real_stack(self, nmaps):
+        b = self.base
+        return torch.empty((nmaps,) + tuple(int(v) for v in b.gridsize), dtype=torch.float32, device=b.device)
+
+    def _filtered_maps(self, src, ell=0, ell2=0, out=None, w=None):
+        """out[q] = C2R(Theta_q L_ell L_ell2 w src) for every squeezed bin q, one batched launch."""
+        b = self.base
+        Nq = self.Nk_squeeze
+        if out is None:
+            out = self._real_stack(Nq)
+        b.c2r(src, out=out, batch=Nq, kop=C.kop(C.KOP_FILTER, bin_table=self.table, bins=self._bins_dev, ell=ell,
+                                                  ell2=ell2, w=w, x_batch_stride=0))
+        return out
+
+    def _g_maps(self, S, out=None):
+        """g_{l,b}(x) for all l, b (bspec.py:931-956) -> stack [Nl*Nk_squeeze, n0, n1, n2]."""
+        b = self.base
+        Nq = self.Nk_squeeze
+        if out is None:
+            out = self._real_stack(self.Nl * Nq)
+        self._filtered_maps(S, 0, out=out[0:Nq])
+        if self.lmax > 0:
+            if b.sightline == 'global':
+                for l in range(2, self.lmax + 1, 2):
+                    self._filtered_maps(S, l, out=out[(l // 2) * Nq:(l // 2 + 1) * Nq])
+            else:
+                S_real = b.c2r(S)
+                for l in range(2, self.lmax + 1, 2):
+                    leg = b.apply_fourier_harmonics(S_real, self.Ylm_real[l], self.Ylm_fourier[l])
+                    self._filtered_maps(leg, 0, out=out[(l // 2) * Nq:(l // 2 + 1) * Nq])
+        return out
+
+    def _triplet(self, stack, i1, i2, i3):
+        """FP64 numpy array of sum_x stack[i1] stack[i2] stack[i3]."""
+        b = self.base
+        nm = stack.shape[0]
+        nvox = int(np.prod(stack.shape[1:]))
+        t = len(i1)
+        idx = torch.from_numpy(np.stack([np.asarray(i1), np.asarray(i2), np.asarray(i3)]).astype(np.int32)).to(b.device)
+        out = torch.empty(t, dtype=torch.float64, device=b.device)
+        prec = 0
+        ws = C.lib().pb3d_triplet_work_size(nvox, t, prec)
+        work = b.work('triplet', ws)
+        C.check(C.lib().pb3d_triplet_sum(stack.data_ptr(), nm, nvox, idx[0].data_ptr(), idx[1].data_ptr(),
+                                         idx[2].data_ptr(), t, prec, out.data_ptr(), work.data_ptr(), b.stream()))
+        return out.cpu().numpy()
+
+    def _compute_symmetry_factor(self):
+        """bspec.py:222-279: l=0 combinatorial; l>0 from u_b = IFFT(Theta_b invPk0) and
+        sum_m IFFT(Theta_b Y_lm invPk0)^2 maps (batched C2R with a squaring epilogue)."""
+        b = self.base
+        Nq = self.Nk_squeeze
+        t = self.all_bins[:self.N3]
+        sym = [6. if (b1 == b2 == b3) else (2. if (b1 == b2 or b1 == b3 or b2 == b3) else 1.) for (b1, b2, b3, _) in t]
+        if self.lmax > 0:
+            stack = self._real_stack(2 * Nq + 1)
+            self._filtered_maps(self.invPk0_c, 0, out=stack[0:Nq])
+            stack[2 * Nq].fill_(1.)
+            for l in range(2, self.lmax + 1, 2):
+                b23 = stack[Nq:2 * Nq]
+                for m in range(2 * l + 1):
+                    b.c2r(self.invPk0_c, out=b23, batch=Nq,
+                          kop=C.kop(C.KOP_YLM, bin_table=self.table, bins=self._bins_dev, ell=l, m=m, x_batch_stride=0),
+                          rop=C.rop(C.ROP_SQUARE, accumulate=m > 0))
+                one = 2 * Nq
+                i1, i2, i3 = [], [], []
+                for (b1, b2, b3, _) in t:
+                    q = b1 if (b1 != b2 and b2 == b3) else b3
+                    i1 += [q, q]
+                    i2 += [Nq + b3, b3]
+                    i3 += [one, b3]
+                r = self._triplet(stack, i1, i2, i3).reshape(-1, 2)
+                for i, (b1, b2, b3, _) in enumerate(t):
+                    if b1 != b2 and b2 != b3:
+                        sym.append(1.)
+                    elif b1 == b2 and b2 != b3:
+                        sym.append(2.)
+                    elif b1 != b2 and b2 == b3:
+                        sym.append(1. + r[i, 0] / r[i, 1])
+                    else:
+                        sym.append(2. * (1. + 2. * r[i, 0] / r[i, 1]))
+            del stack
+        self.sym_factor = np.asarray(sym)
+        assert len(self.sym_factor) == self.N_bins
+
+    # ------------------------------------------------------------------ numerators
+    def _filtered_data(self, data, filtering):
+        b = self.base
+        if filtering == 'ideal':
+            w = self.invPk0
+            if b.pixel_window != 'none':
+                key = ('bk_ideal_pw', id(self))
+                if key not in b._bufs:
+                    pw = b.pixel_window_grid_host
+                    ww = np.zeros(pw.shape)
+                    ww[pw != 0] = self.invPk0_host[pw != 0] / pw[pw != 0]
+                    b._bufs[key] = torch.from_numpy(ww.astype(np.float32)).to(b.device)
+                w = b._bufs[key]
+            return b.r2c(b.as_real(data), kop=C.kop(C.KOP_WEIGHT, w=w, scale=1. / self.cube_mask_mean ** (1. / 3.)))
+        if b.pixel_window != 'none':
+            pw = b.pixel_window_grid_host
+            ww = np.zeros(pw.shape)
+            ww[pw != 0] = 1. / pw[pw != 0]
+            df = b.r2c(b.as_real(data), kop=C.kop(C.KOP_WEIGHT, w=torch.from_numpy(ww.astype(np.float32)).to(b.device)))
+            return b.as_fourier(self.applySinv(df, input_type='fourier', output_type='fourier'))
+        return b.as_fourier(self.applySinv(b.as_real(data), input_type='real', output_type='fourier'))
+
+    def _b3_indices(self):
+        Nq = self.Nk_squeeze
+        ab = self.all_bins
+        return ab[:, 0], ab[:, 1], (ab[:, 3] // 2) * Nq + ab[:, 2]
+
+    def _compute_bk_numerator(self, data, include_linear_term=True, filtering='ideal', verb=False):
+        """bspec.py:895-983."""
+        assert filtering in ['ideal', 'Sinv'], "Unknown filtering option supplied!"
+        b = self.base
+        if (not hasattr(self, 'preload')) and include_linear_term:
+            raise Exception("Need to generate or specify bias simulations!")
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        S = self._filtered_data(data, filtering)
+        g = self._g_maps(S)
+        i1, i2, i3 = self._b3_indices()
+        B3 = self._triplet(g, i1, i2, i3)
+        B1 = np.zeros(self.N_bins)
+        if include_linear_term:
+            NG = g.shape[0]
+            for ii in range(self.N_it):
+                sim = self.g_bl_maps[ii] if self.preload else self.load_sim_data(ii)
+                st = torch.cat([g, sim], 0)
+                # g[0][b1] s[0][b2] s[l][b3] + s[0][b1] g[0][b2] s[l][b3] + s[0][b1] s[0][b2] g[l][b3]
+                r = (self._triplet(st, i1, NG + i2, NG + i3) + self._triplet(st, NG + i1, i2, NG + i3)
+                     + self._triplet(st, NG + i1, NG + i2, i3))
+                B1 += -1. / self.N_it * r
+                del st
+        norm = b.volume / b.gridsize.prod() / self.sym_factor
+        return (B3 + B1) * norm
+
+    def Bk_numerator_ideal(self, data, verb=False):
+        """bspec.py:986-997."""
+        return self._compute_bk_numerator(data, include_linear_term=False, filtering='ideal', verb=verb)
+
+    def Bk_numerator(self, data, include_linear_term=True, verb=False):
+        """bspec.py:429-439."""
+        return self._compute_bk_numerator(data, include_linear_term=include_linear_term, filtering='Sinv', verb=verb)
+
+    # ------------------------------------------------------------------ ideal Fisher
+    def compute_fisher_ideal(self, discreteness_correction=True, verb=False):
+        """bspec.py:999-1083."""
+        b = self.base
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        Nq, N3, Nl = self.Nk_squeeze, self.N3, self.Nl
+        t = self.all_bins[:N3]
+        if discreteness_correction:
+            pairs = [(l, lp) for l in range(0, self.lmax + 1, 2) for lp in range(l, self.lmax + 1, 2)]
+            stack = self._real_stack((Nl + len(pairs)) * Nq)
+            for l in range(0, self.lmax + 1, 2):
+                self._filtered_maps(self.invPk0_c, l, out=stack[(l // 2) * Nq:(l // 2 + 1) * Nq])
+            fish = np.zeros((self.N_bins, self.N_bins))
+            for p, (l, lp) in enumerate(pairs):
+                o = (Nl + p) * Nq
+                self._filtered_maps(self.invPk0_c, l, lp, out=stack[o:o + Nq])
+                # case table of bspec.py:1038-1047: A = (u0[b1], u0[b2], u_llp[b3]),
+                # B = (u0[b1], u_l[b2], u_lp[b3])
+                i1 = list(t[:, 0]) * 2
+                i2 = list(t[:, 1]) + list((l // 2) * Nq + t[:, 1])
+                i3 = list(o + t[:, 2]) + list((lp // 2) * Nq + t[:, 2])
+                r = self._triplet(stack, i1, i2, i3)
+                A, B = r[:N3], r[N3:]
+                for i, (b1, b2, b3, _) in enumerate(t):
+                    if b1 != b2 and b2 != b3:
+                        v = A[i]
+                    elif b1 == b2 and b2 != b3:
+                        v = 2. * A[i]
+                    elif b1 != b2 and b2 == b3:
+                        v = B[i] + A[i]
+                    else:
+                        v = 4. * B[i] + 2. * A[i]
+                    fish[l // 2 * N3 + i, lp // 2 * N3 + i] = v
+            del stack
+            fish = fish + fish.T - np.diag(np.diag(fish))
+            fish = fish / np.outer(self.sym_factor, self.sym_factor) * b.gridsize.prod() ** 2 / b.volume
+        else:
+            stack = self._filtered_maps(self.invPk0_c, 0)
+            Ntri = self._triplet(stack, t[:, 0], t[:, 1], t[:, 2])
+            del stack
+            d = np.concatenate([Ntri / (2. * l + 1.) for l in range(0, self.lmax + 1, 2)])
+            fish = np.diag(d / self.sym_factor) * b.gridsize.prod() ** 2 / b.volume
+        self.fish_ideal = fish
+        self.inv_fish_ideal = np.linalg.inv(fish)
+        self.discreteness_correction = discreteness_correction
+        return fish
+
+    def Bk_ideal(self, data, fish_ideal=[], discreteness_correction=True, verb=False):
+        """bspec.py:1085-1119."""
+        if len(fish_ideal) != 0:
+            self.fish_ideal = np.asarray(fish_ideal)
+            self.inv_fish_ideal = np.linalg.inv(self.fish_ideal)
+        if not hasattr(self, 'inv_fish_ideal'):
+            self.compute_fisher_ideal(discreteness_correction=discreteness_correction, verb=verb)
+        elif getattr(self, 'discreteness_correction', discreteness_correction) != discreteness_correction:
+            self.compute_fisher_ideal(discreteness_correction=discreteness_correction, verb=verb)
+        out = np.matmul(self.inv_fish_ideal, self.Bk_numerator_ideal(data, verb=verb))
+        res, idx = {}, 0
+        for l in range(0, self.lmax + 1, 2):
+            res['b%s' % l] = out[idx:idx + self.N3]
+            idx += self.N3
+        return res
+
+    # ------------------------------------------------------------------ linear-term simulations
+    def apply_pointing(self, map_real, transpose=False):
+        """Multiply by the mask (bspec.py:412-426, no integral constraints)."""
+        return self.base.as_real(map_real) * self.mask
+
+    def _process_sim(self, sim, input_type='real'):
+        """g_{b,l} maps of one simulation (bspec.py:281-326)."""
+        b = self.base
+        if b.pixel_window != 'none':
+            sf = b.to_fourier(sim) if input_type == 'real' else b.as_fourier(sim)
+            pw = b.pixel_window_grid
+            inv = torch.where(pw != 0, 1. / pw, torch.zeros_like(pw))
+            sf = b.kmap(sf, C.kop(C.KOP_WEIGHT, w=inv.contiguous()))
+            S = b.as_fourier(self.applySinv(sf, input_type='fourier', output_type='fourier'))
+        else:
+            S = b.as_fourier(self.applySinv(sim, input_type=input_type, output_type='fourier'))
+        return self._g_maps(S)
+
+    def load_sims(self, load_sim, N_sims, verb=False, input_type='real', preload=True):
+        """bspec.py:328-360."""
+        self.N_it = N_sims
+        if preload:
+            self.preload = True
+            self.g_bl_maps = [self._process_sim(load_sim(ii), input_type=input_type) for ii in range(N_sims)]
+        else:
+            self.preload = False
+            self.load_sim_data = lambda ii: self._process_sim(load_sim(ii), input_type=input_type)
+
+    def generate_sims(self, N_sim, Pk_input=[], verb=False, preload=True):
+        """bspec.py:362-410: GRFs with seeds 1e5+ii, pointing matrix and pixel window applied."""
+        b = self.base
+        self.N_it = N_sim
+
+        def load_sim_data(ii):
+            if self.const_mask:
+                sf = b.generate_data(int(1e5) + ii, Pk_input=Pk_input, output_type='fourier')
+                sf = b.kmap(sf, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
+                return self._process_sim(sf, input_type='fourier')
+            sr = b.generate_data(int(1e5) + ii, Pk_input=Pk_input, output_type='real', include_pixel_window=False)
+            if b.pixel_window != 'none':
+                sf = b.r2c(None, rop=C.rop(C.ROP_MUL, a=sr, b=self.mask),
+                           kop=C.kop(C.KOP_WEIGHT, w=b.pixel_window_grid))
+                return self._process_sim(sf, input_type='fourier')
+            return self._process_sim(self.apply_pointing(sr), input_type='real')
+
+        if preload:
+            self.preload = True
+            self.g_bl_maps = [load_sim_data(ii) for ii in range(N_sim)]
+        else:
+            self.preload = False
+            self.load_sim_data = load_sim_data
+
+    # ------------------------------------------------------------------ MC Fisher
+    def _packed_layout(self):
+        """In-bin modes of the squeezed bins, bin-major (segments of each bin contiguous):
+        modes [M] (linear half-grid index), seg [Nq+1], pack_index [H] (-1 outside), degk [M]."""
+        if getattr(self, '_pk', None) is None:
+            b = self.base
+            idx = b.bin_index_host(self.k_bins_squeeze).reshape(-1)
+            lin = np.nonzero(idx >= 0)[0]
+            order = np.argsort(idx[lin], kind='stable')
+            modes = lin[order]
+            seg = np.searchsorted(idx[modes], np.arange(self.Nk_squeeze + 1))
+            pack_index = np.full(b.H, -1, dtype=np.int32)
+            pack_index[modes] = np.arange(len(modes), dtype=np.int32)
+            degk = np.where(modes % b.half_shape[2] > 0, 2., 1.)
+            dev = b.device
+            self._pk = dict(M=len(modes), seg=seg,
+                            modes=torch.from_numpy(modes.astype(np.int32)).to(dev),
+                            pack_index=torch.from_numpy(pack_index).to(dev),
+                            degk=torch.from_numpy(degk.astype(np.float32)).to(dev))
+        return self._pk
+
+    def _pair_rows(self, pairs):
+        """Row of each (A, B) pair in an FT table and the device [Nq*Nq] lookup (-1 = absent)."""
+        Nq = self.Nk_squeeze
+        rows = {p: r for r, p in enumerate(pairs)}
+        tab = np.full(Nq * Nq, -1, dtype=np.int32)
+        for (A, B), r in rows.items():
+            tab[A * Nq + B] = r
+        return rows, torch.from_numpy(tab).to(self.base.device)
+
+    def _ft_tables(self, G1, G2, pairs, out):
+        """out[l/2][row(A,B)] = packed R2C(g1_{0,A} g1_{l,B} - g2_{0,A} g2_{l,B}) for every pair
+        (the ft_ABl maps of bspec.py:591-613); batched over runs of consecutive B at fixed A."""
+        b = self.base
+        pk = self._packed_layout()
+        Nq, N = self.Nk_squeeze, b.N
+        runs = []
+        for r, (A, B) in enumerate(pairs):
+            if runs and runs[-1][0] == A and runs[-1][1] + runs[-1][2] == B and runs[-1][3] + runs[-1][2] == r:
+                runs[-1][2] += 1
+            else:
+                runs.append([A, B, 1, r])
+        for li in range(self.Nl):
+            for A, B0, nb, r0 in runs:
+                gb1 = G1[li * Nq + B0:li * Nq + B0 + nb]
+                gb2 = G2[li * Nq + B0:li * Nq + B0 + nb]
+                b.r2c(None, out=out[li, r0:r0 + nb], batch=nb,
+                      rop=C.rop(C.ROP_DIFF, a=G1[A], sa=0, b=gb1, sb=N, c=G2[A], sc=0, d=gb2, sd=N),
+                      kop=C.kop(C.KOP_PACK, pack_index=pk['pack_index'], packed_len=pk['M']))
+
+    def _q_ainv(self, FT, pair_tab, QA):
+        """QA[i] = packed S^-1 P applied to the Q[A^-1 a] map of triplet i (bspec.py:615-666)."""
+        b = self.base
+        pk = self._packed_layout()
+        M, Nq = pk['M'], self.Nk_squeeze
+        ab = self.all_bins
+        local = b.sightline == 'local'
+        trip = self._trip_dev
+        per_map = 4 * b.N + 16 * b.H
+        CH = int(max(1, min(64, (4 << 30) // per_map)))
+        common = dict(bin_table=self.table, pack_index=pk['pack_index'], packed_len=M, trip=trip, pair_index=pair_tab,
+                      ft=FT, ft_lstride=FT.shape[1] * M, nk=Nq)
+        t0 = 0
+        while t0 < self.N_bins:
+            ell = int(ab[t0, 3])
+            t1 = t0
+            while t1 < self.N_bins and t1 - t0 < CH and ab[t1, 3] == ell:
+                t1 += 1
+            nb = t1 - t0
+            ylm = local and ell > 0
+            terms = 2 if ylm else 3
+            if self.const_mask:
+                b_ = QA[t0:t1]
+                C.check(C.lib().pb3d_kmap_packed(C.ctypes.byref(b.cgrid), None, pk['modes'].data_ptr(), M, b_.data_ptr(),
+                                                 nb, C.ctypes.byref(C.kop(C.KOP_BKQ, terms=terms, trip0=t0,
+                                                                          leg3_legendre=int(not local),
+                                                                          scale=self.mask_mean, **common)),
+                                                 b.stream()))
+                if ylm:
+                    R = self._ylm_leg3(common, t0, nb, ell)
+                    b.r2c(R, out=QA[t0:t1], batch=nb,
+                          kop=C.kop(C.KOP_PACK, pack_index=pk['pack_index'], packed_len=M, accumulate=1,
+                                    scale=self.mask_mean))
+            else:
+                R = self._real_stack(nb)
+                b.c2r(None, out=R, batch=nb, kop=C.kop(C.KOP_BKQ, terms=terms, trip0=t0,
+                                                       leg3_legendre=int(not local), **common))
+                if ylm:
+                    self._ylm_leg3(common, t0, nb, ell, out=R)
+                b.r2c(None, out=QA[t0:t1], batch=nb, rop=C.rop(C.ROP_MUL, a=R, sa=b.N, b=self.mask, sb=0),
+                      kop=C.kop(C.KOP_PACK, pack_index=pk['pack_index'], packed_len=M))
+            t0 = t1
+
+    def _ylm_leg3(self, common, t0, nb, ell, out=None):
+        """sum_m Y_lm(x) IFFT[Theta_b3 Y_lm(k) FT_0(b1,b2)] for a chunk of local l>0 triplets: the
+        external-leg Legendre factor in real space (apply_real_harmonics, bspec.py:629-636)."""
+        b = self.base
+        R = self._real_stack(nb) if out is None else out
+        acc_first = out is not None
+        for m in range(2 * ell + 1):
+            b.c2r(None, out=R, batch=nb, kop=C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common),
+                  rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=int(acc_first or m > 0)))
+        return R
+
+    def compute_fisher_contribution(self, seed, verb=False):
+        """One MC realization (bspec.py:441-445, 473-676): FP64 numpy [N_bins, N_bins], rows indexed
+        by the S^-1-weighted triplet, not symmetrised (as the reference)."""
+        return self._fisher_contribution_dev(seed).cpu().numpy()
+
+    def _fisher_contribution_dev(self, seed):
+        b = self.base
+        if self.user_Sinv:
+            raise NotImplementedError("compute_fisher with a user S^-1 is not implemented by backend='hip'")
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        pk = self._packed_layout()
+        M, Nq, NB = pk['M'], self.Nk_squeeze, self.N_bins
+        ab = self.all_bins
+        if not hasattr(self, '_trip_dev'):
+            self._trip_dev = torch.from_numpy(ab.astype(np.int32).reshape(-1)).to(b.device)
+            ainv = set()
+            for b1, b2, b3, _ in ab:
+                ainv |= {(b1, b2), (b2, b3), (b1, b3)}
+            self._ainv_pairs = sorted((int(x), int(y)) for x, y in ainv)
+            self._sinv_pairs = sorted(set((int(b1), int(b3)) for b1, _, b3, _ in ab))
+            self._ainv_rows, self._ainv_tab = self._pair_rows(self._ainv_pairs)
+            self._sinv_rows, _ = self._pair_rows(self._sinv_pairs)
+        if getattr(self, '_QA', None) is None:
+            self._QA = torch.empty((NB, M), dtype=torch.complex64, device=b.device)
+        QA = self._QA
+        # GRF pair (bspec.py:648-656)
+        a1 = b.generate_data(seed=seed + int(1e7), output_type='fourier')
+        a2 = b.generate_data(seed=seed + int(2e7), output_type='fourier')
+        # A^-1 side: weighted maps, g maps, FT tables, Q maps (bspec.py:659-666)
+        w = b.ainv_weight()
+        w1 = b.kmap(a1, C.kop(C.KOP_WEIGHT, w=w)) if w is not None else a1
+        w2 = b.kmap(a2, C.kop(C.KOP_WEIGHT, w=w)) if w is not None else a2
+        G1, G2 = self._g_maps(w1), self._g_maps(w2)
+        FT = torch.empty((self.Nl, len(self._ainv_pairs), M), dtype=torch.complex64, device=b.device)
+        self._ft_tables(G1, G2, self._ainv_pairs, FT)
+        del G1, G2
+        self._q_ainv(FT, self._ainv_tab, QA)
+        del FT
+        # S^-1 P side (bspec.py:671, compute_Q('Sinv'))
+        if self.const_mask:
+            v1 = b.kmap(a1, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
+            v2 = b.kmap(a2, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
+        else:
+            v1 = b.r2c(None, rop=C.rop(C.ROP_MUL, a=b.c2r(a1), b=self.mask))
+            v2 = b.r2c(None, rop=C.rop(C.ROP_MUL, a=b.c2r(a2), b=self.mask))
+        H1, H2 = self._g_maps(v1), self._g_maps(v2)
+        LT = torch.empty((self.Nl, len(self._sinv_pairs), M), dtype=torch.complex64, device=b.device)
+        self._ft_tables(H1, H2, self._sinv_pairs, LT)
+        del H1, H2
+        # contraction: F[i, j] = sum_{k in b2(i)} degk Re[conj(L_i) QA_j]  (integrate_row, bspec.py:597)
+        F = torch.zeros((NB, NB), dtype=torch.float64, device=b.device)
+        QAr = torch.view_as_real(QA)
+        nsp = len(self._sinv_pairs)
+        for q in range(Nq):
+            rows = np.nonzero(ab[:, 1] == q)[0]
+            s0, s1 = int(pk['seg'][q]), int(pk['seg'][q + 1])
+            if len(rows) == 0 or s1 == s0:
+                continue
+            src = [(ab[i, 3] // 2) * nsp + self._sinv_rows[(int(ab[i, 0]), int(ab[i, 2]))] for i in rows]
+            src_d = torch.from_numpy(np.asarray(src, dtype=np.int32)).to(b.device)
+            A = torch.empty((len(rows), s1 - s0), dtype=torch.complex64, device=b.device)
+            C.check(C.lib().pb3d_gather_rows(LT.data_ptr(), M, self.Nl * nsp, src_d.data_ptr(), len(rows), s0, s1 - s0,
+                                             pk['degk'].data_ptr(), A.data_ptr(), b.stream()))
+            Ar = torch.view_as_real(A).reshape(len(rows), 2 * (s1 - s0))
+            Br = QAr[:, s0:s1].reshape(NB, 2 * (s1 - s0))
+            F[torch.from_numpy(rows).to(b.device)] = torch.matmul(Ar, Br.t()).double()
+        sym = torch.from_numpy(self.sym_factor).to(b.device)
+        F *= 0.5 * b.volume / b.gridsize.prod() ** 2
+        F /= sym[:, None] * sym[None, :]
+        return F
+
+    def compute_fisher(self, N_it, verb=False):
+        """Mean over N_it realizations, FP64 on the GPU, then inverted (bspec.py:678-693)."""
+        fish = None
+        for seed in range(N_it):
+            if verb and seed % 5 == 0:
+                print("Computing Fisher contribution %d of %d" % (seed + 1, N_it))
+            f = self._fisher_contribution_dev(seed)
+            fish = f / N_it if fish is None else fish + f / N_it
+        self.fish = fish.cpu().numpy()
+        self.inv_fish = torch.linalg.inv(fish).cpu().numpy()
+        return self.fish
+
+    # ------------------------------------------------------------------ unwindowed
+    def Bk_unwindowed(self, data, fish=[], include_linear_term=True, verb=False):
+        """bspec.py:863-893."""
+        if len(fish) != 0:
+            self.fish = np.asarray(fish)
+            self.inv_fish = np.linalg.inv(self.fish)
+        if not hasattr(self, 'inv_fish'):
+            raise Exception("Need to compute Fisher matrix first!")
+        out = np.matmul(self.inv_fish, self.Bk_numerator(data, include_linear_term=include_linear_term, verb=verb))
+        res, idx = {}, 0
+        for l in range(0, self.lmax + 1, 2):
+            res['b%s' % l] = out[idx:idx + self.N3]
+            idx += self.N3
+        return res

@@ -175,6 +175,8 @@ fft_cols(const float2* in, float2* out, int ncols, int cpp, long long plane,
 
   auto load = [&](int p) -> float2 {
     if (!valid) return make_float2(0.f, 0.f);
+    if (MODE == 1 && kop.kind == 4)  // BKQ assembles the map from packed rows; no input read
+      return kop_value(kop, geo, bt, base + p * S, p, mj, ml, make_float2(0.f, 0.f));
     float2 v = src[base + p * S];
     if (MODE == 1) v = kop_value(kop, geo, bt, base + p * S, p, mj, ml, v);
     return v;
@@ -183,6 +185,14 @@ fft_cols(const float2* in, float2* out, int ncols, int cpp, long long plane,
     if (!valid) return;
     if (MODE == 2) {
       long long md = base + p * S;
+      if (kop.kind == 5) {  // packed output (in-bin modes only)
+        const int pi = kop.pack_index[md];
+        if (pi < 0) return;
+        v = cscale(v, kop.scale);
+        if (kop.accumulate) v = cadd(v, dst[pi]);
+        dst[pi] = v;
+        return;
+      }
       v = kop_value(kop, geo, bt, md, p, mj, ml, v);
       v = cscale(v, kop.scale);
       if (kop.accumulate) v = cadd(v, dst[md]);
@@ -343,6 +353,41 @@ c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows
 // ------------------------------------------------------------------------------------------
 static std::atomic<long long> g_fwd{0}, g_rev{0};
 
+// ---- sampled launch timing (roofline reporting): every `period`-th launch of a pass kind is
+// bracketed by hipEvents on its own stream; read() synchronises and sums the durations.
+constexpr int kProfKinds = 5;
+struct ProfSample { int kind; hipEvent_t a, b; double bytes; };
+static int g_prof_period = 0;
+static long long g_prof_count[kProfKinds] = {0, 0, 0, 0, 0};
+static std::vector<ProfSample> g_prof_pending;
+static std::vector<hipEvent_t> g_prof_free;
+static long long g_prof_n[kProfKinds] = {0, 0, 0, 0, 0};
+static double g_prof_ms[kProfKinds] = {0, 0, 0, 0, 0};
+static double g_prof_bytes[kProfKinds] = {0, 0, 0, 0, 0};
+
+static hipEvent_t prof_event() {
+  if (!g_prof_free.empty()) { hipEvent_t e = g_prof_free.back(); g_prof_free.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// returns true and records the start event when this launch is sampled
+static bool prof_begin(int kind, hipStream_t s, ProfSample& smp, double bytes) {
+  if (g_prof_period <= 0) return false;
+  if ((g_prof_count[kind]++ % g_prof_period) != 0) return false;
+  if (g_prof_pending.size() > 4096) return false;
+  smp.kind = kind; smp.bytes = bytes; smp.a = prof_event(); smp.b = prof_event();
+  if (!smp.a || !smp.b) return false;
+  (void)hipEventRecord(smp.a, s);
+  return true;
+}
+static void prof_end(bool on, ProfSample& smp, hipStream_t s) {
+  if (!on) return;
+  (void)hipEventRecord(smp.b, s);
+  g_prof_pending.push_back(smp);
+}
+
 static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
 
 }  // namespace pb3d
@@ -442,7 +487,13 @@ int check_rop(const pb3d_rop* r, bool input_side) {
 
 int check_kop(const pb3d_kop* k) {
   if (!k) return 0;
-  if (k->kind < 0 || k->kind > 3) return fail(PB3D_ERR_ARG, "kop.kind out of range");
+  if (k->kind < 0 || k->kind > 5) return fail(PB3D_ERR_ARG, "kop.kind out of range");
+  if (k->kind == 4 && (!k->bin_table || !k->pack_index || !k->trip || !k->pair_index || !k->ft ||
+                       k->packed_len < 1 || k->nk < 1 || k->nk > 254 || k->trip0 < 0 ||
+                       ((k->terms & 4) && (k->ell < 0 || k->ell > 4 || k->m < 0 || k->m > 2 * k->ell))))
+    return fail(PB3D_ERR_ARG, "kop BKQ: missing tables or bad fields");
+  if (k->kind == 5 && (!k->pack_index || k->packed_len < 1))
+    return fail(PB3D_ERR_ARG, "kop PACK needs pack_index and packed_len");
   if (k->kind == 1 && (!k->bin_table || k->ell < 0 || k->ell > 6 || k->ell2 < 0 || k->ell2 > 6))
     return fail(PB3D_ERR_ARG, "kop FILTER needs bin_table and 0<=ell,ell2<=6");
   if (k->kind == 2 && (k->ell < 0 || k->ell > 4 || k->m < 0 || k->m > 2 * k->ell))
@@ -458,6 +509,28 @@ using namespace pb3d;
 extern "C" int64_t pb3d_fft_counter(int dir) {
   if (dir < 0) { g_fwd = 0; g_rev = 0; return 0; }
   return dir == 0 ? (int64_t)g_fwd.load() : (int64_t)g_rev.load();
+}
+
+extern "C" int pb3d_prof_enable(int period) {
+  if (period < 0) return fail(PB3D_ERR_ARG, "prof_enable: period must be >= 0");
+  g_prof_period = period;
+  for (int k = 0; k < kProfKinds; ++k) { g_prof_count[k] = 0; g_prof_n[k] = 0; g_prof_ms[k] = 0; g_prof_bytes[k] = 0; }
+  return 0;
+}
+
+extern "C" int pb3d_prof_read(int kind, int64_t* sampled, double* ms, double* bytes) {
+  if (kind < 0 || kind >= kProfKinds || !sampled || !ms || !bytes) return fail(PB3D_ERR_ARG, "prof_read: bad args");
+  for (auto& smp : g_prof_pending) {
+    if (hipEventSynchronize(smp.b) != hipSuccess) return fail(PB3D_ERR_HIP, "prof_read: event sync failed");
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, smp.a, smp.b) == hipSuccess) {
+      g_prof_n[smp.kind] += 1; g_prof_ms[smp.kind] += t; g_prof_bytes[smp.kind] += smp.bytes;
+    }
+    g_prof_free.push_back(smp.a); g_prof_free.push_back(smp.b);
+  }
+  g_prof_pending.clear();
+  *sampled = g_prof_n[kind]; *ms = g_prof_ms[kind]; *bytes = g_prof_bytes[kind];
+  return 0;
 }
 
 extern "C" int pb3d_plan_create(pb3d_plan** plan, const pb3d_grid* grid) {
@@ -498,6 +571,8 @@ extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, voi
   if (int e = check_rop(rop_, true)) return e;
   if (int e = check_kop(kop_)) return e;
   if (kop_ && kop_->accumulate && !work_) return fail(PB3D_ERR_ARG, "r2c: accumulating output needs work");
+  if (kop_ && kop_->kind == 5 && !work_) return fail(PB3D_ERR_ARG, "r2c: packed output needs work");
+  if (kop_ && kop_->kind == 4) return fail(PB3D_ERR_ARG, "r2c: BKQ is a C2R prologue");
   hipStream_t s = (hipStream_t)stream;
   const Geo& g = p->geo;
   float2* out = (float2*)out_;
@@ -506,21 +581,32 @@ extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, voi
   KOp kop = to_kop(kop_);
   KOp plain = to_kop(nullptr);
   const long long rows = (long long)batch * g.n0 * g.n1;
+  const double bc = (double)batch * g.H * 16.0, br = (double)batch * (g.N * 4.0 + g.H * 8.0);
+  ProfSample sm{};
+  bool on = prof_begin(0, s, sm, br);
   PB3D_CHECK_HIP(launch_r2c_rows(p->L[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, s));
+  prof_end(on, sm, s);
   // y columns: cols = n0*n2h, cpp = n2h, plane = n1*n2h, stride n2h (in place in work)
+  on = prof_begin(2, s, sm, bc);
   PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, g.n0 * g.n2h, g.n2h, (long long)g.n1 * g.n2h,
                                      g.n2h, g.H, g.H, batch, p->tw[1], g, plain, s)));
-  // x columns: cols = n1*n2h, stride n1*n2h, epilogue kop into out
+  prof_end(on, sm, s);
+  // x columns: cols = n1*n2h, stride n1*n2h, epilogue kop into out (packed rows for PACK)
+  const long long out_bs = (kop_ && kop_->kind == 5) ? kop_->packed_len : g.H;
+  on = prof_begin(4, s, sm, bc);
   PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, out, g.n1 * g.n2h, g.n1 * g.n2h, 0,
-                                     (long long)g.n1 * g.n2h, g.H, g.H, batch, p->tw[0], g, kop, s)));
+                                     (long long)g.n1 * g.n2h, g.H, out_bs, batch, p->tw[0], g, kop, s)));
+  prof_end(on, sm, s);
   g_fwd += batch;
   return 0;
 }
 
 extern "C" int pb3d_fft_c2r(const pb3d_plan* p, const void* in_, float* out, void* work_, int batch,
                             const pb3d_kop* kop_, const pb3d_rop* rop_, void* stream) {
-  if (!p || !in_ || !out || !work_ || batch < 1) return fail(PB3D_ERR_ARG, "c2r: null pointer or batch < 1");
+  if (!p || !out || !work_ || batch < 1) return fail(PB3D_ERR_ARG, "c2r: null pointer or batch < 1");
+  if (!in_ && !(kop_ && kop_->kind == 4)) return fail(PB3D_ERR_ARG, "c2r: null input");
   if (in_ == work_) return fail(PB3D_ERR_ARG, "c2r: work must not alias the input");
+  if (kop_ && kop_->kind == 5) return fail(PB3D_ERR_ARG, "c2r: PACK is an R2C epilogue");
   if (int e = check_rop(rop_, false)) return e;
   if (int e = check_kop(kop_)) return e;
   hipStream_t s = (hipStream_t)stream;
@@ -531,13 +617,21 @@ extern "C" int pb3d_fft_c2r(const pb3d_plan* p, const void* in_, float* out, voi
   KOp kop = to_kop(kop_);
   KOp plain = to_kop(nullptr);
   const long long in_bs = kop_ ? kop_->x_batch_stride : g.H;
+  const double bc = (double)batch * g.H * 16.0, br = (double)batch * (g.N * 4.0 + g.H * 8.0);
+  ProfSample sm{};
+  bool on = prof_begin(3, s, sm, bc);
   PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], in, work, g.n1 * g.n2h, g.n1 * g.n2h, 0,
                                      (long long)g.n1 * g.n2h, in_bs, g.H, batch, p->tw[0], g, kop, s)));
+  prof_end(on, sm, s);
+  on = prof_begin(2, s, sm, bc);
   PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, g.n0 * g.n2h, g.n2h, (long long)g.n1 * g.n2h,
                                      g.n2h, g.H, g.H, batch, p->tw[1], g, plain, s)));
+  prof_end(on, sm, s);
   const long long rows = (long long)batch * g.n0 * g.n1;
+  on = prof_begin(1, s, sm, br);
   PB3D_CHECK_HIP(launch_c2r_rows(p->L[2], work, out, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop,
                                  (float)(1.0 / (double)g.N), s));
+  prof_end(on, sm, s);
   g_rev += batch;
   return 0;
 }

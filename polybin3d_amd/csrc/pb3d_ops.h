@@ -46,6 +46,13 @@ struct KOp {
   const float* w;
   long long x_stride;
   float scale;
+  const int32_t* pack_index;
+  long long packed_len;
+  const int32_t* trip;
+  const int32_t* pair_index;
+  const float2* ft;
+  long long ft_lstride;
+  int nk, terms, trip0, leg3_legendre;
 };
 
 inline ROp to_rop(const pb3d_rop* r) {
@@ -63,6 +70,9 @@ inline KOp to_kop(const pb3d_kop* k) {
   o.kind = k->kind; o.ell = k->ell; o.ell2 = k->ell2; o.m = k->m; o.accumulate = k->accumulate;
   o.bin_table = k->bin_table; o.bins = k->bins; o.w = k->w; o.x_stride = k->x_batch_stride;
   o.scale = k->scale;
+  o.pack_index = k->pack_index; o.packed_len = k->packed_len; o.trip = k->trip;
+  o.pair_index = k->pair_index; o.ft = (const float2*)k->ft; o.ft_lstride = k->ft_lstride;
+  o.nk = k->nk; o.terms = k->terms; o.trip0 = k->trip0; o.leg3_legendre = k->leg3_legendre;
   return o;
 }
 
@@ -139,6 +149,42 @@ __device__ __forceinline__ float2 kop_value(const KOp& op, const Geo& g, long lo
       return cscale(v, f);
     }
     case 3: return cscale(v, op.w[mode]);
+    case 4: {
+      // Q map of one triplet from packed FT[g_A g_B] rows (bspec.py:615-640)
+      const int bin = op.bin_table[mode];
+      const int p = op.pack_index[mode];
+      if (bin == PB3D_NO_BIN || p < 0) return make_float2(0.f, 0.f);
+      const int* tr = op.trip + 4 * (long long)(op.trip0 + bt);
+      const int b1 = tr[0], b2 = tr[1], b3 = tr[2], ell = tr[3];
+      float2 acc = make_float2(0.f, 0.f);
+      if ((op.terms & 5) && bin == b3) {
+        const int pr = op.pair_index[b1 * op.nk + b2];
+        if (pr >= 0) {
+          float2 f = op.ft[(long long)pr * op.packed_len + p];
+          float kx, ky, kz; kvec(g, i, j, l, kx, ky, kz);
+          const float k2 = kx * kx + ky * ky + kz * kz;
+          const float inv = k2 > 0.f ? rsqrtf(k2) : 0.f;
+          if (op.terms & 1) {
+            float fl = 1.f;
+            if (op.leg3_legendre && ell != 0) fl = legendre_f(ell, (g.s0 * kx + g.s1 * ky + g.s2 * kz) * inv);
+            acc = cadd(acc, cscale(f, fl));
+          }
+          if (op.terms & 4) acc = cadd(acc, cscale(f, ylm_f(op.ell, op.m, kx * inv, ky * inv, kz * inv)));
+        }
+      }
+      if (op.terms & 2) {
+        const float2* fl = op.ft + (long long)(ell / 2) * op.ft_lstride;
+        if (bin == b1) {
+          const int pr = op.pair_index[b2 * op.nk + b3];
+          if (pr >= 0) acc = cadd(acc, fl[(long long)pr * op.packed_len + p]);
+        }
+        if (bin == b2) {
+          const int pr = op.pair_index[b1 * op.nk + b3];
+          if (pr >= 0) acc = cadd(acc, fl[(long long)pr * op.packed_len + p]);
+        }
+      }
+      return acc;
+    }
     default: return v;
   }
 }

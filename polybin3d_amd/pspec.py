@@ -50,7 +50,7 @@ class PSpec:
             self.const_mask, self.mask_mean, self.sq_mask_mean = True, 1., 1.
         else:
             m = mask.detach().cpu().numpy() if torch.is_tensor(mask) else np.asarray(mask)
-            m = m.astype(np.float64)
+            m = np.ascontiguousarray(np.broadcast_to(m.astype(np.float64), tuple(int(v) for v in base.gridsize)))
             if np.std(m) < 1e-12:
                 self.const_mask, self.mask_mean = True, float(np.mean(m))
             else:
@@ -389,7 +389,7 @@ class PSpec:
                 b.apply_fourier_harmonics(Sa_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=stack[li])
             r = b.bin_reduce(SNA, self.k_bins, [0] * Nl, y=stack, y_ostride=b.H)[0]
         else:
-            r = b.bin_reduce(SNA, self.k_bins, [2 * li for li in range(Nl)], y=a)[0]
+            r = b.bin_reduce(SNA, self.k_bins, [2 * li for li in range(Nl)], y=a, y_ostride=0)[0]
         shot = 0.5 * r.reshape(-1).cpu().numpy()
         return shot * b.volume / b.gridsize.prod() ** 2
 

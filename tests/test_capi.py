@@ -1,0 +1,70 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol include/pb3d.h declares,
+and rejects bad arguments on the host before any launch."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from polybin3d_amd import _capi as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "pb3d.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pb3d_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_what_the_binding_binds():
+    assert declared_symbols() == sorted(C.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.lib()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    assert lib.pb3d_version() >= 10000
+
+
+def test_struct_layouts_match_header():
+    # sizes of the C structs (x86-64 / amdgcn share this layout): pb3d_grid 88, pb3d_rop 88, pb3d_kop 64
+    assert ctypes.sizeof(C.Grid) == 4 * 4 + 9 * 8
+    assert ctypes.sizeof(C.ROp) == 16 + 32 + 32 + 8
+    assert ctypes.sizeof(C.KOp) == 24 + 24 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 16
+
+
+def test_host_side_argument_validation():
+    lib = C.lib()
+    p = ctypes.c_void_p()
+    for n in ([3, 4, 4], [4, 4, 6], [4, 4, 1024], [1, 4, 4]):
+        g = C.make_grid(n, [1, 1, 1], [0, 0, 0], [0, 0, 1], 0)
+        assert lib.pb3d_plan_create(ctypes.byref(p), ctypes.byref(g)) == -2
+    g = C.make_grid([8, 8, 8], [0, 1, 1], [0, 0, 0], [0, 0, 1], 0)
+    assert lib.pb3d_plan_create(ctypes.byref(p), ctypes.byref(g)) == -1
+    assert b"boxsize" in lib.pb3d_last_error()
+    assert lib.pb3d_fft_r2c(None, None, None, None, 1, None, None, None) == -1
+    assert lib.pb3d_fft_c2r(None, None, None, None, 1, None, None, None) == -1
+    g = C.make_grid([8, 8, 8], [1, 1, 1], [0, 0, 0], [0, 0, 1], 0)
+    assert lib.pb3d_bin_reduce(ctypes.byref(g), None, 0, None, 0, 0, 1, 1, None, None, 0, None, 1, None, None,
+                               None) == -1
+    dummy = ctypes.c_void_p(16)
+    # nout above the limit / bad mode / bad ell are rejected before any launch
+    assert lib.pb3d_bin_reduce(ctypes.byref(g), dummy, 0, None, 0, 0, 1, 17, None, None, 0, dummy, 1, dummy,
+                               dummy, None) == -1
+    assert lib.pb3d_bin_reduce(ctypes.byref(g), dummy, 0, None, 0, 0, 1, 1, None, None, 2, dummy, 1, dummy,
+                               dummy, None) == -1
+    bad = C.int_array([7])
+    assert lib.pb3d_bin_reduce(ctypes.byref(g), dummy, 0, None, 0, 0, 1, 1, bad, None, 0, dummy, 1, dummy,
+                               dummy, None) == -1
+    assert lib.pb3d_triplet_sum(None, 1, 1, None, None, None, 1, 0, None, None, None) == -1
+    assert lib.pb3d_kmap(ctypes.byref(g), dummy, dummy, 1, None, None) == -1
+    k = C.kop(kind=9)
+    assert lib.pb3d_kmap(ctypes.byref(g), dummy, dummy, 1, ctypes.byref(k), None) == -1
+    assert lib.pb3d_triplet_work_size(0, 1, 0) == 0
+
+
+def test_python_errors_are_raised():
+    with pytest.raises(ValueError):
+        C.check(-1)

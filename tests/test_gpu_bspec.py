@@ -1,0 +1,112 @@
+"""GPU parity of the bispectrum path against the FP64 oracle on identical seeded inputs.
+Tolerance: 1e-5 of the per-bin scale (|B| of the oracle numerator; sqrt(F_ii F_jj) for Fisher
+entries, which cross zero off the diagonal)."""
+import numpy as np
+import pytest
+
+from oracle import polybin_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L = 1000.
+
+
+def ng_field(n, seed=0):
+    """delta_G + 0.5 (delta_G^2 - <delta_G^2>) with the power-law GRF (the Bk_ideal workload input)."""
+    kk = np.linspace(0, 3, 6000)
+    P = np.zeros_like(kk)
+    P[1:] = 2e4 * (kk[1:] / 0.1) ** -1.5
+    o = O.PolyBin3D(L, list(n), Pk=[kk, P])
+    d = o.generate_data(seed=seed, output_type='real')
+    return d + 0.5 * (d ** 2 - np.mean(d ** 2))
+
+
+def close(got, ref, tol=1e-5, scale=None):
+    got, ref = np.asarray(got), np.asarray(ref)
+    s = np.abs(ref) if scale is None else scale
+    err = np.max(np.abs(got - ref) / np.maximum(s, 1e-300))
+    assert err < tol, "max scaled error %.3e" % err
+
+
+def fisher_scale(F):
+    d = np.abs(np.diag(F))
+    return np.sqrt(np.outer(d, d))
+
+
+CASES = {
+    "32_global_l0": dict(n=32, kw={}, lmax=0, edges=np.linspace(0.01, 0.09, 7), sq=None, Pk_fid=None),
+    "32_global_l2_squeeze": dict(n=32, kw={}, lmax=2, edges=np.linspace(0.01, 0.05, 3),
+                                 sq=np.linspace(0.01, 0.09, 5), Pk_fid='default'),
+    "32_local_l2": dict(n=32, kw=dict(boxcenter=[0., 1170., 826.], sightline='local'), lmax=2,
+                        edges=np.linspace(0.01, 0.07, 4), sq=None, Pk_fid=None),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_bk_ideal(hip, case):
+    import polybin3d_amd as pb
+    c = CASES[case]
+    n = c['n']
+    Pk = [np.linspace(0, 2, 800), 5e3 / (1 + (np.linspace(0, 2, 800) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **c['kw']), O.PolyBin3D(L, [n] * 3, Pk=Pk, **c['kw'])
+    kw = dict(lmax=c['lmax'], k_bins_squeeze=c['sq'], Pk_fid=c['Pk_fid'])
+    bs, bo = pb.BSpec(b, c['edges'], **kw), O.BSpec(o, c['edges'], **kw)
+    assert bs.N_bins == bo.N_bins and np.array_equal(bs.all_bins, bo.all_bins)
+    d = ng_field((n, n, n))
+    for disc in (True, False):
+        got, ref = bs.Bk_ideal(d, discreteness_correction=disc), bo.Bk_ideal(d, discreteness_correction=disc)
+        close(bs.sym_factor, bo.sym_factor, tol=2e-6)
+        close(bs.fish_ideal, bo.fish_ideal, scale=fisher_scale(bo.fish_ideal))
+        sc = np.abs(bo.Bk_numerator_ideal(d))
+        close(bs.Bk_numerator_ideal(d), bo.Bk_numerator_ideal(d), scale=np.maximum(sc, sc.max() * 1e-3))
+        for k in ref:
+            s = np.abs(ref['b0'])
+            close(got[k], ref[k], tol=2e-5, scale=np.maximum(s, 1e-3 * s.max()))
+
+
+def shell_mask(n, L, center):
+    x = [np.fft.fftshift(np.arange(-n // 2, n // 2)) * L / n + center[i] for i in range(3)]
+    X = np.meshgrid(*x, indexing='ij')
+    r = np.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
+    return 0.2 + ((r >= 150.) & (r <= 480.)).astype(np.float64)
+
+
+@pytest.mark.parametrize("case", ["global_const_l2", "global_mask_l0", "local_mask_l2"])
+def test_bk_mc_fisher_per_seed(hip, case):
+    import polybin3d_amd as pb
+    n = 16
+    edges = np.linspace(0.007, 0.05, 4)
+    if case == "global_const_l2":
+        kw, mask, lmax = {}, 0.7 * np.ones((n, n, n)), 2
+    elif case == "global_mask_l0":
+        kw, mask, lmax = {}, shell_mask(n, L, [0., 0., 0.]), 0
+    else:
+        kw, mask, lmax = dict(boxcenter=[10., 10., 10.], sightline='local'), shell_mask(n, L, [10., 10., 10.]), 2
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    bs, bo = pb.BSpec(b, edges, lmax=lmax, mask=mask), O.BSpec(o, edges, lmax=lmax, mask=mask)
+    for seed in (0, 3):
+        F1 = bs.compute_fisher_contribution(seed)
+        F2 = bo.compute_fisher_contribution(seed)
+        close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+
+
+def test_bk_unwindowed_with_linear_term(hip):
+    import polybin3d_amd as pb
+    n = 16
+    edges = np.linspace(0.007, 0.05, 4)
+    mask = shell_mask(n, L, [0., 0., 0.])
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk), O.PolyBin3D(L, [n] * 3, Pk=Pk)
+    bs, bo = pb.BSpec(b, edges, lmax=2, mask=mask), O.BSpec(o, edges, lmax=2, mask=mask)
+    bs.generate_sims(3)
+    bo.generate_sims(3)
+    fish = bo.compute_fisher(2)
+    close(bs.compute_fisher(2), fish, tol=2e-5, scale=fisher_scale(fish))
+    d = ng_field((n, n, n), 4) * mask
+    n1, n2 = bs.Bk_numerator(d), bo.Bk_numerator(d)
+    close(n1, n2, scale=np.maximum(np.abs(n2), 1e-3 * np.abs(n2).max()))
+    got, ref = bs.Bk_unwindowed(d, fish=fish), bo.Bk_unwindowed(d, fish=fish)
+    for k in ref:
+        s = np.abs(ref['b0'])
+        close(got[k], ref[k], tol=5e-5, scale=np.maximum(s, 1e-3 * s.max()))

@@ -1,0 +1,129 @@
+"""GPU parity of the power-spectrum path against the FP64 oracle on identical seeded inputs.
+Tolerances: spectra and Fisher entries within 1e-5 relative to the per-bin scale (|value| or, for
+entries that cross zero, the row's diagonal scale), the bound the FP32 kernels are held to."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import polybin_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L = 1000.
+
+
+def powerlaw_field(n, seed=0):
+    """float64 GRF with P(k) = 2e4 (k/0.1)^-1.5 (the Pk_ideal workload's input), via the oracle."""
+    kk = np.linspace(0, 2, 4000)
+    P = np.zeros_like(kk)
+    P[1:] = 2e4 * (kk[1:] / 0.1) ** -1.5
+    o = O.PolyBin3D(L, list(n), Pk=[kk, P])
+    return o.generate_data(seed=seed, output_type='real')
+
+
+def close(got, ref, tol=1e-5, scale=None):
+    got, ref = np.asarray(got), np.asarray(ref)
+    s = np.abs(ref) if scale is None else scale
+    err = np.max(np.abs(got - ref) / np.maximum(s, 1e-300))
+    assert err < tol, "max scaled error %.3e" % err
+
+
+def test_bin_reduce_matches_oracle(hip):
+    import polybin3d_amd as pb
+    n = (32, 32, 32)
+    b = pb.PolyBin3D(L, list(n), boxcenter=[300., 200., 800.], sightline='local')
+    o = O.PolyBin3D(L, list(n), boxcenter=[300., 200., 800.], sightline='local')
+    rng = np.random.RandomState(0)
+    s1 = (rng.randn(*o.modk_grid.shape) + 1j * rng.randn(*o.modk_grid.shape)).astype(np.complex64)
+    s2 = (rng.randn(*o.modk_grid.shape) + 1j * rng.randn(*o.modk_grid.shape)).astype(np.complex64)
+    r = rng.rand(*o.modk_grid.shape).astype(np.float32)
+    edges = np.linspace(0.007, 0.1, 13)
+    for l in (0, 2, 4):
+        close(b.integrator.cross_integrate(s1, s2, edges, l), o.integrator.cross_integrate(s1, s2, edges, l),
+              scale=o.integrator.integrate(s1, edges, 0))
+        close(b.integrator.integrate(s1, edges, l), o.integrator.integrate(s1, edges, l),
+              scale=o.integrator.integrate(s1, edges, 0))
+        close(b.integrator.real_integrate_double(r, l, 2, edges), o.integrator.real_integrate_double(r, l, 2, edges),
+              scale=o.integrator.real_integrate_double(r, 0, 0, edges))
+    Q = np.stack([s1, s2, s1 * 2])
+    close(b.integrator.integrate_row(s2, Q, edges[3], edges[4]), o.integrator.integrate_row(s2, Q, edges[3], edges[4]),
+          scale=np.abs(o.integrator.integrate(s2, edges, 0)[3]))
+
+
+@pytest.mark.parametrize("case", ["64_global_l2", "64_local_l4", "128_global_l4"])
+def test_pk_ideal(hip, case):
+    import polybin3d_amd as pb
+    if case == "64_global_l2":
+        n, kw, lmax, edges = 64, {}, 2, np.arange(0.01, 0.2001, 0.01)
+    elif case == "64_local_l4":
+        n, kw, lmax, edges = 64, dict(boxcenter=[0., 1170., 826.], sightline='local'), 4, np.arange(0.01, 0.2001, 0.01)
+    else:
+        n, kw, lmax, edges = 128, {}, 4, np.arange(0.01, 0.3001, 0.01)
+    d = powerlaw_field((n, n, n))
+    b = pb.PolyBin3D(L, [n] * 3, **kw)
+    o = O.PolyBin3D(L, [n] * 3, **kw)
+    ps, po = pb.PSpec(b, edges, lmax=lmax), O.PSpec(o, edges, lmax=lmax)
+    got, ref = ps.Pk_ideal(d), po.Pk_ideal(d)
+    close(ps.fish_ideal, po.fish_ideal, scale=np.abs(np.diag(po.fish_ideal)).max())
+    for k in ref:
+        close(got[k], ref[k], scale=np.abs(ref['p0']))
+    assert (b.n_FFTs_forward, b.n_FFTs_reverse) == (o.n_FFTs_forward, o.n_FFTs_reverse)
+
+
+def test_pk_fisher_ideal_no_discreteness(hip):
+    import polybin3d_amd as pb
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    b, o = pb.PolyBin3D(L, [n] * 3), O.PolyBin3D(L, [n] * 3)
+    f1 = pb.PSpec(b, edges, lmax=4).compute_fisher_ideal(discreteness_correction=False)
+    f2 = O.PSpec(o, edges, lmax=4).compute_fisher_ideal(discreteness_correction=False)
+    close(f1, f2, scale=np.abs(f2).max())
+
+
+def shell_mask(n, L, center, seed=1, r_lo=300., r_hi=480., nholes=200, smooth=5.):
+    """Shell r in [r_lo, r_hi] around the observer at the origin, minus random holes, Gaussian-smoothed."""
+    x = [np.fft.fftshift(np.arange(-n // 2, n // 2)) * L / n + center[i] for i in range(3)]
+    X = np.meshgrid(*x, indexing='ij')
+    r = np.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
+    m = ((r >= r_lo) & (r <= r_hi)).astype(np.float64)
+    rng = np.random.RandomState(seed)
+    for _ in range(nholes):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        c = u * rng.uniform(r_lo, r_hi)
+        rad = rng.uniform(10., 30.)
+        m[(X[0] - c[0]) ** 2 + (X[1] - c[1]) ** 2 + (X[2] - c[2]) ** 2 < rad ** 2] = 0.
+    k = [np.fft.fftfreq(n, L / n) * 2 * np.pi] * 2 + [np.fft.rfftfreq(n, L / n) * 2 * np.pi]
+    K = np.meshgrid(*k, indexing='ij')
+    g = np.exp(-0.5 * smooth ** 2 * (K[0] ** 2 + K[1] ** 2 + K[2] ** 2))
+    return np.fft.irfftn(np.fft.rfftn(m) * g, s=(n, n, n))
+
+
+@pytest.mark.parametrize("case", ["global_const", "local_mask"])
+def test_pk_mc_fisher_and_unwindowed(hip, case):
+    import polybin3d_amd as pb
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    if case == "global_const":
+        kw, mask, lmax = {}, 0.8 * np.ones((n, n, n)), 4
+    else:
+        kw = dict(boxcenter=[10., 10., 10.], sightline='local')
+        mask, lmax = shell_mask(n, L, [10., 10., 10.]), 4
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    ps = pb.PSpec(b, edges, lmax=lmax, mask=mask, mask_shot=mask)
+    po = O.PSpec(o, edges, lmax=lmax, mask=mask, mask_shot=mask)
+    for seed in (0, 1):
+        F1 = ps.compute_fisher_contribution(seed)
+        F2 = po.compute_fisher_contribution(seed)
+        sc = np.sqrt(np.outer(np.abs(np.diag(F2)), np.abs(np.diag(F2))))
+        close(F1, F2, scale=sc)
+    s1, s2 = ps.compute_shot_contribution(5), po.compute_shot_contribution(5)
+    close(s1, s2, scale=np.abs(s2[:ps.Nk]).max())
+    d = powerlaw_field((n, n, n), seed=3) * mask
+    fish = po.compute_fisher(2)
+    close(ps.compute_fisher(2), fish, scale=np.sqrt(np.outer(np.abs(np.diag(fish)), np.abs(np.diag(fish)))))
+    got = ps.Pk_unwindowed(d, fish=fish)
+    ref = po.Pk_unwindowed(d, fish=fish)
+    for k in ref:
+        close(got[k], ref[k], tol=2e-5, scale=np.abs(ref['p0']))
