@@ -1,0 +1,223 @@
+"""Benchmark: Monte Carlo Fisher realizations per second for the power spectrum + bispectrum at 256^3.
+
+One step = one Monte Carlo realization of both Fisher matrices on one GPU:
+  * PSpec MC Fisher: 256^3, box 1000 Mpc/h, local (Yamamoto) sightline, lmax=4, 30 bins 0.01-0.30 h/Mpc;
+  * BSpec MC Fisher: same grid, lmax=2, 40 bins 0.01-0.40 h/Mpc (12700 triangle bins);
+  * mask: shell 300 <= r <= 480 Mpc/h minus 200 random holes (r = 10-30 Mpc/h), smoothed 5 Mpc/h;
+  * GRFs drawn on the GPU (power law P(k) = 2e4 (k/0.1)^-1.5), Fisher rows accumulated in FP64.
+Prints one JSON line with the throughput, the roofline of the dominant kernel (sampled HIP-event
+timing over the timed region) and a CPU baseline (the FP64 oracle's unit operations timed on this
+host and scaled by the reference's exact per-realization operation counts).
+
+usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 256] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Fisher MC realizations/s, Pk+Bk 256³: batched/split-bf16 vs plain FP32"
+HBM_PEAK_GBS = 8000.0
+KIND_NAMES = {0: "r2c_rows", 1: "c2r_rows", 2: "fft_cols (plain)", 3: "fft_cols (C2R first pass, fused prologue)",
+              4: "fft_cols (R2C last pass, fused epilogue)"}
+
+
+def shell_mask(n, L, center, seed=1, r_lo=300., r_hi=480., nholes=200, smooth=5.):
+    """1 inside r_lo <= r <= r_hi (observer at the origin) minus nholes spheres of radius U[10,30]
+    centred uniformly in the shell, smoothed with a Gaussian of width `smooth` (via FFT)."""
+    ax = [np.fft.fftshift(np.arange(-n // 2, n // 2)) * L / n + center[i] for i in range(3)]
+    X = np.meshgrid(*ax, indexing='ij', sparse=True)
+    r2 = X[0] ** 2 + X[1] ** 2 + X[2] ** 2
+    m = ((r2 >= r_lo ** 2) & (r2 <= r_hi ** 2)).astype(np.float32)
+    rng = np.random.RandomState(seed)
+    dx = L / n
+    for _ in range(nholes):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        c = u * rng.uniform(r_lo, r_hi)
+        rad = rng.uniform(10., 30.)
+        sl = []
+        for i in range(3):
+            idx = np.nonzero(np.abs(ax[i] - c[i]) <= rad + dx)[0]
+            sl.append(idx)
+        if any(len(s) == 0 for s in sl):
+            continue
+        sub = np.ix_(*sl)
+        d2 = (ax[0][sl[0]][:, None, None] - c[0]) ** 2 + (ax[1][sl[1]][None, :, None] - c[1]) ** 2 + \
+             (ax[2][sl[2]][None, None, :] - c[2]) ** 2
+        blk = m[sub]
+        blk[d2 < rad ** 2] = 0.
+        m[sub] = blk
+    k = [np.fft.fftfreq(n, L / n) * 2 * np.pi] * 2 + [np.fft.rfftfreq(n, L / n) * 2 * np.pi]
+    K = np.meshgrid(*k, indexing='ij', sparse=True)
+    g = np.exp(-0.5 * smooth ** 2 * (K[0] ** 2 + K[1] ** 2 + K[2] ** 2))
+    return np.fft.irfftn(np.fft.rfftn(m) * g, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32)
+
+
+def powerlaw_pk(kmax):
+    k = np.linspace(0., kmax, 40001)
+    P = np.empty_like(k)
+    P[1:] = 2e4 * (k[1:] / 0.1) ** -1.5
+    P[0] = P[1]
+    return [k, P]
+
+
+def cpu_baseline(n, pk_bins, bk_bins, lmax_pk, lmax_bk, N_bins_bk):
+    """FP64 oracle unit operations at n^3 on this host, scaled by the reference's per-realization
+    operation counts: FFTs (scipy.fft, all cores) + the reference's per-bin full-grid sweeps
+    (oracle/cpu_kernels.c, OpenMP).  Element-wise passes are not counted (a lower bound on the
+    reference's time)."""
+    import scipy.fft
+    from oracle import polybin_oracle as O
+    from oracle import build_ref
+    cores = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS"):
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    lib_path = os.path.join(ROOT, "oracle", "_cpu", "libpb3d_cpu.so")
+    if not os.path.exists(lib_path):
+        build_ref.build_cpu()
+    lib = ctypes.CDLL(lib_path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    t_start = time.perf_counter()
+    rng = np.random.RandomState(0)
+    x = rng.randn(n, n, n)
+
+    def med(f, r=3):
+        f()
+        ts = []
+        for _ in range(r):
+            a = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - a)
+        return float(np.median(ts))
+
+    X = scipy.fft.rfftn(x, workers=cores)
+    t_f = med(lambda: scipy.fft.rfftn(x, workers=cores))
+    t_i = med(lambda: scipy.fft.irfftn(X, s=(n, n, n), workers=cores))
+    kF = 2 * np.pi / 1000.
+    kx = np.fft.fftshift(np.arange(-n // 2, n // 2)) * kF
+    kz = np.arange(n // 2 + 1) * kF
+    modk = np.ascontiguousarray(np.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2))
+    degk = np.ascontiguousarray(np.broadcast_to(1. + (kz > 0), modk.shape)).astype(np.float64)
+    H = modk.size
+    nout = max(4, cores)
+    arr = np.full((nout, H), 1. + 0.5j, dtype=np.complex128)
+    s1 = np.ascontiguousarray(X)
+    edges = np.ascontiguousarray(pk_bins, dtype=np.float64)
+    out = np.zeros(max(nout, len(edges)))
+    lib.cpu_cross_integrate.argtypes = [dp, dp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, dp, ctypes.c_int, dp]
+    lib.cpu_integrate_row.argtypes = [dp, dp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, dp]
+    P = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+    t_cross = med(lambda: lib.cpu_cross_integrate(P(modk), P(degk), s1.ctypes.data, s1.ctypes.data, H, P(edges),
+                                                  len(edges) - 1, P(out)))
+    t_row = med(lambda: lib.cpu_integrate_row(P(modk), P(degk), s1.ctypes.data, arr.ctypes.data, H, nout, 0.1, 0.11,
+                                              P(out)))
+    t_visit = t_row / nout
+    Nk_pk = len(pk_bins) - 1
+    Nl_pk = lmax_pk // 2 + 1
+    pk_f, pk_r = O.pk_fisher_fft_count(Nk_pk, lmax_pk, True, False)
+    ab = O.triangle_bins(bk_bins, bk_bins, lmax_bk)
+    bk_f, bk_r = O.bk_fisher_fft_count(ab, len(bk_bins) - 1, lmax_bk, True, False)
+    # Pk rows: Nl*Nk Q maps x Nl cross_integrate calls (pspec.py:394-413); Bk rows: one
+    # integrate_row over all N_bins Q maps per row (bspec.py:597)
+    T = (pk_f + bk_f) * t_f + (pk_r + bk_r) * t_i + Nl_pk * Nk_pk * Nl_pk * t_cross + N_bins_bk ** 2 * t_visit
+    sample = ("extrapolated: timed at %d^3 on %d host threads rfftn %.3f s, irfftn %.3f s (scipy.fft FP64), one "
+              "%d-bin cross_integrate sweep %.3f s, integrate_row %.2e s per map-sweep (C/OpenMP restatement of the "
+              "reference loops), scaled by %d+%d FFTs, %d cross_integrate calls and %d^2 integrate_row map-sweeps per "
+              "realization; element-wise passes not counted; %.0f s of CPU work in this run"
+              % (n, cores, t_f, t_i, Nk_pk, t_cross, t_visit, pk_f + bk_f, pk_r + bk_r, Nl_pk * Nk_pk * Nl_pk,
+                 N_bins_bk, time.perf_counter() - t_start))
+    return {"value": 1.0 / T, "unit": "realizations/s", "cores": cores, "kind": "port", "sample": sample,
+            "seconds_per_realization": T}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prof-period", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the Fisher realizations of one device and has no "
+              "multi-GPU path (realizations would shard across GPUs with one FP64 all-reduce of the Fisher "
+              "partials); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import polybin3d_amd as pb
+    from polybin3d_amd import _capi as C
+
+    n, L, center = args.n, 1000., [10., 10., 10.]
+    kny = n * np.pi / L
+    pk_bins = np.linspace(0.01, min(0.30, 0.7 * kny), 31)
+    bk_bins = np.linspace(0.01, min(0.40, 0.99 * kny), 41)
+    t_setup = time.perf_counter()
+    base = pb.PolyBin3D(L, [n] * 3, Pk=powerlaw_pk(2 * kny * 1.8), boxcenter=center, sightline='local', rng='device')
+    mask = shell_mask(n, L, center)
+    ps = pb.PSpec(base, pk_bins, lmax=4, mask=mask)
+    bs = pb.BSpec(base, bk_bins, lmax=2, mask=mask)
+    bs._compute_symmetry_factor()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    Fp = torch.zeros((ps.N_bins, ps.N_bins), dtype=torch.float64, device=base.device)
+    Fb = torch.zeros((bs.N_bins, bs.N_bins), dtype=torch.float64, device=base.device)
+
+    def step(seed):
+        Fp.add_(ps._fisher_contribution_dev(seed))
+        Fb.add_(bs._fisher_contribution_dev(seed))
+
+    for w in range(args.warmup):
+        step(10 ** 6 + w)
+        print("warmup step %d done" % w, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    C.check(C.lib().pb3d_prof_enable(args.prof_period))
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step(s)
+        torch.cuda.synchronize()
+        print("timed step %d done (%.2f s)" % (s, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kinds = {}
+    for kind in range(5):
+        cnt, ms, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        C.check(C.lib().pb3d_prof_read(kind, ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(by)))
+        if cnt.value:
+            kinds[kind] = (cnt.value, ms.value, by.value)
+    C.lib().pb3d_prof_enable(0)
+    dom = max(kinds, key=lambda k: kinds[k][1]) if kinds else None
+    roof = None
+    if dom is not None:
+        cnt, ms, by = kinds[dom]
+        achieved = by / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": KIND_NAMES[dom],
+                "avg_launch_ms": round(ms / cnt, 4), "bytes_per_launch": by / cnt,
+                "sampled_launches": cnt, "share_of_sampled_fft_time": round(ms / sum(v[1] for v in kinds.values()), 3)}
+    value = args.steps / dt
+    res = {"metric": METRIC, "value": round(value, 6), "unit": "realizations/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "pk_bk_mc_fisher_%d" % n, "grid": [n] * 3, "boxsize": L, "sightline": "local",
+                      "pk": {"lmax": 4, "nbins": len(pk_bins) - 1, "kmax": float(pk_bins[-1])},
+                      "bk": {"lmax": 2, "nbins": len(bk_bins) - 1, "kmax": float(bk_bins[-1]), "N_bins": bs.N_bins},
+                      "mask": "shell 300-480 Mpc/h, 200 holes, 5 Mpc/h smoothing", "rng": "device",
+                      "fisher_accumulation": "fp64", "setup_s": round(t_setup, 1)},
+           "roofline": roof}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(n, pk_bins, bk_bins, 4, 2, bs.N_bins)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1016,3 +1016,78 @@ class BSpec:
             raise Exception("Need to compute Fisher matrix first!")
         out = self.inv_fish @ self.Bk_numerator(data, include_linear_term=include_linear_term)
         return {'b%d' % l: out[i * self.N3:(i + 1) * self.N3] for i, l in enumerate(range(0, self.lmax + 1, 2))}
+
+
+# ----------------------------------------------------------------------------------------------
+# Structural FFT counts of the reference call graphs (for extrapolating the CPU baseline)
+# ----------------------------------------------------------------------------------------------
+
+def bk_fisher_fft_count(all_bins, Nk_squeeze, lmax, local, const_mask):
+    """(forward, reverse) FFTs of one BSpec._compute_fisher call with trivial S^-1
+    (PolyBin3D/bspec.py:473-676), counted from the loop structure without transforming anything."""
+    ab = np.asarray(all_bins)
+    Nl = lmax // 2 + 1
+    loc = local and lmax > 0
+    harm = sum(2 * l + 1 for l in range(2, lmax + 1, 2))
+    fwd = rev = 0
+    if not const_mask:
+        rev += 2                                                    # a_real1, a_real2 (652-653)
+    for weighting in ('Ainv', 'Sinv'):
+        if weighting == 'Sinv' and not const_mask:
+            fwd += 2                                                # apply_filter of both GRFs
+        if loc:
+            rev += 2                                                # weighted_map_real (543-544)
+            fwd += 2 * harm                                         # leg maps (558-560)
+        rev += 2 * Nk_squeeze                                       # g_{b,0} maps (550-552)
+        for B in range(Nk_squeeze):
+            if lmax > 0:
+                rev += 2 * (Nl - 1)                                 # g_{B,l} maps (570-576)
+            for A in range(B + 1):
+                if weighting == 'Sinv':
+                    if np.any((ab[:, 2] == B) & (ab[:, 0] == A)):
+                        fwd += Nl                                   # ft_ABl (588-593)
+                else:
+                    i1 = np.where((ab[:, 0] == A) & (ab[:, 1] == B))[0]
+                    i2 = np.where((ab[:, 1] == A) & (ab[:, 2] == B))[0]
+                    i3 = np.where((ab[:, 2] == B) & (ab[:, 0] == A))[0]
+                    if len(i1) + len(i2) + len(i3) == 0:
+                        continue
+                    fwd += Nl                                       # ft_ABl (609-613)
+                    if loc:
+                        for ii in i1:
+                            l = ab[ii, 3]
+                            if l > 0:
+                                rev += 2 * l + 1                    # apply_real_harmonics (632)
+                                fwd += 1                            # to_fourier (634/636)
+        if weighting == 'Ainv' and not const_mask:
+            rev += len(ab)                                          # to_real(Q_Ainv[i]) (666)
+            fwd += len(ab)                                          # apply_filter (666)
+    return fwd, rev
+
+
+def pk_fisher_fft_count(Nk, lmax, local, const_mask):
+    """(forward, reverse) FFTs of one PSpec._compute_fisher call with trivial S^-1, even l
+    (PolyBin3D/pspec.py:215-493): the formula the Tutorial 3 counter (452/682) pins."""
+    Nl = lmax // 2 + 1
+    fwd = rev = 0
+    if not const_mask:
+        rev += 1                                                    # a_real (268)
+        fwd += 1                                                    # apply_filter (276)
+    loc = local and lmax > 0
+    if loc:
+        rev += 2                                                    # SinvP_real, Ainv_real (336, 343)
+        fwd += sum(2 * (2 * li) + 1 for li in range(1, Nl))         # leg_maps (352)
+    per_dagger = (1, 1) if not const_mask else (0, 0)               # (fwd, rev) of apply_filter_dagger
+    fwd += Nk * per_dagger[0]
+    rev += Nk * per_dagger[1]                                       # l = 0 rows (444-446)
+    for li in range(1, Nl):
+        ell = 2 * li
+        if not loc:
+            fwd += Nk * per_dagger[0]
+            rev += Nk * per_dagger[1]
+        else:
+            fwd += 2 * ell + 1                                      # leg_map (462)
+            rev += Nk * (2 * ell + 1)                               # real_map pieces (482)
+            fwd += Nk * (per_dagger[0] + (1 if not const_mask else 1))  # two daggers (471, 485)
+            rev += Nk * per_dagger[1]
+    return fwd, rev
