@@ -117,7 +117,23 @@ typedef struct {
   int32_t terms;
   int32_t trip0;
   int32_t leg3_legendre;
+  /* Band limit: as a C2R prologue, the input is zero outside |kx| <= support[0]*kF0,
+   * |ky| <= support[1]*kF1, kz <= support[2]*kF2 (frequency-index radii) and the passes skip those
+   * columns and reads; as an R2C epilogue only modes inside it are produced (PACK outputs, or a
+   * full grid whose modes outside the support are left unwritten).  A negative entry = full axis. */
+  int32_t support[3];
+  int32_t pad3_;
 } pb3d_kop;
+
+/* One sub-map of pb3d_fft_pointing: C2R(kop(in)) weighted in real space by coef * Y_lm(x_hat)
+ * (ell = 0: weight coef). */
+typedef struct {
+  pb3d_kop kop;
+  const void* in;
+  int32_t ell, m;
+  float coef;
+  int32_t pad_;
+} pb3d_submap;
 
 /* ---------------------------------------------------------------- library */
 int pb3d_version(void);
@@ -155,6 +171,18 @@ int pb3d_fft_r2c(const pb3d_plan* plan, const float* in, void* out, void* work, 
  * scratch of batch*n0*n1*(n2/2+1) float2; out: real written through `rop` (NULL = plain). */
 int pb3d_fft_c2r(const pb3d_plan* plan, const void* in, float* out, void* work, int batch,
                  const pb3d_kop* kop, const pb3d_rop* rop, void* stream);
+
+/* The pointing chain of the MC Fisher Q maps with trivial S^-1, fused (apply_filter /
+ * apply_filter_dagger and apply_real_harmonics, PolyBin3D/pspec.py:270-304, bspec.py:499-505,
+ * 615-666, base.py:391-400):
+ *   out[b] = kop_out( R2C( mask(x) * scale * sum_g coef_g Y_g(x_hat) * C2R( kop_g(in_g) )[b] ) )
+ * The sub-map column passes run pruned to each input's support; one row kernel then inverts all
+ * nsub rows, combines them with their real-space weights, applies the mask (NULL = none) and the
+ * forward row transform, so the real-space maps never reach HBM.  work: nsub*batch*n0*n1*(n2/2+1)
+ * float2.  nsub <= 16. */
+int pb3d_fft_pointing(const pb3d_plan* plan, int batch, int nsub, const pb3d_submap* subs,
+                      const float* mask, float scale, const pb3d_kop* kop_out, void* out, void* work,
+                      void* stream);
 
 /* ---------------------------------------------------------------- mode-space maps */
 /* out[batch] = kop(x) over the half grid, without a transform (fourier_filter,

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpb3d.so")
+# PB3D_LIB overrides the library path (an alternative in-tree build of the same sources)
+LIB_PATH = os.environ.get("PB3D_LIB", os.path.join(_HERE, "libpb3d.so"))
 
 c_int32, c_int64, c_float, c_double = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double
 c_void_p, c_size_t = ctypes.c_void_p, ctypes.c_size_t
@@ -28,6 +29,8 @@ EXPORTS = {
     "pb3d_plan_destroy": (ctypes.c_int, [c_void_p]),
     "pb3d_fft_r2c": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
     "pb3d_fft_c2r": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
+    "pb3d_fft_pointing": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_float, c_void_p,
+                                         c_void_p, c_void_p, c_void_p]),
     "pb3d_kmap": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p]),
     "pb3d_kmap_packed": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, ctypes.c_int, c_void_p,
                                         c_void_p]),
@@ -66,7 +69,13 @@ class KOp(ctypes.Structure):
                 ("w", c_void_p), ("x_batch_stride", c_int64), ("scale", c_float), ("pad2_", c_int32),
                 ("pack_index", c_void_p), ("packed_len", c_int64), ("trip", c_void_p), ("pair_index", c_void_p),
                 ("ft", c_void_p), ("ft_lstride", c_int64), ("nk", c_int32), ("terms", c_int32),
-                ("trip0", c_int32), ("leg3_legendre", c_int32)]
+                ("trip0", c_int32), ("leg3_legendre", c_int32), ("support", c_int32 * 3), ("pad3_", c_int32)]
+
+
+class SubMap(ctypes.Structure):
+    """pb3d_submap: one weighted sub-map of pb3d_fft_pointing."""
+    _fields_ = [("kop", KOp), ("inp", c_void_p), ("ell", c_int32), ("m", c_int32), ("coef", c_float),
+                ("pad_", c_int32)]
 
 
 ROP_PLAIN, ROP_MUL, ROP_DIFF, ROP_YLM, ROP_MULYLM, ROP_SQUARE = range(6)
@@ -116,7 +125,7 @@ def rop(kind=ROP_PLAIN, a=None, b=None, c=None, d=None, ell=0, m=0, accumulate=0
 
 def kop(kind=KOP_PLAIN, bin_table=None, bins=None, w=None, ell=0, ell2=0, m=0, accumulate=0, scale=1.0,
         x_batch_stride=0, pack_index=None, packed_len=0, trip=None, pair_index=None, ft=None, ft_lstride=0,
-        nk=0, terms=0, trip0=0, leg3_legendre=0):
+        nk=0, terms=0, trip0=0, leg3_legendre=0, support=None):
     o = KOp()
     o.kind, o.ell, o.ell2, o.m, o.accumulate = kind, ell, ell2, m, int(accumulate)
     o.bin_table, o.bins, o.w = _ptr(bin_table), _ptr(bins), _ptr(w)
@@ -125,6 +134,9 @@ def kop(kind=KOP_PLAIN, bin_table=None, bins=None, w=None, ell=0, ell2=0, m=0, a
     o.pack_index, o.packed_len = _ptr(pack_index), int(packed_len)
     o.trip, o.pair_index, o.ft, o.ft_lstride = _ptr(trip), _ptr(pair_index), _ptr(ft), int(ft_lstride)
     o.nk, o.terms, o.trip0, o.leg3_legendre = int(nk), int(terms), int(trip0), int(leg3_legendre)
+    sup = (-1, -1, -1) if support is None else support
+    for i in range(3):
+        o.support[i] = int(sup[i])
     o._t = {k: v for k, v in (('bin_table', bin_table), ('bins', bins), ('w', w), ('pack_index', pack_index),
                               ('trip', trip), ('pair_index', pair_index), ('ft', ft))
             if v is not None and not isinstance(v, int)}

@@ -364,6 +364,39 @@ class PolyBin3D:
         k = self.as_fourier(_input_kmap)
         return self.c2r(k)
 
+    def support(self, kmax):
+        """Frequency-index radii (per axis) covering every mode with |k| < kmax, for band-limited
+        (pruned) transforms: |fs(i)| * kF_i <= |k| < kmax."""
+        return [int(min(np.floor(kmax / self.kF[i]), self.gridsize[i] // 2)) for i in range(3)]
+
+    def pointing(self, subs, batch, out, mask=None, scale=1.0, kop_out=None):
+        """out[b] = kop_out(R2C(mask * scale * sum_g coef_g Y_g(x) C2R(kop_g(in_g))[b])), fused
+        (pb3d_fft_pointing).  subs: list of (kop, input or None for BKQ, ell, m, coef)."""
+        G = len(subs)
+        arr = (C.SubMap * G)()
+        keep = []
+        for g, (kop, inp, ell, m, coef) in enumerate(subs):
+            self._check_kop(kop, batch, 'pointing sub %d' % g)
+            if inp is not None:
+                self._need(inp, (batch - 1) * kop.x_batch_stride + self.H, 'pointing sub %d input' % g)
+            elif kop.kind != C.KOP_BKQ:
+                raise ValueError("pointing: sub-map %d has no input" % g)
+            arr[g].kop = kop
+            arr[g].inp = C._ptr(inp)
+            arr[g].ell, arr[g].m, arr[g].coef = int(ell), int(m), float(coef)
+            keep.append((kop, inp))
+        if mask is not None:
+            self._need(mask, self.N, 'pointing mask')
+        self._check_kop(kop_out, batch, 'pointing output')
+        packed = kop_out is not None and kop_out.kind == C.KOP_PACK
+        self._need(out, batch * (kop_out.packed_len if packed else self.H), 'pointing output')
+        work = self.work('fft_point', G * batch * self.H * 8)
+        C.check(C.lib().pb3d_fft_pointing(self._plan, batch, G, arr, C._ptr(mask), float(scale),
+                                          C.byref_or_none(kop_out), out.data_ptr(), work.data_ptr(), self.stream()))
+        self.n_FFTs_reverse += batch * G
+        self.n_FFTs_forward += batch
+        return out
+
     def kmap(self, x, kop, out=None, batch=1):
         if out is None:
             out = torch.empty((batch,) + self.half_shape if batch > 1 else self.half_shape,

@@ -122,6 +122,7 @@ class BSpec:
         self.Ylm_fourier = harmonics_tags(lmax, False, 'fourier') if lmax > 0 else None
         self.table = b.bin_table(self.k_bins_squeeze)
         self._bins_dev = torch.arange(self.Nk_squeeze, dtype=torch.int32, device=b.device)
+        self._sup = b.support(self.k_bins_squeeze[-1])
 
     def get_ks(self):
         """Central (k1, k2, k3) of every triangle bin (bspec.py:201-220)."""
@@ -142,7 +143,7 @@ class BSpec:
         if out is None:
             out = self._real_stack(Nq)
         b.c2r(src, out=out, batch=Nq, kop=C.kop(C.KOP_FILTER, bin_table=self.table, bins=self._bins_dev, ell=ell,
-                                                  ell2=ell2, w=w, x_batch_stride=0))
+                                                  ell2=ell2, w=w, x_batch_stride=0, support=self._sup))
         return out
 
     def _g_maps(self, S, out=None):
@@ -441,61 +442,54 @@ class BSpec:
                 gb2 = G2[li * Nq + B0:li * Nq + B0 + nb]
                 b.r2c(None, out=out[li, r0:r0 + nb], batch=nb,
                       rop=C.rop(C.ROP_DIFF, a=G1[A], sa=0, b=gb1, sb=N, c=G2[A], sc=0, d=gb2, sd=N),
-                      kop=C.kop(C.KOP_PACK, pack_index=pk['pack_index'], packed_len=pk['M']))
+                      kop=C.kop(C.KOP_PACK, pack_index=pk['pack_index'], packed_len=pk['M'], support=self._sup))
 
     def _q_ainv(self, FT, pair_tab, QA):
-        """QA[i] = packed S^-1 P applied to the Q[A^-1 a] map of triplet i (bspec.py:615-666)."""
+        """QA[i] = packed S^-1 P applied to the Q[A^-1 a] map of triplet i (bspec.py:615-666), in chunks of
+        triplets of one multipole: the map is assembled from the packed FT rows inside the C2R prologue
+        (kop BKQ) and masked / re-transformed by the fused pointing chain; for local l>0 the external-leg
+        Legendre factor is the real-space sum_m Y_lm(x) IFFT[Theta_b3 Y_lm(k) FT_0(b1,b2)]
+        (apply_real_harmonics, bspec.py:629-636), added as 2l+1 weighted sub-maps."""
         b = self.base
         pk = self._packed_layout()
         M, Nq = pk['M'], self.Nk_squeeze
         ab = self.all_bins
         local = b.sightline == 'local'
-        trip = self._trip_dev
-        per_map = 4 * b.N + 16 * b.H
-        CH = int(max(1, min(64, (4 << 30) // per_map)))
-        common = dict(bin_table=self.table, pack_index=pk['pack_index'], packed_len=M, trip=trip, pair_index=pair_tab,
-                      ft=FT, ft_lstride=FT.shape[1] * M, nk=Nq)
+        common = dict(bin_table=self.table, pack_index=pk['pack_index'], packed_len=M, trip=self._trip_dev,
+                      pair_index=pair_tab, ft=FT, ft_lstride=FT.shape[1] * M, nk=Nq, support=self._sup)
+        pack = dict(pack_index=pk['pack_index'], packed_len=M, support=self._sup)
         t0 = 0
         while t0 < self.N_bins:
             ell = int(ab[t0, 3])
+            ylm = local and ell > 0
+            G = 2 * ell + 2 if ylm else 1
+            CH = int(max(1, min(64, (6 << 30) // (G * 8 * b.H))))
             t1 = t0
             while t1 < self.N_bins and t1 - t0 < CH and ab[t1, 3] == ell:
                 t1 += 1
             nb = t1 - t0
-            ylm = local and ell > 0
             terms = 2 if ylm else 3
             if self.const_mask:
-                b_ = QA[t0:t1]
-                C.check(C.lib().pb3d_kmap_packed(C.ctypes.byref(b.cgrid), None, pk['modes'].data_ptr(), M, b_.data_ptr(),
-                                                 nb, C.ctypes.byref(C.kop(C.KOP_BKQ, terms=terms, trip0=t0,
-                                                                          leg3_legendre=int(not local),
-                                                                          scale=self.mask_mean, **common)),
+                # P = mask_mean: the Theta terms need no real-space step (bspec.py:662-664)
+                C.check(C.lib().pb3d_kmap_packed(C.ctypes.byref(b.cgrid), None, pk['modes'].data_ptr(), M,
+                                                 QA[t0:t1].data_ptr(), nb,
+                                                 C.ctypes.byref(C.kop(C.KOP_BKQ, terms=terms, trip0=t0,
+                                                                      leg3_legendre=int(not local),
+                                                                      scale=self.mask_mean, **common)),
                                                  b.stream()))
                 if ylm:
-                    R = self._ylm_leg3(common, t0, nb, ell)
-                    b.r2c(R, out=QA[t0:t1], batch=nb,
-                          kop=C.kop(C.KOP_PACK, pack_index=pk['pack_index'], packed_len=M, accumulate=1,
-                                    scale=self.mask_mean))
+                    subs = [(C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common), None, ell, m, 1.0)
+                            for m in range(2 * ell + 1)]
+                    b.pointing(subs, nb, QA[t0:t1], mask=None, scale=self.mask_mean,
+                               kop_out=C.kop(C.KOP_PACK, accumulate=1, **pack))
             else:
-                R = self._real_stack(nb)
-                b.c2r(None, out=R, batch=nb, kop=C.kop(C.KOP_BKQ, terms=terms, trip0=t0,
-                                                       leg3_legendre=int(not local), **common))
+                subs = [(C.kop(C.KOP_BKQ, terms=terms, trip0=t0, leg3_legendre=int(not local), **common), None,
+                         0, 0, 1.0)]
                 if ylm:
-                    self._ylm_leg3(common, t0, nb, ell, out=R)
-                b.r2c(None, out=QA[t0:t1], batch=nb, rop=C.rop(C.ROP_MUL, a=R, sa=b.N, b=self.mask, sb=0),
-                      kop=C.kop(C.KOP_PACK, pack_index=pk['pack_index'], packed_len=M))
+                    subs += [(C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common), None, ell, m, 1.0)
+                             for m in range(2 * ell + 1)]
+                b.pointing(subs, nb, QA[t0:t1], mask=self.mask, scale=1.0, kop_out=C.kop(C.KOP_PACK, **pack))
             t0 = t1
-
-    def _ylm_leg3(self, common, t0, nb, ell, out=None):
-        """sum_m Y_lm(x) IFFT[Theta_b3 Y_lm(k) FT_0(b1,b2)] for a chunk of local l>0 triplets: the
-        external-leg Legendre factor in real space (apply_real_harmonics, bspec.py:629-636)."""
-        b = self.base
-        R = self._real_stack(nb) if out is None else out
-        acc_first = out is not None
-        for m in range(2 * ell + 1):
-            b.c2r(None, out=R, batch=nb, kop=C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common),
-                  rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=int(acc_first or m > 0)))
-        return R
 
     def compute_fisher_contribution(self, seed, verb=False):
         """One MC realization (bspec.py:441-445, 473-676): FP64 numpy [N_bins, N_bins], rows indexed

@@ -18,9 +18,11 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Wno-unused-result"]
 
 
-def build(verbose=True, jobs=None):
-    """Compile every source to an object in parallel, then link libpb3d.so."""
-    objdir = os.path.join(HERE, "_build")
+def build(verbose=True, jobs=None, out=None, defines=()):
+    """Compile every source to an object in parallel, then link libpb3d.so (or `out`, with extra
+    -D `defines`, for variant builds)."""
+    lib = out or LIB
+    objdir = os.path.join(HERE, "_build" if out is None else "_build_" + os.path.basename(out).replace(".", "_"))
     os.makedirs(objdir, exist_ok=True)
     t0 = time.time()
     procs = []
@@ -28,7 +30,7 @@ def build(verbose=True, jobs=None):
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [HIPCC, *FLAGS, "-I", os.path.join(HERE, "..", "include"), "-c",
+        cmd = [HIPCC, *FLAGS, *["-D" + d for d in defines], "-I", os.path.join(HERE, "..", "include"), "-c",
                os.path.join(CSRC, src), "-o", obj]
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
     failed = False
@@ -41,15 +43,15 @@ def build(verbose=True, jobs=None):
             sys.stderr.write(out.decode(errors="replace")[-2000:])
     if failed:
         raise RuntimeError("libpb3d build failed")
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError("libpb3d link failed:\n" + r.stdout.decode(errors="replace"))
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built %s in %.1f s" % (LIB, time.time() - t0))
-    return LIB
+        print("built %s in %.1f s" % (lib, time.time() - t0))
+    return lib
 
 
 if __name__ == "__main__":

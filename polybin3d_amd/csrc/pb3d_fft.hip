@@ -139,52 +139,155 @@ __device__ __forceinline__ void pencil_fft(int t, Src0 src0, Lds lds, DstN dstN,
   }
 }
 
-template <int L> struct ColCfg { static constexpr int T = (256 / Decomp<L>::P) > 8 ? 256 / Decomp<L>::P : 8; };
+#ifndef PB3D_COL_THREADS
+#define PB3D_COL_THREADS 512
+#endif
+#ifndef PB3D_POINT_WAVES
+#define PB3D_POINT_WAVES 2
+#endif
+// Last Stockham stage with its outputs delivered to registers: for the last stage NB = L/R = NS,
+// so butterfly j = t + b*P < NS writes positions j + k*NS = t + P*(b + k*NS/P); the sink gets the
+// compile-time slot e = b + k*NS/P (slot e of thread t holds position t + e*P).
+template <int L, int R, int NS, int P, int DIR, typename Src, typename Sink>
+__device__ __forceinline__ void last_stage_to_regs(int t, Src src, Sink sink, const float2* tw) {
+  constexpr int NB = L / R;
+  constexpr int B = NB / P;
+  static_assert(NB == NS && NB % P == 0, "last stage shape");
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + b * P;
+    float2 v[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = src(j + k * NB);
+    if (NS > 1) {
+#pragma unroll
+      for (int k = 1; k < R; ++k) {
+        float2 w = tw[j * k * (L / (NS * R))];
+        if (DIR > 0) w.y = -w.y;
+        v[k] = cmul(v[k], w);
+      }
+    }
+    DFT<R, DIR>::run(v);
+#pragma unroll
+    for (int k = 0; k < R; ++k) sink(b + k * (NS / P), j + k * NS, v[k]);
+  }
+}
+
+// First Stockham stage with inputs from registers: butterfly j = t + b*P reads positions
+// j + k*NB = t + P*(b + k*NB/P), i.e. register slot e = b + k*NB/P of the same layout.
+template <int L, int R, int P, int DIR, typename Get, typename Dst>
+__device__ __forceinline__ void first_stage_from_regs(int t, Get get, Dst dst) {
+  constexpr int NB = L / R;
+  constexpr int B = NB / P;
+  static_assert(NB % P == 0, "first stage shape");
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + b * P;
+    float2 v[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = get(b + k * (NB / P));
+    DFT<R, DIR>::run(v);
+#pragma unroll
+    for (int k = 0; k < R; ++k) dst(j * R + k, v[k]);
+  }
+}
+
+// Inverse pencil FFT whose final stage accumulates into registers (LDS source, all stages but the
+// last in LDS).  Contains barriers: every thread of the block calls it.
+template <int L, typename Lds, typename Sink>
+__device__ __forceinline__ void pencil_inverse_to_regs(int t, Lds lds, Sink sink, const float2* tw) {
+  using D = Decomp<L>;
+  constexpr int P = D::P;
+  if constexpr (D::R1 == 1) {
+    last_stage_to_regs<L, D::R0, 1, P, +1>(t, lds, sink, tw);
+  } else {
+    static_assert(D::R2 == 1, "row transforms use at most two stages");
+    stage<L, D::R0, 1, P, +1>(t, lds, lds, tw, true);
+    __syncthreads();
+    last_stage_to_regs<L, D::R1, D::R0, P, +1>(t, lds, sink, tw);
+  }
+}
+
+// Forward pencil FFT whose first stage reads registers; result left in LDS (natural order).
+template <int L, typename Get, typename Lds>
+__device__ __forceinline__ void pencil_forward_from_regs(int t, Get get, Lds lds, const float2* tw) {
+  using D = Decomp<L>;
+  constexpr int P = D::P;
+  first_stage_from_regs<L, D::R0, P, -1>(t, get, lds);
+  if constexpr (D::R1 != 1) {
+    static_assert(D::R2 == 1, "row transforms use at most two stages");
+    __syncthreads();
+    stage<L, D::R1, D::R0, P, -1>(t, lds, lds, tw, true);
+  }
+}
+
+template <int L> struct ColCfg {
+  static constexpr int T = (PB3D_COL_THREADS / Decomp<L>::P) > 8 ? PB3D_COL_THREADS / Decomp<L>::P : 8;
+};
 template <> struct ColCfg<2>    { static constexpr int T = 256; };
 template <> struct ColCfg<4>    { static constexpr int T = 256; };
 template <> struct ColCfg<8>    { static constexpr int T = 128; };
 template <> struct ColCfg<16>   { static constexpr int T = 128; };
 
 // ------------------------------------------------------------------------------------------
-// Column pass along axis 0 or 1 of the half-complex grid.
-//   column c in [0, ncols): base(c) = (c / cpp) * plane + (c % cpp); element p at base + p*S.
-//   MODE 0: in-place/out-of-place plain.  MODE 1: first C2R pass, input through KOp (broadcast
-//   by x_stride).  MODE 2: last R2C pass, output through KOp (may accumulate).
+// Column pass along axis 0 or 1 of the half-complex grid, optionally pruned to a band limit.
+//   active column c in [0, ncols): outer(q) = q < split ? q : q + gap,
+//   base(c) = outer(c / cpp) * plane + (c % cpp); element p of the pencil at base + p*S.
+//   Elements with p outside the radius rad_in (|fs(p)| > rad_in) are known zeros (not read);
+//   outputs outside rad_out are not needed (not written).  rad < 0 = whole pencil.
+//   MODE 0: plain.  MODE 1: first C2R pass, input through KOp (broadcast by x_stride).
+//   MODE 2: last R2C pass, output through KOp (may accumulate / pack).
 // ------------------------------------------------------------------------------------------
+struct ColMap {
+  int ncols, cpp, split, gap;
+  long long plane, S;
+  int rad_in, rad_out;
+};
+
+__device__ __forceinline__ bool in_radius(int p, int L, int rad) {
+  return rad < 0 || p <= rad || p >= L - rad;
+}
+
 template <int L, int DIR, int MODE>
 __global__ void __launch_bounds__(Decomp<L>::P * ColCfg<L>::T)
-fft_cols(const float2* in, float2* out, int ncols, int cpp, long long plane,
-         long long S, long long in_bstride, long long out_bstride, const float2* __restrict__ tw, Geo geo,
-         KOp kop) {
-  constexpr int P = Decomp<L>::P;
+fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long long out_bstride,
+         const float2* __restrict__ tw, Geo geo, KOp kop) {
   constexpr int T = ColCfg<L>::T;
   constexpr int LD = T + 1;
   __shared__ float2 lds[(Decomp<L>::R1 == 1) ? 1 : L * LD];
+  __shared__ float2 stw[L];
   const int tid = threadIdx.x;
+  // twiddles to LDS; first used after the barrier that follows stage 0 (stage 0 needs none)
+  for (int i = tid; i < L; i += Decomp<L>::P * T) stw[i] = tw[i];
   const int col = tid % T;
   const int t = tid / T;
   const long long bt = blockIdx.y;
   const int c = blockIdx.x * T + col;
-  const bool valid = c < ncols;
-  const long long base = valid ? (long long)(c / cpp) * plane + (c % cpp) : 0;
+  const bool valid = c < cm.ncols;
+  long long base = 0;
+  if (valid) {
+    int q = c / cm.cpp;
+    q = q < cm.split ? q : q + cm.gap;
+    base = (long long)q * cm.plane + (c % cm.cpp);
+  }
   const float2* src = in + bt * in_bstride;
   float2* dst = out + bt * out_bstride;
-  // mode coordinates for KOp (only the x-column pass uses them: axis 0, cpp = n1*n2h)
+  // mode coordinates for KOp (x-column pass only: base = j*n2h + l)
   int mj = 0, ml = 0;
   if (MODE != 0) { mj = (int)(base / geo.n2h); ml = (int)(base % geo.n2h); }
 
   auto load = [&](int p) -> float2 {
-    if (!valid) return make_float2(0.f, 0.f);
+    if (!valid || !in_radius(p, L, cm.rad_in)) return make_float2(0.f, 0.f);
     if (MODE == 1 && kop.kind == 4)  // BKQ assembles the map from packed rows; no input read
-      return kop_value(kop, geo, bt, base + p * S, p, mj, ml, make_float2(0.f, 0.f));
-    float2 v = src[base + p * S];
-    if (MODE == 1) v = kop_value(kop, geo, bt, base + p * S, p, mj, ml, v);
+      return kop_value(kop, geo, bt, base + p * cm.S, p, mj, ml, make_float2(0.f, 0.f));
+    float2 v = src[base + p * cm.S];
+    if (MODE == 1) v = kop_value(kop, geo, bt, base + p * cm.S, p, mj, ml, v);
     return v;
   };
   auto store = [&](int p, float2 v) {
-    if (!valid) return;
+    if (!valid || !in_radius(p, L, cm.rad_out)) return;
     if (MODE == 2) {
-      long long md = base + p * S;
+      long long md = base + p * cm.S;
       if (kop.kind == 5) {  // packed output (in-bin modes only)
         const int pi = kop.pack_index[md];
         if (pi < 0) return;
@@ -198,7 +301,7 @@ fft_cols(const float2* in, float2* out, int ncols, int cpp, long long plane,
       if (kop.accumulate) v = cadd(v, dst[md]);
       dst[md] = v;
     } else {
-      dst[base + p * S] = v;
+      dst[base + p * cm.S] = v;
     }
   };
   struct LdsRW {
@@ -206,13 +309,18 @@ fft_cols(const float2* in, float2* out, int ncols, int cpp, long long plane,
     __device__ float2 operator()(int p) const { return l[p * LDv + colv]; }
     __device__ void operator()(int p, float2 v) const { l[p * LDv + colv] = v; }
   } lrw{lds, LD, col};
-  pencil_fft<L, DIR>(t, load, lrw, store, tw, false);
+  pencil_fft<L, DIR>(t, load, lrw, store, stw, false);
 }
 
 // ------------------------------------------------------------------------------------------
 // Row pass (axis 2).  R2C: real rows of n2 = 2M floats packed as M complex, FFT_M, then split
-// into the n2/2+1 half spectrum.  C2R: the inverse (numpy/pocketfft semantics: Im of the kz=0
-// and kz=M entries ignored), with 1/N and the real-space epilogue fused into the store.
+// into the n2/2+1 half spectrum (only kz < lz_out written).  C2R: the inverse (numpy/pocketfft
+// semantics: Im of the kz=0 and kz=M entries ignored; entries kz >= lz_in known zero), with 1/N
+// and the real-space epilogue fused into the store.
+// LDS holds a tile of T rows transposed, [position][row] with the row index fastest, and the
+// lanes of a wave walk the rows: every butterfly read and write is then bank-conflict free (the
+// layout of the column kernel).  Global rows are contiguous, so loads and stores stage through
+// the tile with coalesced flat indexing.  Twiddles are copied to LDS once per block.
 // ------------------------------------------------------------------------------------------
 template <int M> struct RowCfg { static constexpr int T = (256 / Decomp<M>::P) > 8 ? 256 / Decomp<M>::P : 8; };
 template <> struct RowCfg<2>  { static constexpr int T = 128; };
@@ -220,16 +328,80 @@ template <> struct RowCfg<4>  { static constexpr int T = 128; };
 template <> struct RowCfg<8>  { static constexpr int T = 64; };
 template <> struct RowCfg<16> { static constexpr int T = 64; };
 
+// accessor for row r of a transposed tile: element p at l[p * LDT + r]
+struct RowT {
+  float2* l; int LDT; int r;
+  __device__ float2 operator()(int p) const { return l[p * LDT + r]; }
+  __device__ void operator()(int p, float2 v) const { l[p * LDT + r] = v; }
+};
+
+// LDS pitch of the transposed tile: T rows (+1 to spread the flat staging accesses)
+template <int M> struct RowTile {
+  static constexpr int T = RowCfg<M>::T;
+  static constexpr int P = Decomp<M>::P;
+  static constexpr int NT = P * T;
+  static constexpr int LDT = T + 1;
+  static constexpr int SIZE = (M + 1) * LDT;
+};
+
 template <int M>
-__global__ void __launch_bounds__(Decomp<M>::P * RowCfg<M>::T)
+__device__ __forceinline__ void stage_twiddles(float2* stw, float2* stwN, const float2* __restrict__ tw,
+                                               const float2* __restrict__ twN, int tid, int nt) {
+  for (int i = tid; i < M; i += nt) stw[i] = tw[i];
+  for (int i = tid; i <= M; i += nt) stwN[i] = twN[i];
+}
+
+// X[k] = E[k] + W^k O[k], E = (Z[k]+conj Z[M-k])/2, O = (Z[k]-conj Z[M-k])/(2i)   (row r of a tile)
+template <int M>
+__device__ __forceinline__ float2 r2c_split(const float2* b, int LDT, int r, int k, const float2* twN) {
+  const float2 zk = b[(k == M ? 0 : k) * LDT + r];
+  const float2 zm = cconj(b[(k == 0 ? 0 : M - k) * LDT + r]);
+  const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+  const float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y));
+  const float2 od = make_float2(d.y, -d.x);
+  return cadd(e, cmul(twN[k], od));
+}
+
+// Load a tile of T half-spectrum rows from global memory and apply the C2R pre-pass on the fly
+// (pairs (k, M-k) read together, Z' written to the transposed LDS tile): one staging step, no
+// separate LDS pass.  Entries kz >= lz_in are known zeros; Im of kz=0 and kz=M is ignored.
+template <int M, int T, int NT>
+__device__ __forceinline__ void load_prepass(float2* buf, int LDT, const float2* __restrict__ src,
+                                             long long rows_left, int lz_in, int tid, const float2* twN) {
+  for (int f = tid; f < T * (M / 2 + 1); f += NT) {
+    const int rr = f / (M / 2 + 1), k = f % (M / 2 + 1);
+    float2 xk = make_float2(0.f, 0.f), xm = make_float2(0.f, 0.f);
+    if (rr < rows_left) {
+      if (k < lz_in) xk = src[rr * (M + 1) + k];
+      if (M - k < lz_in) xm = src[rr * (M + 1) + M - k];
+    }
+    if (k == 0) { xk.y = 0.f; xm.y = 0.f; }   // (k, M) pair: both self-conjugate entries
+    if (2 * k == M) xm = xk;
+    {
+      const float2 sm = cadd(xk, cconj(xm));
+      const float2 d = csub(xk, cconj(xm));
+      const float2 wd = cmul(cconj(twN[k]), d);
+      buf[k * LDT + rr] = make_float2(sm.x - wd.y, sm.y + wd.x);
+    }
+    if (k != 0 && 2 * k != M) {
+      const float2 sm = cadd(xm, cconj(xk));
+      const float2 d = csub(xm, cconj(xk));
+      const float2 wd = cmul(cconj(twN[M - k]), d);
+      buf[(M - k) * LDT + rr] = make_float2(sm.x - wd.y, sm.y + wd.x);
+    }
+  }
+}
+
+template <int M>
+__global__ void __launch_bounds__(RowTile<M>::NT)
 r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows, int rows_per_batch,
-         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop) {
-  constexpr int P = Decomp<M>::P;
-  constexpr int T = RowCfg<M>::T;
-  constexpr int NT = P * T;
-  constexpr int LD = M + 1;
-  __shared__ float2 buf[T * LD];
+         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop, int lz_out) {
+  using RT = RowTile<M>;
+  constexpr int T = RT::T, NT = RT::NT, LDT = RT::LDT;
+  __shared__ float2 buf[RT::SIZE];
+  __shared__ float2 stw[M], stwN[M + 1];
   const int tid = threadIdx.x;
+  stage_twiddles<M>(stw, stwN, tw, twN, tid, NT);
   const long long row0 = (long long)blockIdx.x * T;
   // 1. coalesced load of T rows (T*M float2) with the voxel prologue
   for (int f = tid; f < T * M; f += NT) {
@@ -249,80 +421,41 @@ r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows
       }
       v = make_float2(x0 * rop.scale, x1 * rop.scale);
     }
-    buf[r * LD + m] = v;
+    buf[m * LDT + r] = v;
   }
   __syncthreads();
-  // 2. complex FFT of each row in LDS
-  const int r = tid / P, t = tid % P;
-  struct RowRW {
-    float2* l;
-    __device__ float2 operator()(int p) const { return l[p]; }
-    __device__ void operator()(int p, float2 v) const { l[p] = v; }
-  } rw{buf + r * LD};
-  pencil_fft<M, -1>(t, rw, rw, rw, tw, true);
+  // 2. complex FFT of each row in LDS (lanes walk the rows)
+  const int r = tid % T, t = tid / T;
+  RowT rw{buf, LDT, r};
+  pencil_fft<M, -1>(t, rw, rw, rw, stw, true);
   __syncthreads();
-  // 3. split: X[k] = E[k] + W^k O[k], E = (Z[k]+conj Z[M-k])/2, O = (Z[k]-conj Z[M-k])/(2i)
+  // 3. split into the half spectrum
   float2* o = out + row0 * (M + 1);
   for (int f = tid; f < T * (M + 1); f += NT) {
     const int rr = f / (M + 1), k = f % (M + 1);
-    if (row0 + rr >= nrows) continue;
-    const float2 zk = buf[rr * LD + (k == M ? 0 : k)];
-    const float2 zm = cconj(buf[rr * LD + (k == 0 ? 0 : M - k)]);
-    const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
-    const float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y));
-    const float2 od = make_float2(d.y, -d.x);  // d / i
-    o[f] = cadd(e, cmul(twN[k], od));
+    if (row0 + rr >= nrows || k >= lz_out) continue;
+    o[f] = r2c_split<M>(buf, LDT, rr, k, stwN);
   }
 }
 
 template <int M>
-__global__ void __launch_bounds__(Decomp<M>::P * RowCfg<M>::T)
+__global__ void __launch_bounds__(RowTile<M>::NT)
 c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows, int rows_per_batch,
-         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop, float norm) {
-  constexpr int P = Decomp<M>::P;
-  constexpr int T = RowCfg<M>::T;
-  constexpr int NT = P * T;
-  constexpr int LD = M + 1;
-  __shared__ float2 buf[T * LD];
+         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop, float norm, int lz_in) {
+  using RT = RowTile<M>;
+  constexpr int T = RT::T, NT = RT::NT, LDT = RT::LDT;
+  __shared__ float2 buf[RT::SIZE];
+  __shared__ float2 stw[M], stwN[M + 1];
   const int tid = threadIdx.x;
+  stage_twiddles<M>(stw, stwN, tw, twN, tid, NT);
   const long long row0 = (long long)blockIdx.x * T;
   const float2* src = in + row0 * (M + 1);
-  for (int f = tid; f < T * (M + 1); f += NT) {
-    const int rr = f / (M + 1), k = f % (M + 1);
-    float2 v = make_float2(0.f, 0.f);
-    if (row0 + rr < nrows) v = src[f];
-    if (k == 0 || k == M) v.y = 0.f;
-    buf[rr * LD + k] = v;
-  }
+  __syncthreads();  // twiddles staged
+  load_prepass<M, T, NT>(buf, LDT, src, nrows - row0, lz_in, tid, stwN);
   __syncthreads();
-  // Z'[k] = (X[k] + conj X[M-k]) + i W^{-k} (X[k] - conj X[M-k]),  k = 0..M-1; pairs (k, M-k)
-  for (int f = tid; f < T * (M / 2 + 1); f += NT) {
-    const int rr = f / (M / 2 + 1), k = f % (M / 2 + 1);
-    float2* b = buf + rr * LD;
-    const float2 xk = b[k], xm = b[M - k];
-    {
-      const float2 s = cadd(xk, cconj(xm));
-      const float2 d = csub(xk, cconj(xm));
-      const float2 w = cconj(twN[k]);
-      const float2 wd = cmul(w, d);
-      b[k] = make_float2(s.x - wd.y, s.y + wd.x);
-    }
-    if (k != 0 && 2 * k != M) {
-      const float2 s = cadd(xm, cconj(xk));
-      const float2 d = csub(xm, cconj(xk));
-      const float2 w = cconj(twN[M - k]);
-      const float2 wd = cmul(w, d);
-      b[M - k] = make_float2(s.x - wd.y, s.y + wd.x);
-    }
-  }
-  __syncthreads();
-  const int r = tid / P, t = tid % P;
-  struct RowRW {
-    float2* l;
-    __device__ float2 operator()(int p) const { return l[p]; }
-    __device__ void operator()(int p, float2 v) const { l[p] = v; }
-  } rw{buf + r * LD};
-  pencil_fft<M, +1>(t, rw, rw, rw, tw, true);
+  const int r = tid % T, t = tid / T;
+  RowT rw{buf, LDT, r};
+  pencil_fft<M, +1>(t, rw, rw, rw, stw, true);
   __syncthreads();
   for (int f = tid; f < T * M; f += NT) {
     const int rr = f / M, m = f % M;
@@ -331,7 +464,7 @@ c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows
     const long long bt = row / rows_per_batch;
     const long long rw_ = row % rows_per_batch;
     const long long vox = rw_ * (2 * M) + 2 * m;
-    const float2 z = buf[rr * LD + m];
+    const float2 z = buf[m * LDT + rr];
     float x0 = z.x * norm, x1 = z.y * norm;
     float* o = out + bt * (long long)rows_per_batch * 2 * M;
     if (rop.kind != 0) {
@@ -345,6 +478,123 @@ c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows
       x0 += prev.x; x1 += prev.y;
     }
     *reinterpret_cast<float2*>(o + vox) = make_float2(x0, x1);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Fused pointing row kernel: for output map t and row (i0,i1): invert the rows of all nsub
+// sub-maps (work[g*batch + t]), accumulate coef_g * Y_g(x_hat) * z in registers, multiply by the
+// mask and scale, forward-transform the row and write kz < lz_out into work[t] (in place: a block
+// owns its rows in every sub-map).
+// ------------------------------------------------------------------------------------------
+constexpr int kMaxSub = 16;
+constexpr int kMaxMono = 15;  // monomials of a homogeneous polynomial of degree <= 4
+// Real-space weight of sub-map g: coef * Y_{ell,m}(r_hat) written as the homogeneous polynomial
+// sum_{k<=ell} sum_{a<=ell-k} poly[idx] rx^a ry^(ell-k-a) rz^k / |r|^ell (idx runs over (k, a) in
+// that order); y0 = Y(0) at r = 0, where the reference's zero-vector convention applies.
+struct SubW {
+  int n;
+  int ell[kMaxSub];
+  float coef[kMaxSub];
+  float y0[kMaxSub];
+  float poly[kMaxSub][kMaxMono];
+};
+
+template <int M>
+__global__ void __launch_bounds__(RowTile<M>::NT, PB3D_POINT_WAVES)
+pointing_rows(float2* work, long long rows_per_map, int batch, SubW sw, const float* __restrict__ mask,
+              float scale, int lz_in, int lz_out, const float2* __restrict__ tw, const float2* __restrict__ twN,
+              Geo geo) {
+  using RT = RowTile<M>;
+  constexpr int T = RT::T, NT = RT::NT, LDT = RT::LDT;
+  constexpr int E = (T * M) / NT;  // values per thread (<= 16)
+  __shared__ float2 buf[RT::SIZE];
+  __shared__ float2 stw[M], stwN[M + 1];
+  const int tid = threadIdx.x;
+  stage_twiddles<M>(stw, stwN, tw, twN, tid, NT);
+  const long long t = blockIdx.y;
+  const long long row0 = (long long)blockIdx.x * T;
+  const long long H = geo.H;
+  const int r = tid % T, tp = tid / T;
+  RowT rw{buf, LDT, r};
+  // this thread's voxels after the inverse transforms: positions m = tp + e*P of row r
+  const long long row = row0 + r;
+  const long long rowc = row < rows_per_map ? row : rows_per_map - 1;
+  const int i0 = (int)(rowc / geo.n1), i1 = (int)(rowc % geo.n1);
+  const float rx = freq_index(i0, geo.n0) * geo.dx0 + geo.c0;
+  const float ry = freq_index(i1, geo.n1) * geo.dx1 + geo.c1;
+  const float rxy2 = rx * rx + ry * ry;
+  float acc[E][2];
+#pragma unroll
+  for (int e = 0; e < E; ++e) { acc[e][0] = 0.f; acc[e][1] = 0.f; }
+  for (int g = 0; g < sw.n; ++g) {
+    const float2* src = work + ((long long)g * batch + t) * H + row0 * (M + 1);
+    __syncthreads();  // previous readers of buf are done, twiddles staged
+    load_prepass<M, T, NT>(buf, LDT, src, rows_per_map - row0, lz_in, tid, stwN);
+    __syncthreads();
+    const int ell = sw.ell[g];
+    const float cf = sw.coef[g];
+    // z-polynomial coefficients of this row: D[k] = sum_a poly rx^a ry^(ell-k-a)
+    float D[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if (ell > 0) {
+      int idx = 0;
+      for (int k = 0; k <= ell; ++k) {
+        float sacc = 0.f;
+        for (int a = 0; a <= ell - k; ++a) {
+          float mono = 1.f;
+          for (int q = 0; q < a; ++q) mono *= rx;
+          for (int q = 0; q < ell - k - a; ++q) mono *= ry;
+          sacc += sw.poly[g][idx++] * mono;
+        }
+        if (k == 0) D[0] = sacc; else if (k == 1) D[1] = sacc; else if (k == 2) D[2] = sacc;
+        else if (k == 3) D[3] = sacc; else D[4] = sacc;
+      }
+    }
+    const float y0 = sw.y0[g];
+    // inverse row transform; its last stage adds w(x) * z straight into the accumulators
+    auto sink = [&](int e, int m, float2 z) {
+      float w[2] = {cf, cf};
+      if (ell != 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const float rz = freq_index(2 * m + h, geo.n2) * geo.dx2 + geo.c2;
+          const float r2 = rxy2 + rz * rz;
+          float pz = D[4];
+          pz = pz * rz + D[3];
+          pz = pz * rz + D[2];
+          pz = pz * rz + D[1];
+          pz = pz * rz + D[0];
+          const float ri = r2 > 0.f ? rsqrtf(r2) : 0.f;
+          float rl = 1.f;
+          for (int q = 0; q < ell; ++q) rl *= ri;
+          w[h] *= r2 > 0.f ? pz * rl : y0;
+        }
+      }
+      acc[e][0] += z.x * w[0];
+      acc[e][1] += z.y * w[1];
+    };
+    pencil_inverse_to_regs<M>(tp, rw, sink, stw);
+  }
+  __syncthreads();  // all reads of buf by the last inverse stage are done
+  // mask and scale in registers, then the forward row transform (first stage from registers)
+  const float norm = scale / (float)geo.N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int m = tp + e * RT::P;
+    acc[e][0] *= norm; acc[e][1] *= norm;
+    if (mask && row < rows_per_map) {
+      const float2 mk = *reinterpret_cast<const float2*>(mask + row * (2 * M) + 2 * m);
+      acc[e][0] *= mk.x; acc[e][1] *= mk.y;
+    }
+  }
+  auto get = [&](int e) { return make_float2(acc[e][0], acc[e][1]); };
+  pencil_forward_from_regs<M>(tp, get, rw, stw);
+  __syncthreads();
+  float2* o = work + t * H + row0 * (M + 1);
+  for (int f = tid; f < T * (M + 1); f += NT) {
+    const int rr = f / (M + 1), k = f % (M + 1);
+    if (row0 + rr >= rows_per_map || k >= lz_out) continue;
+    o[f] = r2c_split<M>(buf, LDT, rr, k, stwN);
   }
 }
 
@@ -403,15 +653,14 @@ struct pb3d_plan {
 namespace pb3d {
 
 template <int DIR, int MODE>
-static hipError_t launch_cols(int L, const float2* in, float2* out, int ncols, int cpp, long long plane,
-                              long long S, long long in_bs, long long out_bs, int batch, const float2* tw,
-                              const Geo& geo, const KOp& kop, hipStream_t s) {
+static hipError_t launch_cols(int L, const float2* in, float2* out, const ColMap& cm, long long in_bs, long long out_bs,
+                              int batch, const float2* tw, const Geo& geo, const KOp& kop, hipStream_t s) {
+  if (cm.ncols <= 0) return hipSuccess;
 #define PB3D_COLS(LL)                                                                          \
   case LL: {                                                                                   \
     constexpr int T = ColCfg<LL>::T;                                                           \
-    dim3 grid((ncols + T - 1) / T, batch);                                                     \
-    fft_cols<LL, DIR, MODE><<<grid, Decomp<LL>::P * T, 0, s>>>(in, out, ncols, cpp, plane, S,  \
-                                                             in_bs, out_bs, tw, geo, kop);     \
+    dim3 grid((cm.ncols + T - 1) / T, batch);                                                  \
+    fft_cols<LL, DIR, MODE><<<grid, Decomp<LL>::P * T, 0, s>>>(in, out, cm, in_bs, out_bs, tw, geo, kop); \
     return hipGetLastError();                                                                  \
   }
   switch (L) {
@@ -423,13 +672,13 @@ static hipError_t launch_cols(int L, const float2* in, float2* out, int ncols, i
 }
 
 static hipError_t launch_r2c_rows(int M, const float* in, float2* out, long long nrows, int rpb,
-                                  const float2* tw, const float2* twN, const Geo& geo, const ROp& rop,
+                                  const float2* tw, const float2* twN, const Geo& geo, const ROp& rop, int lz_out,
                                   hipStream_t s) {
 #define PB3D_ROWS(MM)                                                                            \
   case MM: {                                                                                     \
     constexpr int T = RowCfg<MM>::T;                                                             \
-    r2c_rows<MM><<<(unsigned)((nrows + T - 1) / T), Decomp<MM>::P * T, 0, s>>>(in, out, nrows, rpb, tw, \
-                                                                              twN, geo, rop);    \
+    r2c_rows<MM><<<(unsigned)((nrows + T - 1) / T), RowTile<MM>::NT, 0, s>>>(in, out, nrows, rpb, tw, \
+                                                                              twN, geo, rop, lz_out); \
     return hipGetLastError();                                                                    \
   }
   switch (M) {
@@ -442,12 +691,12 @@ static hipError_t launch_r2c_rows(int M, const float* in, float2* out, long long
 
 static hipError_t launch_c2r_rows(int M, const float2* in, float* out, long long nrows, int rpb,
                                   const float2* tw, const float2* twN, const Geo& geo, const ROp& rop,
-                                  float norm, hipStream_t s) {
+                                  float norm, int lz_in, hipStream_t s) {
 #define PB3D_ROWS(MM)                                                                            \
   case MM: {                                                                                     \
     constexpr int T = RowCfg<MM>::T;                                                             \
-    c2r_rows<MM><<<(unsigned)((nrows + T - 1) / T), Decomp<MM>::P * T, 0, s>>>(in, out, nrows, rpb, tw, \
-                                                                              twN, geo, rop, norm); \
+    c2r_rows<MM><<<(unsigned)((nrows + T - 1) / T), RowTile<MM>::NT, 0, s>>>(in, out, nrows, rpb, tw, \
+                                                                              twN, geo, rop, norm, lz_in); \
     return hipGetLastError();                                                                    \
   }
   switch (M) {
@@ -456,6 +705,112 @@ static hipError_t launch_c2r_rows(int M, const float2* in, float* out, long long
     default: return hipErrorInvalidValue;
   }
 #undef PB3D_ROWS
+}
+
+static hipError_t launch_pointing_rows(int M, float2* work, long long rpm, int batch, const SubW& sw,
+                                       const float* mask, float scale, int lz_in, int lz_out, const float2* tw,
+                                       const float2* twN, const Geo& geo, hipStream_t s) {
+#define PB3D_PROWS(MM)                                                                           \
+  case MM: {                                                                                     \
+    constexpr int T = RowCfg<MM>::T;                                                             \
+    dim3 grid((unsigned)((rpm + T - 1) / T), batch);                                             \
+    pointing_rows<MM><<<grid, RowTile<MM>::NT, 0, s>>>(work, rpm, batch, sw, mask, scale, lz_in, lz_out, \
+                                                         tw, twN, geo);                          \
+    return hipGetLastError();                                                                    \
+  }
+  switch (M) {
+    PB3D_PROWS(2) PB3D_PROWS(4) PB3D_PROWS(8) PB3D_PROWS(16) PB3D_PROWS(32) PB3D_PROWS(64)
+    PB3D_PROWS(128) PB3D_PROWS(256)
+    default: return hipErrorInvalidValue;
+  }
+#undef PB3D_PROWS
+}
+
+// Homogeneous-polynomial form of Y_{ell,m} (degree ell <= 4) on the unit sphere, fitted by least
+// squares on fixed sample directions in FP64 (exact up to rounding: the homogeneous polynomials of
+// degree ell restricted to the sphere are linearly independent).  Coefficient order: z power k
+// outer, x power a inner, y power ell-k-a.
+static bool ylm_poly(int ell, int m, float* coef) {
+  int K = 0;
+  int pa[kMaxMono], pb[kMaxMono], pk[kMaxMono];
+  for (int k = 0; k <= ell; ++k)
+    for (int a = 0; a <= ell - k; ++a) { pa[K] = a; pb[K] = ell - k - a; pk[K] = k; ++K; }
+  double A[kMaxMono][kMaxMono] = {}, rhs[kMaxMono] = {};
+  const int ns = 97;
+  for (int s = 0; s < ns; ++s) {
+    // deterministic spread of directions (spherical Fibonacci lattice)
+    const double zz = 1.0 - (2.0 * s + 1.0) / ns, rr = std::sqrt(1.0 - zz * zz);
+    const double ph = 2.399963229728653 * s;
+    const double x = rr * std::cos(ph), y = rr * std::sin(ph), z = zz;
+    double mono[kMaxMono];
+    for (int i = 0; i < K; ++i) mono[i] = std::pow(x, pa[i]) * std::pow(y, pb[i]) * std::pow(z, pk[i]);
+    const double yv = (double)ylm_f(ell, m, (float)x, (float)y, (float)z);
+    for (int i = 0; i < K; ++i) {
+      rhs[i] += mono[i] * yv;
+      for (int j = 0; j < K; ++j) A[i][j] += mono[i] * mono[j];
+    }
+  }
+  // Gaussian elimination with partial pivoting
+  for (int c = 0; c < K; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < K; ++r) if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+    if (std::fabs(A[piv][c]) < 1e-12) return false;
+    for (int j = 0; j < K; ++j) std::swap(A[c][j], A[piv][j]);
+    std::swap(rhs[c], rhs[piv]);
+    for (int r = 0; r < K; ++r) {
+      if (r == c) continue;
+      const double f = A[r][c] / A[c][c];
+      for (int j = 0; j < K; ++j) A[r][j] -= f * A[c][j];
+      rhs[r] -= f * rhs[c];
+    }
+  }
+  for (int i = 0; i < K; ++i) coef[i] = (float)(rhs[i] / A[i][i]);
+  return true;
+}
+
+// Band-limit bookkeeping: radii r0, r1 on the two complex axes (-1 = whole axis) and the number of
+// kz planes lz in the support.
+struct Prune { int r0, r1, lz; };
+static Prune make_prune(const Geo& g, const pb3d_kop* k) {
+  Prune pr{-1, -1, g.n2h};
+  if (!k) return pr;
+  if (k->support[0] >= 0 && 2 * k->support[0] < g.n0) pr.r0 = k->support[0];
+  if (k->support[1] >= 0 && 2 * k->support[1] < g.n1) pr.r1 = k->support[1];
+  if (k->support[2] >= 0 && k->support[2] + 1 < g.n2h) pr.lz = k->support[2] + 1;
+  return pr;
+}
+static Prune merge_prune(const Geo& g, Prune a, Prune b) {
+  Prune o;
+  o.r0 = (a.r0 < 0 || b.r0 < 0) ? -1 : (a.r0 > b.r0 ? a.r0 : b.r0);
+  o.r1 = (a.r1 < 0 || b.r1 < 0) ? -1 : (a.r1 > b.r1 ? a.r1 : b.r1);
+  o.lz = a.lz > b.lz ? a.lz : b.lz;
+  (void)g;
+  return o;
+}
+// x-column pass (axis 0) over the active (j, l) columns
+static ColMap xcols(const Geo& g, const Prune& pr, bool c2r) {
+  ColMap cm{};
+  cm.S = (long long)g.n1 * g.n2h;
+  if (pr.r1 < 0 && pr.lz == g.n2h) {
+    cm.ncols = g.n1 * g.n2h; cm.cpp = g.n1 * g.n2h; cm.plane = 0; cm.split = 1 << 30; cm.gap = 0;
+  } else {
+    const int J = pr.r1 < 0 ? g.n1 : 2 * pr.r1 + 1;
+    cm.ncols = J * pr.lz; cm.cpp = pr.lz; cm.plane = g.n2h;
+    cm.split = pr.r1 < 0 ? (1 << 30) : pr.r1 + 1;
+    cm.gap = pr.r1 < 0 ? 0 : g.n1 - J;
+  }
+  cm.rad_in = c2r ? pr.r0 : -1;
+  cm.rad_out = c2r ? -1 : pr.r0;
+  return cm;
+}
+// y-column pass (axis 1) over the active (i, l) columns
+static ColMap ycols(const Geo& g, const Prune& pr, bool c2r) {
+  ColMap cm{};
+  cm.S = g.n2h;
+  cm.ncols = g.n0 * pr.lz; cm.cpp = pr.lz; cm.plane = (long long)g.n1 * g.n2h; cm.split = 1 << 30; cm.gap = 0;
+  cm.rad_in = c2r ? pr.r1 : -1;
+  cm.rad_out = c2r ? -1 : pr.r1;
+  return cm;
 }
 
 static float2* upload_twiddles(int L, int count) {
@@ -566,7 +921,7 @@ extern "C" int pb3d_plan_destroy(pb3d_plan* p) {
 
 extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, void* work_, int batch,
                             const pb3d_rop* rop_, const pb3d_kop* kop_, void* stream) {
-  if (!p || !out_ || batch < 1) return fail(PB3D_ERR_ARG, "r2c: null plan/output or batch < 1");
+  if (!p || !out_ || batch < 1 || batch > 65535) return fail(PB3D_ERR_ARG, "r2c: null plan/output or batch out of [1, 65535]");
   if (!in && !(rop_ && rop_->a)) return fail(PB3D_ERR_ARG, "r2c: no input");
   if (int e = check_rop(rop_, true)) return e;
   if (int e = check_kop(kop_)) return e;
@@ -580,22 +935,19 @@ extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, voi
   ROp rop = to_rop(rop_);
   KOp kop = to_kop(kop_);
   KOp plain = to_kop(nullptr);
+  const Prune pr = make_prune(g, kop_);
   const long long rows = (long long)batch * g.n0 * g.n1;
   const double bc = (double)batch * g.H * 16.0, br = (double)batch * (g.N * 4.0 + g.H * 8.0);
   ProfSample sm{};
   bool on = prof_begin(0, s, sm, br);
-  PB3D_CHECK_HIP(launch_r2c_rows(p->L[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, s));
+  PB3D_CHECK_HIP(launch_r2c_rows(p->L[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, pr.lz, s));
   prof_end(on, sm, s);
-  // y columns: cols = n0*n2h, cpp = n2h, plane = n1*n2h, stride n2h (in place in work)
   on = prof_begin(2, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, g.n0 * g.n2h, g.n2h, (long long)g.n1 * g.n2h,
-                                     g.n2h, g.H, g.H, batch, p->tw[1], g, plain, s)));
+  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, ycols(g, pr, false), g.H, g.H, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
-  // x columns: cols = n1*n2h, stride n1*n2h, epilogue kop into out (packed rows for PACK)
   const long long out_bs = (kop_ && kop_->kind == 5) ? kop_->packed_len : g.H;
   on = prof_begin(4, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, out, g.n1 * g.n2h, g.n1 * g.n2h, 0,
-                                     (long long)g.n1 * g.n2h, g.H, out_bs, batch, p->tw[0], g, kop, s)));
+  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, out, xcols(g, pr, false), g.H, out_bs, batch, p->tw[0], g, kop, s)));
   prof_end(on, sm, s);
   g_fwd += batch;
   return 0;
@@ -603,7 +955,7 @@ extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, voi
 
 extern "C" int pb3d_fft_c2r(const pb3d_plan* p, const void* in_, float* out, void* work_, int batch,
                             const pb3d_kop* kop_, const pb3d_rop* rop_, void* stream) {
-  if (!p || !out || !work_ || batch < 1) return fail(PB3D_ERR_ARG, "c2r: null pointer or batch < 1");
+  if (!p || !out || !work_ || batch < 1 || batch > 65535) return fail(PB3D_ERR_ARG, "c2r: null pointer or batch out of [1, 65535]");
   if (!in_ && !(kop_ && kop_->kind == 4)) return fail(PB3D_ERR_ARG, "c2r: null input");
   if (in_ == work_) return fail(PB3D_ERR_ARG, "c2r: work must not alias the input");
   if (kop_ && kop_->kind == 5) return fail(PB3D_ERR_ARG, "c2r: PACK is an R2C epilogue");
@@ -616,22 +968,86 @@ extern "C" int pb3d_fft_c2r(const pb3d_plan* p, const void* in_, float* out, voi
   ROp rop = to_rop(rop_);
   KOp kop = to_kop(kop_);
   KOp plain = to_kop(nullptr);
+  const Prune pr = make_prune(g, kop_);
   const long long in_bs = kop_ ? kop_->x_batch_stride : g.H;
   const double bc = (double)batch * g.H * 16.0, br = (double)batch * (g.N * 4.0 + g.H * 8.0);
   ProfSample sm{};
   bool on = prof_begin(3, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], in, work, g.n1 * g.n2h, g.n1 * g.n2h, 0,
-                                     (long long)g.n1 * g.n2h, in_bs, g.H, batch, p->tw[0], g, kop, s)));
+  PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], in, work, xcols(g, pr, true), in_bs, g.H, batch, p->tw[0], g, kop, s)));
   prof_end(on, sm, s);
   on = prof_begin(2, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, g.n0 * g.n2h, g.n2h, (long long)g.n1 * g.n2h,
-                                     g.n2h, g.H, g.H, batch, p->tw[1], g, plain, s)));
+  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, ycols(g, pr, true), g.H, g.H, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   const long long rows = (long long)batch * g.n0 * g.n1;
   on = prof_begin(1, s, sm, br);
   PB3D_CHECK_HIP(launch_c2r_rows(p->L[2], work, out, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop,
-                                 (float)(1.0 / (double)g.N), s));
+                                 (float)(1.0 / (double)g.N), pr.lz, s));
   prof_end(on, sm, s);
   g_rev += batch;
+  return 0;
+}
+
+extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const pb3d_submap* subs, const float* mask,
+                                 float scale, const pb3d_kop* kop_out, void* out_, void* work_, void* stream) {
+  if (!p || !subs || !out_ || !work_ || batch < 1 || batch > 65535)
+    return fail(PB3D_ERR_ARG, "pointing: null pointer or batch out of [1, 65535]");
+  if (nsub < 1 || nsub > kMaxSub) return fail(PB3D_ERR_ARG, "pointing: nsub out of range [1, 16]");
+  if (kop_out && kop_out->kind == 4) return fail(PB3D_ERR_ARG, "pointing: BKQ is a prologue");
+  if (int e = check_kop(kop_out)) return e;
+  hipStream_t s = (hipStream_t)stream;
+  const Geo& g = p->geo;
+  float2* work = (float2*)work_;
+  SubW sw{};
+  sw.n = nsub;
+  Prune pin = make_prune(g, &subs[0].kop);
+  for (int k = 0; k < nsub; ++k) {
+    const pb3d_submap& sb = subs[k];
+    if (int e = check_kop(&sb.kop)) return e;
+    if (sb.kop.kind == 5) return fail(PB3D_ERR_ARG, "pointing: PACK is an R2C epilogue");
+    if (!sb.in && sb.kop.kind != 4) return fail(PB3D_ERR_ARG, "pointing: sub-map input missing");
+    if (sb.ell < 0 || sb.ell > 4 || (sb.ell > 0 && (sb.m < 0 || sb.m > 2 * sb.ell)))
+      return fail(PB3D_ERR_ARG, "pointing: sub-map Y_lm index out of range (l<=4)");
+    sw.ell[k] = sb.ell; sw.coef[k] = sb.coef;
+    sw.y0[k] = sb.ell > 0 ? ylm_f(sb.ell, sb.m, 0.f, 0.f, 0.f) : 1.f;
+    if (sb.ell > 0 && !ylm_poly(sb.ell, sb.m, sw.poly[k]))
+      return fail(PB3D_ERR_ARG, "pointing: Y_lm polynomial fit failed");
+    pin = merge_prune(g, pin, make_prune(g, &sb.kop));
+  }
+  KOp plain = to_kop(nullptr);
+  const double bc = (double)batch * g.H * 16.0;
+  ProfSample sm{};
+  // 1. pruned x-column C2R pass of every sub-map into work[k*batch + b]
+  for (int k = 0; k < nsub; ++k) {
+    const pb3d_submap& sb = subs[k];
+    KOp kk = to_kop(&sb.kop);
+    bool on = prof_begin(3, s, sm, bc);
+    PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], (const float2*)sb.in, work + (long long)k * batch * g.H,
+                                       xcols(g, make_prune(g, &sb.kop), true), sb.kop.x_batch_stride, g.H, batch,
+                                       p->tw[0], g, kk, s)));
+    prof_end(on, sm, s);
+  }
+  // 2. pruned y-column C2R pass over all sub-maps
+  bool on = prof_begin(2, s, sm, bc * nsub);
+  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, ycols(g, pin, true), g.H, g.H, batch * nsub, p->tw[1], g,
+                                     plain, s)));
+  prof_end(on, sm, s);
+  // 3. fused rows: inverse rows of every sub-map, weights, mask, forward row -> work[b]
+  const Prune pout = make_prune(g, kop_out);
+  on = prof_begin(1, s, sm, (double)batch * (g.H * 8.0 * nsub + g.N * 4.0 + g.H * 8.0));
+  PB3D_CHECK_HIP(launch_pointing_rows(p->L[2], work, (long long)g.n0 * g.n1, batch, sw, mask, scale, pin.lz, pout.lz,
+                                      p->tw[2], p->twN, g, s));
+  prof_end(on, sm, s);
+  // 4. pruned forward y and x column passes, epilogue into out
+  on = prof_begin(2, s, sm, bc);
+  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, ycols(g, pout, false), g.H, g.H, batch, p->tw[1], g, plain, s)));
+  prof_end(on, sm, s);
+  KOp ko = to_kop(kop_out);
+  const long long out_bs = (kop_out && kop_out->kind == 5) ? kop_out->packed_len : g.H;
+  on = prof_begin(4, s, sm, bc);
+  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, (float2*)out_, xcols(g, pout, false), g.H, out_bs, batch, p->tw[0],
+                                     g, ko, s)));
+  prof_end(on, sm, s);
+  g_rev += (long long)batch * nsub;
+  g_fwd += batch;
   return 0;
 }

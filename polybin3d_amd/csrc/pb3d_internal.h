@@ -51,8 +51,9 @@ __device__ __forceinline__ float legendre_f(int l, float mu) {
 
 // Real spherical harmonic Y_lm(n) for unit n=(x,y,z), normalised so that sum_m Y_lm(a)Y_lm(b) =
 // L_l(a.b) (the convention of compute_real_harmonics, PolyBin3D/cython/utils.pyx:518-609).  The m
-// order is this library's own; every caller sums over m.  n = 0 gives 0 for l > 0.
-__device__ __forceinline__ float ylm_f(int l, int m, float x, float y, float z) {
+// order is this library's own; every caller sums over m.  n = 0 evaluates the same polynomials at
+// the origin (Y_40(0) = 3/8), as the reference does for a zero coordinate vector.
+__host__ __device__ __forceinline__ float ylm_f(int l, int m, float x, float y, float z) {
   const float s3 = 1.7320508075688772f, s5 = 2.2360679774997897f, s15 = 3.872983346207417f;
   const float s35 = 5.916079783099616f;
   float x2 = x * x, y2 = y * y, z2 = z * z;

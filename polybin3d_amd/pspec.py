@@ -200,42 +200,27 @@ class PSpec:
         return res
 
     # ------------------------------------------------------------------ MC Fisher
-    def _pointing_rop(self, scale=1.0):
-        """Real-space epilogue applying the pointing matrix P (mask) (pspec.py:199-213)."""
-        if self.const_mask:
-            return C.rop(C.ROP_PLAIN, scale=scale * self.mask_mean)
-        return C.rop(C.ROP_MUL, b=self.mask, scale=scale)
-
-    def _q_maps(self, src, li, Q, R):
+    def _q_maps(self, src, li, Q):
         """Q_{l,b} = (S^-1 P)^dag Theta_b L_l S^-1 P a for all bins b of multipole index li, written into
-        Q[Nk, H] (pspec.py:444-488) with trivial S^-1.  R is real scratch [Nk, N].
+        Q[Nk, H] (pspec.py:444-488) with trivial S^-1, as one fused pointing chain:
           global / l=0 : Q = R2C(P * C2R(Theta_b L_l SinvP))
           local  l>0   : Q = 0.5 R2C(P * [C2R(Theta_b leg) + sum_m Y_m(x) C2R(Theta_b Y_m(k) SinvP)])
-        (the reference applies (S^-1 P)^dag to both local half-terms separately; the sum is linear, so one
-        R2C of the summed real map is the same operator)."""
+        (the reference applies (S^-1 P)^dag to both local half-terms separately; the operator is linear,
+        so one R2C of the summed real map is the same).  Only modes inside the k-bins are written."""
         b = self.base
         Nk = self.Nk
         bins = self._bins_dev
         SinvP, leg = src
         ell = 2 * li
+        sup = b.support(self.k_bins[-1])
+        common = dict(bin_table=self.table, bins=bins, x_batch_stride=0, support=sup)
         if b.sightline == 'global' or li == 0:
-            b.c2r(SinvP, out=R, batch=Nk,
-                  kop=C.kop(C.KOP_FILTER, bin_table=self.table, bins=bins, ell=ell, x_batch_stride=0))
-            b.r2c(None, out=Q, batch=Nk, rop=self._fisher_rop(R, 1.0))
+            subs = [(C.kop(C.KOP_FILTER, ell=ell, **common), SinvP, 0, 0, 1.0)]
         else:
-            b.c2r(leg[li], out=R, batch=Nk,
-                  kop=C.kop(C.KOP_FILTER, bin_table=self.table, bins=bins, ell=0, x_batch_stride=0))
-            for m in range(2 * ell + 1):
-                b.c2r(SinvP, out=R, batch=Nk,
-                      kop=C.kop(C.KOP_YLM, bin_table=self.table, bins=bins, ell=ell, m=m, x_batch_stride=0),
-                      rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=1))
-            b.r2c(None, out=Q, batch=Nk, rop=self._fisher_rop(R, 0.5))
-
-    def _fisher_rop(self, R, scale):
-        b = self.base
-        if self.const_mask:
-            return C.rop(C.ROP_PLAIN, a=R, sa=b.N, scale=scale * self.mask_mean)
-        return C.rop(C.ROP_MUL, a=R, sa=b.N, b=self.mask, sb=0, scale=scale)
+            subs = [(C.kop(C.KOP_FILTER, ell=0, **common), leg[li], 0, 0, 0.5)]
+            subs += [(C.kop(C.KOP_YLM, ell=ell, m=m, **common), SinvP, ell, m, 0.5) for m in range(2 * ell + 1)]
+        mask, scale = (None, self.mask_mean) if self.const_mask else (self.mask, 1.0)
+        b.pointing(subs, Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup))
 
     def compute_fisher_contribution(self, seed, verb=False):
         """One MC realization of the Fisher matrix (pspec.py:141-147, 215-493; compute_cov=False).
@@ -275,11 +260,10 @@ class PSpec:
                 lefts[li].copy_(Ainv)
         ells = [0] * Nl if local else [2 * li for li in range(Nl)]
         Q = torch.empty((Nk,) + b.half_shape, dtype=torch.complex64, device=b.device)
-        R = torch.empty((Nk,) + tuple(int(v) for v in b.gridsize), dtype=torch.float32, device=b.device)
         F = torch.zeros((self.N_bins, self.N_bins), dtype=torch.float64, device=b.device)
         scaling = 0.5 * b.volume / b.gridsize.prod() ** 2
         for li in range(Nl):
-            self._q_maps((SinvP, leg), li, Q, R)
+            self._q_maps((SinvP, leg), li, Q)
             r = b.bin_reduce(Q, self.k_bins, ells, y=lefts, y_qstride=0, y_ostride=b.H, nq=Nk)  # [Nk, Nl, Nk]
             F[li * Nk:(li + 1) * Nk, :] = r.reshape(Nk, Nl * Nk) * scaling
         return F

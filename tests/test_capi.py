@@ -29,10 +29,11 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header():
-    # sizes of the C structs (x86-64 / amdgcn share this layout): pb3d_grid 88, pb3d_rop 88, pb3d_kop 64
+    # sizes of the C structs as laid out by the host compiler (natural alignment)
     assert ctypes.sizeof(C.Grid) == 4 * 4 + 9 * 8
     assert ctypes.sizeof(C.ROp) == 16 + 32 + 32 + 8
-    assert ctypes.sizeof(C.KOp) == 24 + 24 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 16
+    assert ctypes.sizeof(C.KOp) == 24 + 24 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8 + 16 + 16
+    assert ctypes.sizeof(C.SubMap) == ctypes.sizeof(C.KOp) + 8 + 16
 
 
 def test_host_side_argument_validation():
@@ -68,3 +69,21 @@ def test_host_side_argument_validation():
 def test_python_errors_are_raised():
     with pytest.raises(ValueError):
         C.check(-1)
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """The ctypes mirrors have the sizes and field offsets the C compiler gives include/pb3d.h."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no host C compiler")
+    src = tmp_path / "sizes.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"pb3d.h\"\n"
+                   "int main(){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(pb3d_grid), sizeof(pb3d_rop),"
+                   " sizeof(pb3d_kop), sizeof(pb3d_submap), offsetof(pb3d_kop, support), offsetof(pb3d_kop, ft),"
+                   " offsetof(pb3d_submap, coef));}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == [ctypes.sizeof(C.Grid), ctypes.sizeof(C.ROp), ctypes.sizeof(C.KOp), ctypes.sizeof(C.SubMap),
+                   C.KOp.support.offset, C.KOp.ft.offset, C.SubMap.coef.offset]

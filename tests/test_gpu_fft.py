@@ -121,3 +121,66 @@ def test_device_grf_statistics(hip):
     assert rel(b.to_fourier(r).cpu().numpy(), a[1].cpu().numpy()) < 1e-5
     assert torch.equal(a[2], b.generate_data(seed=2, output_type='fourier'))
     assert not torch.equal(a[2], a[3])
+
+
+def test_pruned_transforms(hip):
+    """Band-limited C2R (input zero outside the support) and R2C with packed in-support output match
+    numpy; the pruned passes skip only known-zero or unused data."""
+    from polybin3d_amd import _capi as C
+    n = (64, 32, 64)
+    b = make_base(n)
+    o = O.PolyBin3D(1000., list(n))
+    kmax = 0.09
+    sup = b.support(kmax)
+    rng = np.random.RandomState(5)
+    X = rng.randn(*o.modk_grid.shape) + 1j * rng.randn(*o.modk_grid.shape)
+    X[o.modk_grid >= kmax] = 0.
+    Xd = torch.from_numpy(X.astype(np.complex64)).cuda()
+    got = b.c2r(Xd, kop=C.kop(C.KOP_PLAIN, support=sup, x_batch_stride=b.H)).cpu().numpy()
+    assert rel(got, np.fft.irfftn(X, s=n, axes=(0, 1, 2))) < 1e-5
+    x = rng.randn(*n)
+    edges = np.array([0.01, 0.05, kmax])
+    idx = b.bin_index_host(edges).reshape(-1)
+    modes = np.nonzero(idx >= 0)[0]
+    modes = modes[np.argsort(idx[modes], kind='stable')]
+    pi = np.full(b.H, -1, dtype=np.int32)
+    pi[modes] = np.arange(len(modes), dtype=np.int32)
+    pid = torch.from_numpy(pi).cuda()
+    out = torch.zeros(2 * len(modes), dtype=torch.complex64, device='cuda')
+    xd = torch.from_numpy(np.stack([x, 2 * x]).astype(np.float32)).cuda()
+    b.r2c(xd, out=out, kop=C.kop(C.KOP_PACK, pack_index=pid, packed_len=len(modes), support=sup))
+    ref = np.fft.rfftn(x).reshape(-1)[modes]
+    got = out.cpu().numpy().reshape(2, -1)
+    assert rel(got[0], ref) < 1e-5 and rel(got[1], 2 * ref) < 1e-5
+
+
+def test_fused_pointing_chain(hip):
+    """pb3d_fft_pointing = R2C(mask * scale * sum_g coef_g Y_g(x) C2R(kop_g(in))) composed from numpy."""
+    from polybin3d_amd import _capi as C
+    n = (32, 32, 32)
+    center = [400., 250., -300.]
+    b = make_base(n, boxcenter=center, sightline='local')
+    o = O.PolyBin3D(1000., list(n), boxcenter=center, sightline='local')
+    rng = np.random.RandomState(6)
+    A = np.fft.rfftn(rng.randn(*n))
+    mask = rng.rand(*n)
+    edges = np.linspace(0.01, 0.08, 5)
+    tab = b.bin_table(edges)
+    bins = torch.arange(4, dtype=torch.int32, device='cuda')
+    sup = b.support(edges[-1])
+    Ad = torch.from_numpy(A.astype(np.complex64)).cuda()
+    Md = b.as_real(mask)
+    common = dict(bin_table=tab, bins=bins, x_batch_stride=0, support=sup)
+    subs = [(C.kop(C.KOP_FILTER, ell=2, **common), Ad, 0, 0, 0.5)]
+    subs += [(C.kop(C.KOP_YLM, ell=2, m=m, **common), Ad, 2, m, 0.25) for m in range(5)]
+    out = torch.empty((4,) + b.half_shape, dtype=torch.complex64, device='cuda')
+    b.pointing(subs, 4, out, mask=Md, scale=1.5, kop_out=None)
+    Yr = O.real_harmonics(np.asarray(o.r_grids), 2, False)[2]
+    Yk = O.real_harmonics(np.asarray(o.k_grids), 2, False)[2]
+    got = out.cpu().numpy()
+    for q in range(4):
+        r = 0.5 * o.to_real(o.map_utils.fourier_filter(A, 2, edges[q], edges[q + 1]))
+        for m in range(5):
+            r += 0.25 * Yr[m] * o.to_real(o.map_utils.prod_fourier_filter(A, Yk[m], edges[q], edges[q + 1]))
+        ref = o.to_fourier(1.5 * mask * r)
+        assert rel(got[q], ref) < 2e-5
