@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Fisher MC realizations/s, Pk+Bk 256³: batched/split-bf16 vs plain FP32"
 HBM_PEAK_GBS = 8000.0
 KIND_NAMES = {0: "r2c_rows", 1: "c2r_rows", 2: "fft_cols (plain)", 3: "fft_cols (C2R first pass, fused prologue)",
-              4: "fft_cols (R2C last pass, fused epilogue)"}
+              4: "fft_cols (R2C last pass, fused epilogue)", 5: "pointing_rows (fused Q-map rows)"}
 
 
 def shell_mask(n, L, center, seed=1, r_lo=300., r_hi=480., nholes=200, smooth=5.):
@@ -189,7 +189,7 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     kinds = {}
-    for kind in range(5):
+    for kind in range(6):
         cnt, ms, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
         C.check(C.lib().pb3d_prof_read(kind, ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(by)))
         if cnt.value:

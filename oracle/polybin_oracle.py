@@ -25,9 +25,10 @@ How each part is pinned (tests/test_oracle_*.py):
   * MC Fisher (global, constant mask): per-seed closed form from the GRF amplitudes; local and
     masked branches: dense-operator restatement on 6^3-8^3 grids via the addition theorem;
     expectation E[F_MC] -> F_ideal statistically.
-Functions with no check beyond restatement are listed as "parity unpinned" in DESIGN.md:
-BSpec._compute_fisher's per-seed values for local sightline with l>0 (checked only in expectation
-against the ideal Fisher).
+Parity unpinned (no check beyond restatement and the compiled utils.pyx kernels they call):
+BSpec._compute_fisher per-seed values; BSpec._compute_symmetry_factor and
+compute_fisher_ideal(discreteness_correction=True) for l>0; the local-sightline Bk numerator;
+PSpec.compute_shot_contribution.
 """
 import numpy as np
 import scipy.fft

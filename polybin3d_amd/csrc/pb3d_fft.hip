@@ -605,15 +605,15 @@ static std::atomic<long long> g_fwd{0}, g_rev{0};
 
 // ---- sampled launch timing (roofline reporting): every `period`-th launch of a pass kind is
 // bracketed by hipEvents on its own stream; read() synchronises and sums the durations.
-constexpr int kProfKinds = 5;
+constexpr int kProfKinds = 6;  // 0 r2c rows, 1 c2r rows, 2 plain cols, 3 C2R first cols, 4 R2C last cols, 5 pointing rows
 struct ProfSample { int kind; hipEvent_t a, b; double bytes; };
 static int g_prof_period = 0;
-static long long g_prof_count[kProfKinds] = {0, 0, 0, 0, 0};
+static long long g_prof_count[kProfKinds] = {};
 static std::vector<ProfSample> g_prof_pending;
 static std::vector<hipEvent_t> g_prof_free;
-static long long g_prof_n[kProfKinds] = {0, 0, 0, 0, 0};
-static double g_prof_ms[kProfKinds] = {0, 0, 0, 0, 0};
-static double g_prof_bytes[kProfKinds] = {0, 0, 0, 0, 0};
+static long long g_prof_n[kProfKinds] = {};
+static double g_prof_ms[kProfKinds] = {};
+static double g_prof_bytes[kProfKinds] = {};
 
 static hipEvent_t prof_event() {
   if (!g_prof_free.empty()) { hipEvent_t e = g_prof_free.back(); g_prof_free.pop_back(); return e; }
@@ -813,6 +813,18 @@ static ColMap ycols(const Geo& g, const Prune& pr, bool c2r) {
   return cm;
 }
 
+// Algorithmic HBM bytes of one launch (what the pass must read and write, after pruning): 8 B per
+// complex element read or written; k-space prologue/epilogue tables (bin tables, packed-row
+// indices) are not counted.
+static int npos(int rad, int L) { return rad < 0 ? L : (2 * rad + 1 < L ? 2 * rad + 1 : L); }
+static double col_bytes(const ColMap& cm, int L, int batch) {
+  return 8.0 * batch * cm.ncols * (double)(npos(cm.rad_in, L) + npos(cm.rad_out, L));
+}
+static int rop_operands(const pb3d_rop* r) {
+  if (!r) return 1;
+  return 1 + (r->b ? 1 : 0) + (r->c ? 1 : 0) + (r->d ? 1 : 0);
+}
+
 static float2* upload_twiddles(int L, int count) {
   std::vector<float2> h(count);
   const double twopi = 6.283185307179586476925286766559;
@@ -937,17 +949,18 @@ extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, voi
   KOp plain = to_kop(nullptr);
   const Prune pr = make_prune(g, kop_);
   const long long rows = (long long)batch * g.n0 * g.n1;
-  const double bc = (double)batch * g.H * 16.0, br = (double)batch * (g.N * 4.0 + g.H * 8.0);
+  const ColMap cy = ycols(g, pr, false), cx = xcols(g, pr, false);
+  const double br = (double)batch * (g.N * 4.0 * rop_operands(rop_) + (double)g.n0 * g.n1 * pr.lz * 8.0);
   ProfSample sm{};
   bool on = prof_begin(0, s, sm, br);
   PB3D_CHECK_HIP(launch_r2c_rows(p->L[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, pr.lz, s));
   prof_end(on, sm, s);
-  on = prof_begin(2, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, ycols(g, pr, false), g.H, g.H, batch, p->tw[1], g, plain, s)));
+  on = prof_begin(2, s, sm, col_bytes(cy, p->L[1], batch));
+  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cy, g.H, g.H, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   const long long out_bs = (kop_ && kop_->kind == 5) ? kop_->packed_len : g.H;
-  on = prof_begin(4, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, out, xcols(g, pr, false), g.H, out_bs, batch, p->tw[0], g, kop, s)));
+  on = prof_begin(4, s, sm, col_bytes(cx, p->L[0], batch));
+  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, out, cx, g.H, out_bs, batch, p->tw[0], g, kop, s)));
   prof_end(on, sm, s);
   g_fwd += batch;
   return 0;
@@ -970,13 +983,15 @@ extern "C" int pb3d_fft_c2r(const pb3d_plan* p, const void* in_, float* out, voi
   KOp plain = to_kop(nullptr);
   const Prune pr = make_prune(g, kop_);
   const long long in_bs = kop_ ? kop_->x_batch_stride : g.H;
-  const double bc = (double)batch * g.H * 16.0, br = (double)batch * (g.N * 4.0 + g.H * 8.0);
+  const ColMap cx = xcols(g, pr, true), cy = ycols(g, pr, true);
+  const double br = (double)batch * ((double)g.n0 * g.n1 * pr.lz * 8.0 +
+                                     g.N * 4.0 * (rop_operands(rop_) + (rop_ && rop_->accumulate ? 1 : 0)));
   ProfSample sm{};
-  bool on = prof_begin(3, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], in, work, xcols(g, pr, true), in_bs, g.H, batch, p->tw[0], g, kop, s)));
+  bool on = prof_begin(3, s, sm, col_bytes(cx, p->L[0], batch));
+  PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], in, work, cx, in_bs, g.H, batch, p->tw[0], g, kop, s)));
   prof_end(on, sm, s);
-  on = prof_begin(2, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, ycols(g, pr, true), g.H, g.H, batch, p->tw[1], g, plain, s)));
+  on = prof_begin(2, s, sm, col_bytes(cy, p->L[1], batch));
+  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, cy, g.H, g.H, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   const long long rows = (long long)batch * g.n0 * g.n1;
   on = prof_begin(1, s, sm, br);
@@ -1014,38 +1029,38 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
     pin = merge_prune(g, pin, make_prune(g, &sb.kop));
   }
   KOp plain = to_kop(nullptr);
-  const double bc = (double)batch * g.H * 16.0;
   ProfSample sm{};
   // 1. pruned x-column C2R pass of every sub-map into work[k*batch + b]
   for (int k = 0; k < nsub; ++k) {
     const pb3d_submap& sb = subs[k];
     KOp kk = to_kop(&sb.kop);
-    bool on = prof_begin(3, s, sm, bc);
-    PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], (const float2*)sb.in, work + (long long)k * batch * g.H,
-                                       xcols(g, make_prune(g, &sb.kop), true), sb.kop.x_batch_stride, g.H, batch,
-                                       p->tw[0], g, kk, s)));
+    const ColMap cx = xcols(g, make_prune(g, &sb.kop), true);
+    bool on = prof_begin(3, s, sm, col_bytes(cx, p->L[0], batch));
+    PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], (const float2*)sb.in, work + (long long)k * batch * g.H, cx,
+                                       sb.kop.x_batch_stride, g.H, batch, p->tw[0], g, kk, s)));
     prof_end(on, sm, s);
   }
   // 2. pruned y-column C2R pass over all sub-maps
-  bool on = prof_begin(2, s, sm, bc * nsub);
-  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, ycols(g, pin, true), g.H, g.H, batch * nsub, p->tw[1], g,
-                                     plain, s)));
+  const ColMap cyi = ycols(g, pin, true);
+  bool on = prof_begin(2, s, sm, col_bytes(cyi, p->L[1], batch * nsub));
+  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, cyi, g.H, g.H, batch * nsub, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   // 3. fused rows: inverse rows of every sub-map, weights, mask, forward row -> work[b]
   const Prune pout = make_prune(g, kop_out);
-  on = prof_begin(1, s, sm, (double)batch * (g.H * 8.0 * nsub + g.N * 4.0 + g.H * 8.0));
+  const double rows1 = (double)g.n0 * g.n1;
+  on = prof_begin(5, s, sm, (double)batch * (rows1 * 8.0 * (pin.lz * (double)nsub + pout.lz) + (mask ? g.N * 4.0 : 0.0)));
   PB3D_CHECK_HIP(launch_pointing_rows(p->L[2], work, (long long)g.n0 * g.n1, batch, sw, mask, scale, pin.lz, pout.lz,
                                       p->tw[2], p->twN, g, s));
   prof_end(on, sm, s);
   // 4. pruned forward y and x column passes, epilogue into out
-  on = prof_begin(2, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, ycols(g, pout, false), g.H, g.H, batch, p->tw[1], g, plain, s)));
+  const ColMap cyo = ycols(g, pout, false), cxo = xcols(g, pout, false);
+  on = prof_begin(2, s, sm, col_bytes(cyo, p->L[1], batch));
+  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cyo, g.H, g.H, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   KOp ko = to_kop(kop_out);
   const long long out_bs = (kop_out && kop_out->kind == 5) ? kop_out->packed_len : g.H;
-  on = prof_begin(4, s, sm, bc);
-  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, (float2*)out_, xcols(g, pout, false), g.H, out_bs, batch, p->tw[0],
-                                     g, ko, s)));
+  on = prof_begin(4, s, sm, col_bytes(cxo, p->L[0], batch));
+  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, (float2*)out_, cxo, g.H, out_bs, batch, p->tw[0], g, ko, s)));
   prof_end(on, sm, s);
   g_rev += (long long)batch * nsub;
   g_fwd += batch;
