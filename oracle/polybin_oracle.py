@@ -63,8 +63,9 @@ def legendre(l, mu):
 def real_harmonics(coords, lmax, odd_l):
     """Real spherical harmonics Y_lm(n) for l=1..lmax, normalised so that
     sum_m Y_lm(a) Y_lm(b) = L_l(a.b) (the convention of compute_real_harmonics,
-    PolyBin3D/cython/utils.pyx:518-609).  Odd l only when odd_l.  Zero vectors map to Y=0 (l>0
-    members vanish at the origin there too, utils.pyx:534-542).
+    PolyBin3D/cython/utils.pyx:518-609).  Odd l only when odd_l.  A zero vector is normalised to
+    (0, 0, 0) (utils.pyx:534-538) and the polynomials are evaluated there: every member vanishes
+    except the l=4 one (35 z^4 - 30 z^2 + 3)/8 = 3/8 (utils.pyx:599).
 
     Returns {l: array[2l+1, ...]}.  The basis is built from the standard tesseral harmonics
     (Condon-Shortley phases dropped) times sqrt(4 pi / (2l+1)); any orthonormal real basis gives

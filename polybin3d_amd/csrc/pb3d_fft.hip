@@ -81,10 +81,23 @@ template <> struct Decomp<128>  { static constexpr int R0 = 16, R1 = 8,  R2 = 1,
 template <> struct Decomp<256>  { static constexpr int R0 = 16, R1 = 16, R2 = 1, P = 16; };
 template <> struct Decomp<512>  { static constexpr int R0 = 8,  R1 = 8,  R2 = 8, P = 64; };
 
+// LDS ordering between the lanes of one wave: a wave's LDS instructions execute in issue order,
+// so a compiler-level barrier is enough (no s_barrier).  Block variant: __syncthreads.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) wave_lds_sync(); else __syncthreads();
+}
+
 // One Stockham stage: butterflies j = t + b*P, inputs src(j + k*L/R), twiddles from the length-L
 // table tw[m] = exp(-2 pi i m / L) (conjugated for DIR=+1), outputs dst((j/NS)*NS*R + j%NS + k*NS).
-// If `barrier` the block synchronises between the reads and the writes (src/dst share LDS).
-template <int L, int R, int NS, int P, int DIR, typename Src, typename Dst>
+// If `barrier` the block (WAVE: the wave) synchronises between the reads and the writes (src/dst
+// share LDS).
+template <int L, int R, int NS, int P, int DIR, bool WAVE = false, typename Src, typename Dst>
 __device__ __forceinline__ void stage(int t, Src src, Dst dst, const float2* __restrict__ tw,
                                       bool barrier) {
   constexpr int NB = L / R;
@@ -107,7 +120,7 @@ __device__ __forceinline__ void stage(int t, Src src, Dst dst, const float2* __r
     }
     DFT<R, DIR>::run(v[b]);
   }
-  if (barrier) __syncthreads();
+  if (barrier) lds_sync<WAVE>();
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const int j = t + b * P;
@@ -194,30 +207,34 @@ __device__ __forceinline__ void first_stage_from_regs(int t, Get get, Dst dst) {
 
 // Inverse pencil FFT whose final stage accumulates into registers (LDS source, all stages but the
 // last in LDS).  Contains barriers: every thread of the block calls it.
-template <int L, typename Lds, typename Sink>
-__device__ __forceinline__ void pencil_inverse_to_regs(int t, Lds lds, Sink sink, const float2* tw) {
+// Split in two so that callers can specialise the sink per call: head = every stage but the last
+// (in LDS, ends synchronised), tail = the last stage into registers.
+template <int L, bool WAVE, typename Lds>
+__device__ __forceinline__ void pencil_inverse_head(int t, Lds lds, const float2* tw) {
   using D = Decomp<L>;
-  constexpr int P = D::P;
-  if constexpr (D::R1 == 1) {
-    last_stage_to_regs<L, D::R0, 1, P, +1>(t, lds, sink, tw);
-  } else {
+  if constexpr (D::R1 != 1) {
     static_assert(D::R2 == 1, "row transforms use at most two stages");
-    stage<L, D::R0, 1, P, +1>(t, lds, lds, tw, true);
-    __syncthreads();
-    last_stage_to_regs<L, D::R1, D::R0, P, +1>(t, lds, sink, tw);
+    stage<L, D::R0, 1, D::P, +1, WAVE>(t, lds, lds, tw, true);
+    lds_sync<WAVE>();
   }
+}
+template <int L, typename Lds, typename Sink>
+__device__ __forceinline__ void pencil_inverse_tail(int t, Lds lds, Sink sink, const float2* tw) {
+  using D = Decomp<L>;
+  if constexpr (D::R1 == 1) last_stage_to_regs<L, D::R0, 1, D::P, +1>(t, lds, sink, tw);
+  else last_stage_to_regs<L, D::R1, D::R0, D::P, +1>(t, lds, sink, tw);
 }
 
 // Forward pencil FFT whose first stage reads registers; result left in LDS (natural order).
-template <int L, typename Get, typename Lds>
+template <int L, bool WAVE, typename Get, typename Lds>
 __device__ __forceinline__ void pencil_forward_from_regs(int t, Get get, Lds lds, const float2* tw) {
   using D = Decomp<L>;
   constexpr int P = D::P;
   first_stage_from_regs<L, D::R0, P, -1>(t, get, lds);
   if constexpr (D::R1 != 1) {
     static_assert(D::R2 == 1, "row transforms use at most two stages");
-    __syncthreads();
-    stage<L, D::R1, D::R0, P, -1>(t, lds, lds, tw, true);
+    lds_sync<WAVE>();
+    stage<L, D::R1, D::R0, P, -1, WAVE>(t, lds, lds, tw, true);
   }
 }
 
@@ -364,28 +381,112 @@ __device__ __forceinline__ float2 r2c_split(const float2* b, int LDT, int r, int
 
 // Load a tile of T half-spectrum rows from global memory and apply the C2R pre-pass on the fly
 // (pairs (k, M-k) read together, Z' written to the transposed LDS tile): one staging step, no
-// separate LDS pass.  Entries kz >= lz_in are known zeros; Im of kz=0 and kz=M is ignored.
+// separate LDS pass.  Entries kz >= lz_in are known zeros; Im of kz=0 and kz=M is ignored.  The
+// trip count is a compile-time constant and every load is issued before the first one is used, so
+// a thread waits for HBM once per tile rather than once per element.
 template <int M, int T, int NT>
 __device__ __forceinline__ void load_prepass(float2* buf, int LDT, const float2* __restrict__ src,
                                              long long rows_left, int lz_in, int tid, const float2* twN) {
-  for (int f = tid; f < T * (M / 2 + 1); f += NT) {
-    const int rr = f / (M / 2 + 1), k = f % (M / 2 + 1);
-    float2 xk = make_float2(0.f, 0.f), xm = make_float2(0.f, 0.f);
-    if (rr < rows_left) {
-      if (k < lz_in) xk = src[rr * (M + 1) + k];
-      if (M - k < lz_in) xm = src[rr * (M + 1) + M - k];
+  constexpr int K = M / 2 + 1;
+  constexpr int TOT = T * K;
+  constexpr int IT = (TOT + NT - 1) / NT;
+  float2 xk[IT], xm[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int f = tid + it * NT;
+    const int rr = f / K, k = f % K;
+    xk[it] = make_float2(0.f, 0.f);
+    xm[it] = make_float2(0.f, 0.f);
+    if ((IT * NT == TOT || f < TOT) && rr < rows_left) {
+      if (k < lz_in) xk[it] = src[rr * (M + 1) + k];
+      if (M - k < lz_in) xm[it] = src[rr * (M + 1) + M - k];
     }
-    if (k == 0) { xk.y = 0.f; xm.y = 0.f; }   // (k, M) pair: both self-conjugate entries
-    if (2 * k == M) xm = xk;
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int f = tid + it * NT;
+    if (IT * NT != TOT && f >= TOT) continue;
+    const int rr = f / K, k = f % K;
+    float2 a = xk[it], c = xm[it];
+    if (k == 0) { a.y = 0.f; c.y = 0.f; }   // (k, M) pair: both self-conjugate entries
+    if (2 * k == M) c = a;
     {
-      const float2 sm = cadd(xk, cconj(xm));
-      const float2 d = csub(xk, cconj(xm));
+      const float2 sm = cadd(a, cconj(c));
+      const float2 d = csub(a, cconj(c));
       const float2 wd = cmul(cconj(twN[k]), d);
       buf[k * LDT + rr] = make_float2(sm.x - wd.y, sm.y + wd.x);
     }
     if (k != 0 && 2 * k != M) {
-      const float2 sm = cadd(xm, cconj(xk));
-      const float2 d = csub(xm, cconj(xk));
+      const float2 sm = cadd(c, cconj(a));
+      const float2 d = csub(c, cconj(a));
+      const float2 wd = cmul(cconj(twN[M - k]), d);
+      buf[(M - k) * LDT + rr] = make_float2(sm.x - wd.y, sm.y + wd.x);
+    }
+  }
+}
+
+// The same staging with lanes walking the rows: lane (rr = tid % T, kb = tid / T) handles
+// k = kb + it * (NT / T), so every address offset is a compile-time immediate and no division is
+// needed.  Used where a tile has few rows (one wave's rows).  Needs rows_left >= 1: rows past the
+// end are read as row 0 and replaced by zeros.  Three uniform regimes: lz_in == M/2+1 (every
+// entry of the left half live, none of the right: unconditional loads), lz_in < M/2+1
+// (predicated loads, right half zero), lz_in > M/2+1 (both halves read).
+template <int M, int T, int NT>
+__device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const float2* __restrict__ src,
+                                                  long long rows_left, int lz_in, int tid, const float2* twN) {
+  constexpr int K = M / 2 + 1;
+  constexpr int KS = NT / T;
+  constexpr int IT = (K + KS - 1) / KS;
+  static_assert(NT % T == 0, "tile shape");
+  const int rr = tid % T, kb = tid / T;
+  const bool rv = rr < rows_left;
+  const float2* rowp = src + (rv ? rr : 0) * (M + 1);
+  const float2 zero = make_float2(0.f, 0.f);
+  float2 xk[IT], xm[IT];
+  if (lz_in == K) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int k = kb + it * KS;
+      const float2 v = rowp[k < K ? k : K - 1];
+      xk[it] = rv ? v : zero;
+      xm[it] = zero;
+    }
+  } else if (lz_in < K) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int k = kb + it * KS;
+      xk[it] = zero;
+      xm[it] = zero;
+      if (rv && k < lz_in) xk[it] = rowp[k];
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int k = kb + it * KS;
+      xk[it] = zero;
+      xm[it] = zero;
+      if (rv && (IT * KS == K || k < K)) {
+        xk[it] = rowp[k];
+        if (M - k < lz_in) xm[it] = rowp[M - k];
+      }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int k = kb + it * KS;
+    if (IT * KS != K && k >= K) continue;
+    float2 a = xk[it], c = xm[it];
+    if (k == 0) { a.y = 0.f; c.y = 0.f; }   // (k, M) pair: both self-conjugate entries
+    if (2 * k == M) c = a;
+    {
+      const float2 sm = cadd(a, cconj(c));
+      const float2 d = csub(a, cconj(c));
+      const float2 wd = cmul(cconj(twN[k]), d);
+      buf[k * LDT + rr] = make_float2(sm.x - wd.y, sm.y + wd.x);
+    }
+    if (k != 0 && 2 * k != M) {
+      const float2 sm = cadd(c, cconj(a));
+      const float2 d = csub(c, cconj(a));
       const float2 wd = cmul(cconj(twN[M - k]), d);
       buf[(M - k) * LDT + rr] = make_float2(sm.x - wd.y, sm.y + wd.x);
     }
@@ -403,8 +504,11 @@ r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows
   const int tid = threadIdx.x;
   stage_twiddles<M>(stw, stwN, tw, twN, tid, NT);
   const long long row0 = (long long)blockIdx.x * T;
-  // 1. coalesced load of T rows (T*M float2) with the voxel prologue
-  for (int f = tid; f < T * M; f += NT) {
+  // 1. coalesced load of T rows (T*M float2) with the voxel prologue (T*M is a multiple of NT)
+  static_assert((T * M) % NT == 0, "row tile");
+#pragma unroll
+  for (int it = 0; it < (T * M) / NT; ++it) {
+    const int f = tid + it * NT;
     const int r = f / M, m = f % M;
     const long long row = row0 + r;
     float2 v = make_float2(0.f, 0.f);
@@ -431,9 +535,12 @@ r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows
   __syncthreads();
   // 3. split into the half spectrum
   float2* o = out + row0 * (M + 1);
-  for (int f = tid; f < T * (M + 1); f += NT) {
+  constexpr int ITS = (T * (M + 1) + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < ITS; ++it) {
+    const int f = tid + it * NT;
     const int rr = f / (M + 1), k = f % (M + 1);
-    if (row0 + rr >= nrows || k >= lz_out) continue;
+    if (f >= T * (M + 1) || row0 + rr >= nrows || k >= lz_out) continue;
     o[f] = r2c_split<M>(buf, LDT, rr, k, stwN);
   }
 }
@@ -457,7 +564,9 @@ c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows
   RowT rw{buf, LDT, r};
   pencil_fft<M, +1>(t, rw, rw, rw, stw, true);
   __syncthreads();
-  for (int f = tid; f < T * M; f += NT) {
+#pragma unroll
+  for (int it = 0; it < (T * M) / NT; ++it) {
+    const int f = tid + it * NT;
     const int rr = f / M, m = f % M;
     const long long row = row0 + rr;
     if (row >= nrows) continue;
@@ -484,8 +593,10 @@ c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows
 // ------------------------------------------------------------------------------------------
 // Fused pointing row kernel: for output map t and row (i0,i1): invert the rows of all nsub
 // sub-maps (work[g*batch + t]), accumulate coef_g * Y_g(x_hat) * z in registers, multiply by the
-// mask and scale, forward-transform the row and write kz < lz_out into work[t] (in place: a block
+// mask and scale, forward-transform the row and write kz < lz_out into work[t] (in place: a wave
 // owns its rows in every sub-map).
+// Each wave works on its own RW = 64/P rows with its own transposed LDS tile, so the exchanges
+// between FFT stages need only wave-level ordering: the sub-map loop has no block barrier.
 // ------------------------------------------------------------------------------------------
 constexpr int kMaxSub = 16;
 constexpr int kMaxMono = 15;  // monomials of a homogeneous polynomial of degree <= 4
@@ -500,100 +611,172 @@ struct SubW {
   float poly[kMaxSub][kMaxMono];
 };
 
+template <int M> struct PointCfg {
+  static constexpr int P = Decomp<M>::P;      // lanes per row
+  static constexpr int RW = 64 / P;           // rows per wave
+  static constexpr int WAVES = 4;
+  static constexpr int NT = 64 * WAVES;
+  static constexpr int RB = RW * WAVES;       // rows per block
+  static constexpr int LDT = RW + 1;          // tile pitch: element p of row r at p * LDT + r
+  static constexpr int SIZE = (M + 1) * LDT;  // one wave's tile
+  static constexpr int E = M / P;             // values per lane
+  static_assert(64 % P == 0, "lanes per row must divide the wave");
+};
+
 template <int M>
-__global__ void __launch_bounds__(RowTile<M>::NT, PB3D_POINT_WAVES)
+__global__ void __launch_bounds__(PointCfg<M>::NT, PB3D_POINT_WAVES)
 pointing_rows(float2* work, long long rows_per_map, int batch, SubW sw, const float* __restrict__ mask,
               float scale, int lz_in, int lz_out, const float2* __restrict__ tw, const float2* __restrict__ twN,
               Geo geo) {
-  using RT = RowTile<M>;
-  constexpr int T = RT::T, NT = RT::NT, LDT = RT::LDT;
-  constexpr int E = (T * M) / NT;  // values per thread (<= 16)
-  __shared__ float2 buf[RT::SIZE];
+  using PC = PointCfg<M>;
+  constexpr int RW = PC::RW, LDT = PC::LDT, E = PC::E, P = PC::P;
+  __shared__ float2 tiles[PC::WAVES * PC::SIZE];
   __shared__ float2 stw[M], stwN[M + 1];
   const int tid = threadIdx.x;
-  stage_twiddles<M>(stw, stwN, tw, twN, tid, NT);
+  stage_twiddles<M>(stw, stwN, tw, twN, tid, PC::NT);
+  __syncthreads();  // twiddles staged: the only block barrier
+  const int wave = tid / 64, lane = tid % 64;
+  float2* buf = tiles + wave * PC::SIZE;
   const long long t = blockIdx.y;
-  const long long row0 = (long long)blockIdx.x * T;
+  const long long row0 = (long long)blockIdx.x * PC::RB + wave * RW;  // this wave's first row
+  if (row0 >= rows_per_map) return;  // no rows for this wave (no block barrier follows)
   const long long H = geo.H;
-  const int r = tid % T, tp = tid / T;
+  const int r = lane % RW, tp = lane / RW;
   RowT rw{buf, LDT, r};
-  // this thread's voxels after the inverse transforms: positions m = tp + e*P of row r
+  // this lane's voxels after the inverse transforms: positions m = tp + e*P of row r
   const long long row = row0 + r;
   const long long rowc = row < rows_per_map ? row : rows_per_map - 1;
   const int i0 = (int)(rowc / geo.n1), i1 = (int)(rowc % geo.n1);
   const float rx = freq_index(i0, geo.n0) * geo.dx0 + geo.c0;
   const float ry = freq_index(i1, geo.n1) * geo.dx1 + geo.c1;
   const float rxy2 = rx * rx + ry * ry;
+  const float rzb = (float)(2 * tp) * geo.dx2 + geo.c2;  // z coordinate of position 2*tp
+  static_assert((M / 2) % P == 0, "negative-frequency slots must be whole");
   float acc[E][2];
 #pragma unroll
   for (int e = 0; e < E; ++e) { acc[e][0] = 0.f; acc[e][1] = 0.f; }
   for (int g = 0; g < sw.n; ++g) {
     const float2* src = work + ((long long)g * batch + t) * H + row0 * (M + 1);
-    __syncthreads();  // previous readers of buf are done, twiddles staged
-    load_prepass<M, T, NT>(buf, LDT, src, rows_per_map - row0, lz_in, tid, stwN);
-    __syncthreads();
+    // Opaque copies of the lane indices: index arithmetic derived from them is recomputed per
+    // sub-map instead of being hoisted out of the loop and held in (spilled) registers.
+    int lane_g = lane, tp_g = tp;
+    float rzb_g = rzb;
+    asm volatile("" : "+v"(lane_g), "+v"(tp_g), "+v"(rzb_g));
+    wave_lds_sync();  // the previous sub-map's reads of the tile are issued
+    load_prepass_rows<M, RW, 64>(buf, LDT, src, rows_per_map - row0, lz_in, lane_g, stwN);
+    wave_lds_sync();
     const int ell = sw.ell[g];
     const float cf = sw.coef[g];
-    // z-polynomial coefficients of this row: D[k] = sum_a poly rx^a ry^(ell-k-a)
-    float D[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-    if (ell > 0) {
-      int idx = 0;
-      for (int k = 0; k <= ell; ++k) {
-        float sacc = 0.f;
-        for (int a = 0; a <= ell - k; ++a) {
-          float mono = 1.f;
-          for (int q = 0; q < a; ++q) mono *= rx;
-          for (int q = 0; q < ell - k - a; ++q) mono *= ry;
-          sacc += sw.poly[g][idx++] * mono;
+    const float* pc = sw.poly[g];
+    pencil_inverse_head<M, true>(tp_g, rw, stw);
+    // Last inverse stage: adds w(x) * z straight into the accumulators.  w = coef Y(r_hat) =
+    // D(rz) / r^ell with the z-polynomial coefficients of this row D[k] = coef sum_a poly rx^a
+    // ry^(ell-k-a), in a form specialised to ell (uniform per sub-map).  Slot e of a lane holds z
+    // position m = tp + e*P, and m >= M/2 (negative frequency) exactly when e >= E/2, so the
+    // z coordinate of slot e is one FMA from the lane base rzb.
+    const float rzb_neg = rzb_g - (float)geo.n2 * geo.dx2;
+    auto rz_of = [&](int e) { return fmaf((float)(2 * e * P), geo.dx2, 2 * e >= E ? rzb_neg : rzb_g); };
+    switch (ell) {
+      case 0: {
+        auto sink = [&](int e, int, float2 z) { acc[e][0] += z.x * cf; acc[e][1] += z.y * cf; };
+        pencil_inverse_tail<M>(tp_g, rw, sink, stw);
+        break;
+      }
+      case 2: {
+        const float D0 = cf * (pc[0] * ry * ry + pc[1] * rx * ry + pc[2] * rx * rx);
+        const float D1 = cf * (pc[3] * ry + pc[4] * rx);
+        const float D2 = cf * pc[5];
+        // at r = 0 (only possible with rx = ry = rz = 0) pz = 0 exactly and Y_2m(0) = 0
+        auto sink = [&](int e, int, float2 z) {
+          const float rz0 = rz_of(e), rz1 = rz0 + geo.dx2;
+          const float w0 = fmaf(fmaf(D2, rz0, D1), rz0, D0) * __builtin_amdgcn_rcpf(fmaxf(fmaf(rz0, rz0, rxy2), 1e-30f));
+          const float w1 = fmaf(fmaf(D2, rz1, D1), rz1, D0) * __builtin_amdgcn_rcpf(fmaxf(fmaf(rz1, rz1, rxy2), 1e-30f));
+          acc[e][0] = fmaf(z.x, w0, acc[e][0]);
+          acc[e][1] = fmaf(z.y, w1, acc[e][1]);
+        };
+        pencil_inverse_tail<M>(tp_g, rw, sink, stw);
+        break;
+      }
+      case 4: {
+        const float rx2 = rx * rx, ry2 = ry * ry;
+        const float D0 = cf * (pc[0] * ry2 * ry2 + pc[1] * rx * ry2 * ry + pc[2] * rx2 * ry2 + pc[3] * rx2 * rx * ry +
+                               pc[4] * rx2 * rx2);
+        const float D1 = cf * (pc[5] * ry2 * ry + pc[6] * rx * ry2 + pc[7] * rx2 * ry + pc[8] * rx2 * rx);
+        const float D2 = cf * (pc[9] * ry2 + pc[10] * rx * ry + pc[11] * rx2);
+        const float D3 = cf * (pc[12] * ry + pc[13] * rx);
+        const float D4 = cf * pc[14];
+        const float y0c = sw.y0[g] * cf;  // Y_4m(0) is not zero for every m
+        auto sink = [&](int e, int, float2 z) {
+          float w[2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const float rz = rz_of(e) + h * geo.dx2;
+            const float r2 = fmaf(rz, rz, rxy2);
+            const float pz = fmaf(fmaf(fmaf(fmaf(D4, rz, D3), rz, D2), rz, D1), rz, D0);
+            const float ri2 = __builtin_amdgcn_rcpf(r2);
+            w[h] = r2 > 0.f ? pz * ri2 * ri2 : y0c;
+          }
+          acc[e][0] = fmaf(z.x, w[0], acc[e][0]);
+          acc[e][1] = fmaf(z.y, w[1], acc[e][1]);
+        };
+        pencil_inverse_tail<M>(tp_g, rw, sink, stw);
+        break;
+      }
+      default: {  // odd ell (1, 3): Y(0) = 0, and pz = 0 exactly at r = 0
+        float D[4] = {0.f, 0.f, 0.f, 0.f};
+        int idx = 0;
+        for (int k = 0; k <= ell; ++k) {
+          float sacc = 0.f;
+          for (int a = 0; a <= ell - k; ++a) {
+            float mono = 1.f;
+            for (int q = 0; q < a; ++q) mono *= rx;
+            for (int q = 0; q < ell - k - a; ++q) mono *= ry;
+            sacc += pc[idx++] * mono;
+          }
+          sacc *= cf;
+          if (k == 0) D[0] = sacc; else if (k == 1) D[1] = sacc; else if (k == 2) D[2] = sacc; else D[3] = sacc;
         }
-        if (k == 0) D[0] = sacc; else if (k == 1) D[1] = sacc; else if (k == 2) D[2] = sacc;
-        else if (k == 3) D[3] = sacc; else D[4] = sacc;
+        auto sink = [&](int e, int, float2 z) {
+          float w[2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const float rz = rz_of(e) + h * geo.dx2;
+            const float ri = rsqrtf(fmaxf(fmaf(rz, rz, rxy2), 1e-30f));
+            const float pz = fmaf(fmaf(fmaf(D[3], rz, D[2]), rz, D[1]), rz, D[0]);
+            w[h] = pz * (ell == 1 ? ri : ri * ri * ri);
+          }
+          acc[e][0] = fmaf(z.x, w[0], acc[e][0]);
+          acc[e][1] = fmaf(z.y, w[1], acc[e][1]);
+        };
+        pencil_inverse_tail<M>(tp_g, rw, sink, stw);
+        break;
       }
     }
-    const float y0 = sw.y0[g];
-    // inverse row transform; its last stage adds w(x) * z straight into the accumulators
-    auto sink = [&](int e, int m, float2 z) {
-      float w[2] = {cf, cf};
-      if (ell != 0) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const float rz = freq_index(2 * m + h, geo.n2) * geo.dx2 + geo.c2;
-          const float r2 = rxy2 + rz * rz;
-          float pz = D[4];
-          pz = pz * rz + D[3];
-          pz = pz * rz + D[2];
-          pz = pz * rz + D[1];
-          pz = pz * rz + D[0];
-          const float ri = r2 > 0.f ? rsqrtf(r2) : 0.f;
-          float rl = 1.f;
-          for (int q = 0; q < ell; ++q) rl *= ri;
-          w[h] *= r2 > 0.f ? pz * rl : y0;
-        }
-      }
-      acc[e][0] += z.x * w[0];
-      acc[e][1] += z.y * w[1];
-    };
-    pencil_inverse_to_regs<M>(tp, rw, sink, stw);
   }
-  __syncthreads();  // all reads of buf by the last inverse stage are done
+  wave_lds_sync();  // all reads of the tile by the last inverse stage are issued
   // mask and scale in registers, then the forward row transform (first stage from registers)
   const float norm = scale / (float)geo.N;
+  if (mask && row < rows_per_map) {
+    float2 mk[E];  // all mask loads in flight together
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int m = tp + e * RT::P;
-    acc[e][0] *= norm; acc[e][1] *= norm;
-    if (mask && row < rows_per_map) {
-      const float2 mk = *reinterpret_cast<const float2*>(mask + row * (2 * M) + 2 * m);
-      acc[e][0] *= mk.x; acc[e][1] *= mk.y;
-    }
+    for (int e = 0; e < E; ++e)
+      mk[e] = *reinterpret_cast<const float2*>(mask + row * (2 * M) + 2 * (tp + e * P));
+#pragma unroll
+    for (int e = 0; e < E; ++e) { acc[e][0] *= norm * mk[e].x; acc[e][1] *= norm * mk[e].y; }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) { acc[e][0] *= norm; acc[e][1] *= norm; }
   }
   auto get = [&](int e) { return make_float2(acc[e][0], acc[e][1]); };
-  pencil_forward_from_regs<M>(tp, get, rw, stw);
-  __syncthreads();
+  pencil_forward_from_regs<M, true>(tp, get, rw, stw);
+  wave_lds_sync();
   float2* o = work + t * H + row0 * (M + 1);
-  for (int f = tid; f < T * (M + 1); f += NT) {
+  constexpr int ITS = (RW * (M + 1) + 63) / 64;
+#pragma unroll
+  for (int it = 0; it < ITS; ++it) {
+    const int f = lane + it * 64;
     const int rr = f / (M + 1), k = f % (M + 1);
-    if (row0 + rr >= rows_per_map || k >= lz_out) continue;
+    if (f >= RW * (M + 1) || row0 + rr >= rows_per_map || k >= lz_out) continue;
     o[f] = r2c_split<M>(buf, LDT, rr, k, stwN);
   }
 }
@@ -712,9 +895,9 @@ static hipError_t launch_pointing_rows(int M, float2* work, long long rpm, int b
                                        const float2* twN, const Geo& geo, hipStream_t s) {
 #define PB3D_PROWS(MM)                                                                           \
   case MM: {                                                                                     \
-    constexpr int T = RowCfg<MM>::T;                                                             \
-    dim3 grid((unsigned)((rpm + T - 1) / T), batch);                                             \
-    pointing_rows<MM><<<grid, RowTile<MM>::NT, 0, s>>>(work, rpm, batch, sw, mask, scale, lz_in, lz_out, \
+    constexpr int RB = PointCfg<MM>::RB;                                                         \
+    dim3 grid((unsigned)((rpm + RB - 1) / RB), batch);                                           \
+    pointing_rows<MM><<<grid, PointCfg<MM>::NT, 0, s>>>(work, rpm, batch, sw, mask, scale, lz_in, lz_out, \
                                                          tw, twN, geo);                          \
     return hipGetLastError();                                                                    \
   }

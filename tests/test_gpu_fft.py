@@ -154,10 +154,16 @@ def test_pruned_transforms(hip):
     assert rel(got[0], ref) < 1e-5 and rel(got[1], 2 * ref) < 1e-5
 
 
-def test_fused_pointing_chain(hip):
+# (shape, l of the Y_lm sub-maps): row lengths n2/2 = 16, 128, 32, 64, 256 and every weight form
+# (l = 1..4); (4, 4, 256) has fewer rows than one block holds
+POINT_CASES = [((32, 32, 32), 2), ((4, 4, 256), 2), ((16, 8, 64), 2), ((8, 16, 128), 2), ((8, 8, 512), 2),
+               ((32, 32, 32), 1), ((32, 32, 32), 3), ((32, 32, 32), 4), ((8, 16, 128), 4)]
+
+
+@pytest.mark.parametrize("n,L", POINT_CASES, ids=lambda c: "x".join(map(str, c)) if isinstance(c, tuple) else "l%d" % c)
+def test_fused_pointing_chain(hip, n, L):
     """pb3d_fft_pointing = R2C(mask * scale * sum_g coef_g Y_g(x) C2R(kop_g(in))) composed from numpy."""
     from polybin3d_amd import _capi as C
-    n = (32, 32, 32)
     center = [400., 250., -300.]
     b = make_base(n, boxcenter=center, sightline='local')
     o = O.PolyBin3D(1000., list(n), boxcenter=center, sightline='local')
@@ -172,15 +178,15 @@ def test_fused_pointing_chain(hip):
     Md = b.as_real(mask)
     common = dict(bin_table=tab, bins=bins, x_batch_stride=0, support=sup)
     subs = [(C.kop(C.KOP_FILTER, ell=2, **common), Ad, 0, 0, 0.5)]
-    subs += [(C.kop(C.KOP_YLM, ell=2, m=m, **common), Ad, 2, m, 0.25) for m in range(5)]
+    subs += [(C.kop(C.KOP_YLM, ell=L, m=m, **common), Ad, L, m, 0.25) for m in range(2 * L + 1)]
     out = torch.empty((4,) + b.half_shape, dtype=torch.complex64, device='cuda')
     b.pointing(subs, 4, out, mask=Md, scale=1.5, kop_out=None)
-    Yr = O.real_harmonics(np.asarray(o.r_grids), 2, False)[2]
-    Yk = O.real_harmonics(np.asarray(o.k_grids), 2, False)[2]
+    Yr = O.real_harmonics(np.asarray(o.r_grids), L, True)[L]
+    Yk = O.real_harmonics(np.asarray(o.k_grids), L, True)[L]
     got = out.cpu().numpy()
     for q in range(4):
         r = 0.5 * o.to_real(o.map_utils.fourier_filter(A, 2, edges[q], edges[q + 1]))
-        for m in range(5):
+        for m in range(2 * L + 1):
             r += 0.25 * Yr[m] * o.to_real(o.map_utils.prod_fourier_filter(A, Yk[m], edges[q], edges[q + 1]))
         ref = o.to_fourier(1.5 * mask * r)
         assert rel(got[q], ref) < 2e-5
