@@ -203,7 +203,10 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": KIND_NAMES[dom],
                 "avg_launch_ms": round(ms / cnt, 4), "bytes_per_launch": by / cnt,
-                "sampled_launches": cnt, "share_of_sampled_fft_time": round(ms / sum(v[1] for v in kinds.values()), 3)}
+                "sampled_launches": cnt, "share_of_sampled_fft_time": round(ms / sum(v[1] for v in kinds.values()), 3),
+                "passes": {KIND_NAMES[k]: {"sampled_launches": v[0], "avg_launch_ms": round(v[1] / v[0], 4),
+                                           "GB_per_s": round(v[2] / (v[1] * 1e-3) / 1e9, 1)}
+                           for k, v in sorted(kinds.items())}}
     value = args.steps / dt
     res = {"metric": METRIC, "value": round(value, 6), "unit": "realizations/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 2), "higher_is_better": True,

@@ -22,7 +22,7 @@ def build(verbose=True, jobs=None, out=None, defines=()):
     """Compile every source to an object in parallel, then link libpb3d.so (or `out`, with extra
     -D `defines`, for variant builds)."""
     lib = out or LIB
-    objdir = os.path.join(HERE, "_build" if out is None else "_build_" + os.path.basename(out).replace(".", "_"))
+    objdir = os.path.join(HERE, "_build") if out is None else os.path.join(HERE, "_build", "variant_" + os.path.basename(out).replace(".", "_"))
     os.makedirs(objdir, exist_ok=True)
     t0 = time.time()
     procs = []

@@ -153,7 +153,7 @@ __device__ __forceinline__ void pencil_fft(int t, Src0 src0, Lds lds, DstN dstN,
 }
 
 #ifndef PB3D_COL_THREADS
-#define PB3D_COL_THREADS 512
+#define PB3D_COL_THREADS 1024
 #endif
 #ifndef PB3D_POINT_WAVES
 #define PB3D_POINT_WAVES 2
