@@ -91,6 +91,11 @@ typedef struct {
  *             FT_0(b1,b2).  FT_l(A,B) = ft + (l/2)*ft_lstride + pair_index[A*nk+B]*packed_len.
  *   5 PACK    R2C epilogue: out[batch*packed_len + pack_index[k]] = v for modes with
  *             pack_index >= 0 (the in-bin modes, stored bin-major).
+ *   6 BINPOLY v = x * w(k) * sum_{q<nk} coef[q*terms + bin(k)] * L_{2q}(mu) * (2 on the kz = n2/2
+ *             plane, else 1), coef = (const float*)ft, terms = number of bins, 0 outside every
+ *             bin: the mode-space weight of the Pk_ideal adjoint (no counterpart in the reference;
+ *             the factor 2 is the reference's degeneracy degk = 1 + (kz > 0), base.py:94, on the
+ *             self-conjugate Nyquist plane).
  * For R2C epilogues x is the transform output; out = scale*v or out += scale*v.
  * `x_batch_stride` (complex elements) advances the C2R input per batch entry (0 = broadcast:
  * one input map filtered into many bins). */
@@ -241,6 +246,26 @@ size_t pb3d_triplet_work_size(int64_t nvox, int ntrip, int precision);
 int pb3d_triplet_sum(const float* g, int nmaps, int64_t nvox, const int32_t* i1, const int32_t* i2,
                      const int32_t* i3, int ntrip, int precision, double* out, void* work,
                      void* stream);
+
+/* Triplet sums on the matrix cores, as one GEMM (assemble_b3 / sum_triplet, utils.pyx:357-366,
+ * 614-636, restated as T = G_A P^T):
+ *   out[a][p] = sum_x g[amap[a]](x) g[pb[p]](x) g[pc[p]](x)      (FP64 out[na][np])
+ * for na <= 128 A maps and np pairs of the float32 stack g[nmaps <= 256][nvox] (index arrays on
+ * the device).  precision 1 = split BF16, 3 products (hi*hi + hi*mid + mid*hi, ~2^-16 relative
+ * per product); 2 = split BF16, 6 products (~2^-24).  FP32 accumulation over runs of 1024 voxels,
+ * FP64 beyond, reduced across voxel chunks in a fixed order.  An index outside the stack gives NaN for its entries.
+ * work: pb3d_pair_gemm_work_size() bytes. */
+size_t pb3d_pair_gemm_work_size(int64_t nvox, int na, int np);
+int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int32_t* amap, int na, const int32_t* pb,
+                   const int32_t* pc, int np, int precision, double* out, void* work, void* stream);
+
+/* Adjoint of the triplet sums (no counterpart in the reference; used by the Bk_ideal gradient):
+ *   out[a][x] = sum_p U[a][p] g[pb[p]](x) g[pc[p]](x)           (float32 out[na][nvox])
+ * U: float32 [na <= 128][np] on the device; g: float32 stack [nmaps <= 128][nvox].  precision as
+ * for pb3d_pair_gemm.  work: pb3d_pair_adjoint_work_size() bytes. */
+size_t pb3d_pair_adjoint_work_size(int na, int np, int precision);
+int pb3d_pair_adjoint(const float* g, int nmaps, int64_t nvox, const int32_t* pb, const int32_t* pc, int np,
+                      const float* U, int na, int precision, float* out, void* work, void* stream);
 
 #ifdef __cplusplus
 }

@@ -7,5 +7,6 @@ ctypes in _capi.py; it must be built in-tree (python -m polybin3d_amd.build) bef
 from .base import PolyBin3D
 from .pspec import PSpec
 from .bspec import BSpec
+from . import autograd
 
-__all__ = ["PolyBin3D", "PSpec", "BSpec"]
+__all__ = ["PolyBin3D", "PSpec", "BSpec", "autograd"]

@@ -42,6 +42,12 @@ EXPORTS = {
     "pb3d_bin_reduce": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, ctypes.c_int,
                                        ctypes.c_int, c_void_p, c_void_p, ctypes.c_int, c_void_p, ctypes.c_int,
                                        c_void_p, c_void_p, c_void_p]),
+    "pb3d_pair_gemm_work_size": (c_size_t, [c_int64, ctypes.c_int, ctypes.c_int]),
+    "pb3d_pair_gemm": (ctypes.c_int, [c_void_p, ctypes.c_int, c_int64, c_void_p, ctypes.c_int, c_void_p, c_void_p,
+                                      ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
+    "pb3d_pair_adjoint_work_size": (c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pb3d_pair_adjoint": (ctypes.c_int, [c_void_p, ctypes.c_int, c_int64, c_void_p, c_void_p, ctypes.c_int, c_void_p,
+                                         ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
     "pb3d_triplet_work_size": (c_size_t, [c_int64, ctypes.c_int, ctypes.c_int]),
     "pb3d_triplet_sum": (ctypes.c_int, [c_void_p, ctypes.c_int, c_int64, c_void_p, c_void_p, c_void_p, ctypes.c_int,
                                         ctypes.c_int, c_void_p, c_void_p, c_void_p]),
@@ -79,7 +85,7 @@ class SubMap(ctypes.Structure):
 
 
 ROP_PLAIN, ROP_MUL, ROP_DIFF, ROP_YLM, ROP_MULYLM, ROP_SQUARE = range(6)
-KOP_PLAIN, KOP_FILTER, KOP_YLM, KOP_WEIGHT, KOP_BKQ, KOP_PACK = range(6)
+KOP_PLAIN, KOP_FILTER, KOP_YLM, KOP_WEIGHT, KOP_BKQ, KOP_PACK, KOP_BINPOLY = range(7)
 
 _lib = None
 

@@ -47,7 +47,7 @@ class BSpec:
 
     def __init__(self, base, k_bins, lmax=0, mask=None, applySinv=None, k_bins_squeeze=None,
                  include_partial_triangles=False, add_GIC=False, mask_IC=None, add_RIC=False, radial_bins_RIC=[],
-                 Pk_fid=None, mask_shot=None, applySinv_transpose=None, verb=False, precision='fp32'):
+                 Pk_fid=None, mask_shot=None, applySinv_transpose=None, verb=False, precision='bf16x6'):
         b = self.base = base
         self.k_bins = np.asarray(k_bins, dtype=np.float64)
         self.lmax = lmax
@@ -80,6 +80,8 @@ class BSpec:
         assert self.lmax <= 4, "Only l-max<=4 is currently supported!"
         if add_GIC or add_RIC:
             raise NotImplementedError("integral constraints are not implemented by backend='hip'")
+        if precision not in self._PREC:
+            raise ValueError("precision must be one of %s" % sorted(self._PREC))
         if mask is None:
             self.const_mask, self.mask_mean, self.cube_mask_mean = True, 1., 1.
         else:
@@ -164,8 +166,43 @@ class BSpec:
                     self._filtered_maps(leg, 0, out=out[(l // 2) * Nq:(l // 2 + 1) * Nq])
         return out
 
+    _PREC = {'fp32': 0, 'bf16x3': 1, 'bf16x6': 2}
+
     def _triplet(self, stack, i1, i2, i3):
-        """FP64 numpy array of sum_x stack[i1] stack[i2] stack[i3]."""
+        """FP64 numpy array of sum_x stack[i1] stack[i2] stack[i3] (assemble_b3 / sum_triplet,
+        utils.pyx:357-366, 614-636).  precision 'bf16x3' / 'bf16x6': one split-BF16 MFMA GEMM
+        T[a][p] = sum_x g_a g_b(p) g_c(p) over the distinct third maps a = i3 and pairs (i1, i2);
+        'fp32': FP32 products with FP64 accumulation on the vector ALU."""
+        prec = self._PREC[self.precision]
+        if prec > 0 and stack.shape[0] <= 256:
+            return self._triplet_mfma(stack, np.asarray(i1), np.asarray(i2), np.asarray(i3), prec)
+        return self._triplet_vec(stack, i1, i2, i3)
+
+    def _triplet_mfma(self, stack, i1, i2, i3, prec):
+        b = self.base
+        nm = stack.shape[0]
+        nvox = int(np.prod(stack.shape[1:]))
+        lo, hi = np.minimum(i1, i2), np.maximum(i1, i2)
+        ua, ia = np.unique(i3, return_inverse=True)
+        out = np.zeros(len(i1))
+        for c0 in range(0, len(ua), 128):
+            sel = np.nonzero((ia >= c0) & (ia < c0 + 128))[0]
+            amap = ua[c0:c0 + 128]
+            up, ip = np.unique(lo[sel] * nm + hi[sel], return_inverse=True)
+            dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(b.device)
+            amap_d, pb_d, pc_d = dev(amap), dev(up // nm), dev(up % nm)
+            na, npair = len(amap), len(up)
+            T = torch.empty((na, npair), dtype=torch.float64, device=b.device)
+            ws = C.lib().pb3d_pair_gemm_work_size(nvox, na, npair)
+            work = b.work('pair_gemm', ws)
+            C.check(C.lib().pb3d_pair_gemm(stack.data_ptr(), nm, nvox, amap_d.data_ptr(), na, pb_d.data_ptr(),
+                                           pc_d.data_ptr(), npair, prec, T.data_ptr(), work.data_ptr(), b.stream()))
+            rows = torch.from_numpy(ia[sel] - c0).to(b.device)
+            cols = torch.from_numpy(ip).to(b.device)
+            out[sel] = T[rows, cols].cpu().numpy()
+        return out
+
+    def _triplet_vec(self, stack, i1, i2, i3):
         b = self.base
         nm = stack.shape[0]
         nvox = int(np.prod(stack.shape[1:]))
@@ -341,6 +378,64 @@ class BSpec:
             res['b%s' % l] = out[idx:idx + self.N3]
             idx += self.N3
         return res
+
+    def _bk_ideal_adjoint(self, data, grad_bk):
+        """d(grad_bk . Bk_ideal)/d(data) for a real map `data` (global sightline; no counterpart in the
+        reference).  With u = (F^-T grad_bk) * V/N/sym, the g-maps g_m = C2R(F_m a d) are recomputed,
+        the adjoint maps Gamma_m = sum_i u_i d(g_i1 g_i2 g_i3)/d g_m come from one split-BF16 MFMA
+        contraction, and the gradient is C2R( sum_m F_m a R2C(Gamma_m) ) with F_(l,b) = Theta_b L_l."""
+        b = self.base
+        if b.sightline != 'global' and self.lmax > 0:
+            raise NotImplementedError("the Bk_ideal adjoint is implemented for the global sightline")
+        Nq = self.Nk_squeeze
+        Finv = torch.from_numpy(np.ascontiguousarray(self.inv_fish_ideal)).to(b.device)
+        u = (Finv.t() @ grad_bk.to(torch.float64).reshape(-1)).cpu().numpy()
+        u = u * b.volume / float(b.gridsize.prod()) / self.sym_factor
+        i1, i2, i3 = [np.asarray(v, dtype=np.int64) for v in self._b3_indices()]
+        Nm = self.Nl * Nq
+        # U[m][pair] with Gamma_m = sum_pairs U[m][pair] g_b g_c
+        rows = np.concatenate([i1, i2, i3])
+        pa = np.concatenate([i2, i1, i1])
+        pb_ = np.concatenate([i3, i3, i2])
+        keys = np.minimum(pa, pb_) * Nm + np.maximum(pa, pb_)
+        up, ip = np.unique(keys, return_inverse=True)
+        U = np.zeros((Nm, len(up)))
+        np.add.at(U, (rows, ip), np.concatenate([u, u, u]))
+        S = self._filtered_data(data, 'ideal')
+        g = self._g_maps(S)
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(b.device)
+        pb_d, pc_d = dev(up // Nm), dev(up % Nm)
+        U_d = torch.from_numpy(U.astype(np.float32)).to(b.device)
+        if Nm > 128:
+            raise NotImplementedError("the Bk_ideal adjoint supports at most 128 g-maps (Nl * Nk_squeeze)")
+        gam = torch.empty_like(g)
+        prec = 1 if self.precision == 'bf16x3' else 2
+        ws = C.lib().pb3d_pair_adjoint_work_size(Nm, len(up), prec)
+        work = b.work('pair_adjoint', ws)
+        C.check(C.lib().pb3d_pair_adjoint(g.data_ptr(), Nm, int(b.N), pb_d.data_ptr(), pc_d.data_ptr(), len(up),
+                                          U_d.data_ptr(), Nm, prec, gam.data_ptr(), work.data_ptr(), b.stream()))
+        del g
+        w = self.invPk0
+        if b.pixel_window != 'none':
+            w = self._filtered_weight_pw()
+        scale = 1. / self.cube_mask_mean ** (1. / 3.)
+        Z = torch.zeros(b.half_shape, dtype=torch.complex64, device=b.device)
+        for li in range(self.Nl):
+            for q in range(Nq):
+                b.r2c(gam[li * Nq + q], out=Z,
+                      kop=C.kop(C.KOP_FILTER, bin_table=self.table, bins=self._bins_dev[q:q + 1], ell=2 * li, w=w,
+                                scale=scale, accumulate=1))
+        return b.c2r(Z)
+
+    def _filtered_weight_pw(self):
+        b = self.base
+        key = ('bk_ideal_pw', id(self))
+        if key not in b._bufs:
+            pw = b.pixel_window_grid_host
+            ww = np.zeros(pw.shape)
+            ww[pw != 0] = self.invPk0_host[pw != 0] / pw[pw != 0]
+            b._bufs[key] = torch.from_numpy(ww.astype(np.float32)).to(b.device)
+        return b._bufs[key]
 
     # ------------------------------------------------------------------ linear-term simulations
     def apply_pointing(self, map_real, transpose=False):

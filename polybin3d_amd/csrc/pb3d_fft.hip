@@ -1037,7 +1037,9 @@ int check_rop(const pb3d_rop* r, bool input_side) {
 
 int check_kop(const pb3d_kop* k) {
   if (!k) return 0;
-  if (k->kind < 0 || k->kind > 5) return fail(PB3D_ERR_ARG, "kop.kind out of range");
+  if (k->kind < 0 || k->kind > 6) return fail(PB3D_ERR_ARG, "kop.kind out of range");
+  if (k->kind == 6 && (!k->bin_table || !k->ft || k->nk < 1 || k->nk > 4 || k->terms < 1 || k->terms > 254))
+    return fail(PB3D_ERR_ARG, "kop BINPOLY needs bin_table, coef (ft), 1<=nk<=4 and 1<=terms<=254");
   if (k->kind == 4 && (!k->bin_table || !k->pack_index || !k->trip || !k->pair_index || !k->ft ||
                        k->packed_len < 1 || k->nk < 1 || k->nk > 254 || k->trip0 < 0 ||
                        ((k->terms & 4) && (k->ell < 0 || k->ell > 4 || k->m < 0 || k->m > 2 * k->ell))))

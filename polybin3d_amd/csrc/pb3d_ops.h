@@ -185,6 +185,21 @@ __device__ __forceinline__ float2 kop_value(const KOp& op, const Geo& g, long lo
       }
       return acc;
     }
+    case 6: {
+      const int bin = op.bin_table[mode];
+      if (bin == PB3D_NO_BIN) return make_float2(0.f, 0.f);
+      const float* coef = reinterpret_cast<const float*>(op.ft);
+      float f = coef[bin];
+      if (op.nk > 1) {
+        float kx, ky, kz; kvec(g, i, j, l, kx, ky, kz);
+        const float k2 = kx * kx + ky * ky + kz * kz;
+        const float mu = k2 > 0.f ? (g.s0 * kx + g.s1 * ky + g.s2 * kz) * rsqrtf(k2) : 0.f;
+        for (int q = 1; q < op.nk; ++q) f += coef[q * op.terms + bin] * legendre_f(2 * q, mu);
+      }
+      if (op.w) f *= op.w[mode];
+      if ((g.n2 & 1) == 0 && l == g.n2h - 1) f *= 2.f;
+      return cscale(v, f);
+    }
     default: return v;
   }
 }

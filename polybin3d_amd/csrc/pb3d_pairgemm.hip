@@ -1,0 +1,342 @@
+// Triplet contractions as split-BF16 GEMMs on the gfx950 matrix cores (v_mfma_f32_32x32x16_bf16,
+// FP32 accumulation).
+//
+// Forward (assemble_b3 / sum_triplet, PolyBin3D/cython/utils.pyx:357-366, 614-636):
+//   T[a][p] = sum_x g[amap[a]](x) * g[pb[p]](x) * g[pc[p]](x)
+// i.e. the GEMM T = G_A . P^T with K = voxels and the pair products P_p = g_b g_c formed on the fly;
+// every triplet sum is one entry of T.  Adjoint (no counterpart in the reference):
+//   out[a][x] = sum_p U[a][p] * g[pb[p]](x) * g[pc[p]](x)
+// the GEMM U . P with voxels as the N axis.
+//
+// Split BF16: an FP32 operand x = hi + mid (+ lo), each part bf16 (x - hi - mid exact in FP32).
+// precision 1 = 3-term product (hi*hi + hi*mid + mid*hi: ~2^-16 relative per product),
+// precision 2 = 6-term product (adds hi*lo + lo*hi + mid*mid: ~2^-24, FP32-like).
+//
+// MFMA 32x32x16 lane layout (checked on the device): A[i][k] from lane l: i = l%32,
+// k = 8*(l/32) + r (r = 0..7 of an 8 x bf16 operand); B[k][j]: j = l%32, same k; accumulator
+// register v of lane l holds C[8*(v/4) + 4*(l/32) + v%4][l%32].
+#include "pb3d_ops.h"
+
+namespace pb3d {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+constexpr int kPgKS = 16;         // voxels (forward) / pairs (adjoint) per MFMA k-step
+constexpr int kPgWaves = 4;       // waves per block
+constexpr int kPgThreads = 64 * kPgWaves;
+constexpr int kPgMaxM = 128;      // A rows (maps) per launch: up to 4 M-tiles of 32
+constexpr int kPgMaxMaps = 256;   // maps in the stack the forward kernel stages
+
+// x = p[0] + p[1] (+ p[2]) in bf16 parts; NP = 2 or 3
+template <int NP>
+__device__ __forceinline__ void split8(const float* x, bf16x8* p) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    float v = x[r];
+    const __bf16 h = (__bf16)v;
+    v -= (float)h;
+    const __bf16 m = (__bf16)v;
+    p[0][r] = h;
+    p[1][r] = m;
+    if constexpr (NP == 3) {
+      v -= (float)m;
+      p[2][r] = (__bf16)v;
+    }
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ f32x16 mma_split(const bf16x8* a, const bf16x8* b, f32x16 c) {
+  if constexpr (NP == 3) {  // small terms first
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[2], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2], b[0], c, 0, 0, 0);
+  }
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[0], c, 0, 0, 0);
+  return c;
+}
+
+// ------------------------------------------------------------------------------------------
+// Forward.  Block (blockIdx.x = group of kPgWaves N-tiles of 32 pairs, blockIdx.y = voxel chunk):
+// stages kPgKS voxels of every map of the stack in LDS, splits the A rows once into bf16 parts
+// (shared by the waves), then each wave forms the products of its 32 pairs and runs NP-part
+// MFMAs for every M-tile.  Partial T of the chunk -> part[chunk][a][p] (FP64; the FP32 MFMA
+// accumulators are flushed into FP64 registers every kPgFlush steps).
+// ------------------------------------------------------------------------------------------
+constexpr int kPgFlush = 64;  // k-steps (1024 voxels) between FP32 -> FP64 accumulator flushes
+
+template <int NP, int MT>
+__global__ void __launch_bounds__(kPgThreads)
+pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int* __restrict__ amap, int na,
+              const int* __restrict__ pb, const int* __restrict__ pc, int np, long long chunk,
+              double* __restrict__ part) {
+  __shared__ float sg[kPgMaxMaps][kPgKS];              // staged map values [map][voxel]
+  __shared__ bf16x8 sa[NP][MT * 32][kPgKS / 8];        // split A rows [part][row][k/8]
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const long long x0 = (long long)blockIdx.y * chunk;
+  long long x1 = x0 + chunk;
+  if (x1 > nvox) x1 = nvox;
+  // this lane's pair (column j = lane%32 of the wave's N-tile)
+  const int p = (blockIdx.x * kPgWaves + wave) * 32 + lane % 32;
+  const bool pvalid = p < np;
+  int mb = pvalid ? pb[p] : 0, mc = pvalid ? pc[p] : 0;
+  // an index outside the stack gives NaN for that pair (no stray LDS read)
+  const bool pbad = mb < 0 || mb >= nmaps || mc < 0 || mc >= nmaps;
+  if (pbad) { mb = 0; mc = 0; }
+  const int kh = 8 * (lane / 32);  // this lane's k offset inside a step
+  f32x16 acc[MT];
+  double acc64[MT][16];
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) { acc[t][v] = 0.f; acc64[t][v] = 0.; }
+  int nstep = 0;
+  for (long long xs = x0; xs < x1; xs += kPgKS) {
+    if (++nstep == kPgFlush) {  // bound the FP32 accumulation length (uniform)
+      nstep = 0;
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) { acc64[t][v] += (double)acc[t][v]; acc[t][v] = 0.f; }
+    }
+    __syncthreads();  // previous step's readers of sg / sa are done
+    // stage: element e = m * kPgKS + k
+    for (int e = tid; e < nmaps * kPgKS; e += kPgThreads) {
+      const int m = e / kPgKS, k = e % kPgKS;
+      const long long x = xs + k;
+      sg[m][k] = x < x1 ? g[(long long)m * nvox + x] : 0.f;
+    }
+    __syncthreads();
+    // split the A rows (rows beyond na are zero)
+    for (int e = tid; e < MT * 32 * (kPgKS / 8); e += kPgThreads) {
+      const int row = e / (kPgKS / 8), kb = e % (kPgKS / 8);
+      float v[8];
+      int am = row < na ? amap[row] : 0;
+      const bool abad = am < 0 || am >= nmaps;
+      if (abad) am = 0;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = row >= na ? 0.f : (abad ? __builtin_nanf("") : sg[am][kb * 8 + r]);
+      bf16x8 parts[NP];
+      split8<NP>(v, parts);
+#pragma unroll
+      for (int q = 0; q < NP; ++q) sa[q][row][kb] = parts[q];
+    }
+    // B operand: this lane's 8 pair products
+    float pr[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) pr[r] = !pvalid ? 0.f : (pbad ? __builtin_nanf("") : sg[mb][kh + r] * sg[mc][kh + r]);
+    bf16x8 bparts[NP];
+    split8<NP>(pr, bparts);
+    __syncthreads();  // sa ready
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      bf16x8 aparts[NP];
+#pragma unroll
+      for (int q = 0; q < NP; ++q) aparts[q] = sa[q][t * 32 + lane % 32][lane / 32];
+      acc[t] = mma_split<NP>(aparts, bparts, acc[t]);
+    }
+  }
+  // partial results: C[row][col] with col = lane%32 (pair), row = 8*(v/4) + 4*(lane/32) + v%4
+  double* out = part + (long long)blockIdx.y * na * np;
+  if (pvalid) {
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int row = t * 32 + 8 * (v / 4) + 4 * (lane / 32) + v % 4;
+        if (row < na) out[(long long)row * np + p] = acc64[t][v] + (double)acc[t][v];
+      }
+  }
+}
+
+__global__ void pair_gemm_reduce(const double* __restrict__ part, int nchunks, long long n, double* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.;
+  for (int c = 0; c < nchunks; ++c) s += part[(long long)c * n + i];
+  out[i] = s;
+}
+
+// ------------------------------------------------------------------------------------------
+// Adjoint.  Block = kPgWaves N-tiles of 32 voxels; the maps of the block's voxels are staged in
+// LDS once; the K loop runs over pairs (kPgKS per step) with the A operand = split U rows read
+// from global memory (small, cache resident) and the B operand = pair products of this lane's
+// voxel.  Output rows a < na of out[a][x] (FP32).
+// ------------------------------------------------------------------------------------------
+constexpr int kAdjVox = 32 * kPgWaves;  // voxels per block
+constexpr int kAdjMaxMaps = 128;
+
+template <int NP, int MT>
+__global__ void __launch_bounds__(kPgThreads)
+pair_adjoint_kernel(const float* __restrict__ g, int nmaps, long long nvox, const int* __restrict__ pb,
+                    const int* __restrict__ pc, int np_pad, const bf16x8* __restrict__ usplit, int na,
+                    float* __restrict__ out) {
+  __shared__ float sg[kAdjMaxMaps][kAdjVox];
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const long long xb = (long long)blockIdx.x * kAdjVox;
+  for (int e = tid; e < nmaps * kAdjVox; e += kPgThreads) {
+    const int m = e / kAdjVox, k = e % kAdjVox;
+    const long long x = xb + k;
+    sg[m][k] = x < nvox ? g[(long long)m * nvox + x] : 0.f;
+  }
+  __syncthreads();
+  const int col = wave * 32 + lane % 32;  // this lane's voxel inside the block
+  const int kh = 8 * (lane / 32);
+  f32x16 acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+  const int ksteps = np_pad / kPgKS;
+  for (int s = 0; s < ksteps; ++s) {
+    const int p0 = s * kPgKS + kh;
+    float pr[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int b = pb[p0 + r], c = pc[p0 + r];
+      const bool ok = b >= 0 && b < nmaps && c >= 0 && c < nmaps;  // else NaN, no stray LDS read
+      pr[r] = ok ? sg[b][col] * sg[c][col] : __builtin_nanf("");
+    }
+    bf16x8 bparts[NP];
+    split8<NP>(pr, bparts);
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      bf16x8 aparts[NP];
+      // usplit layout: [part][row (MT*32 padded)][np_pad / 8] bf16x8
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        aparts[q] = usplit[((long long)q * (MT * 32) + t * 32 + lane % 32) * (np_pad / 8) + (p0 / 8)];
+      acc[t] = mma_split<NP>(aparts, bparts, acc[t]);
+    }
+  }
+  const long long x = xb + col;
+  if (x < nvox) {
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int row = t * 32 + 8 * (v / 4) + 4 * (lane / 32) + v % 4;
+        if (row < na) out[(long long)row * nvox + x] = acc[t][v];
+      }
+  }
+}
+
+// U [na][np] FP32 -> split parts [NP][rows][np_pad/8] bf16x8 (rows = MT*32, zero padded)
+template <int NP>
+__global__ void split_rows_kernel(const float* __restrict__ U, int na, int np, int rows, int np_pad,
+                                  bf16x8* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // (row, k8)
+  const long long n = (long long)rows * (np_pad / 8);
+  if (i >= n) return;
+  const int row = (int)(i / (np_pad / 8)), k8 = (int)(i % (np_pad / 8));
+  float v[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int p = k8 * 8 + r;
+    v[r] = (row < na && p < np) ? U[(long long)row * np + p] : 0.f;
+  }
+  bf16x8 parts[NP];
+  split8<NP>(v, parts);
+#pragma unroll
+  for (int q = 0; q < NP; ++q) out[(long long)q * n + i] = parts[q];
+}
+
+// pair list padded to a multiple of kPgKS with pair (0, 0) and zero U columns
+__global__ void pad_pairs_kernel(const int* __restrict__ pb, const int* __restrict__ pc, int np, int np_pad,
+                                 int* __restrict__ ob, int* __restrict__ oc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np_pad) return;
+  ob[i] = i < np ? pb[i] : 0;
+  oc[i] = i < np ? pc[i] : 0;
+}
+
+static int pg_mtiles(int na) { return (na + 31) / 32; }
+static long long pg_chunks(long long nvox, int ngroups) {
+  // enough blocks to fill the device several times over, chunks of at least 4096 voxels
+  long long c = (2048 + ngroups - 1) / ngroups;
+  const long long maxc = (nvox + 4095) / 4096;
+  if (c > maxc) c = maxc;
+  if (c < 1) c = 1;
+  return c;
+}
+static int pad16(int n) { return (n + kPgKS - 1) / kPgKS * kPgKS; }
+
+}  // namespace pb3d
+
+using namespace pb3d;
+
+extern "C" size_t pb3d_pair_gemm_work_size(int64_t nvox, int na, int np) {
+  if (nvox < 1 || na < 1 || np < 1) return 0;
+  const int ngroups = (np + 32 * kPgWaves - 1) / (32 * kPgWaves);
+  return (size_t)pg_chunks(nvox, ngroups) * na * np * sizeof(double);
+}
+
+extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int32_t* amap, int na, const int32_t* pb,
+                              const int32_t* pc, int np, int precision, double* out, void* work, void* stream) {
+  if (!g || !amap || !pb || !pc || !out || !work) return fail(PB3D_ERR_ARG, "pair_gemm: null pointer");
+  if (nmaps < 1 || nmaps > kPgMaxMaps) return fail(PB3D_ERR_ARG, "pair_gemm: nmaps out of [1, 256]");
+  if (nvox < 1 || na < 1 || na > kPgMaxM || np < 1) return fail(PB3D_ERR_ARG, "pair_gemm: na out of [1, 128] or empty input");
+  if (precision != 1 && precision != 2) return fail(PB3D_ERR_ARG, "pair_gemm: precision must be 1 (bf16 x3) or 2 (bf16 x6)");
+  hipStream_t s = (hipStream_t)stream;
+  const int ngroups = (np + 32 * kPgWaves - 1) / (32 * kPgWaves);
+  const long long nc = pg_chunks(nvox, ngroups);
+  long long chunk = (nvox + nc - 1) / nc;
+  chunk = (chunk + kPgKS - 1) / kPgKS * kPgKS;
+  dim3 grid((unsigned)ngroups, (unsigned)nc);
+  double* part = (double*)work;
+  const int mt = pg_mtiles(na);
+#define PB3D_PG(NP, MT) pair_gemm_fwd<NP, MT><<<grid, kPgThreads, 0, s>>>(g, nmaps, nvox, amap, na, pb, pc, np, chunk, part)
+  if (precision == 1) {
+    switch (mt) { case 1: PB3D_PG(2, 1); break; case 2: PB3D_PG(2, 2); break; case 3: PB3D_PG(2, 3); break; default: PB3D_PG(2, 4); }
+  } else {
+    switch (mt) { case 1: PB3D_PG(3, 1); break; case 2: PB3D_PG(3, 2); break; case 3: PB3D_PG(3, 3); break; default: PB3D_PG(3, 4); }
+  }
+#undef PB3D_PG
+  PB3D_CHECK_LAUNCH();
+  const long long n = (long long)na * np;
+  pair_gemm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, (int)nc, n, out);
+  PB3D_CHECK_LAUNCH();
+  return 0;
+}
+
+extern "C" size_t pb3d_pair_adjoint_work_size(int na, int np, int precision) {
+  if (na < 1 || np < 1) return 0;
+  const int nparts = precision == 2 ? 3 : 2;
+  const size_t rows = (size_t)pg_mtiles(na) * 32;
+  const size_t npp = (size_t)pad16(np);
+  return nparts * rows * npp * 2 + 2 * npp * sizeof(int32_t) + 256;
+}
+
+extern "C" int pb3d_pair_adjoint(const float* g, int nmaps, int64_t nvox, const int32_t* pb, const int32_t* pc, int np,
+                                 const float* U, int na, int precision, float* out, void* work, void* stream) {
+  if (!g || !pb || !pc || !U || !out || !work) return fail(PB3D_ERR_ARG, "pair_adjoint: null pointer");
+  if (nmaps < 1 || nmaps > kAdjMaxMaps) return fail(PB3D_ERR_ARG, "pair_adjoint: nmaps out of [1, 128]");
+  if (nvox < 1 || na < 1 || na > kPgMaxM || np < 1) return fail(PB3D_ERR_ARG, "pair_adjoint: na out of [1, 128] or empty input");
+  if (precision != 1 && precision != 2) return fail(PB3D_ERR_ARG, "pair_adjoint: precision must be 1 (bf16 x3) or 2 (bf16 x6)");
+  hipStream_t s = (hipStream_t)stream;
+  const int mt = pg_mtiles(na), rows = mt * 32, npp = pad16(np);
+  const int nparts = precision == 2 ? 3 : 2;
+  bf16x8* us = (bf16x8*)work;
+  const size_t ubytes = (size_t)nparts * rows * npp * 2;
+  int* ob = (int*)((char*)work + ((ubytes + 255) / 256) * 256);
+  int* oc = ob + npp;
+  const long long nsplit = (long long)rows * (npp / 8);
+  if (precision == 1) split_rows_kernel<2><<<(unsigned)((nsplit + 255) / 256), 256, 0, s>>>(U, na, np, rows, npp, us);
+  else split_rows_kernel<3><<<(unsigned)((nsplit + 255) / 256), 256, 0, s>>>(U, na, np, rows, npp, us);
+  PB3D_CHECK_LAUNCH();
+  pad_pairs_kernel<<<(npp + 255) / 256, 256, 0, s>>>(pb, pc, np, npp, ob, oc);
+  PB3D_CHECK_LAUNCH();
+  const unsigned blocks = (unsigned)((nvox + kAdjVox - 1) / kAdjVox);
+#define PB3D_PA(NP, MT) pair_adjoint_kernel<NP, MT><<<blocks, kPgThreads, 0, s>>>(g, nmaps, nvox, ob, oc, npp, us, na, out)
+  if (precision == 1) {
+    switch (mt) { case 1: PB3D_PA(2, 1); break; case 2: PB3D_PA(2, 2); break; case 3: PB3D_PA(2, 3); break; default: PB3D_PA(2, 4); }
+  } else {
+    switch (mt) { case 1: PB3D_PA(3, 1); break; case 2: PB3D_PA(3, 2); break; case 3: PB3D_PA(3, 3); break; default: PB3D_PA(3, 4); }
+  }
+#undef PB3D_PA
+  PB3D_CHECK_LAUNCH();
+  return 0;
+}

@@ -120,8 +120,12 @@ class PSpec:
         return b._bufs[key]
 
     def _compute_pk_numerator(self, data, filtering='ideal'):
-        """pspec.py:613-686.  Global: one bin pass for all multipoles.  Local: S_real, then for each
-        l>0 the harmonics sum (2l+1 fused R2Cs) and one cross-integral pass."""
+        """pspec.py:613-686 (numpy FP64 result)."""
+        return self._pk_numerator_dev(data, filtering).cpu().numpy()
+
+    def _pk_numerator_dev(self, data, filtering='ideal'):
+        """pspec.py:613-686 as an FP64 device tensor.  Global: one bin pass for all multipoles.  Local:
+        S_real, then for each l>0 the harmonics sum (2l+1 fused R2Cs) and one cross-integral pass."""
         assert filtering in ['ideal', 'Sinv'], "Unknown filtering option supplied!"
         b = self.base
         Nl = self.Nl if filtering == 'Sinv' else self.Nl_even
@@ -136,8 +140,44 @@ class PSpec:
             for li in range(1, Nl):
                 b.apply_fourier_harmonics(S_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=stack[li])
             out = b.bin_reduce(stack[0], self.k_bins, [0] * Nl, y=stack, y_ostride=b.H)[0]
-        Pk = 0.5 * out.reshape(-1).cpu().numpy()
-        return Pk * b.volume / b.gridsize.prod() ** 2
+        return out.reshape(-1) * (0.5 * b.volume / float(b.gridsize.prod()) ** 2)
+
+    def _ideal_weight_sq(self):
+        """a(k)^2 for the ideal filter S = a(k) d(k) (a = invPk0 [/W] / sqrt(<mask^2>)), FP32 device
+        half grid, or None when a is the constant 1/sqrt(<mask^2>)."""
+        b = self.base
+        key = ('pk_ideal_w2', id(self))
+        if key not in b._bufs:
+            if b.pixel_window != 'none':
+                pw = b.pixel_window_grid_host
+                w = np.zeros(pw.shape)
+                w[pw != 0] = b.invPk0_grid_host[pw != 0] / pw[pw != 0]
+            elif b.invPk0_is_one:
+                w = None
+            else:
+                w = b.invPk0_grid_host
+            b._bufs[key] = None if w is None else \
+                torch.from_numpy((w ** 2 / self.sq_mask_mean).astype(np.float32)).to(b.device)
+        return b._bufs[key]
+
+    def _pk_ideal_adjoint(self, data, grad_pk):
+        """d(grad_pk . Pk_ideal)/d(data) for a real map `data` (global sightline): with v = F^-T grad_pk,
+        the gradient is (V/N) C2R[ a^2 d(k) sum_l v[l, b(k)] L_l(mu) (x2 on the Nyquist plane) ], one
+        R2C and one fused-prologue C2R (no counterpart in the reference)."""
+        b = self.base
+        if b.sightline != 'global' and self.lmax > 0:
+            raise NotImplementedError("the Pk_ideal adjoint is implemented for the global sightline")
+        Nl, Nk = self.Nl_even, self.Nk
+        Finv = torch.from_numpy(np.ascontiguousarray(self.inv_fish_ideal)).to(b.device)
+        v = Finv.t() @ grad_pk.to(torch.float64).reshape(-1)
+        scale = b.volume / float(b.gridsize.prod())
+        w2 = self._ideal_weight_sq()
+        if w2 is None:
+            scale /= self.sq_mask_mean
+        coef = (v.reshape(Nl, Nk) * scale).to(torch.float32).contiguous()
+        Z = b.r2c(b.as_real(data))
+        return b.c2r(Z, kop=C.kop(C.KOP_BINPOLY, bin_table=b.bin_table(self.k_bins), w=w2, ft=coef, nk=Nl, terms=Nk,
+                                  support=b.support(self.k_bins[-1])))
 
     def Pk_numerator_ideal(self, data):
         """pspec.py:689-700."""
