@@ -89,18 +89,23 @@ def cpu_baseline(n, pk_bins, bk_bins, lmax_pk, lmax_bk, N_bins_bk):
     rng = np.random.RandomState(0)
     x = rng.randn(n, n, n)
 
-    def med(f, r=3):
+    reps = {}
+
+    def med(f, budget=2.5, name=None):
+        """median time of f over repetitions filling about `budget` seconds (at least 3)"""
         f()
         ts = []
-        for _ in range(r):
+        while len(ts) < 3 or sum(ts) < budget:
             a = time.perf_counter()
             f()
             ts.append(time.perf_counter() - a)
+        if name:
+            reps[name] = len(ts)
         return float(np.median(ts))
 
     X = scipy.fft.rfftn(x, workers=cores)
-    t_f = med(lambda: scipy.fft.rfftn(x, workers=cores))
-    t_i = med(lambda: scipy.fft.irfftn(X, s=(n, n, n), workers=cores))
+    t_f = med(lambda: scipy.fft.rfftn(x, workers=cores), name='rfftn')
+    t_i = med(lambda: scipy.fft.irfftn(X, s=(n, n, n), workers=cores), name='irfftn')
     kF = 2 * np.pi / 1000.
     kx = np.fft.fftshift(np.arange(-n // 2, n // 2)) * kF
     kz = np.arange(n // 2 + 1) * kF
@@ -117,9 +122,9 @@ def cpu_baseline(n, pk_bins, bk_bins, lmax_pk, lmax_bk, N_bins_bk):
                                       ctypes.c_double, ctypes.c_double, dp]
     P = lambda a: a.ctypes.data_as(dp)  # noqa: E731
     t_cross = med(lambda: lib.cpu_cross_integrate(P(modk), P(degk), s1.ctypes.data, s1.ctypes.data, H, P(edges),
-                                                  len(edges) - 1, P(out)))
+                                                  len(edges) - 1, P(out)), name='cross_integrate')
     t_row = med(lambda: lib.cpu_integrate_row(P(modk), P(degk), s1.ctypes.data, arr.ctypes.data, H, nout, 0.1, 0.11,
-                                              P(out)))
+                                              P(out)), name='integrate_row')
     t_visit = t_row / nout
     Nk_pk = len(pk_bins) - 1
     Nl_pk = lmax_pk // 2 + 1
@@ -129,12 +134,14 @@ def cpu_baseline(n, pk_bins, bk_bins, lmax_pk, lmax_bk, N_bins_bk):
     # Pk rows: Nl*Nk Q maps x Nl cross_integrate calls (pspec.py:394-413); Bk rows: one
     # integrate_row over all N_bins Q maps per row (bspec.py:597)
     T = (pk_f + bk_f) * t_f + (pk_r + bk_r) * t_i + Nl_pk * Nk_pk * Nl_pk * t_cross + N_bins_bk ** 2 * t_visit
-    sample = ("extrapolated: timed at %d^3 on %d host threads rfftn %.3f s, irfftn %.3f s (scipy.fft FP64), one "
-              "%d-bin cross_integrate sweep %.3f s, integrate_row %.2e s per map-sweep (C/OpenMP restatement of the "
-              "reference loops), scaled by %d+%d FFTs, %d cross_integrate calls and %d^2 integrate_row map-sweeps per "
-              "realization; element-wise passes not counted; %.0f s of CPU work in this run"
-              % (n, cores, t_f, t_i, Nk_pk, t_cross, t_visit, pk_f + bk_f, pk_r + bk_r, Nl_pk * Nk_pk * Nl_pk,
-                 N_bins_bk, time.perf_counter() - t_start))
+    sample = ("extrapolated: median times at %d^3 on %d host threads, each op repeated to fill ~2.5 s "
+              "(rfftn x%d: %.3f s, irfftn x%d: %.3f s, scipy.fft FP64; one %d-bin cross_integrate sweep x%d: %.3f s; "
+              "integrate_row over %d maps x%d: %.2e s per map-sweep; C/OpenMP restatement of the reference loops), "
+              "scaled by the reference's %d+%d FFTs, %d cross_integrate calls and %d^2 integrate_row map-sweeps per "
+              "realization; element-wise passes not counted; %.1f s of CPU work in this run"
+              % (n, cores, reps['rfftn'], t_f, reps['irfftn'], t_i, Nk_pk, reps['cross_integrate'], t_cross, nout,
+                 reps['integrate_row'], t_visit, pk_f + bk_f, pk_r + bk_r, Nl_pk * Nk_pk * Nl_pk, N_bins_bk,
+                 time.perf_counter() - t_start))
     return {"value": 1.0 / T, "unit": "realizations/s", "cores": cores, "kind": "port", "sample": sample,
             "seconds_per_realization": T}
 
@@ -147,6 +154,13 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-period", type=int, default=8)
+    ap.add_argument("--workload", default="mc_fisher", choices=["mc_fisher", "bk_ideal"],
+                    help="mc_fisher: the Pk+Bk MC Fisher metric (default); bk_ideal: Bk_ideal evaluations "
+                         "and the MFMA triplet contraction (--n, --bk-bins, --lmax, --precision)")
+    ap.add_argument("--bk-bins", type=int, default=16)
+    ap.add_argument("--bk-kmax", type=float, default=0.17)
+    ap.add_argument("--lmax", type=int, default=0)
+    ap.add_argument("--precision", default="bf16x6", choices=["fp32", "bf16x3", "bf16x6"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the Fisher realizations of one device and has no "
@@ -156,6 +170,8 @@ def main():
     import torch
     import polybin3d_amd as pb
     from polybin3d_amd import _capi as C
+    if args.workload == "bk_ideal":
+        return bk_ideal_workload(args, torch, pb)
 
     n, L, center = args.n, 1000., [10., 10., 10.]
     kny = n * np.pi / L
@@ -219,6 +235,72 @@ def main():
            "roofline": roof}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(n, pk_bins, bk_bins, 4, 2, bs.N_bins)
+    print(json.dumps(res), flush=True)
+
+
+def bk_ideal_workload(args, torch, pb):
+    """Bk_ideal on a quadratic non-Gaussian field (delta_G + (delta_G^2 - <delta_G^2>)/2, power-law GRF),
+    global sightline, linear bins 0.01..bk_kmax.  One step = one Bk_ideal evaluation (1 R2C, Nl*Nk
+    filtered C2Rs, the triplet contraction, the cached inverse ideal Fisher).  The triplet kernel is
+    timed with HIP events on the launch stream; its bf16 MFMA rate counts the split products executed
+    (3 or 6 per multiply-add) over the dense T[a][p] block."""
+    n, L = args.n, 1000.
+    edges = np.linspace(0.01, args.bk_kmax, args.bk_bins + 1)
+    b = pb.PolyBin3D(L, [n] * 3, Pk=powerlaw_pk(2 * n * np.pi / L * 1.8), rng='device')
+    bs = pb.BSpec(b, edges, lmax=args.lmax, precision=args.precision)
+    dG = b.generate_data(seed=0, output_type='real')
+    field = (dG + 0.5 * (dG ** 2 - (dG ** 2).mean())).contiguous()
+    del dG
+    bs.Bk_ideal(field)  # ideal Fisher and symmetry factors (data independent), first call
+    orig = bs._triplet
+    trip_ms, trip_calls = [], []
+
+    def timed_triplet(stack, i1, i2, i3):
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        r = orig(stack, i1, i2, i3)
+        e1.record(s)
+        e1.synchronize()
+        trip_ms.append(e0.elapsed_time(e1))
+        i3 = np.asarray(i3)
+        pairs = np.unique(np.minimum(i1, i2) * stack.shape[0] + np.maximum(i1, i2))
+        trip_calls.append((len(np.unique(i3)), len(pairs), int(np.prod(stack.shape[1:])), len(i3)))
+        return r
+    bs._triplet = timed_triplet
+    for _ in range(args.warmup):
+        bs.Bk_ideal(field)
+    trip_ms.clear(); trip_calls.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bs.Bk_ideal(field)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nterm = {"fp32": 1, "bf16x3": 3, "bf16x6": 6}[args.precision]
+    pad = lambda v, m: (v + m - 1) // m * m  # noqa: E731
+    flops = sum(2. * pad(na, 32) * pad(np_, 32) * nv * nterm for na, np_, nv, _ in trip_calls)
+    useful = sum(2. * na * np_ * nv for na, np_, nv, _ in trip_calls)
+    vec = sum(3. * nt * nv for _, _, nv, nt in trip_calls)  # 2 multiplies + 1 add per triplet and voxel
+    ms = sum(trip_ms)
+    mfma = args.precision != "fp32"
+    res = {"metric": "Bk_ideal evaluations/s", "value": round(args.steps / dt, 4), "unit": "evaluations/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": args.precision, "data": "synthetic",
+           "config": {"workload": "bk_ideal_%d" % n, "grid": [n] * 3, "boxsize": L, "sightline": "global",
+                      "lmax": args.lmax, "nbins": args.bk_bins, "kmax": args.bk_kmax, "N_bins": bs.N_bins,
+                      "triplet_precision": args.precision},
+           "triplet": {"ms_per_step": round(ms / args.steps, 3), "share_of_step": round(ms * 1e-3 / dt, 3),
+                       "A_maps_pairs_voxels_triplets": trip_calls[0] if trip_calls else None,
+                       "fp32_equivalent_TFLOP_s": round(useful / (ms * 1e-3) / 1e12, 2)},
+           "roofline": ({"bound": "mfma", "achieved": round(flops / (ms * 1e-3) / 1e12, 2), "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": round(flops / (ms * 1e-3) / 1e12 / 2500.0, 4), "traffic": None,
+                         "kernel": "pair_gemm_fwd (bf16 MFMA, %d products per FP32 multiply-add)" % nterm}
+                        if mfma else {"bound": "valu", "achieved": round(vec / (ms * 1e-3) / 1e12, 2),
+                                      "peak": 157.3, "unit": "TFLOP/s",
+                                      "frac": round(vec / (ms * 1e-3) / 1e12 / 157.3, 4), "traffic": None,
+                                      "kernel": "triplet_vec_kernel (FP32 products, FP64 sums, vector ALU)"})}
     print(json.dumps(res), flush=True)
 
 

@@ -252,8 +252,8 @@ int pb3d_triplet_sum(const float* g, int nmaps, int64_t nvox, const int32_t* i1,
  *   out[a][p] = sum_x g[amap[a]](x) g[pb[p]](x) g[pc[p]](x)      (FP64 out[na][np])
  * for na <= 128 A maps and np pairs of the float32 stack g[nmaps <= 256][nvox] (index arrays on
  * the device).  precision 1 = split BF16, 3 products (hi*hi + hi*mid + mid*hi, ~2^-16 relative
- * per product); 2 = split BF16, 6 products (~2^-24).  FP32 accumulation over runs of 1024 voxels,
- * FP64 beyond, reduced across voxel chunks in a fixed order.  An index outside the stack gives NaN for its entries.
+ * per product); 2 = split BF16, 6 products (~2^-24).  Two-level FP32 accumulation (runs of 1024
+ * voxels, then their sum) within a voxel chunk, FP64 across chunks in a fixed order.  An index outside the stack gives NaN for its entries.
  * work: pb3d_pair_gemm_work_size() bytes. */
 size_t pb3d_pair_gemm_work_size(int64_t nvox, int na, int np);
 int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int32_t* amap, int na, const int32_t* pb,

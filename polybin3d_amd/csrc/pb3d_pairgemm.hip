@@ -61,20 +61,30 @@ __device__ __forceinline__ f32x16 mma_split(const bf16x8* a, const bf16x8* b, f3
 
 // ------------------------------------------------------------------------------------------
 // Forward.  Block (blockIdx.x = group of kPgWaves N-tiles of 32 pairs, blockIdx.y = voxel chunk):
-// stages kPgKS voxels of every map of the stack in LDS, splits the A rows once into bf16 parts
-// (shared by the waves), then each wave forms the products of its 32 pairs and runs NP-part
-// MFMAs for every M-tile.  Partial T of the chunk -> part[chunk][a][p] (FP64; the FP32 MFMA
-// accumulators are flushed into FP64 registers every kPgFlush steps).
+// stages kPgStage voxels of every map of the stack in LDS (the next stage's loads are issued into
+// registers before the current stage is used), splits the A rows once into bf16 parts shared by
+// the waves, then each wave forms the products of its 32 pairs and runs NP-part MFMAs for every
+// M-tile, kPgStage/kPgKS k-steps per stage.  Partial T of the chunk -> part[chunk][a][p] (FP64),
+// accumulated in two FP32 levels (MFMA accumulator over 1024 voxels, then a second accumulator).
+// LDS (dynamic): sg[nmaps][kPgStage] floats, then sa[NP][MT*32][kPgStage/8] bf16x8.
 // ------------------------------------------------------------------------------------------
-constexpr int kPgFlush = 64;  // k-steps (1024 voxels) between FP32 -> FP64 accumulator flushes
+constexpr int kPgStage = 32;                          // voxels per LDS stage
+constexpr int kPgFlush = 32;                          // stages (1024 voxels) per FP64 flush
 
-template <int NP, int MT>
+static size_t pg_fwd_lds(int nmaps, int np_parts, int mt) {
+  return (size_t)nmaps * kPgStage * sizeof(float) + (size_t)np_parts * mt * 32 * (kPgStage / 8) * 16;
+}
+
+// PF = prefetch registers per thread: nmaps * kPgStage <= PF * kPgThreads
+template <int NP, int MT, int PF>
 __global__ void __launch_bounds__(kPgThreads)
 pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int* __restrict__ amap, int na,
               const int* __restrict__ pb, const int* __restrict__ pc, int np, long long chunk,
               double* __restrict__ part) {
-  __shared__ float sg[kPgMaxMaps][kPgKS];              // staged map values [map][voxel]
-  __shared__ bf16x8 sa[NP][MT * 32][kPgKS / 8];        // split A rows [part][row][k/8]
+  extern __shared__ __align__(16) char smem[];
+  float* sg = reinterpret_cast<float*>(smem);                                         // [nmaps][kPgStage]
+  bf16x8* sa = reinterpret_cast<bf16x8*>(smem + (size_t)nmaps * kPgStage * sizeof(float));
+  constexpr int K8 = kPgStage / 8;                                                     // bf16x8 per row
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const long long x0 = (long long)blockIdx.y * chunk;
   long long x1 = x0 + chunk;
@@ -86,57 +96,80 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
   // an index outside the stack gives NaN for that pair (no stray LDS read)
   const bool pbad = mb < 0 || mb >= nmaps || mc < 0 || mc >= nmaps;
   if (pbad) { mb = 0; mc = 0; }
-  const int kh = 8 * (lane / 32);  // this lane's k offset inside a step
-  f32x16 acc[MT];
-  double acc64[MT][16];
+  const int kh = 8 * (lane / 32);  // this lane's k offset inside a k-step
+  // two-level FP32 accumulation: the MFMA accumulator acc covers at most kPgFlush stages (1024
+  // voxels), then is added into acc2 (rounding grows like sqrt(steps per level), not sqrt(steps))
+  f32x16 acc[MT], acc2[MT];
 #pragma unroll
   for (int t = 0; t < MT; ++t)
 #pragma unroll
-    for (int v = 0; v < 16; ++v) { acc[t][v] = 0.f; acc64[t][v] = 0.; }
-  int nstep = 0;
-  for (long long xs = x0; xs < x1; xs += kPgKS) {
-    if (++nstep == kPgFlush) {  // bound the FP32 accumulation length (uniform)
-      nstep = 0;
+    for (int v = 0; v < 16; ++v) { acc[t][v] = 0.f; acc2[t][v] = 0.f; }
+  // staged element e = m * kPgStage + k, k = e % kPgStage: thread tid loads e = tid + i * kPgThreads
+  const int nel = nmaps * kPgStage;
+  float pf[PF];
+  auto fetch = [&](long long xs) {
 #pragma unroll
-      for (int t = 0; t < MT; ++t)
+    for (int i = 0; i < PF; ++i) {
+      const int e = tid + i * kPgThreads;
+      float v = 0.f;
+      if (e < nel) {
+        const long long x = xs + (e % kPgStage);
+        if (x < x1) v = g[(long long)(e / kPgStage) * nvox + x];
+      }
+      pf[i] = v;
+    }
+  };
+  if (x0 < x1) fetch(x0);
+  int nstage = 0;
+  for (long long xs = x0; xs < x1; xs += kPgStage) {
+    __syncthreads();  // previous stage's readers of sg / sa are done
 #pragma unroll
-        for (int v = 0; v < 16; ++v) { acc64[t][v] += (double)acc[t][v]; acc[t][v] = 0.f; }
+    for (int i = 0; i < PF; ++i) {
+      const int e = tid + i * kPgThreads;
+      if (e < nel) sg[e] = pf[i];
     }
-    __syncthreads();  // previous step's readers of sg / sa are done
-    // stage: element e = m * kPgKS + k
-    for (int e = tid; e < nmaps * kPgKS; e += kPgThreads) {
-      const int m = e / kPgKS, k = e % kPgKS;
-      const long long x = xs + k;
-      sg[m][k] = x < x1 ? g[(long long)m * nvox + x] : 0.f;
-    }
+    if (xs + kPgStage < x1) fetch(xs + kPgStage);  // next stage in flight while this one computes
     __syncthreads();
     // split the A rows (rows beyond na are zero)
-    for (int e = tid; e < MT * 32 * (kPgKS / 8); e += kPgThreads) {
-      const int row = e / (kPgKS / 8), kb = e % (kPgKS / 8);
+    for (int e = tid; e < MT * 32 * K8; e += kPgThreads) {
+      const int row = e / K8, kb = e % K8;
       float v[8];
       int am = row < na ? amap[row] : 0;
       const bool abad = am < 0 || am >= nmaps;
       if (abad) am = 0;
 #pragma unroll
-      for (int r = 0; r < 8; ++r) v[r] = row >= na ? 0.f : (abad ? __builtin_nanf("") : sg[am][kb * 8 + r]);
+      for (int r = 0; r < 8; ++r)
+        v[r] = row >= na ? 0.f : (abad ? __builtin_nanf("") : sg[am * kPgStage + kb * 8 + r]);
       bf16x8 parts[NP];
       split8<NP>(v, parts);
 #pragma unroll
-      for (int q = 0; q < NP; ++q) sa[q][row][kb] = parts[q];
+      for (int q = 0; q < NP; ++q) sa[((long long)q * MT * 32 + row) * K8 + kb] = parts[q];
     }
-    // B operand: this lane's 8 pair products
-    float pr[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) pr[r] = !pvalid ? 0.f : (pbad ? __builtin_nanf("") : sg[mb][kh + r] * sg[mc][kh + r]);
-    bf16x8 bparts[NP];
-    split8<NP>(pr, bparts);
     __syncthreads();  // sa ready
 #pragma unroll
-    for (int t = 0; t < MT; ++t) {
-      bf16x8 aparts[NP];
+    for (int ks = 0; ks < kPgStage / kPgKS; ++ks) {
+      // B operand: this lane's 8 pair products of k-step ks
+      float pr[8];
+      const int k0 = ks * kPgKS + kh;
 #pragma unroll
-      for (int q = 0; q < NP; ++q) aparts[q] = sa[q][t * 32 + lane % 32][lane / 32];
-      acc[t] = mma_split<NP>(aparts, bparts, acc[t]);
+      for (int r = 0; r < 8; ++r)
+        pr[r] = !pvalid ? 0.f : (pbad ? __builtin_nanf("") : sg[mb * kPgStage + k0 + r] * sg[mc * kPgStage + k0 + r]);
+      bf16x8 bparts[NP];
+      split8<NP>(pr, bparts);
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        bf16x8 aparts[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) aparts[q] = sa[((long long)q * MT * 32 + t * 32 + lane % 32) * K8 + k0 / 8];
+        acc[t] = mma_split<NP>(aparts, bparts, acc[t]);
+      }
+    }
+    if (++nstage == kPgFlush) {  // bound the first-level accumulation length (uniform)
+      nstage = 0;
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) { acc2[t][v] += acc[t][v]; acc[t][v] = 0.f; }
     }
   }
   // partial results: C[row][col] with col = lane%32 (pair), row = 8*(v/4) + 4*(lane/32) + v%4
@@ -147,7 +180,7 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int row = t * 32 + 8 * (v / 4) + 4 * (lane / 32) + v % 4;
-        if (row < na) out[(long long)row * np + p] = acc64[t][v] + (double)acc[t][v];
+        if (row < na) out[(long long)row * np + p] = (double)acc2[t][v] + (double)acc[t][v];
       }
   }
 }
@@ -284,17 +317,27 @@ extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int
   const int ngroups = (np + 32 * kPgWaves - 1) / (32 * kPgWaves);
   const long long nc = pg_chunks(nvox, ngroups);
   long long chunk = (nvox + nc - 1) / nc;
-  chunk = (chunk + kPgKS - 1) / kPgKS * kPgKS;
+  chunk = (chunk + kPgStage - 1) / kPgStage * kPgStage;
   dim3 grid((unsigned)ngroups, (unsigned)nc);
   double* part = (double*)work;
   const int mt = pg_mtiles(na);
-#define PB3D_PG(NP, MT) pair_gemm_fwd<NP, MT><<<grid, kPgThreads, 0, s>>>(g, nmaps, nvox, amap, na, pb, pc, np, chunk, part)
+  const size_t lds = pg_fwd_lds(nmaps, precision == 2 ? 3 : 2, mt < 4 ? mt : 4);
+  const int pfb = nmaps <= 32 ? 0 : nmaps <= 64 ? 1 : nmaps <= 128 ? 2 : 3;  // prefetch depth bucket
+#define PB3D_PG3(NP, MT, PF) pair_gemm_fwd<NP, MT, PF><<<grid, kPgThreads, lds, s>>>(g, nmaps, nvox, amap, na, pb, pc, np, chunk, part)
+#define PB3D_PG(NP, MT)                                                   \
+  switch (pfb) {                                                          \
+    case 0: PB3D_PG3(NP, MT, 32 * kPgStage / kPgThreads); break;          \
+    case 1: PB3D_PG3(NP, MT, 64 * kPgStage / kPgThreads); break;          \
+    case 2: PB3D_PG3(NP, MT, 128 * kPgStage / kPgThreads); break;         \
+    default: PB3D_PG3(NP, MT, 256 * kPgStage / kPgThreads); break;        \
+  }
   if (precision == 1) {
     switch (mt) { case 1: PB3D_PG(2, 1); break; case 2: PB3D_PG(2, 2); break; case 3: PB3D_PG(2, 3); break; default: PB3D_PG(2, 4); }
   } else {
     switch (mt) { case 1: PB3D_PG(3, 1); break; case 2: PB3D_PG(3, 2); break; case 3: PB3D_PG(3, 3); break; default: PB3D_PG(3, 4); }
   }
 #undef PB3D_PG
+#undef PB3D_PG3
   PB3D_CHECK_LAUNCH();
   const long long n = (long long)na * np;
   pair_gemm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, (int)nc, n, out);
