@@ -548,10 +548,8 @@ class BSpec:
         b = self.base
         pk = self._packed_layout()
         M, Nq = pk['M'], self.Nk_squeeze
-        ab = self.all_bins
+        ab = self.all_bins[self._qperm]  # rows of QA follow this order (sorted by l, then b3)
         local = b.sightline == 'local'
-        common = dict(bin_table=self.table, pack_index=pk['pack_index'], packed_len=M, trip=self._trip_dev,
-                      pair_index=pair_tab, ft=FT, ft_lstride=FT.shape[1] * M, nk=Nq, support=self._sup)
         pack = dict(pack_index=pk['pack_index'], packed_len=M, support=self._sup)
         t0 = 0
         while t0 < self.N_bins:
@@ -563,6 +561,11 @@ class BSpec:
             while t1 < self.N_bins and t1 - t0 < CH and ab[t1, 3] == ell:
                 t1 += 1
             nb = t1 - t0
+            # before the mask, the Q maps of triplets (b1 <= b2 <= b3) live on modes below edge[b3+1]:
+            # the inverse passes are pruned to the chunk's largest b3 (the forward passes keep k_max)
+            sup_in = b.support(self.k_bins_squeeze[int(ab[t0:t1, 2].max()) + 1])
+            common = dict(bin_table=self.table, pack_index=pk['pack_index'], packed_len=M, trip=self._trip_dev,
+                          pair_index=pair_tab, ft=FT, ft_lstride=FT.shape[1] * M, nk=Nq, support=sup_in)
             terms = 2 if ylm else 3
             if self.const_mask:
                 # P = mask_mean: the Theta terms need no real-space step (bspec.py:662-664)
@@ -601,7 +604,10 @@ class BSpec:
         M, Nq, NB = pk['M'], self.Nk_squeeze, self.N_bins
         ab = self.all_bins
         if not hasattr(self, '_trip_dev'):
-            self._trip_dev = torch.from_numpy(ab.astype(np.int32).reshape(-1)).to(b.device)
+            # Q maps are built in the order (l, b3) so that each chunk has a tight input band limit;
+            # the Fisher columns are put back in triplet order after the contraction
+            self._qperm = np.lexsort((ab[:, 2], ab[:, 3]))
+            self._trip_dev = torch.from_numpy(ab[self._qperm].astype(np.int32).reshape(-1)).to(b.device)
             ainv = set()
             for b1, b2, b3, _ in ab:
                 ainv |= {(b1, b2), (b2, b3), (b1, b3)}
@@ -653,6 +659,10 @@ class BSpec:
             Ar = torch.view_as_real(A).reshape(len(rows), 2 * (s1 - s0))
             Br = QAr[:, s0:s1].reshape(NB, 2 * (s1 - s0))
             F[torch.from_numpy(rows).to(b.device)] = torch.matmul(Ar, Br.t()).double()
+        # columns of F follow the Q-map order: F[:, qperm[p]] = F_perm[:, p]
+        inv = np.empty_like(self._qperm)
+        inv[self._qperm] = np.arange(NB)
+        F = F[:, torch.from_numpy(inv).to(b.device)]
         sym = torch.from_numpy(self.sym_factor).to(b.device)
         F *= 0.5 * b.volume / b.gridsize.prod() ** 2
         F /= sym[:, None] * sym[None, :]
