@@ -22,13 +22,15 @@ How each part is pinned (tests/test_oracle_*.py):
   * Pk_ideal: plane-wave known answer P = A^2 V / (2 N_modes);
   * Bk numerator: exact Fourier-space triple sum by brute force on 8^3 grids;
   * FFT counts of the whole call graphs: the notebook counters 1/0, 15/1, 2/96 and 452/682;
-  * MC Fisher (global, constant mask): per-seed closed form from the GRF amplitudes; local and
+  * Pk MC Fisher (global, constant mask): per-seed closed form from the GRF amplitudes; local and
     masked branches: dense-operator restatement on 6^3-8^3 grids via the addition theorem;
-    expectation E[F_MC] -> F_ideal statistically.
+  * Bk MC Fisher: in expectation over 200 seeds (unit mask, ideal weights) equal to the ideal Fisher
+    with the discreteness correction, a separate code path, which it thereby also pins for l > 0;
+  * Bk symmetry factors for l > 0: lattice triangle sums with the addition theorem (brute force);
+  * local-sightline Bk numerator: dense DFT-matrix restatement with L_l(k_hat . x_hat).
 Parity unpinned (no check beyond restatement and the compiled utils.pyx kernels they call):
-BSpec._compute_fisher per-seed values; BSpec._compute_symmetry_factor and
-compute_fisher_ideal(discreteness_correction=True) for l>0; the local-sightline Bk numerator;
-PSpec.compute_shot_contribution.
+BSpec._compute_fisher per-seed values with a mask or a local sightline (pinned in expectation only
+for the unit mask); PSpec.compute_shot_contribution.
 """
 import numpy as np
 import scipy.fft
