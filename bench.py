@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Fisher MC realizations/s, Pk+Bk 256³: batched/split-bf16 vs plain FP32"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled x2.000 / x1.000
+# by a 1 GiB device copy) over one realization of this workload, for the pass kind they were taken on
+# (profiles/fft_cols_hbm_traffic_pk_bk_mc_fisher_256.md); other kinds report null.
+MEASURED_TRAFFIC = {"fft_cols (plain)": 2.297e9}
 KIND_NAMES = {0: "r2c_rows", 1: "c2r_rows", 2: "fft_cols (plain)", 3: "fft_cols (C2R first pass, fused prologue)",
               4: "fft_cols (R2C last pass, fused epilogue)", 5: "pointing_rows (fused Q-map rows)"}
 
@@ -217,7 +221,8 @@ def main():
         cnt, ms, by = kinds[dom]
         achieved = by / (ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": KIND_NAMES[dom],
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get(KIND_NAMES[dom]),
+                "kernel": KIND_NAMES[dom],
                 "avg_launch_ms": round(ms / cnt, 4), "bytes_per_launch": by / cnt,
                 "sampled_launches": cnt, "share_of_sampled_fft_time": round(ms / sum(v[1] for v in kinds.values()), 3),
                 "passes": {KIND_NAMES[k]: {"sampled_launches": v[0], "avg_launch_ms": round(v[1] / v[0], 4),
