@@ -31,6 +31,7 @@ EXPORTS = {
     "pb3d_fft_c2r": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
     "pb3d_fft_pointing": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_float, c_void_p,
                                          c_void_p, c_void_p, c_void_p]),
+    "pb3d_fft_pointing_work_size": (c_int64, [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p]),
     "pb3d_kmap": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p]),
     "pb3d_kmap_packed": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, ctypes.c_int, c_void_p,
                                         c_void_p]),

@@ -390,7 +390,10 @@ class PolyBin3D:
         self._check_kop(kop_out, batch, 'pointing output')
         packed = kop_out is not None and kop_out.kind == C.KOP_PACK
         self._need(out, batch * (kop_out.packed_len if packed else self.H), 'pointing output')
-        work = self.work('fft_point', G * batch * self.H * 8)
+        nw = C.lib().pb3d_fft_pointing_work_size(C.ctypes.byref(self.cgrid), batch, G, arr, C.byref_or_none(kop_out))
+        if nw < 0:
+            C.check(-1)
+        work = self.work('fft_point', nw * 8)
         C.check(C.lib().pb3d_fft_pointing(self._plan, batch, G, arr, C._ptr(mask), float(scale),
                                           C.byref_or_none(kop_out), out.data_ptr(), work.data_ptr(), self.stream()))
         self.n_FFTs_reverse += batch * G

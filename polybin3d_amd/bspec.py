@@ -556,7 +556,10 @@ class BSpec:
             ell = int(ab[t0, 3])
             ylm = local and ell > 0
             G = 2 * ell + 2 if ylm else 1
-            CH = int(max(1, min(64, (6 << 30) // (G * 8 * b.H))))
+            # chunk of Q maps per fused chain: its work maps (compact, z planes kz <= support) within
+            # 12 GiB; larger chunks amortise the pass tails (measured: 60 maps 8 % faster than 15)
+            per_map = int(b.gridsize[0]) * int(b.gridsize[1]) * min(b.half_shape[2], self._sup[2] + 1)
+            CH = int(max(1, min(64, (12 << 30) // (G * 8 * per_map))))
             t1 = t0
             while t1 < self.N_bins and t1 - t0 < CH and ab[t1, 3] == ell:
                 t1 += 1

@@ -155,8 +155,9 @@ __device__ __forceinline__ void pencil_fft(int t, Src0 src0, Lds lds, DstN dstN,
 #ifndef PB3D_COL_THREADS
 #define PB3D_COL_THREADS 1024
 #endif
+// minimum waves per SIMD the fused pointing rows are compiled for (4: at most 128 VGPRs)
 #ifndef PB3D_POINT_WAVES
-#define PB3D_POINT_WAVES 2
+#define PB3D_POINT_WAVES 4
 #endif
 // Last Stockham stage with its outputs delivered to registers: for the last stage NB = L/R = NS,
 // so butterfly j = t + b*P < NS writes positions j + k*NS = t + P*(b + k*NS/P); the sink gets the
@@ -249,7 +250,10 @@ template <> struct ColCfg<16>   { static constexpr int T = 128; };
 // ------------------------------------------------------------------------------------------
 // Column pass along axis 0 or 1 of the half-complex grid, optionally pruned to a band limit.
 //   active column c in [0, ncols): outer(q) = q < split ? q : q + gap,
-//   base(c) = outer(c / cpp) * plane + (c % cpp); element p of the pencil at base + p*S.
+//   base(c) = outer(c / cpp) * plane + (c % cpp); element p of the pencil at base + p*S in
+//   half-complex mode coordinates (what the KOp sees).  The input and output arrays are addressed
+//   the same way with their own (plane, S): the half-complex layout itself, or a compact layout
+//   that keeps only the kz < pitch planes of a band-limited map (pitch along z instead of n2/2+1).
 //   Elements with p outside the radius rad_in (|fs(p)| > rad_in) are known zeros (not read);
 //   outputs outside rad_out are not needed (not written).  rad < 0 = whole pencil.
 //   MODE 0: plain.  MODE 1: first C2R pass, input through KOp (broadcast by x_stride).
@@ -257,7 +261,9 @@ template <> struct ColCfg<16>   { static constexpr int T = 128; };
 // ------------------------------------------------------------------------------------------
 struct ColMap {
   int ncols, cpp, split, gap;
-  long long plane, S;
+  long long plane, S;          // mode coordinates
+  long long in_plane, in_S;    // input addressing
+  long long out_plane, out_S;  // output addressing
   int rad_in, rad_out;
 };
 
@@ -281,11 +287,14 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
   const long long bt = blockIdx.y;
   const int c = blockIdx.x * T + col;
   const bool valid = c < cm.ncols;
-  long long base = 0;
+  long long base = 0, bin = 0, bout = 0;
   if (valid) {
     int q = c / cm.cpp;
+    const int l = c % cm.cpp;
     q = q < cm.split ? q : q + cm.gap;
-    base = (long long)q * cm.plane + (c % cm.cpp);
+    base = (long long)q * cm.plane + l;
+    bin = (long long)q * cm.in_plane + l;
+    bout = (long long)q * cm.out_plane + l;
   }
   const float2* src = in + bt * in_bstride;
   float2* dst = out + bt * out_bstride;
@@ -297,7 +306,7 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
     if (!valid || !in_radius(p, L, cm.rad_in)) return make_float2(0.f, 0.f);
     if (MODE == 1 && kop.kind == 4)  // BKQ assembles the map from packed rows; no input read
       return kop_value(kop, geo, bt, base + p * cm.S, p, mj, ml, make_float2(0.f, 0.f));
-    float2 v = src[base + p * cm.S];
+    float2 v = src[bin + p * cm.in_S];
     if (MODE == 1) v = kop_value(kop, geo, bt, base + p * cm.S, p, mj, ml, v);
     return v;
   };
@@ -315,10 +324,11 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
       }
       v = kop_value(kop, geo, bt, md, p, mj, ml, v);
       v = cscale(v, kop.scale);
-      if (kop.accumulate) v = cadd(v, dst[md]);
-      dst[md] = v;
+      const long long od = bout + p * cm.out_S;
+      if (kop.accumulate) v = cadd(v, dst[od]);
+      dst[od] = v;
     } else {
-      dst[base + p * cm.S] = v;
+      dst[bout + p * cm.out_S] = v;
     }
   };
   struct LdsRW {
@@ -383,9 +393,10 @@ __device__ __forceinline__ float2 r2c_split(const float2* b, int LDT, int r, int
 // (pairs (k, M-k) read together, Z' written to the transposed LDS tile): one staging step, no
 // separate LDS pass.  Entries kz >= lz_in are known zeros; Im of kz=0 and kz=M is ignored.  The
 // trip count is a compile-time constant and every load is issued before the first one is used, so
-// a thread waits for HBM once per tile rather than once per element.
+// a thread waits for HBM once per tile rather than once per element.  Row rr of the source starts
+// at src + rr * pitch (pitch >= lz_in).
 template <int M, int T, int NT>
-__device__ __forceinline__ void load_prepass(float2* buf, int LDT, const float2* __restrict__ src,
+__device__ __forceinline__ void load_prepass(float2* buf, int LDT, const float2* __restrict__ src, int pitch,
                                              long long rows_left, int lz_in, int tid, const float2* twN) {
   constexpr int K = M / 2 + 1;
   constexpr int TOT = T * K;
@@ -398,8 +409,8 @@ __device__ __forceinline__ void load_prepass(float2* buf, int LDT, const float2*
     xk[it] = make_float2(0.f, 0.f);
     xm[it] = make_float2(0.f, 0.f);
     if ((IT * NT == TOT || f < TOT) && rr < rows_left) {
-      if (k < lz_in) xk[it] = src[rr * (M + 1) + k];
-      if (M - k < lz_in) xm[it] = src[rr * (M + 1) + M - k];
+      if (k < lz_in) xk[it] = src[(long long)rr * pitch + k];
+      if (M - k < lz_in) xm[it] = src[(long long)rr * pitch + M - k];
     }
   }
 #pragma unroll
@@ -430,9 +441,10 @@ __device__ __forceinline__ void load_prepass(float2* buf, int LDT, const float2*
 // needed.  Used where a tile has few rows (one wave's rows).  Needs rows_left >= 1: rows past the
 // end are read as row 0 and replaced by zeros.  Three uniform regimes: lz_in == M/2+1 (every
 // entry of the left half live, none of the right: unconditional loads), lz_in < M/2+1
-// (predicated loads, right half zero), lz_in > M/2+1 (both halves read).
+// (predicated loads, right half zero), lz_in > M/2+1 (both halves read).  Row rr starts at
+// src + rr * pitch (pitch >= lz_in).
 template <int M, int T, int NT>
-__device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const float2* __restrict__ src,
+__device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const float2* __restrict__ src, int pitch,
                                                   long long rows_left, int lz_in, int tid, const float2* twN) {
   constexpr int K = M / 2 + 1;
   constexpr int KS = NT / T;
@@ -440,7 +452,7 @@ __device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const fl
   static_assert(NT % T == 0, "tile shape");
   const int rr = tid % T, kb = tid / T;
   const bool rv = rr < rows_left;
-  const float2* rowp = src + (rv ? rr : 0) * (M + 1);
+  const float2* rowp = src + (rv ? rr : 0) * pitch;
   const float2 zero = make_float2(0.f, 0.f);
   float2 xk[IT], xm[IT];
   if (lz_in == K) {
@@ -496,7 +508,7 @@ __device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const fl
 template <int M>
 __global__ void __launch_bounds__(RowTile<M>::NT)
 r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows, int rows_per_batch,
-         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop, int lz_out) {
+         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop, int lz_out, int pitch) {
   using RT = RowTile<M>;
   constexpr int T = RT::T, NT = RT::NT, LDT = RT::LDT;
   __shared__ float2 buf[RT::SIZE];
@@ -533,22 +545,23 @@ r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows
   RowT rw{buf, LDT, r};
   pencil_fft<M, -1>(t, rw, rw, rw, stw, true);
   __syncthreads();
-  // 3. split into the half spectrum
-  float2* o = out + row0 * (M + 1);
+  // 3. split into the half spectrum: kz < lz_out of row r at out + r * pitch
+  float2* o = out + row0 * pitch;
   constexpr int ITS = (T * (M + 1) + NT - 1) / NT;
 #pragma unroll
   for (int it = 0; it < ITS; ++it) {
     const int f = tid + it * NT;
     const int rr = f / (M + 1), k = f % (M + 1);
     if (f >= T * (M + 1) || row0 + rr >= nrows || k >= lz_out) continue;
-    o[f] = r2c_split<M>(buf, LDT, rr, k, stwN);
+    o[(long long)rr * pitch + k] = r2c_split<M>(buf, LDT, rr, k, stwN);
   }
 }
 
 template <int M>
 __global__ void __launch_bounds__(RowTile<M>::NT)
 c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows, int rows_per_batch,
-         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop, float norm, int lz_in) {
+         const float2* __restrict__ tw, const float2* __restrict__ twN, Geo geo, ROp rop, float norm, int lz_in,
+         int pitch) {
   using RT = RowTile<M>;
   constexpr int T = RT::T, NT = RT::NT, LDT = RT::LDT;
   __shared__ float2 buf[RT::SIZE];
@@ -556,9 +569,9 @@ c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows
   const int tid = threadIdx.x;
   stage_twiddles<M>(stw, stwN, tw, twN, tid, NT);
   const long long row0 = (long long)blockIdx.x * T;
-  const float2* src = in + row0 * (M + 1);
+  const float2* src = in + row0 * pitch;
   __syncthreads();  // twiddles staged
-  load_prepass<M, T, NT>(buf, LDT, src, nrows - row0, lz_in, tid, stwN);
+  load_prepass<M, T, NT>(buf, LDT, src, pitch, nrows - row0, lz_in, tid, stwN);
   __syncthreads();
   const int r = tid % T, t = tid / T;
   RowT rw{buf, LDT, r};
@@ -594,7 +607,8 @@ c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows
 // Fused pointing row kernel: for output map t and row (i0,i1): invert the rows of all nsub
 // sub-maps (work[g*batch + t]), accumulate coef_g * Y_g(x_hat) * z in registers, multiply by the
 // mask and scale, forward-transform the row and write kz < lz_out into work[t] (in place: a wave
-// owns its rows in every sub-map).
+// owns its rows in every sub-map).  Work map j starts at work + j * slot; row r of a map at
+// + r * pitch, with pitch >= max(lz_in, lz_out) so that the in-place rows never overlap.
 // Each wave works on its own RW = 64/P rows with its own transposed LDS tile, so the exchanges
 // between FFT stages need only wave-level ordering: the sub-map loop has no block barrier.
 // ------------------------------------------------------------------------------------------
@@ -625,9 +639,9 @@ template <int M> struct PointCfg {
 
 template <int M>
 __global__ void __launch_bounds__(PointCfg<M>::NT, PB3D_POINT_WAVES)
-pointing_rows(float2* work, long long rows_per_map, int batch, SubW sw, const float* __restrict__ mask,
-              float scale, int lz_in, int lz_out, const float2* __restrict__ tw, const float2* __restrict__ twN,
-              Geo geo) {
+pointing_rows(float2* work, long long slot, int pitch, long long rows_per_map, int batch, SubW sw,
+              const float* __restrict__ mask, float scale, int lz_in, int lz_out, const float2* __restrict__ tw,
+              const float2* __restrict__ twN, Geo geo) {
   using PC = PointCfg<M>;
   constexpr int RW = PC::RW, LDT = PC::LDT, E = PC::E, P = PC::P;
   __shared__ float2 tiles[PC::WAVES * PC::SIZE];
@@ -640,7 +654,6 @@ pointing_rows(float2* work, long long rows_per_map, int batch, SubW sw, const fl
   const long long t = blockIdx.y;
   const long long row0 = (long long)blockIdx.x * PC::RB + wave * RW;  // this wave's first row
   if (row0 >= rows_per_map) return;  // no rows for this wave (no block barrier follows)
-  const long long H = geo.H;
   const int r = lane % RW, tp = lane / RW;
   RowT rw{buf, LDT, r};
   // this lane's voxels after the inverse transforms: positions m = tp + e*P of row r
@@ -656,14 +669,14 @@ pointing_rows(float2* work, long long rows_per_map, int batch, SubW sw, const fl
 #pragma unroll
   for (int e = 0; e < E; ++e) { acc[e][0] = 0.f; acc[e][1] = 0.f; }
   for (int g = 0; g < sw.n; ++g) {
-    const float2* src = work + ((long long)g * batch + t) * H + row0 * (M + 1);
+    const float2* src = work + ((long long)g * batch + t) * slot + row0 * pitch;
     // Opaque copies of the lane indices: index arithmetic derived from them is recomputed per
     // sub-map instead of being hoisted out of the loop and held in (spilled) registers.
     int lane_g = lane, tp_g = tp;
     float rzb_g = rzb;
     asm volatile("" : "+v"(lane_g), "+v"(tp_g), "+v"(rzb_g));
     wave_lds_sync();  // the previous sub-map's reads of the tile are issued
-    load_prepass_rows<M, RW, 64>(buf, LDT, src, rows_per_map - row0, lz_in, lane_g, stwN);
+    load_prepass_rows<M, RW, 64>(buf, LDT, src, pitch, rows_per_map - row0, lz_in, lane_g, stwN);
     wave_lds_sync();
     const int ell = sw.ell[g];
     const float cf = sw.coef[g];
@@ -770,14 +783,14 @@ pointing_rows(float2* work, long long rows_per_map, int batch, SubW sw, const fl
   auto get = [&](int e) { return make_float2(acc[e][0], acc[e][1]); };
   pencil_forward_from_regs<M, true>(tp, get, rw, stw);
   wave_lds_sync();
-  float2* o = work + t * H + row0 * (M + 1);
+  float2* o = work + t * slot + row0 * pitch;
   constexpr int ITS = (RW * (M + 1) + 63) / 64;
 #pragma unroll
   for (int it = 0; it < ITS; ++it) {
     const int f = lane + it * 64;
     const int rr = f / (M + 1), k = f % (M + 1);
     if (f >= RW * (M + 1) || row0 + rr >= rows_per_map || k >= lz_out) continue;
-    o[f] = r2c_split<M>(buf, LDT, rr, k, stwN);
+    o[(long long)rr * pitch + k] = r2c_split<M>(buf, LDT, rr, k, stwN);
   }
 }
 
@@ -856,12 +869,12 @@ static hipError_t launch_cols(int L, const float2* in, float2* out, const ColMap
 
 static hipError_t launch_r2c_rows(int M, const float* in, float2* out, long long nrows, int rpb,
                                   const float2* tw, const float2* twN, const Geo& geo, const ROp& rop, int lz_out,
-                                  hipStream_t s) {
+                                  int pitch, hipStream_t s) {
 #define PB3D_ROWS(MM)                                                                            \
   case MM: {                                                                                     \
     constexpr int T = RowCfg<MM>::T;                                                             \
     r2c_rows<MM><<<(unsigned)((nrows + T - 1) / T), RowTile<MM>::NT, 0, s>>>(in, out, nrows, rpb, tw, \
-                                                                              twN, geo, rop, lz_out); \
+                                                                              twN, geo, rop, lz_out, pitch); \
     return hipGetLastError();                                                                    \
   }
   switch (M) {
@@ -874,12 +887,12 @@ static hipError_t launch_r2c_rows(int M, const float* in, float2* out, long long
 
 static hipError_t launch_c2r_rows(int M, const float2* in, float* out, long long nrows, int rpb,
                                   const float2* tw, const float2* twN, const Geo& geo, const ROp& rop,
-                                  float norm, int lz_in, hipStream_t s) {
+                                  float norm, int lz_in, int pitch, hipStream_t s) {
 #define PB3D_ROWS(MM)                                                                            \
   case MM: {                                                                                     \
     constexpr int T = RowCfg<MM>::T;                                                             \
     c2r_rows<MM><<<(unsigned)((nrows + T - 1) / T), RowTile<MM>::NT, 0, s>>>(in, out, nrows, rpb, tw, \
-                                                                              twN, geo, rop, norm, lz_in); \
+                                                                              twN, geo, rop, norm, lz_in, pitch); \
     return hipGetLastError();                                                                    \
   }
   switch (M) {
@@ -890,15 +903,16 @@ static hipError_t launch_c2r_rows(int M, const float2* in, float* out, long long
 #undef PB3D_ROWS
 }
 
-static hipError_t launch_pointing_rows(int M, float2* work, long long rpm, int batch, const SubW& sw,
+static hipError_t launch_pointing_rows(int M, float2* work, long long slot, int pitch, long long rpm, int batch,
+                                       const SubW& sw,
                                        const float* mask, float scale, int lz_in, int lz_out, const float2* tw,
                                        const float2* twN, const Geo& geo, hipStream_t s) {
 #define PB3D_PROWS(MM)                                                                           \
   case MM: {                                                                                     \
     constexpr int RB = PointCfg<MM>::RB;                                                         \
     dim3 grid((unsigned)((rpm + RB - 1) / RB), batch);                                           \
-    pointing_rows<MM><<<grid, PointCfg<MM>::NT, 0, s>>>(work, rpm, batch, sw, mask, scale, lz_in, lz_out, \
-                                                         tw, twN, geo);                          \
+    pointing_rows<MM><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot, pitch, rpm, batch, sw, mask, scale, lz_in, \
+                                                         lz_out, tw, twN, geo);                  \
     return hipGetLastError();                                                                    \
   }
   switch (M) {
@@ -970,8 +984,29 @@ static Prune merge_prune(const Geo& g, Prune a, Prune b) {
   (void)g;
   return o;
 }
-// x-column pass (axis 0) over the active (j, l) columns
-static ColMap xcols(const Geo& g, const Prune& pr, bool c2r) {
+// Row pitch of a compact work map holding the z planes kz < lz (unpadded: the column passes read
+// and write whole compact planes, measured faster than padded pitches).
+static int compact_pitch(const Geo& g, int lz) { return lz < g.n2h ? lz : g.n2h; }
+// Band limits of a pointing chain: inverse side (all sub-maps) and forward side (kop_out).
+static void pointing_prunes(const Geo& g, int nsub, const pb3d_submap* subs, const pb3d_kop* kop_out, Prune& pin,
+                            Prune& pout) {
+  pin = make_prune(g, &subs[0].kop);
+  for (int k = 1; k < nsub; ++k) pin = merge_prune(g, pin, make_prune(g, &subs[k].kop));
+  pout = make_prune(g, kop_out);
+}
+// Column-pass array addressing.  pitch 0: the half-complex layout; pitch > 0: the compact layout
+// of a band-limited map, z planes kz < pitch only (row (i, j) at (i * n1 + j) * pitch).
+static void x_side(const Geo& g, const ColMap& cm, int pitch, long long& plane, long long& S) {
+  if (pitch <= 0 || cm.plane == 0) { plane = cm.plane; S = cm.S; return; }  // whole planes: pitch == n2h
+  plane = pitch; S = (long long)g.n1 * pitch;
+}
+static void y_side(const Geo& g, const ColMap& cm, int pitch, long long& plane, long long& S) {
+  if (pitch <= 0) { plane = cm.plane; S = cm.S; return; }
+  plane = (long long)g.n1 * pitch; S = pitch;
+}
+// x-column pass (axis 0) over the active (j, l) columns; pitch_in / pitch_out select the layout of
+// the input / output array (0 = half-complex)
+static ColMap xcols(const Geo& g, const Prune& pr, bool c2r, int pitch_in = 0, int pitch_out = 0) {
   ColMap cm{};
   cm.S = (long long)g.n1 * g.n2h;
   if (pr.r1 < 0 && pr.lz == g.n2h) {
@@ -984,15 +1019,19 @@ static ColMap xcols(const Geo& g, const Prune& pr, bool c2r) {
   }
   cm.rad_in = c2r ? pr.r0 : -1;
   cm.rad_out = c2r ? -1 : pr.r0;
+  x_side(g, cm, pitch_in, cm.in_plane, cm.in_S);
+  x_side(g, cm, pitch_out, cm.out_plane, cm.out_S);
   return cm;
 }
 // y-column pass (axis 1) over the active (i, l) columns
-static ColMap ycols(const Geo& g, const Prune& pr, bool c2r) {
+static ColMap ycols(const Geo& g, const Prune& pr, bool c2r, int pitch_in = 0, int pitch_out = 0) {
   ColMap cm{};
   cm.S = g.n2h;
   cm.ncols = g.n0 * pr.lz; cm.cpp = pr.lz; cm.plane = (long long)g.n1 * g.n2h; cm.split = 1 << 30; cm.gap = 0;
   cm.rad_in = c2r ? pr.r1 : -1;
   cm.rad_out = c2r ? -1 : pr.r1;
+  y_side(g, cm, pitch_in, cm.in_plane, cm.in_S);
+  y_side(g, cm, pitch_out, cm.out_plane, cm.out_S);
   return cm;
 }
 
@@ -1134,18 +1173,23 @@ extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, voi
   KOp plain = to_kop(nullptr);
   const Prune pr = make_prune(g, kop_);
   const long long rows = (long long)batch * g.n0 * g.n1;
-  const ColMap cy = ycols(g, pr, false), cx = xcols(g, pr, false);
+  // a separate work array holds the band-limited intermediate compactly (z planes kz < lz only);
+  // without one the output array is the work array and keeps the half-complex layout
+  const int pitch = work == out ? 0 : compact_pitch(g, pr.lz);
+  const long long wbs = pitch ? (long long)g.n0 * g.n1 * pitch : g.H;
+  const ColMap cy = ycols(g, pr, false, pitch, pitch), cx = xcols(g, pr, false, pitch, 0);
   const double br = (double)batch * (g.N * 4.0 * rop_operands(rop_) + (double)g.n0 * g.n1 * pr.lz * 8.0);
   ProfSample sm{};
   bool on = prof_begin(0, s, sm, br);
-  PB3D_CHECK_HIP(launch_r2c_rows(p->L[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, pr.lz, s));
+  PB3D_CHECK_HIP(launch_r2c_rows(p->L[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, pr.lz,
+                                 pitch ? pitch : g.n2h, s));
   prof_end(on, sm, s);
   on = prof_begin(2, s, sm, col_bytes(cy, p->L[1], batch));
-  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cy, g.H, g.H, batch, p->tw[1], g, plain, s)));
+  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cy, wbs, wbs, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   const long long out_bs = (kop_ && kop_->kind == 5) ? kop_->packed_len : g.H;
   on = prof_begin(4, s, sm, col_bytes(cx, p->L[0], batch));
-  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, out, cx, g.H, out_bs, batch, p->tw[0], g, kop, s)));
+  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, out, cx, wbs, out_bs, batch, p->tw[0], g, kop, s)));
   prof_end(on, sm, s);
   g_fwd += batch;
   return 0;
@@ -1168,23 +1212,39 @@ extern "C" int pb3d_fft_c2r(const pb3d_plan* p, const void* in_, float* out, voi
   KOp plain = to_kop(nullptr);
   const Prune pr = make_prune(g, kop_);
   const long long in_bs = kop_ ? kop_->x_batch_stride : g.H;
-  const ColMap cx = xcols(g, pr, true), cy = ycols(g, pr, true);
+  // the band-limited intermediate is kept compactly in work (z planes kz < lz only)
+  const int pitch = compact_pitch(g, pr.lz);
+  const long long wbs = (long long)g.n0 * g.n1 * pitch;
+  const ColMap cx = xcols(g, pr, true, 0, pitch), cy = ycols(g, pr, true, pitch, pitch);
   const double br = (double)batch * ((double)g.n0 * g.n1 * pr.lz * 8.0 +
                                      g.N * 4.0 * (rop_operands(rop_) + (rop_ && rop_->accumulate ? 1 : 0)));
   ProfSample sm{};
   bool on = prof_begin(3, s, sm, col_bytes(cx, p->L[0], batch));
-  PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], in, work, cx, in_bs, g.H, batch, p->tw[0], g, kop, s)));
+  PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], in, work, cx, in_bs, wbs, batch, p->tw[0], g, kop, s)));
   prof_end(on, sm, s);
   on = prof_begin(2, s, sm, col_bytes(cy, p->L[1], batch));
-  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, cy, g.H, g.H, batch, p->tw[1], g, plain, s)));
+  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, cy, wbs, wbs, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   const long long rows = (long long)batch * g.n0 * g.n1;
   on = prof_begin(1, s, sm, br);
   PB3D_CHECK_HIP(launch_c2r_rows(p->L[2], work, out, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop,
-                                 (float)(1.0 / (double)g.N), pr.lz, s));
+                                 (float)(1.0 / (double)g.N), pr.lz, pitch, s));
   prof_end(on, sm, s);
   g_rev += batch;
   return 0;
+}
+
+extern "C" int64_t pb3d_fft_pointing_work_size(const pb3d_grid* grid, int batch, int nsub, const pb3d_submap* subs,
+                                               const pb3d_kop* kop_out) {
+  if (!grid || !subs || batch < 1 || nsub < 1 || nsub > kMaxSub) {
+    fail(PB3D_ERR_ARG, "pointing_work_size: null pointer, batch < 1 or nsub out of [1, 16]");
+    return -1;
+  }
+  const Geo g = make_geo(*grid);
+  Prune pin, pout;
+  pointing_prunes(g, nsub, subs, kop_out, pin, pout);
+  const int pz = compact_pitch(g, pin.lz > pout.lz ? pin.lz : pout.lz);
+  return (int64_t)nsub * batch * g.n0 * g.n1 * pz;
 }
 
 extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const pb3d_submap* subs, const float* mask,
@@ -1199,7 +1259,6 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   float2* work = (float2*)work_;
   SubW sw{};
   sw.n = nsub;
-  Prune pin = make_prune(g, &subs[0].kop);
   for (int k = 0; k < nsub; ++k) {
     const pb3d_submap& sb = subs[k];
     if (int e = check_kop(&sb.kop)) return e;
@@ -1211,41 +1270,46 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
     sw.y0[k] = sb.ell > 0 ? ylm_f(sb.ell, sb.m, 0.f, 0.f, 0.f) : 1.f;
     if (sb.ell > 0 && !ylm_poly(sb.ell, sb.m, sw.poly[k]))
       return fail(PB3D_ERR_ARG, "pointing: Y_lm polynomial fit failed");
-    pin = merge_prune(g, pin, make_prune(g, &sb.kop));
   }
+  Prune pin, pout;
+  pointing_prunes(g, nsub, subs, kop_out, pin, pout);
   KOp plain = to_kop(nullptr);
   ProfSample sm{};
-  // 1. pruned x-column C2R pass of every sub-map into work[k*batch + b]
+  // Work maps use the compact layout: z planes kz < pz only, pz covering both the inverse-side and
+  // the forward-side band limits (the fused rows are rewritten in place), one map every ws entries.
+  const int pz = compact_pitch(g, pin.lz > pout.lz ? pin.lz : pout.lz);
+  const long long ws = (long long)g.n0 * g.n1 * pz;
+  // 1. pruned x-column C2R pass of every sub-map into work[k*batch + b], over the common support
+  //    (so that the y pass reads no entry that was not written)
   for (int k = 0; k < nsub; ++k) {
     const pb3d_submap& sb = subs[k];
     KOp kk = to_kop(&sb.kop);
-    const ColMap cx = xcols(g, make_prune(g, &sb.kop), true);
+    const ColMap cx = xcols(g, pin, true, 0, pz);
     bool on = prof_begin(3, s, sm, col_bytes(cx, p->L[0], batch));
-    PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], (const float2*)sb.in, work + (long long)k * batch * g.H, cx,
-                                       sb.kop.x_batch_stride, g.H, batch, p->tw[0], g, kk, s)));
+    PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], (const float2*)sb.in, work + (long long)k * batch * ws, cx,
+                                       sb.kop.x_batch_stride, ws, batch, p->tw[0], g, kk, s)));
     prof_end(on, sm, s);
   }
   // 2. pruned y-column C2R pass over all sub-maps
-  const ColMap cyi = ycols(g, pin, true);
+  const ColMap cyi = ycols(g, pin, true, pz, pz);
   bool on = prof_begin(2, s, sm, col_bytes(cyi, p->L[1], batch * nsub));
-  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, cyi, g.H, g.H, batch * nsub, p->tw[1], g, plain, s)));
+  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, cyi, ws, ws, batch * nsub, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   // 3. fused rows: inverse rows of every sub-map, weights, mask, forward row -> work[b]
-  const Prune pout = make_prune(g, kop_out);
   const double rows1 = (double)g.n0 * g.n1;
   on = prof_begin(5, s, sm, (double)batch * (rows1 * 8.0 * (pin.lz * (double)nsub + pout.lz) + (mask ? g.N * 4.0 : 0.0)));
-  PB3D_CHECK_HIP(launch_pointing_rows(p->L[2], work, (long long)g.n0 * g.n1, batch, sw, mask, scale, pin.lz, pout.lz,
-                                      p->tw[2], p->twN, g, s));
+  PB3D_CHECK_HIP(launch_pointing_rows(p->L[2], work, ws, pz, (long long)g.n0 * g.n1, batch, sw, mask, scale, pin.lz,
+                                      pout.lz, p->tw[2], p->twN, g, s));
   prof_end(on, sm, s);
   // 4. pruned forward y and x column passes, epilogue into out
-  const ColMap cyo = ycols(g, pout, false), cxo = xcols(g, pout, false);
+  const ColMap cyo = ycols(g, pout, false, pz, pz), cxo = xcols(g, pout, false, pz, 0);
   on = prof_begin(2, s, sm, col_bytes(cyo, p->L[1], batch));
-  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cyo, g.H, g.H, batch, p->tw[1], g, plain, s)));
+  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cyo, ws, ws, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   KOp ko = to_kop(kop_out);
   const long long out_bs = (kop_out && kop_out->kind == 5) ? kop_out->packed_len : g.H;
   on = prof_begin(4, s, sm, col_bytes(cxo, p->L[0], batch));
-  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, (float2*)out_, cxo, g.H, out_bs, batch, p->tw[0], g, ko, s)));
+  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, (float2*)out_, cxo, ws, out_bs, batch, p->tw[0], g, ko, s)));
   prof_end(on, sm, s);
   g_rev += (long long)batch * nsub;
   g_fwd += batch;
