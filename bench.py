@@ -28,9 +28,17 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled x2.000 / x1.000
 # by a 1 GiB device copy) over one realization of this workload, for the pass kind they were taken on
 # (profiles/fft_cols_hbm_traffic_pk_bk_mc_fisher_256.md); other kinds report null.
-MEASURED_TRAFFIC = {"fft_cols (plain)": 2.297e9}
-KIND_NAMES = {0: "r2c_rows", 1: "c2r_rows", 2: "fft_cols (plain)", 3: "fft_cols (C2R first pass, fused prologue)",
-              4: "fft_cols (R2C last pass, fused epilogue)", 5: "pointing_rows (fused Q-map rows)"}
+MEASURED_TRAFFIC = {}
+KIND_NAMES = {0: "r2c_rows", 1: "c2r_rows", 2: "fft_cols (plain C2R)", 3: "fft_cols (C2R first pass, fused prologue)",
+              4: "fft_cols (R2C last pass, fused epilogue)", 5: "pointing_rows (fused Q-map rows)",
+              6: "fft_cols (plain R2C)"}
+
+
+def kind_kernel(kind, n):
+    """Kernel template of a sampled pass kind at grid n^3 (the name rocprofv3 reports)."""
+    return {0: "r2c_rows<%d>" % (n // 2), 1: "c2r_rows<%d>" % (n // 2), 2: "fft_cols<%d, 1, 0>" % n,
+            3: "fft_cols<%d, 1, 1>" % n, 4: "fft_cols<%d, -1, 2>" % n, 5: "pointing_rows<%d>" % (n // 2),
+            6: "fft_cols<%d, -1, 0>" % n}[kind]
 
 
 def shell_mask(n, L, center, seed=1, r_lo=300., r_hi=480., nholes=200, smooth=5.):
@@ -157,7 +165,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--prof-period", type=int, default=8)
+    # odd sampling period: launches of one kind come in patterns of even length (sub-maps, y passes of
+    # both directions), which an even period would sample at one phase only
+    ap.add_argument("--prof-period", type=int, default=7)
     ap.add_argument("--workload", default="mc_fisher", choices=["mc_fisher", "bk_ideal"],
                     help="mc_fisher: the Pk+Bk MC Fisher metric (default); bk_ideal: Bk_ideal evaluations "
                          "and the MFMA triplet contraction (--n, --bk-bins, --lmax, --precision)")
@@ -209,7 +219,7 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     kinds = {}
-    for kind in range(6):
+    for kind in sorted(KIND_NAMES):
         cnt, ms, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
         C.check(C.lib().pb3d_prof_read(kind, ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(by)))
         if cnt.value:
@@ -221,8 +231,8 @@ def main():
         cnt, ms, by = kinds[dom]
         achieved = by / (ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get(KIND_NAMES[dom]),
-                "kernel": KIND_NAMES[dom],
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get(kind_kernel(dom, n)),
+                "kernel": kind_kernel(dom, n), "pass": KIND_NAMES[dom],
                 "avg_launch_ms": round(ms / cnt, 4), "bytes_per_launch": by / cnt,
                 "sampled_launches": cnt, "share_of_sampled_fft_time": round(ms / sum(v[1] for v in kinds.values()), 3),
                 "passes": {KIND_NAMES[k]: {"sampled_launches": v[0], "avg_launch_ms": round(v[1] / v[0], 4),

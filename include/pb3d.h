@@ -149,8 +149,9 @@ int64_t pb3d_fft_counter(int dir);
 
 /* Sampled launch timing for roofline reports (no counterpart in the reference): with period P > 0
  * every P-th launch of each FFT pass kind is bracketed by hipEvents on its stream.  kinds:
- * 0 R2C z-rows, 1 C2R z-rows, 2 plain column pass, 3 C2R first column pass (mode prologue),
- * 4 R2C last column pass (mode epilogue), 5 fused pointing rows (pb3d_fft_pointing step 3).
+ * 0 R2C z-rows, 1 C2R z-rows, 2 plain C2R column pass, 3 C2R first column pass (mode prologue),
+ * 4 R2C last column pass (mode epilogue), 5 fused pointing rows (pb3d_fft_pointing step 3),
+ * 6 plain R2C column pass.
  * pb3d_prof_read synchronises the sampled events and returns the number of sampled launches, their
  * summed duration (ms) and their summed algorithmic bytes: the elements the pass must read and
  * write after band-limit pruning, 8 B per complex mode and 4 B per real voxel (k-space table reads

@@ -801,7 +801,9 @@ static std::atomic<long long> g_fwd{0}, g_rev{0};
 
 // ---- sampled launch timing (roofline reporting): every `period`-th launch of a pass kind is
 // bracketed by hipEvents on its own stream; read() synchronises and sums the durations.
-constexpr int kProfKinds = 6;  // 0 r2c rows, 1 c2r rows, 2 plain cols, 3 C2R first cols, 4 R2C last cols, 5 pointing rows
+// 0 r2c rows, 1 c2r rows, 2 plain C2R cols, 3 C2R first cols, 4 R2C last cols, 5 pointing rows,
+// 6 plain R2C cols: one kernel template per kind
+constexpr int kProfKinds = 7;
 struct ProfSample { int kind; hipEvent_t a, b; double bytes; };
 static int g_prof_period = 0;
 static long long g_prof_count[kProfKinds] = {};
@@ -1184,7 +1186,7 @@ extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, voi
   PB3D_CHECK_HIP(launch_r2c_rows(p->L[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, pr.lz,
                                  pitch ? pitch : g.n2h, s));
   prof_end(on, sm, s);
-  on = prof_begin(2, s, sm, col_bytes(cy, p->L[1], batch));
+  on = prof_begin(6, s, sm, col_bytes(cy, p->L[1], batch));
   PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cy, wbs, wbs, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   const long long out_bs = (kop_ && kop_->kind == 5) ? kop_->packed_len : g.H;
@@ -1303,7 +1305,7 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   prof_end(on, sm, s);
   // 4. pruned forward y and x column passes, epilogue into out
   const ColMap cyo = ycols(g, pout, false, pz, pz), cxo = xcols(g, pout, false, pz, 0);
-  on = prof_begin(2, s, sm, col_bytes(cyo, p->L[1], batch));
+  on = prof_begin(6, s, sm, col_bytes(cyo, p->L[1], batch));
   PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cyo, ws, ws, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   KOp ko = to_kop(kop_out);
