@@ -168,9 +168,12 @@ def main():
     # odd sampling period: launches of one kind come in patterns of even length (sub-maps, y passes of
     # both directions), which an even period would sample at one phase only
     ap.add_argument("--prof-period", type=int, default=7)
-    ap.add_argument("--workload", default="mc_fisher", choices=["mc_fisher", "bk_ideal"],
+    ap.add_argument("--workload", default="mc_fisher", choices=["mc_fisher", "bk_ideal", "pk_mc_fisher"],
                     help="mc_fisher: the Pk+Bk MC Fisher metric (default); bk_ideal: Bk_ideal evaluations "
-                         "and the MFMA triplet contraction (--n, --bk-bins, --lmax, --precision)")
+                         "and the MFMA triplet contraction (--n, --bk-bins, --lmax, --precision); pk_mc_fisher: "
+                         "PSpec MC Fisher realizations at 128^3 and 256^3 (--realizations per call)")
+    ap.add_argument("--realizations", type=int, default=1,
+                    help="pk_mc_fisher: realizations per compute_fisher batch")
     ap.add_argument("--bk-bins", type=int, default=16)
     ap.add_argument("--bk-kmax", type=float, default=0.17)
     ap.add_argument("--lmax", type=int, default=0)
@@ -186,6 +189,8 @@ def main():
     from polybin3d_amd import _capi as C
     if args.workload == "bk_ideal":
         return bk_ideal_workload(args, torch, pb)
+    if args.workload == "pk_mc_fisher":
+        return pk_fisher_workload(args, torch, pb)
 
     n, L, center = args.n, 1000., [10., 10., 10.]
     kny = n * np.pi / L
@@ -250,6 +255,47 @@ def main():
            "roofline": roof}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(n, pk_bins, bk_bins, 4, 2, bs.N_bins)
+    print(json.dumps(res), flush=True)
+
+
+def pk_fisher_workload(args, torch, pb):
+    """PSpec MC Fisher (local sightline, lmax 4, 30 bins 0.01..0.30, shell mask, device GRFs, FP64
+    accumulation) on a 128^3 and a 256^3 grid.  One step = args.realizations realizations on each grid;
+    value = realizations per second over both grids."""
+    L, center = 1000., [10., 10., 10.]
+    grids = (128, 256)
+    specs = {}
+    for n in grids:
+        b = pb.PolyBin3D(L, [n] * 3, Pk=powerlaw_pk(2 * n * np.pi / L * 1.8), boxcenter=center, sightline='local',
+                         rng='device')
+        specs[n] = pb.PSpec(b, np.linspace(0.01, 0.30, 31), lmax=4, mask=shell_mask(n, L, center))
+    R = max(1, args.realizations)
+    F = {n: torch.zeros((specs[n].N_bins,) * 2, dtype=torch.float64, device=specs[n].base.device) for n in grids}
+    per = {n: 0. for n in grids}
+
+    def step(seed0, timed):
+        for n in grids:
+            t = time.perf_counter()
+            F[n].add_(specs[n]._fisher_contributions_dev(list(range(seed0, seed0 + R))))
+            torch.cuda.synchronize()
+            if timed:
+                per[n] += time.perf_counter() - t
+    for w in range(args.warmup):
+        step(10 ** 6 + w * R, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step(s * R, True)
+    dt = time.perf_counter() - t0
+    nreal = args.steps * R * len(grids)
+    res = {"metric": "Pk MC Fisher realizations/s (128^3 + 256^3)", "value": round(nreal / dt, 4),
+           "unit": "realizations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(1e3 * dt / args.steps, 2), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "pk_mc_fisher_128_256", "grids": list(grids), "boxsize": L, "sightline": "local",
+                      "lmax": 4, "nbins": 30, "kmax": 0.3, "mask": "shell", "realizations_per_batch": R,
+                      "fisher_accumulation": "fp64"},
+           "ms_per_realization": {str(n): round(1e3 * per[n] / (args.steps * R), 2) for n in grids}}
     print(json.dumps(res), flush=True)
 
 

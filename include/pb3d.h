@@ -98,7 +98,8 @@ typedef struct {
  *             self-conjugate Nyquist plane).
  * For R2C epilogues x is the transform output; out = scale*v or out += scale*v.
  * `x_batch_stride` (complex elements) advances the C2R input per batch entry (0 = broadcast:
- * one input map filtered into many bins). */
+ * one input map filtered into many bins); with `x_batch_group` = g > 1 it advances once per g
+ * consecutive entries (entry t reads input t / g: several realizations, each filtered into g bins). */
 typedef struct {
   int32_t kind;
   int32_t ell, ell2;
@@ -127,7 +128,7 @@ typedef struct {
    * columns and reads; as an R2C epilogue only modes inside it are produced (PACK outputs, or a
    * full grid whose modes outside the support are left unwritten).  A negative entry = full axis. */
   int32_t support[3];
-  int32_t pad3_;
+  int32_t x_batch_group;  /* 0 or 1: one input step per batch entry */
 } pb3d_kop;
 
 /* One sub-map of pb3d_fft_pointing: C2R(kop(in)) weighted in real space by coef * Y_lm(x_hat)

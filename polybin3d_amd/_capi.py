@@ -76,7 +76,7 @@ class KOp(ctypes.Structure):
                 ("w", c_void_p), ("x_batch_stride", c_int64), ("scale", c_float), ("pad2_", c_int32),
                 ("pack_index", c_void_p), ("packed_len", c_int64), ("trip", c_void_p), ("pair_index", c_void_p),
                 ("ft", c_void_p), ("ft_lstride", c_int64), ("nk", c_int32), ("terms", c_int32),
-                ("trip0", c_int32), ("leg3_legendre", c_int32), ("support", c_int32 * 3), ("pad3_", c_int32)]
+                ("trip0", c_int32), ("leg3_legendre", c_int32), ("support", c_int32 * 3), ("x_batch_group", c_int32)]
 
 
 class SubMap(ctypes.Structure):
@@ -132,11 +132,12 @@ def rop(kind=ROP_PLAIN, a=None, b=None, c=None, d=None, ell=0, m=0, accumulate=0
 
 def kop(kind=KOP_PLAIN, bin_table=None, bins=None, w=None, ell=0, ell2=0, m=0, accumulate=0, scale=1.0,
         x_batch_stride=0, pack_index=None, packed_len=0, trip=None, pair_index=None, ft=None, ft_lstride=0,
-        nk=0, terms=0, trip0=0, leg3_legendre=0, support=None):
+        nk=0, terms=0, trip0=0, leg3_legendre=0, support=None, x_batch_group=0):
     o = KOp()
     o.kind, o.ell, o.ell2, o.m, o.accumulate = kind, ell, ell2, m, int(accumulate)
     o.bin_table, o.bins, o.w = _ptr(bin_table), _ptr(bins), _ptr(w)
     o.x_batch_stride = int(x_batch_stride)
+    o.x_batch_group = int(x_batch_group)
     o.scale = float(scale)
     o.pack_index, o.packed_len = _ptr(pack_index), int(packed_len)
     o.trip, o.pair_index, o.ft, o.ft_lstride = _ptr(trip), _ptr(pair_index), _ptr(ft), int(ft_lstride)

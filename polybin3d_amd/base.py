@@ -253,6 +253,13 @@ class PolyBin3D:
     def stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
 
+    def _input_extent(self, kop, batch):
+        """Complex elements a k-space input of `batch` entries spans: one H-map per
+        kop.x_batch_group entries, x_batch_stride apart."""
+        if kop is None:
+            return batch * self.H
+        return ((batch - 1) // max(1, int(kop.x_batch_group))) * int(kop.x_batch_stride) + self.H
+
     def _need(self, t, n, what):
         """Raise unless tensor t is a contiguous device tensor holding at least n elements of its
         kind (complex64 modes or float32 voxels), so no kernel reads or writes past it."""
@@ -341,9 +348,8 @@ class PolyBin3D:
             g = tuple(int(v) for v in self.gridsize)
             out = torch.empty((batch,) + g if batch > 1 else g, dtype=torch.float32, device=self.device)
         work = self.work('fft_c2r', batch * self.H * 8)
-        xs = self.H if kop is None else kop.x_batch_stride
         if k is not None:
-            self._need(k, (batch - 1) * xs + self.H, 'c2r input')
+            self._need(k, self._input_extent(kop, batch), 'c2r input')
         elif kop is None or kop.kind != C.KOP_BKQ:
             raise ValueError("c2r: no input")
         self._check_kop(kop, batch, 'c2r')
@@ -378,7 +384,7 @@ class PolyBin3D:
         for g, (kop, inp, ell, m, coef) in enumerate(subs):
             self._check_kop(kop, batch, 'pointing sub %d' % g)
             if inp is not None:
-                self._need(inp, (batch - 1) * kop.x_batch_stride + self.H, 'pointing sub %d input' % g)
+                self._need(inp, self._input_extent(kop, batch), 'pointing sub %d input' % g)
             elif kop.kind != C.KOP_BKQ:
                 raise ValueError("pointing: sub-map %d has no input" % g)
             arr[g].kop = kop
@@ -404,7 +410,7 @@ class PolyBin3D:
         if out is None:
             out = torch.empty((batch,) + self.half_shape if batch > 1 else self.half_shape,
                               dtype=torch.complex64, device=self.device)
-        self._need(x, (batch - 1) * kop.x_batch_stride + self.H, 'kmap input')
+        self._need(x, self._input_extent(kop, batch), 'kmap input')
         self._need(out, batch * self.H, 'kmap output')
         self._check_kop(kop, batch, 'kmap')
         C.check(C.lib().pb3d_kmap(C.ctypes.byref(self.cgrid), x.data_ptr(), out.data_ptr(), batch,

@@ -296,7 +296,8 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
     bin = (long long)q * cm.in_plane + l;
     bout = (long long)q * cm.out_plane + l;
   }
-  const float2* src = in + bt * in_bstride;
+  // the input of a C2R first pass steps once per kop.x_group batch entries
+  const float2* src = in + (MODE == 1 ? bt / kop.x_group : bt) * in_bstride;
   float2* dst = out + bt * out_bstride;
   // mode coordinates for KOp (x-column pass only: base = j*n2h + l)
   int mj = 0, ml = 0;
@@ -617,8 +618,10 @@ constexpr int kMaxMono = 15;  // monomials of a homogeneous polynomial of degree
 // Real-space weight of sub-map g: coef * Y_{ell,m}(r_hat) written as the homogeneous polynomial
 // sum_{k<=ell} sum_{a<=ell-k} poly[idx] rx^a ry^(ell-k-a) rz^k / |r|^ell (idx runs over (k, a) in
 // that order); y0 = Y(0) at r = 0, where the reference's zero-vector convention applies.
+// The first nfact sub-maps are the ell = 2 ones: their weights are accumulated without the common
+// 1/|r|^2, which multiplies the sum once before the remaining sub-maps are added.
 struct SubW {
-  int n;
+  int n, nfact;
   int ell[kMaxSub];
   float coef[kMaxSub];
   float y0[kMaxSub];
@@ -668,6 +671,17 @@ pointing_rows(float2* work, long long slot, int pitch, long long rows_per_map, i
   float acc[E][2];
 #pragma unroll
   for (int e = 0; e < E; ++e) { acc[e][0] = 0.f; acc[e][1] = 0.f; }
+  // acc *= 1/|r|^2 (the ell = 2 weights accumulated so far carry only their numerators); at r = 0
+  // those numerators vanish exactly, as Y_2m(0) = 0 does
+  auto rescale = [&](float rzb_v) {
+    const float rzn = rzb_v - (float)geo.n2 * geo.dx2;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float rz0 = fmaf((float)(2 * e * P), geo.dx2, 2 * e >= E ? rzn : rzb_v), rz1 = rz0 + geo.dx2;
+      acc[e][0] *= __builtin_amdgcn_rcpf(fmaxf(fmaf(rz0, rz0, rxy2), 1e-30f));
+      acc[e][1] *= __builtin_amdgcn_rcpf(fmaxf(fmaf(rz1, rz1, rxy2), 1e-30f));
+    }
+  };
   for (int g = 0; g < sw.n; ++g) {
     const float2* src = work + ((long long)g * batch + t) * slot + row0 * pitch;
     // Opaque copies of the lane indices: index arithmetic derived from them is recomputed per
@@ -675,6 +689,7 @@ pointing_rows(float2* work, long long slot, int pitch, long long rows_per_map, i
     int lane_g = lane, tp_g = tp;
     float rzb_g = rzb;
     asm volatile("" : "+v"(lane_g), "+v"(tp_g), "+v"(rzb_g));
+    if (g == sw.nfact && g > 0) rescale(rzb_g);
     wave_lds_sync();  // the previous sub-map's reads of the tile are issued
     load_prepass_rows<M, RW, 64>(buf, LDT, src, pitch, rows_per_map - row0, lz_in, lane_g, stwN);
     wave_lds_sync();
@@ -699,13 +714,11 @@ pointing_rows(float2* work, long long slot, int pitch, long long rows_per_map, i
         const float D0 = cf * (pc[0] * ry * ry + pc[1] * rx * ry + pc[2] * rx * rx);
         const float D1 = cf * (pc[3] * ry + pc[4] * rx);
         const float D2 = cf * pc[5];
-        // at r = 0 (only possible with rx = ry = rz = 0) pz = 0 exactly and Y_2m(0) = 0
+        // numerator only (the ell = 2 sub-maps come first; rescale() divides by |r|^2)
         auto sink = [&](int e, int, float2 z) {
           const float rz0 = rz_of(e), rz1 = rz0 + geo.dx2;
-          const float w0 = fmaf(fmaf(D2, rz0, D1), rz0, D0) * __builtin_amdgcn_rcpf(fmaxf(fmaf(rz0, rz0, rxy2), 1e-30f));
-          const float w1 = fmaf(fmaf(D2, rz1, D1), rz1, D0) * __builtin_amdgcn_rcpf(fmaxf(fmaf(rz1, rz1, rxy2), 1e-30f));
-          acc[e][0] = fmaf(z.x, w0, acc[e][0]);
-          acc[e][1] = fmaf(z.y, w1, acc[e][1]);
+          acc[e][0] = fmaf(z.x, fmaf(fmaf(D2, rz0, D1), rz0, D0), acc[e][0]);
+          acc[e][1] = fmaf(z.y, fmaf(fmaf(D2, rz1, D1), rz1, D0), acc[e][1]);
         };
         pencil_inverse_tail<M>(tp_g, rw, sink, stw);
         break;
@@ -766,6 +779,7 @@ pointing_rows(float2* work, long long slot, int pitch, long long rows_per_map, i
       }
     }
   }
+  if (sw.nfact == sw.n) rescale(rzb);
   wave_lds_sync();  // all reads of the tile by the last inverse stage are issued
   // mask and scale in registers, then the forward row transform (first stage from registers)
   const float norm = scale / (float)geo.N;
@@ -1078,6 +1092,7 @@ int check_rop(const pb3d_rop* r, bool input_side) {
 
 int check_kop(const pb3d_kop* k) {
   if (!k) return 0;
+  if (k->x_batch_group < 0) return fail(PB3D_ERR_ARG, "kop.x_batch_group must be >= 0");
   if (k->kind < 0 || k->kind > 6) return fail(PB3D_ERR_ARG, "kop.kind out of range");
   if (k->kind == 6 && (!k->bin_table || !k->ft || k->nk < 1 || k->nk > 4 || k->terms < 1 || k->terms > 254))
     return fail(PB3D_ERR_ARG, "kop BINPOLY needs bin_table, coef (ft), 1<=nk<=4 and 1<=terms<=254");
@@ -1261,8 +1276,14 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   float2* work = (float2*)work_;
   SubW sw{};
   sw.n = nsub;
+  // sub-map order in the fused rows (and work slots): the ell = 2 sub-maps first
+  int order[kMaxSub];
+  int no = 0;
+  for (int k = 0; k < nsub; ++k) if (subs[k].ell == 2) order[no++] = k;
+  sw.nfact = no;
+  for (int k = 0; k < nsub; ++k) if (subs[k].ell != 2) order[no++] = k;
   for (int k = 0; k < nsub; ++k) {
-    const pb3d_submap& sb = subs[k];
+    const pb3d_submap& sb = subs[order[k]];
     if (int e = check_kop(&sb.kop)) return e;
     if (sb.kop.kind == 5) return fail(PB3D_ERR_ARG, "pointing: PACK is an R2C epilogue");
     if (!sb.in && sb.kop.kind != 4) return fail(PB3D_ERR_ARG, "pointing: sub-map input missing");
@@ -1284,7 +1305,7 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   // 1. pruned x-column C2R pass of every sub-map into work[k*batch + b], over the common support
   //    (so that the y pass reads no entry that was not written)
   for (int k = 0; k < nsub; ++k) {
-    const pb3d_submap& sb = subs[k];
+    const pb3d_submap& sb = subs[order[k]];
     KOp kk = to_kop(&sb.kop);
     const ColMap cx = xcols(g, pin, true, 0, pz);
     bool on = prof_begin(3, s, sm, col_bytes(cx, p->L[0], batch));

@@ -14,7 +14,7 @@ __global__ void kmap_kernel(const float2* __restrict__ x, float2* __restrict__ o
     const int i = (int)(e / n12);
     const long long r = e % n12;
     const int j = (int)(r / g.n2h), l = (int)(r % g.n2h);
-    float2 v = kop_value(op, g, bt, e, i, j, l, x[bt * op.x_stride + e]);
+    float2 v = kop_value(op, g, bt, e, i, j, l, x[(bt / op.x_group) * op.x_stride + e]);
     v = cscale(v, op.scale);
     float2* o = out + bt * g.H + e;
     if (op.accumulate) v = cadd(v, *o);
@@ -94,7 +94,7 @@ __global__ void kmap_packed_kernel(const float2* __restrict__ x, const int32_t* 
     const int i = (int)(e / n12);
     const long long r = e % n12;
     const int j = (int)(r / g.n2h), l = (int)(r % g.n2h);
-    const float2 xv = x ? x[bt * op.x_stride + e] : make_float2(0.f, 0.f);
+    const float2 xv = x ? x[(bt / op.x_group) * op.x_stride + e] : make_float2(0.f, 0.f);
     float2 v = cscale(kop_value(op, g, bt, e, i, j, l, xv), op.scale);
     float2* o = out + bt * M + p;
     if (op.accumulate) v = cadd(v, *o);

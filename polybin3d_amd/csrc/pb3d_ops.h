@@ -45,6 +45,7 @@ struct KOp {
   const int32_t* bins;
   const float* w;
   long long x_stride;
+  int x_group;  // batch entries per input step (>= 1)
   float scale;
   const int32_t* pack_index;
   long long packed_len;
@@ -66,9 +67,10 @@ inline ROp to_rop(const pb3d_rop* r) {
 }
 inline KOp to_kop(const pb3d_kop* k) {
   KOp o{};
-  if (!k) { o.kind = 0; o.scale = 1.f; return o; }
+  if (!k) { o.kind = 0; o.scale = 1.f; o.x_group = 1; return o; }
   o.kind = k->kind; o.ell = k->ell; o.ell2 = k->ell2; o.m = k->m; o.accumulate = k->accumulate;
   o.bin_table = k->bin_table; o.bins = k->bins; o.w = k->w; o.x_stride = k->x_batch_stride;
+  o.x_group = k->x_batch_group > 1 ? k->x_batch_group : 1;
   o.scale = k->scale;
   o.pack_index = k->pack_index; o.packed_len = k->packed_len; o.trip = k->trip;
   o.pair_index = k->pair_index; o.ft = (const float2*)k->ft; o.ft_lstride = k->ft_lstride;

@@ -308,6 +308,87 @@ class PSpec:
             F[li * Nk:(li + 1) * Nk, :] = r.reshape(Nk, Nl * Nk) * scaling
         return F
 
+    def _fisher_contributions_dev(self, seeds):
+        """Sum over `seeds` of the MC Fisher contributions (pspec.py:215-493 for each seed), FP64 on the
+        device.  The seeds go through every pass together: their GRFs, filters and Q maps are batch
+        entries of the same launches (a sub-map input is shared by the Nk bin entries of its seed), so
+        small grids fill the GPU.  Chunks of seeds keep the fused chain's work maps within 16 GiB."""
+        b = self.base
+        seeds = [int(s) for s in seeds]
+        if self.user_Sinv or len(seeds) == 1:
+            F = None
+            for s in seeds:
+                f = self._fisher_contribution_dev(s).to(b.device)
+                F = f if F is None else F + f
+            return F
+        Nk, Nl, H, N = self.Nk, self.Nl, b.H, b.N
+        local = b.sightline == 'local' and self.lmax > 0
+        gmax = 2 * (2 * (Nl - 1)) + 2 if local else 1
+        pz = min(b.half_shape[2], b.support(self.k_bins[-1])[2] + 1)  # z planes of a compact work map
+        per_seed = gmax * Nk * int(b.gridsize[0]) * int(b.gridsize[1]) * pz * 8
+        chunk = max(1, min(len(seeds), (16 << 30) // per_seed))
+        F = torch.zeros((self.N_bins, self.N_bins), dtype=torch.float64, device=b.device)
+        for c0 in range(0, len(seeds), chunk):
+            F += self._fisher_batch(seeds[c0:c0 + chunk])
+        return F
+
+    def _fisher_batch(self, seeds):
+        b = self.base
+        R, Nk, Nl, H, N = len(seeds), self.Nk, self.Nl, b.H, b.N
+        if not hasattr(self, '_bins_rep'):
+            self._bins_rep = {}
+        if R not in self._bins_rep:
+            self._bins_rep[R] = torch.arange(Nk, dtype=torch.int32, device=b.device).repeat(R)
+        bins = self._bins_rep[R]
+        a = torch.stack([b.generate_data(seed=s + int(1e7), output_type='fourier') for s in seeds])
+        w = b.ainv_weight()
+        Ainv = b.kmap(a, C.kop(C.KOP_WEIGHT, w=w, x_batch_stride=H), batch=R) if w is not None else a
+        if self.const_mask:
+            SinvP = b.kmap(a, C.kop(C.KOP_PLAIN, scale=self.mask_mean, x_batch_stride=H), batch=R)
+        else:
+            a_real = b.c2r(a, batch=R)
+            SinvP = b.r2c(None, rop=C.rop(C.ROP_MUL, a=a_real, b=self.mask, sa=N, sb=0), batch=R)
+            del a_real
+        local = b.sightline == 'local' and self.lmax > 0
+        lefts = torch.empty((R, Nl) + b.half_shape, dtype=torch.complex64, device=b.device)
+        lefts[:, 0].copy_(Ainv)
+        leg = {}
+        if local:
+            SinvP_real = b.c2r(SinvP, batch=R)
+            Ainv_real = b.c2r(Ainv, batch=R)
+            for li in range(1, Nl):
+                leg[li] = torch.empty((R,) + b.half_shape, dtype=torch.complex64, device=b.device)
+                for r in range(R):
+                    b.apply_fourier_harmonics(Ainv_real[r], self.Ylm_real[2 * li], self.Ylm_fourier[2 * li],
+                                              out=lefts[r, li])
+                    b.apply_fourier_harmonics(SinvP_real[r], self.Ylm_real[2 * li], self.Ylm_fourier[2 * li],
+                                              out=leg[li][r])
+            del SinvP_real, Ainv_real
+        else:
+            for li in range(1, Nl):
+                lefts[:, li].copy_(Ainv)
+        ells = [0] * Nl if local else [2 * li for li in range(Nl)]
+        Q = torch.empty((R * Nk,) + b.half_shape, dtype=torch.complex64, device=b.device)
+        F = torch.zeros((self.N_bins, self.N_bins), dtype=torch.float64, device=b.device)
+        scaling = 0.5 * b.volume / b.gridsize.prod() ** 2
+        sup = b.support(self.k_bins[-1])
+        common = dict(bin_table=self.table, bins=bins, x_batch_stride=H, x_batch_group=Nk, support=sup)
+        mask, scale = (None, self.mask_mean) if self.const_mask else (self.mask, 1.0)
+        for li in range(Nl):
+            ell = 2 * li
+            # the Q maps of _q_maps for every seed: batch entry r*Nk + bin reads seed r's maps
+            if not local or li == 0:
+                subs = [(C.kop(C.KOP_FILTER, ell=ell, **common), SinvP, 0, 0, 1.0)]
+            else:
+                subs = [(C.kop(C.KOP_FILTER, ell=0, **common), leg[li], 0, 0, 0.5)]
+                subs += [(C.kop(C.KOP_YLM, ell=ell, m=m, **common), SinvP, ell, m, 0.5) for m in range(2 * ell + 1)]
+            b.pointing(subs, R * Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup))
+            for r in range(R):
+                red = b.bin_reduce(Q[r * Nk:(r + 1) * Nk], self.k_bins, ells, y=lefts[r], y_qstride=0,
+                                   y_ostride=H, nq=Nk)
+                F[li * Nk:(li + 1) * Nk, :] += red.reshape(Nk, Nl * Nk) * scaling
+        return F
+
     def _fisher_contribution_generic(self, seed):
         """Reference call sequence with a user-supplied S^-1 (pspec.py:215-493), maps on the GPU."""
         b = self.base
@@ -370,13 +451,18 @@ class PSpec:
                     add_row(li * Nk + ki, Qm)
         return F
 
-    def compute_fisher(self, N_it, verb=False):
-        """Mean of N_it realizations, accumulated in FP64 on the GPU, then inverted (pspec.py:553-569)."""
+    def compute_fisher(self, N_it, verb=False, batch=1):
+        """Mean of N_it realizations, accumulated in FP64 on the GPU, then inverted (pspec.py:553-569).
+        batch > 1 (no counterpart in the reference) processes that many seeds per pass (same result up
+        to FP64 summation order)."""
         fish = None
-        for seed in range(N_it):
-            if verb and seed % 5 == 0:
-                print("Computing Fisher contribution %d of %d" % (seed + 1, N_it))
-            f = self._fisher_contribution_dev(seed)
+        batch = max(1, int(batch))
+        for s0 in range(0, N_it, batch):
+            if verb and (batch > 1 or s0 % 5 == 0):
+                print("Computing Fisher contribution %d of %d" % (s0 + 1, N_it))
+            seeds = list(range(s0, min(N_it, s0 + batch)))
+            f = self._fisher_contributions_dev(seeds) if len(seeds) > 1 else self._fisher_contribution_dev(seeds[0])
+            f = f.to(self.base.device)
             fish = f / N_it if fish is None else fish + f / N_it
         self.fish = fish.cpu().numpy()
         self.inv_fish = torch.linalg.inv(fish).cpu().numpy()

@@ -127,3 +127,26 @@ def test_pk_mc_fisher_and_unwindowed(hip, case):
     ref = po.Pk_unwindowed(d, fish=fish)
     for k in ref:
         close(got[k], ref[k], tol=2e-5, scale=np.abs(ref['p0']))
+
+
+@pytest.mark.parametrize("case", ["global_const", "local_mask"])
+def test_pk_mc_fisher_batched_seeds(hip, case):
+    """compute_fisher(N_it, batch=R): the R seeds of a batch share every pass (their GRFs, filters and
+    Q maps are batch entries of the same launches, sub-map inputs grouped per seed); the result is the
+    oracle's mean over the same seeds."""
+    import polybin3d_amd as pb
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    if case == "global_const":
+        kw, mask = {}, 0.8 * np.ones((n, n, n))
+    else:
+        kw = dict(boxcenter=[10., 10., 10.], sightline='local')
+        mask = shell_mask(n, L, [10., 10., 10.])
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    ps = pb.PSpec(b, edges, lmax=4, mask=mask)
+    po = O.PSpec(o, edges, lmax=4, mask=mask)
+    fish = po.compute_fisher(3)
+    sc = np.sqrt(np.outer(np.abs(np.diag(fish)), np.abs(np.diag(fish))))
+    close(ps.compute_fisher(3, batch=3), fish, scale=sc)
+    close(ps.compute_fisher(3, batch=2), fish, scale=sc)
