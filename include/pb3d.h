@@ -188,14 +188,14 @@ int pb3d_fft_c2r(const pb3d_plan* plan, const void* in, float* out, void* work, 
  * The sub-map column passes run pruned to each input's support; one row kernel then inverts all
  * nsub rows, combines them with their real-space weights, applies the mask (NULL = none) and the
  * forward row transform, so the real-space maps never reach HBM.  work: the float2 count given by
- * pb3d_fft_pointing_work_size (at most nsub*batch*n0*n1*(n2/2+1)): the intermediate maps keep only
- * the z planes inside the band limits.  nsub <= 16. */
+ * pb3d_fft_pointing_work_size (at most (nsub+1)*batch*n0*n1*(n2/2+1)): the intermediate maps keep
+ * only the z planes inside the band limits.  nsub <= 16. */
 int pb3d_fft_pointing(const pb3d_plan* plan, int batch, int nsub, const pb3d_submap* subs,
                       const float* mask, float scale, const pb3d_kop* kop_out, void* out, void* work,
                       void* stream);
 /* float2 entries of work that pb3d_fft_pointing needs for these sub-maps and output op: nsub*batch
- * maps of n0*n1*pz, pz = the largest z band limit + 1 over the sub-map kops and kop_out (n2/2+1
- * where one has none).  Host-only; -1 on a bad argument.  (No counterpart in the reference, whose
+ * maps of n0*n1*pzi and batch maps of n0*n1*pzo, pzi = the largest z band limit + 1 over the
+ * sub-map kops, pzo = that of kop_out (n2/2+1 where one has none).  Host-only; -1 on a bad argument.  (No counterpart in the reference, whose
  * pointing allocates full maps per step.) */
 int64_t pb3d_fft_pointing_work_size(const pb3d_grid* grid, int batch, int nsub, const pb3d_submap* subs,
                                     const pb3d_kop* kop_out);

@@ -559,7 +559,7 @@ class BSpec:
             # chunk of Q maps per fused chain: its work maps (compact, z planes kz <= support) within
             # 12 GiB; larger chunks amortise the pass tails (measured: 60 maps 8 % faster than 15)
             per_map = int(b.gridsize[0]) * int(b.gridsize[1]) * min(b.half_shape[2], self._sup[2] + 1)
-            CH = int(max(1, min(64, (12 << 30) // (G * 8 * per_map))))
+            CH = int(max(1, min(64, (12 << 30) // ((G + 1) * 8 * per_map))))
             t1 = t0
             while t1 < self.N_bins and t1 - t0 < CH and ab[t1, 3] == ell:
                 t1 += 1

@@ -259,8 +259,10 @@ template <> struct ColCfg<16>   { static constexpr int T = 128; };
 //   MODE 0: plain.  MODE 1: first C2R pass, input through KOp (broadcast by x_stride).
 //   MODE 2: last R2C pass, output through KOp (may accumulate / pack).
 // ------------------------------------------------------------------------------------------
+//   tpp > 0: plane-aligned tiles, tpp tiles per plane (tile b covers columns (b % tpp)*T + col of
+//   plane b / tpp), so that no tile straddles two planes, whose data lie far apart in the y pass.
 struct ColMap {
-  int ncols, cpp, split, gap;
+  int ncols, cpp, split, gap, tpp;
   long long plane, S;          // mode coordinates
   long long in_plane, in_S;    // input addressing
   long long out_plane, out_S;  // output addressing
@@ -285,8 +287,16 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
   const int col = tid % T;
   const int t = tid / T;
   const long long bt = blockIdx.y;
-  const int c = blockIdx.x * T + col;
-  const bool valid = c < cm.ncols;
+  int c;
+  bool valid;
+  if (cm.tpp > 0) {
+    const int lc = (blockIdx.x % cm.tpp) * T + col;
+    valid = lc < cm.cpp;
+    c = (blockIdx.x / cm.tpp) * cm.cpp + lc;
+  } else {
+    c = blockIdx.x * T + col;
+    valid = c < cm.ncols;
+  }
   long long base = 0, bin = 0, bout = 0;
   if (valid) {
     int q = c / cm.cpp;
@@ -606,10 +616,10 @@ c2r_rows(const float2* __restrict__ in, float* __restrict__ out, long long nrows
 
 // ------------------------------------------------------------------------------------------
 // Fused pointing row kernel: for output map t and row (i0,i1): invert the rows of all nsub
-// sub-maps (work[g*batch + t]), accumulate coef_g * Y_g(x_hat) * z in registers, multiply by the
-// mask and scale, forward-transform the row and write kz < lz_out into work[t] (in place: a wave
-// owns its rows in every sub-map).  Work map j starts at work + j * slot; row r of a map at
-// + r * pitch, with pitch >= max(lz_in, lz_out) so that the in-place rows never overlap.
+// sub-maps (work map g*batch + t), accumulate coef_g * Y_g(x_hat) * z in registers, multiply by the
+// mask and scale, forward-transform the row and write kz < lz_out into output map t.  Work map j
+// starts at work + j * slot_in, output map t at out + t * slot_out; row r of a map at + r * pitch
+// (compact maps: pitch_in >= lz_in, pitch_out >= lz_out).
 // Each wave works on its own RW = 64/P rows with its own transposed LDS tile, so the exchanges
 // between FFT stages need only wave-level ordering: the sub-map loop has no block barrier.
 // ------------------------------------------------------------------------------------------
@@ -642,7 +652,8 @@ template <int M> struct PointCfg {
 
 template <int M>
 __global__ void __launch_bounds__(PointCfg<M>::NT, PB3D_POINT_WAVES)
-pointing_rows(float2* work, long long slot, int pitch, long long rows_per_map, int batch, SubW sw,
+pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, long long slot_out, int pitch_out,
+              long long rows_per_map, int batch, SubW sw,
               const float* __restrict__ mask, float scale, int lz_in, int lz_out, const float2* __restrict__ tw,
               const float2* __restrict__ twN, Geo geo) {
   using PC = PointCfg<M>;
@@ -683,7 +694,7 @@ pointing_rows(float2* work, long long slot, int pitch, long long rows_per_map, i
     }
   };
   for (int g = 0; g < sw.n; ++g) {
-    const float2* src = work + ((long long)g * batch + t) * slot + row0 * pitch;
+    const float2* src = work + ((long long)g * batch + t) * slot_in + row0 * pitch_in;
     // Opaque copies of the lane indices: index arithmetic derived from them is recomputed per
     // sub-map instead of being hoisted out of the loop and held in (spilled) registers.
     int lane_g = lane, tp_g = tp;
@@ -691,7 +702,7 @@ pointing_rows(float2* work, long long slot, int pitch, long long rows_per_map, i
     asm volatile("" : "+v"(lane_g), "+v"(tp_g), "+v"(rzb_g));
     if (g == sw.nfact && g > 0) rescale(rzb_g);
     wave_lds_sync();  // the previous sub-map's reads of the tile are issued
-    load_prepass_rows<M, RW, 64>(buf, LDT, src, pitch, rows_per_map - row0, lz_in, lane_g, stwN);
+    load_prepass_rows<M, RW, 64>(buf, LDT, src, pitch_in, rows_per_map - row0, lz_in, lane_g, stwN);
     wave_lds_sync();
     const int ell = sw.ell[g];
     const float cf = sw.coef[g];
@@ -797,14 +808,14 @@ pointing_rows(float2* work, long long slot, int pitch, long long rows_per_map, i
   auto get = [&](int e) { return make_float2(acc[e][0], acc[e][1]); };
   pencil_forward_from_regs<M, true>(tp, get, rw, stw);
   wave_lds_sync();
-  float2* o = work + t * slot + row0 * pitch;
+  float2* o = out + t * slot_out + row0 * pitch_out;
   constexpr int ITS = (RW * (M + 1) + 63) / 64;
 #pragma unroll
   for (int it = 0; it < ITS; ++it) {
     const int f = lane + it * 64;
     const int rr = f / (M + 1), k = f % (M + 1);
     if (f >= RW * (M + 1) || row0 + rr >= rows_per_map || k >= lz_out) continue;
-    o[(long long)rr * pitch + k] = r2c_split<M>(buf, LDT, rr, k, stwN);
+    o[(long long)rr * pitch_out + k] = r2c_split<M>(buf, LDT, rr, k, stwN);
   }
 }
 
@@ -868,11 +879,17 @@ template <int DIR, int MODE>
 static hipError_t launch_cols(int L, const float2* in, float2* out, const ColMap& cm, long long in_bs, long long out_bs,
                               int batch, const float2* tw, const Geo& geo, const KOp& kop, hipStream_t s) {
   if (cm.ncols <= 0) return hipSuccess;
+  // Plane-aligned tiles for the y pass when a plane's columns leave at least half a tile over:
+  // a straddling tile would read a few columns of the next plane at a stride.
 #define PB3D_COLS(LL)                                                                          \
   case LL: {                                                                                   \
     constexpr int T = ColCfg<LL>::T;                                                           \
-    dim3 grid((cm.ncols + T - 1) / T, batch);                                                  \
-    fft_cols<LL, DIR, MODE><<<grid, Decomp<LL>::P * T, 0, s>>>(in, out, cm, in_bs, out_bs, tw, geo, kop); \
+    ColMap c2 = cm;                                                                            \
+    c2.tpp = 0;                                                                                \
+    if (cm.tpp && cm.cpp % T >= T / 2) c2.tpp = (cm.cpp + T - 1) / T;                          \
+    const long long nt = c2.tpp ? (long long)(cm.ncols / cm.cpp) * c2.tpp : (cm.ncols + T - 1) / T; \
+    dim3 grid((unsigned)nt, batch);                                                            \
+    fft_cols<LL, DIR, MODE><<<grid, Decomp<LL>::P * T, 0, s>>>(in, out, c2, in_bs, out_bs, tw, geo, kop); \
     return hipGetLastError();                                                                  \
   }
   switch (L) {
@@ -919,15 +936,16 @@ static hipError_t launch_c2r_rows(int M, const float2* in, float* out, long long
 #undef PB3D_ROWS
 }
 
-static hipError_t launch_pointing_rows(int M, float2* work, long long slot, int pitch, long long rpm, int batch,
-                                       const SubW& sw,
+static hipError_t launch_pointing_rows(int M, const float2* work, long long slot_in, int pitch_in, float2* out,
+                                       long long slot_out, int pitch_out, long long rpm, int batch, const SubW& sw,
                                        const float* mask, float scale, int lz_in, int lz_out, const float2* tw,
                                        const float2* twN, const Geo& geo, hipStream_t s) {
 #define PB3D_PROWS(MM)                                                                           \
   case MM: {                                                                                     \
     constexpr int RB = PointCfg<MM>::RB;                                                         \
     dim3 grid((unsigned)((rpm + RB - 1) / RB), batch);                                           \
-    pointing_rows<MM><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot, pitch, rpm, batch, sw, mask, scale, lz_in, \
+    pointing_rows<MM><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, pitch_out, rpm,  \
+                                                         batch, sw, mask, scale, lz_in,          \
                                                          lz_out, tw, twN, geo);                  \
     return hipGetLastError();                                                                    \
   }
@@ -1044,6 +1062,7 @@ static ColMap ycols(const Geo& g, const Prune& pr, bool c2r, int pitch_in = 0, i
   ColMap cm{};
   cm.S = g.n2h;
   cm.ncols = g.n0 * pr.lz; cm.cpp = pr.lz; cm.plane = (long long)g.n1 * g.n2h; cm.split = 1 << 30; cm.gap = 0;
+  cm.tpp = 1;  // plane-aligned tiles allowed (the launcher decides)
   cm.rad_in = c2r ? pr.r1 : -1;
   cm.rad_out = c2r ? -1 : pr.r1;
   y_side(g, cm, pitch_in, cm.in_plane, cm.in_S);
@@ -1260,8 +1279,7 @@ extern "C" int64_t pb3d_fft_pointing_work_size(const pb3d_grid* grid, int batch,
   const Geo g = make_geo(*grid);
   Prune pin, pout;
   pointing_prunes(g, nsub, subs, kop_out, pin, pout);
-  const int pz = compact_pitch(g, pin.lz > pout.lz ? pin.lz : pout.lz);
-  return (int64_t)nsub * batch * g.n0 * g.n1 * pz;
+  return (int64_t)batch * g.n0 * g.n1 * ((int64_t)nsub * compact_pitch(g, pin.lz) + compact_pitch(g, pout.lz));
 }
 
 extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const pb3d_submap* subs, const float* mask,
@@ -1298,41 +1316,43 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   pointing_prunes(g, nsub, subs, kop_out, pin, pout);
   KOp plain = to_kop(nullptr);
   ProfSample sm{};
-  // Work maps use the compact layout: z planes kz < pz only, pz covering both the inverse-side and
-  // the forward-side band limits (the fused rows are rewritten in place), one map every ws entries.
-  const int pz = compact_pitch(g, pin.lz > pout.lz ? pin.lz : pout.lz);
-  const long long ws = (long long)g.n0 * g.n1 * pz;
+  // Work maps use the compact layout: the nsub*batch sub-maps keep the z planes kz < pzi of the
+  // inverse-side band limit, one map every wsi entries; the batch output maps after them keep
+  // kz < pzo of the forward-side limit, one every wso entries.
+  const int pzi = compact_pitch(g, pin.lz), pzo = compact_pitch(g, pout.lz);
+  const long long wsi = (long long)g.n0 * g.n1 * pzi, wso = (long long)g.n0 * g.n1 * pzo;
+  float2* outw = work + (long long)nsub * batch * wsi;
   // 1. pruned x-column C2R pass of every sub-map into work[k*batch + b], over the common support
   //    (so that the y pass reads no entry that was not written)
   for (int k = 0; k < nsub; ++k) {
     const pb3d_submap& sb = subs[order[k]];
     KOp kk = to_kop(&sb.kop);
-    const ColMap cx = xcols(g, pin, true, 0, pz);
+    const ColMap cx = xcols(g, pin, true, 0, pzi);
     bool on = prof_begin(3, s, sm, col_bytes(cx, p->L[0], batch));
-    PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], (const float2*)sb.in, work + (long long)k * batch * ws, cx,
-                                       sb.kop.x_batch_stride, ws, batch, p->tw[0], g, kk, s)));
+    PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], (const float2*)sb.in, work + (long long)k * batch * wsi, cx,
+                                       sb.kop.x_batch_stride, wsi, batch, p->tw[0], g, kk, s)));
     prof_end(on, sm, s);
   }
   // 2. pruned y-column C2R pass over all sub-maps
-  const ColMap cyi = ycols(g, pin, true, pz, pz);
+  const ColMap cyi = ycols(g, pin, true, pzi, pzi);
   bool on = prof_begin(2, s, sm, col_bytes(cyi, p->L[1], batch * nsub));
-  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, cyi, ws, ws, batch * nsub, p->tw[1], g, plain, s)));
+  PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, cyi, wsi, wsi, batch * nsub, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
-  // 3. fused rows: inverse rows of every sub-map, weights, mask, forward row -> work[b]
+  // 3. fused rows: inverse rows of every sub-map, weights, mask, forward row -> output map b
   const double rows1 = (double)g.n0 * g.n1;
   on = prof_begin(5, s, sm, (double)batch * (rows1 * 8.0 * (pin.lz * (double)nsub + pout.lz) + (mask ? g.N * 4.0 : 0.0)));
-  PB3D_CHECK_HIP(launch_pointing_rows(p->L[2], work, ws, pz, (long long)g.n0 * g.n1, batch, sw, mask, scale, pin.lz,
-                                      pout.lz, p->tw[2], p->twN, g, s));
+  PB3D_CHECK_HIP(launch_pointing_rows(p->L[2], work, wsi, pzi, outw, wso, pzo, (long long)g.n0 * g.n1, batch, sw,
+                                      mask, scale, pin.lz, pout.lz, p->tw[2], p->twN, g, s));
   prof_end(on, sm, s);
   // 4. pruned forward y and x column passes, epilogue into out
-  const ColMap cyo = ycols(g, pout, false, pz, pz), cxo = xcols(g, pout, false, pz, 0);
+  const ColMap cyo = ycols(g, pout, false, pzo, pzo), cxo = xcols(g, pout, false, pzo, 0);
   on = prof_begin(6, s, sm, col_bytes(cyo, p->L[1], batch));
-  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cyo, ws, ws, batch, p->tw[1], g, plain, s)));
+  PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], outw, outw, cyo, wso, wso, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
   KOp ko = to_kop(kop_out);
   const long long out_bs = (kop_out && kop_out->kind == 5) ? kop_out->packed_len : g.H;
   on = prof_begin(4, s, sm, col_bytes(cxo, p->L[0], batch));
-  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, (float2*)out_, cxo, ws, out_bs, batch, p->tw[0], g, ko, s)));
+  PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], outw, (float2*)out_, cxo, wso, out_bs, batch, p->tw[0], g, ko, s)));
   prof_end(on, sm, s);
   g_rev += (long long)batch * nsub;
   g_fwd += batch;

@@ -325,7 +325,7 @@ class PSpec:
         local = b.sightline == 'local' and self.lmax > 0
         gmax = 2 * (2 * (Nl - 1)) + 2 if local else 1
         pz = min(b.half_shape[2], b.support(self.k_bins[-1])[2] + 1)  # z planes of a compact work map
-        per_seed = gmax * Nk * int(b.gridsize[0]) * int(b.gridsize[1]) * pz * 8
+        per_seed = (gmax + 1) * Nk * int(b.gridsize[0]) * int(b.gridsize[1]) * pz * 8
         chunk = max(1, min(len(seeds), (16 << 30) // per_seed))
         F = torch.zeros((self.N_bins, self.N_bins), dtype=torch.float64, device=b.device)
         for c0 in range(0, len(seeds), chunk):

@@ -67,21 +67,24 @@ def test_host_side_argument_validation():
 
 
 def test_pointing_work_size():
-    """The pointing chain's work maps keep only the z planes inside the band limits: pz = the largest
-    support[2] + 1 over the sub-maps and the output op, n2/2+1 where one has no support."""
+    """The pointing chain's work maps keep only the z planes inside the band limits: nsub*batch
+    sub-maps with pzi = the largest support[2] + 1 over the sub-maps, batch output maps with
+    pzo = support[2] + 1 of the output op; n2/2+1 where one has no support."""
     lib = C.lib()
     g = C.make_grid([256, 256, 256], [1000.] * 3, [0, 0, 0], [0, 0, 1], 0)
     arr = (C.SubMap * 3)()
     for j, sz in enumerate((40, 63, 20)):
         arr[j].kop = C.kop(C.KOP_PLAIN, support=[sz, sz, sz])
     out63 = C.kop(C.KOP_PLAIN, support=[63, 63, 63])
-    assert lib.pb3d_fft_pointing_work_size(ctypes.byref(g), 5, 3, arr, ctypes.byref(out63)) == 3 * 5 * 256 * 256 * 64
+    assert lib.pb3d_fft_pointing_work_size(ctypes.byref(g), 5, 3, arr, ctypes.byref(out63)) == \
+        5 * 256 * 256 * (3 * 64 + 64)
     out70 = C.kop(C.KOP_PLAIN, support=[70, 70, 70])
-    assert lib.pb3d_fft_pointing_work_size(ctypes.byref(g), 5, 1, arr, ctypes.byref(out70)) == 5 * 256 * 256 * 71
+    assert lib.pb3d_fft_pointing_work_size(ctypes.byref(g), 5, 1, arr, ctypes.byref(out70)) == \
+        5 * 256 * 256 * (41 + 71)
     # no output band limit, or a support reaching the Nyquist plane: whole half-complex rows
-    assert lib.pb3d_fft_pointing_work_size(ctypes.byref(g), 2, 2, arr, None) == 2 * 2 * 256 * 256 * 129
+    assert lib.pb3d_fft_pointing_work_size(ctypes.byref(g), 2, 2, arr, None) == 2 * 256 * 256 * (2 * 64 + 129)
     nyq = C.kop(C.KOP_PLAIN, support=[128, 128, 128])
-    assert lib.pb3d_fft_pointing_work_size(ctypes.byref(g), 1, 1, arr, ctypes.byref(nyq)) == 256 * 256 * 129
+    assert lib.pb3d_fft_pointing_work_size(ctypes.byref(g), 1, 1, arr, ctypes.byref(nyq)) == 256 * 256 * (41 + 129)
     assert lib.pb3d_fft_pointing_work_size(ctypes.byref(g), 1, 17, arr, None) == -1
     assert lib.pb3d_fft_pointing_work_size(None, 1, 1, arr, None) == -1
 
