@@ -450,10 +450,10 @@ __device__ __forceinline__ void load_prepass(float2* buf, int LDT, const float2*
 // The same staging with lanes walking the rows: lane (rr = tid % T, kb = tid / T) handles
 // k = kb + it * (NT / T), so every address offset is a compile-time immediate and no division is
 // needed.  Used where a tile has few rows (one wave's rows).  Needs rows_left >= 1: rows past the
-// end are read as row 0 and replaced by zeros.  Three uniform regimes: lz_in == M/2+1 (every
-// entry of the left half live, none of the right: unconditional loads), lz_in < M/2+1
-// (predicated loads, right half zero), lz_in > M/2+1 (both halves read).  Row rr starts at
-// src + rr * pitch (pitch >= lz_in).
+// end are read as row 0 and replaced by zeros.  Two uniform regimes: lz_in <= M/2+1 (the right
+// half X[M-k] is a known zero apart from k = M/2, so each pair needs one load and one complex
+// multiply per output; loads are unpredicated at clamped addresses), lz_in > M/2+1 (both halves
+// read, general pair formula).  Row rr starts at src + rr * pitch (pitch >= lz_in).
 template <int M, int T, int NT>
 __device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const float2* __restrict__ src, int pitch,
                                                   long long rows_left, int lz_in, int tid, const float2* twN) {
@@ -465,24 +465,37 @@ __device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const fl
   const bool rv = rr < rows_left;
   const float2* rowp = src + (rv ? rr : 0) * pitch;
   const float2 zero = make_float2(0.f, 0.f);
+  if (lz_in <= K) {
+    // With X[M-k] = 0 (k != M/2): Z'[k] = a (1 + i conj W^k) and Z'[M-k] = conj(a) (1 - i conj W^(M-k)),
+    // W^k = twN[k]; at k = M/2 the pair is (a, a): Z' = (2 Re a, -2 Im a).
+    float2 xa[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int k = kb + it * KS;
+      const float2 v = rowp[k < lz_in ? k : lz_in - 1];
+      xa[it] = (rv && k < lz_in) ? v : zero;
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int k = kb + it * KS;
+      if (IT * KS != K && k >= K) continue;
+      float2 a = xa[it];
+      if (k == 0) a.y = 0.f;  // Im of the DC entry is ignored
+      const float2 w = twN[k];
+      const float p = 1.f + w.y;
+      float2 z1 = make_float2(fmaf(a.x, p, -a.y * w.x), fmaf(a.x, w.x, a.y * p));
+      if (2 * k == M) z1 = make_float2(2.f * a.x, -2.f * a.y);
+      buf[k * LDT + rr] = z1;
+      if (k != 0 && 2 * k != M) {
+        const float2 w2 = twN[M - k];
+        const float q = 1.f - w2.y;
+        buf[(M - k) * LDT + rr] = make_float2(fmaf(a.x, q, -a.y * w2.x), -fmaf(a.x, w2.x, a.y * q));
+      }
+    }
+    return;
+  }
   float2 xk[IT], xm[IT];
-  if (lz_in == K) {
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int k = kb + it * KS;
-      const float2 v = rowp[k < K ? k : K - 1];
-      xk[it] = rv ? v : zero;
-      xm[it] = zero;
-    }
-  } else if (lz_in < K) {
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      const int k = kb + it * KS;
-      xk[it] = zero;
-      xm[it] = zero;
-      if (rv && k < lz_in) xk[it] = rowp[k];
-    }
-  } else {
+  {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int k = kb + it * KS;
