@@ -155,6 +155,10 @@ __device__ __forceinline__ void pencil_fft(int t, Src0 src0, Lds lds, DstN dstN,
 #ifndef PB3D_COL_THREADS
 #define PB3D_COL_THREADS 1024
 #endif
+// plane-aligned y-pass tiles (see launch_cols)
+#ifndef PB3D_COL_ALIGN
+#define PB3D_COL_ALIGN 1
+#endif
 // minimum waves per SIMD the fused pointing rows are compiled for (4: at most 128 VGPRs)
 #ifndef PB3D_POINT_WAVES
 #define PB3D_POINT_WAVES 4
@@ -899,7 +903,7 @@ static hipError_t launch_cols(int L, const float2* in, float2* out, const ColMap
     constexpr int T = ColCfg<LL>::T;                                                           \
     ColMap c2 = cm;                                                                            \
     c2.tpp = 0;                                                                                \
-    if (cm.tpp && cm.cpp % T >= T / 2) c2.tpp = (cm.cpp + T - 1) / T;                          \
+    if (PB3D_COL_ALIGN && cm.tpp && cm.cpp % T >= T / 2) c2.tpp = (cm.cpp + T - 1) / T;       \
     const long long nt = c2.tpp ? (long long)(cm.ncols / cm.cpp) * c2.tpp : (cm.ncols + T - 1) / T; \
     dim3 grid((unsigned)nt, batch);                                                            \
     fft_cols<LL, DIR, MODE><<<grid, Decomp<LL>::P * T, 0, s>>>(in, out, c2, in_bs, out_bs, tw, geo, kop); \
