@@ -26,9 +26,9 @@ sys.path.insert(0, ROOT)
 METRIC = "Fisher MC realizations/s, Pk+Bk 256³: batched/split-bf16 vs plain FP32"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled x2.000 / x1.000
-# by a 1 GiB device copy) over one realization of this workload, for the pass kind they were taken on
-# (profiles/fft_cols_hbm_traffic_pk_bk_mc_fisher_256.md); other kinds report null.
-MEASURED_TRAFFIC = {}
+# by a 1 GiB device copy), averaged over the launches of one realization of this workload, for the
+# kernel they were taken on (profiles/hbm_traffic_pk_bk_mc_fisher_256.md); other kernels report null.
+MEASURED_TRAFFIC = {"pointing_rows<128>": 11.03e9}
 KIND_NAMES = {0: "r2c_rows", 1: "c2r_rows", 2: "fft_cols (plain C2R)", 3: "fft_cols (C2R first pass, fused prologue)",
               4: "fft_cols (R2C last pass, fused epilogue)", 5: "pointing_rows (fused Q-map rows)",
               6: "fft_cols (plain R2C)"}
