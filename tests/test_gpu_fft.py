@@ -123,14 +123,15 @@ def test_device_grf_statistics(hip):
     assert not torch.equal(a[2], a[3])
 
 
-def test_pruned_transforms(hip):
+# (shape, kmax): the second has 40 z planes in the band limit against 64-column y-pass tiles, so the
+# y passes run plane-aligned tiles over compact work maps
+@pytest.mark.parametrize("n,kmax", [((64, 32, 64), 0.09), ((64, 256, 128), 0.25)], ids=["small", "aligned_y_tiles"])
+def test_pruned_transforms(hip, n, kmax):
     """Band-limited C2R (input zero outside the support) and R2C with packed in-support output match
     numpy; the pruned passes skip only known-zero or unused data."""
     from polybin3d_amd import _capi as C
-    n = (64, 32, 64)
     b = make_base(n)
     o = O.PolyBin3D(1000., list(n))
-    kmax = 0.09
     sup = b.support(kmax)
     rng = np.random.RandomState(5)
     X = rng.randn(*o.modk_grid.shape) + 1j * rng.randn(*o.modk_grid.shape)
@@ -155,9 +156,10 @@ def test_pruned_transforms(hip):
 
 
 # (shape, l of the Y_lm sub-maps): row lengths n2/2 = 16, 128, 32, 64, 256 and every weight form
-# (l = 1..4); (4, 4, 256) has fewer rows than one block holds
+# (l = 1..4); (4, 4, 256) has fewer rows than one block holds; (16, 256, 64) runs the forward y pass
+# on plane-aligned tiles (33 z planes against 64-column tiles)
 POINT_CASES = [((32, 32, 32), 2), ((4, 4, 256), 2), ((16, 8, 64), 2), ((8, 16, 128), 2), ((8, 8, 512), 2),
-               ((32, 32, 32), 1), ((32, 32, 32), 3), ((32, 32, 32), 4), ((8, 16, 128), 4)]
+               ((32, 32, 32), 1), ((32, 32, 32), 3), ((32, 32, 32), 4), ((8, 16, 128), 4), ((16, 256, 64), 2)]
 
 
 @pytest.mark.parametrize("n,L", POINT_CASES, ids=lambda c: "x".join(map(str, c)) if isinstance(c, tuple) else "l%d" % c)
