@@ -16,7 +16,7 @@ How each part is pinned (tests/test_oracle_*.py):
     (Tutorial 1 cell 8) plus mode counting by brute force;
   * FFT wrappers: brute-force DFT on small grids, Parseval, plane wave <-> delta;
   * utils kernels (integrate, cross_integrate, real_integrate_double, integrate_row, filters,
-    sum_pair/sum_triplet, assemble_b3/b1): explicit per-bin loops on tiny grids, and the compiled
+    sum_pair/sum_triplet, assemble_b3/b1/bshot): explicit per-bin loops on tiny grids, and the compiled
     reference kernels in oracle/_ref when that build is present;
   * real spherical harmonics: the addition theorem sum_m Y_lm(a) Y_lm(b) = L_l(a.b);
   * Pk_ideal: plane-wave known answer P = A^2 V / (2 N_modes);
@@ -30,7 +30,8 @@ How each part is pinned (tests/test_oracle_*.py):
   * local-sightline Bk numerator: dense DFT-matrix restatement with L_l(k_hat . x_hat).
 Parity unpinned (no check beyond restatement and the compiled utils.pyx kernels they call):
 BSpec._compute_fisher per-seed values with a mask or a local sightline (pinned in expectation only
-for the unit mask); PSpec.compute_shot_contribution.
+for the unit mask); PSpec.compute_shot_contribution; BSpec.compute_shot_contribution as a whole (its
+parts, the 1/P_fid GRF draws, A^-1, the g-maps and assemble_bshot, are each pinned above).
 """
 import numpy as np
 import scipy.fft
@@ -211,6 +212,18 @@ def assemble_b3(g, all_bins):
     out = np.zeros(len(all_bins))
     for i, (b1, b2, b3, l) in enumerate(all_bins):
         out[i] = np.sum(g[0][b1] * g[0][b2] * g[l // 2][b3])
+    return out
+
+
+def assemble_bshot(gA, gB, gC, all_bins):
+    """Shot-noise triple sums: the six assignments of the map sets (A, B, C) to the legs (b1, b2,
+    b3 with its multipole) (utils.pyx:641-663)."""
+    out = np.zeros(len(all_bins))
+    for i, (b1, b2, b3, l) in enumerate(all_bins):
+        L = l // 2
+        out[i] = np.sum(gA[0][b1] * gB[0][b2] * gC[L][b3] + gA[0][b1] * gC[0][b2] * gB[L][b3]
+                        + gB[0][b1] * gA[0][b2] * gC[L][b3] + gB[0][b1] * gC[0][b2] * gA[L][b3]
+                        + gC[0][b1] * gA[0][b2] * gB[L][b3] + gC[0][b1] * gB[0][b2] * gA[L][b3])
     return out
 
 
@@ -688,7 +701,7 @@ class BSpec:
     """FP64 restatement of PolyBin3D.BSpec (PolyBin3D/bspec.py:12-1119), hot-path members only."""
 
     def __init__(self, base, k_bins, lmax=0, mask=None, applySinv=None, k_bins_squeeze=None,
-                 include_partial_triangles=False, Pk_fid=None, applySinv_transpose=None):
+                 include_partial_triangles=False, Pk_fid=None, applySinv_transpose=None, mask_shot=None):
         b = self.base = base
         self.k_bins = np.asarray(k_bins, dtype=np.float64)
         self.lmax = lmax
@@ -737,6 +750,8 @@ class BSpec:
         # bspec.py:158-169
         self.Ylm_real = real_harmonics(np.asarray(b.r_grids), lmax, False) if (b.sightline == 'local' and lmax > 0) else None
         self.Ylm_fourier = real_harmonics(np.asarray(b.k_grids), lmax, False) if lmax > 0 else None
+        if mask_shot is not None:  # bspec.py:113-114
+            self.mask_shot = np.asarray(mask_shot, dtype=np.float64)
 
     def get_ks(self):
         """bspec.py:201-220."""
@@ -821,6 +836,50 @@ class BSpec:
 
     def Bk_numerator_ideal(self, data):
         return self._compute_bk_numerator(data, include_linear_term=False, filtering='ideal')
+
+    def compute_shot_contribution(self, seed, data=None, data2=None, cubic_only=False, return_scale=False):
+        """bspec.py:695-861: shot-noise numerator from a GRF pair drawn with 1/P_fid (seeds seed+1e7,
+        seed+2e7) and weighted by A^-1 = P_fid.  Returns quadratic - 2 cubic, where the quadratic
+        term also uses the data's g-maps and data2 (the doubly weighted data, default data); or the
+        cubic term alone.  return_scale (for tests): also the same sums over |g| maps and |terms|,
+        the magnitude that FP32 rounding acts on."""
+        b = self.base
+        if not cubic_only:
+            assert data is not None
+            data = np.asarray(data, dtype=np.float64)
+            data2 = data if data2 is None else np.asarray(data2, dtype=np.float64)
+        assert hasattr(self, 'mask_shot')
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        if not cubic_only:
+            if b.pixel_window != 'none':  # bspec.py:739-744
+                df = MapUtils.div_fourier(b.to_fourier(data), b.pixel_window_grid)
+                Sd = self.applySinv(df, input_type='fourier', output_type='fourier')
+            else:
+                Sd = self.applySinv(data, input_type='real', output_type='fourier')
+            g = self._g_maps(Sd)
+        PkA = [b.Pfid[0], 1. / b.Pfid[1]]  # bspec.py:777-779
+        a1 = b.generate_data(seed=seed + int(1e7), Pk_input=PkA, output_type='fourier')
+        a2 = b.generate_data(seed=seed + int(2e7), Pk_input=PkA, output_type='fourier')
+        A1 = b.applyAinv(a1, invPk0_grid=b.Pk0_grid, input_type='fourier', output_type='real')
+        A2 = b.applyAinv(a2, invPk0_grid=b.Pk0_grid, input_type='fourier', output_type='real')
+        gC = self._g_maps(self.applySinv(A1 * A2 * self.mask_shot, input_type='real', output_type='fourier'))
+        if not cubic_only:
+            gD = self._g_maps(self.applySinv(A1 * data2, input_type='real', output_type='fourier'))
+            gE = self._g_maps(self.applySinv(A2 * data2, input_type='real', output_type='fourier'))
+        gA = self._g_maps(self.applySinv(a1, input_type='fourier', output_type='fourier'))
+        gB = self._g_maps(self.applySinv(a2, input_type='fourier', output_type='fourier'))
+        norm = b.volume / b.gridsize.prod() / self.sym_factor
+        ab = self.all_bins
+        cubic = assemble_bshot(gA, gB, gC, ab) / 6. * norm  # bspec.py:846-847
+        scale = np.abs(assemble_bshot(np.abs(gA), np.abs(gB), np.abs(gC), ab) / 6. * norm)
+        if cubic_only:
+            return (cubic, scale) if return_scale else cubic
+        quad = 0.5 * assemble_bshot(g, gA, gD, ab) + 0.5 * assemble_bshot(g, gB, gE, ab)
+        quad *= 0.5 * norm  # bspec.py:851-854
+        scale = 2. * scale + np.abs(0.25 * norm * (assemble_bshot(np.abs(g), np.abs(gA), np.abs(gD), ab)
+                                                   + assemble_bshot(np.abs(g), np.abs(gB), np.abs(gE), ab)))
+        return (quad - 2. * cubic, scale) if return_scale else quad - 2. * cubic
 
     def Bk_numerator(self, data, include_linear_term=True):
         return self._compute_bk_numerator(data, include_linear_term=include_linear_term, filtering='Sinv')

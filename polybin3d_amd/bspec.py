@@ -592,6 +592,69 @@ class BSpec:
                 b.pointing(subs, nb, QA[t0:t1], mask=self.mask, scale=1.0, kop_out=C.kop(C.KOP_PACK, **pack))
             t0 = t1
 
+    # ------------------------------------------------------------------ shot noise
+    def _bshot(self, gA, gB, gC):
+        """assemble_bshot (utils.pyx:641-663): sum over the six assignments of the map sets A, B, C
+        to the legs (b1, b2, b3 with its multipole), as triplet sums over one stack [A, B, C]."""
+        NG = gA.shape[0]
+        st = torch.cat([gA, gB, gC], 0)
+        i1, i2, i3 = self._b3_indices()
+        n = len(i1)
+        sets = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+        j1 = np.concatenate([x * NG + i1 for x, _, _ in sets])
+        j2 = np.concatenate([y * NG + i2 for _, y, _ in sets])
+        j3 = np.concatenate([z * NG + i3 for _, _, z in sets])
+        r = self._triplet(st, j1, j2, j3)
+        del st
+        return r.reshape(6, n).sum(0)
+
+    def compute_shot_contribution(self, seed, verb=False, data=None, data2=None, cubic_only=False):
+        """Shot-noise numerator from one pair of GRFs drawn with 1/P_fid (bspec.py:695-861): the cubic
+        term assemble_bshot(g_A, g_B, g_C) / 6 and, unless cubic_only, the quadratic term from the
+        singly and doubly weighted data (data, data2 = data by default); returns quadratic - 2 cubic
+        (or the cubic term), normalised by V/N/sym."""
+        b = self.base
+        if not cubic_only:
+            assert data is not None, "Must supply data to compute the quadratic shot-noise contribution!"
+            if data2 is None:
+                print("## Caution: assuming uniform weights!")
+                data2 = data
+        else:
+            print("## Caution: Dropping quadratic (~ P(k)/n) term!")
+        assert hasattr(self, 'mask_shot'), "Must supply triply-weighted mask to compute the shot-noise contribution!"
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        if not cubic_only:
+            g = self._g_maps(self._filtered_data(data, 'Sinv'))
+        if verb:
+            print("Computing shot-noise g_{b,l}(r) maps")
+        PkA = [b.Pfid[0], 1. / b.Pfid[1]]
+        a1 = b.generate_data(seed=seed + int(1e7), Pk_input=PkA, output_type='fourier')
+        a2 = b.generate_data(seed=seed + int(2e7), Pk_input=PkA, output_type='fourier')
+        A1 = b.applyAinv(a1, invPk0_grid=b.Pk0_grid_host, input_type='fourier', output_type='real')
+        A2 = b.applyAinv(a2, invPk0_grid=b.Pk0_grid_host, input_type='fourier', output_type='real')
+        Sinv = lambda m, it: b.as_fourier(self.applySinv(m, input_type=it, output_type='fourier'))  # noqa: E731
+        gC = self._g_maps(Sinv(A1 * A2 * self.mask_shot, 'real'))
+        if not cubic_only:
+            d2 = b.as_real(data2)
+            gD = self._g_maps(Sinv(A1 * d2, 'real'))
+            gE = self._g_maps(Sinv(A2 * d2, 'real'))
+        del A1, A2
+        gA = self._g_maps(Sinv(a1, 'fourier'))
+        gB = self._g_maps(Sinv(a2, 'fourier'))
+        del a1, a2
+        norm = b.volume / b.gridsize.prod() / self.sym_factor
+        if verb:
+            print("Assembling cubic shot-noise term")
+        cubic = self._bshot(gA, gB, gC) / 6. * norm
+        if cubic_only:
+            return cubic
+        if verb:
+            print("Assembling quadratic shot-noise term")
+        quad = 0.5 * self._bshot(g, gA, gD) + 0.5 * self._bshot(g, gB, gE)
+        quad *= 0.5 * norm
+        return quad - 2. * cubic
+
     def compute_fisher_contribution(self, seed, verb=False):
         """One MC realization (bspec.py:441-445, 473-676): FP64 numpy [N_bins, N_bins], rows indexed
         by the S^-1-weighted triplet, not symmetrised (as the reference)."""

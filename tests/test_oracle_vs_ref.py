@@ -93,3 +93,5 @@ def test_triplet_assembly_matches_reference(setup):
     bins = O.triangle_bins([0.01, 0.02, 0.03, 0.04], [0.01, 0.02, 0.03, 0.04], 2)
     np.testing.assert_allclose(O.assemble_b3(g, bins), U.assemble_b3(g, bins, 2), rtol=1e-12)
     np.testing.assert_allclose(O.assemble_b1(g, s, bins), U.assemble_b1(g, s, bins, 2), rtol=1e-12)
+    c = rng.randn(2, 3, *b.gridsize)
+    np.testing.assert_allclose(O.assemble_bshot(g, s, c, bins), U.assemble_bshot(g, s, c, bins, 2), rtol=1e-12)
