@@ -141,6 +141,21 @@ typedef struct {
   int32_t pad_;
 } pb3d_submap;
 
+/* Per-bin weighted sums taken inside pb3d_fft_pointing, for the integral-constrained pointing
+ * matrices (PolyBin3D/cython/utils.pyx:405-513; pspec.py:199-213; bspec.py:412-426): for output map b
+ * and bin z,
+ *   sums[b*nbins + z] += sum over voxels x with bin[x] == z of m_b(x) * weight[x],
+ * where m_b = scale * sum_g coef_g Y_g(x_hat) C2R(kop_g(in_g))[b] is the combined real map before the
+ * mask.  bin == NULL puts every voxel in bin 0 (global constraint, nbins = 1); entries >= nbins are
+ * in no bin.  1 <= nbins <= 64.  The caller zeroes `sums`; FP64 accumulation. */
+typedef struct {
+  const float* weight;
+  const uint8_t* bin;
+  int32_t nbins;
+  int32_t pad_;
+  double* sums;
+} pb3d_icsum;
+
 /* ---------------------------------------------------------------- library */
 int pb3d_version(void);
 const char* pb3d_last_error(void);
@@ -187,12 +202,13 @@ int pb3d_fft_c2r(const pb3d_plan* plan, const void* in, float* out, void* work, 
  *   out[b] = kop_out( R2C( mask(x) * scale * sum_g coef_g Y_g(x_hat) * C2R( kop_g(in_g) )[b] ) )
  * The sub-map column passes run pruned to each input's support; one row kernel then inverts all
  * nsub rows, combines them with their real-space weights, applies the mask (NULL = none) and the
- * forward row transform, so the real-space maps never reach HBM.  work: the float2 count given by
+ * forward row transform, so the real-space maps never reach HBM.  ic (NULL = none): per-bin sums of
+ * the combined real map, taken in the same kernel (see pb3d_icsum).  work: the float2 count given by
  * pb3d_fft_pointing_work_size (at most (nsub+1)*batch*n0*n1*(n2/2+1)): the intermediate maps keep
  * only the z planes inside the band limits.  nsub <= 16. */
 int pb3d_fft_pointing(const pb3d_plan* plan, int batch, int nsub, const pb3d_submap* subs,
-                      const float* mask, float scale, const pb3d_kop* kop_out, void* out, void* work,
-                      void* stream);
+                      const float* mask, float scale, const pb3d_kop* kop_out, const pb3d_icsum* ic,
+                      void* out, void* work, void* stream);
 /* float2 entries of work that pb3d_fft_pointing needs for these sub-maps and output op: nsub*batch
  * maps of n0*n1*pzi and batch maps of n0*n1*pzo, pzi = the largest z band limit + 1 over the
  * sub-map kops, pzo = that of kop_out (n2/2+1 where one has none).  Host-only; -1 on a bad argument.  (No counterpart in the reference, whose

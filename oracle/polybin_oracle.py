@@ -16,7 +16,8 @@ How each part is pinned (tests/test_oracle_*.py):
     (Tutorial 1 cell 8) plus mode counting by brute force;
   * FFT wrappers: brute-force DFT on small grids, Parseval, plane wave <-> delta;
   * utils kernels (integrate, cross_integrate, real_integrate_double, integrate_row, filters,
-    sum_pair/sum_triplet, assemble_b3/b1/bshot): explicit per-bin loops on tiny grids, and the compiled
+    sum_pair/sum_triplet, assemble_b3/b1/bshot, the integral-constrained pointing_GIC/RIC and their
+    transposes): explicit per-bin loops on tiny grids, transposes as adjoints, and the compiled
     reference kernels in oracle/_ref when that build is present;
   * real spherical harmonics: the addition theorem sum_m Y_lm(a) Y_lm(b) = L_l(a.b);
   * Pk_ideal: plane-wave known answer P = A^2 V / (2 N_modes);
@@ -213,6 +214,75 @@ def assemble_b3(g, all_bins):
     for i, (b1, b2, b3, l) in enumerate(all_bins):
         out[i] = np.sum(g[0][b1] * g[0][b2] * g[l // 2][b3])
     return out
+
+
+def pointing_GIC(m, mask, mask_IC):
+    """mask (m - <m mask_IC> / <mask_IC>): pointing with the global integral constraint
+    (utils.pyx:405-423)."""
+    return mask * (m - np.sum(m * mask_IC) / np.sum(mask_IC))
+
+
+def pointing_GIC_transpose(m, mask, mask_IC):
+    """mask m - mask_IC <m mask> / <mask_IC> (utils.pyx:428-446)."""
+    return mask * m - mask_IC * (np.sum(m * mask) / np.sum(mask_IC))
+
+
+def _radial_bin(modr, edges):
+    """Half-open radial bin index per voxel, -1 outside every bin (utils.pyx:462-463)."""
+    z = np.full(modr.shape, -1, dtype=np.int64)
+    for iz in range(len(edges) - 1):
+        z[(modr >= edges[iz]) & (modr < edges[iz + 1])] = iz
+    return z
+
+
+def pointing_RIC(m, mask, mask_IC, edges, modr):
+    """mask (m - av_z) in radial bin z, av_z = sum_z m mask_IC / sum_z mask_IC; 0 outside every bin
+    (utils.pyx:451-481)."""
+    z = _radial_bin(modr, edges)
+    out = np.zeros(m.shape)
+    for iz in range(len(edges) - 1):
+        s = z == iz
+        out[s] = mask[s] * (m[s] - np.sum(m[s] * mask_IC[s]) / np.sum(mask_IC[s]))
+    return out
+
+
+def pointing_RIC_transpose(m, mask, mask_IC, edges, modr):
+    """mask m - mask_IC av_z in radial bin z, av_z = sum_z m mask / sum_z mask_IC; 0 outside every bin
+    (utils.pyx:486-513)."""
+    z = _radial_bin(modr, edges)
+    out = np.zeros(m.shape)
+    for iz in range(len(edges) - 1):
+        s = z == iz
+        out[s] = mask[s] * m[s] - mask_IC[s] * (np.sum(m[s] * mask[s]) / np.sum(mask_IC[s]))
+    return out
+
+
+def apply_pointing_ic(spec, m, transpose):
+    """PSpec/BSpec.apply_pointing (pspec.py:199-213, bspec.py:412-426): mask multiply, or the
+    integral-constrained pointing matrix."""
+    if spec.add_RIC:
+        f = pointing_RIC_transpose if transpose else pointing_RIC
+        return f(m, spec.mask, spec.mask_IC, spec.radial_bins_RIC, spec.base.modr_grid)
+    if spec.add_GIC:
+        f = pointing_GIC_transpose if transpose else pointing_GIC
+        return f(m, spec.mask, spec.mask_IC)
+    return m * spec.mask
+
+
+def _setup_ic(spec, mask, add_GIC, mask_IC, add_RIC, radial_bins_RIC):
+    """Integral-constraint options (pspec.py:60-72, 101-109; bspec.py:86-112)."""
+    spec.add_GIC, spec.add_RIC = add_GIC, add_RIC
+    if mask is None:
+        assert not add_GIC and not add_RIC
+    if mask_IC is not None:
+        spec.mask_IC = np.asarray(mask_IC, dtype=np.float64)
+    if add_GIC:
+        assert hasattr(spec, 'mask_IC')
+    if add_RIC:
+        assert not add_GIC and hasattr(spec, 'mask_IC') and len(radial_bins_RIC) > 0
+        b = spec.base
+        b.modr_grid = np.sqrt(b.r_grids[0] ** 2. + b.r_grids[1] ** 2. + b.r_grids[2] ** 2.)
+        spec.radial_bins_RIC = np.asarray(radial_bins_RIC, dtype=np.float64)
 
 
 def assemble_bshot(gA, gB, gC, all_bins):
@@ -419,7 +489,8 @@ class PSpec:
     (no covariance/theory matrices, no integral constraints)."""
 
     def __init__(self, base, k_bins, lmax=2, mask=None, applySinv=None, odd_l=False,
-                 applySinv_transpose=None, mask_shot=None):
+                 applySinv_transpose=None, mask_shot=None, add_GIC=False, mask_IC=None, add_RIC=False,
+                 radial_bins_RIC=()):
         # pspec.py:27-120
         self.base = base
         self.k_bins = np.asarray(k_bins, dtype=np.float64)
@@ -447,6 +518,7 @@ class PSpec:
             self.sq_mask_mean = np.mean(self.mask ** 2)
         if mask_shot is not None:
             self.mask_shot = np.asarray(mask_shot, dtype=np.float64)
+        _setup_ic(self, mask, add_GIC, mask_IC, add_RIC, radial_bins_RIC)
         if applySinv is None:
             self.applySinv = base.applySinv_trivial
             self.applySinv_transpose = base.applySinv_trivial if applySinv_transpose is None else applySinv_transpose
@@ -464,8 +536,8 @@ class PSpec:
         return 0.5 * (self.k_bins[1:] + self.k_bins[:-1])
 
     def apply_pointing(self, m, transpose=False):
-        """mask multiply (pspec.py:199-213, no integral constraints)."""
-        return m * self.mask
+        """pspec.py:199-213."""
+        return apply_pointing_ic(self, m, transpose)
 
     # --- numerator ------------------------------------------------------------------------
     def _compute_pk_numerator(self, data, filtering='ideal'):
@@ -701,7 +773,8 @@ class BSpec:
     """FP64 restatement of PolyBin3D.BSpec (PolyBin3D/bspec.py:12-1119), hot-path members only."""
 
     def __init__(self, base, k_bins, lmax=0, mask=None, applySinv=None, k_bins_squeeze=None,
-                 include_partial_triangles=False, Pk_fid=None, applySinv_transpose=None, mask_shot=None):
+                 include_partial_triangles=False, Pk_fid=None, applySinv_transpose=None, mask_shot=None,
+                 add_GIC=False, mask_IC=None, add_RIC=False, radial_bins_RIC=()):
         b = self.base = base
         self.k_bins = np.asarray(k_bins, dtype=np.float64)
         self.lmax = lmax
@@ -752,6 +825,7 @@ class BSpec:
         self.Ylm_fourier = real_harmonics(np.asarray(b.k_grids), lmax, False) if lmax > 0 else None
         if mask_shot is not None:  # bspec.py:113-114
             self.mask_shot = np.asarray(mask_shot, dtype=np.float64)
+        _setup_ic(self, mask, add_GIC, mask_IC, add_RIC, radial_bins_RIC)
 
     def get_ks(self):
         """bspec.py:201-220."""
@@ -938,7 +1012,8 @@ class BSpec:
 
     # --- linear term simulations ------------------------------------------------------------
     def apply_pointing(self, m, transpose=False):
-        return m * self.mask
+        """bspec.py:412-426."""
+        return apply_pointing_ic(self, m, transpose)
 
     def _process_sim(self, sim, input_type='real'):
         """bspec.py:281-326."""

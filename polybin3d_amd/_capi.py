@@ -30,7 +30,7 @@ EXPORTS = {
     "pb3d_fft_r2c": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
     "pb3d_fft_c2r": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
     "pb3d_fft_pointing": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_float, c_void_p,
-                                         c_void_p, c_void_p, c_void_p]),
+                                         c_void_p, c_void_p, c_void_p, c_void_p]),
     "pb3d_fft_pointing_work_size": (c_int64, [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p]),
     "pb3d_kmap": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p]),
     "pb3d_kmap_packed": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, ctypes.c_int, c_void_p,
@@ -83,6 +83,11 @@ class SubMap(ctypes.Structure):
     """pb3d_submap: one weighted sub-map of pb3d_fft_pointing."""
     _fields_ = [("kop", KOp), ("inp", c_void_p), ("ell", c_int32), ("m", c_int32), ("coef", c_float),
                 ("pad_", c_int32)]
+
+
+class IcSum(ctypes.Structure):
+    """pb3d_icsum: per-bin weighted sums of the pointing chain's combined real map."""
+    _fields_ = [("weight", c_void_p), ("bin", c_void_p), ("nbins", c_int32), ("pad_", c_int32), ("sums", c_void_p)]
 
 
 ROP_PLAIN, ROP_MUL, ROP_DIFF, ROP_YLM, ROP_MULYLM, ROP_SQUARE = range(6)

@@ -375,9 +375,11 @@ class PolyBin3D:
         (pruned) transforms: |fs(i)| * kF_i <= |k| < kmax."""
         return [int(min(np.floor(kmax / self.kF[i]), self.gridsize[i] // 2)) for i in range(3)]
 
-    def pointing(self, subs, batch, out, mask=None, scale=1.0, kop_out=None):
+    def pointing(self, subs, batch, out, mask=None, scale=1.0, kop_out=None, ic=None):
         """out[b] = kop_out(R2C(mask * scale * sum_g coef_g Y_g(x) C2R(kop_g(in_g))[b])), fused
-        (pb3d_fft_pointing).  subs: list of (kop, input or None for BKQ, ell, m, coef)."""
+        (pb3d_fft_pointing).  subs: list of (kop, input or None for BKQ, ell, m, coef).  ic: optional
+        (weight, bin or None, nbins, sums): sums[b, z] += sum over bin z of m_b * weight, m_b the
+        combined real map before the mask (integral constraints)."""
         G = len(subs)
         arr = (C.SubMap * G)()
         keep = []
@@ -400,8 +402,20 @@ class PolyBin3D:
         if nw < 0:
             C.check(-1)
         work = self.work('fft_point', nw * 8)
+        icd = None
+        if ic is not None:
+            weight, rbin, nbins, sums = ic
+            if weight.dtype != torch.float32 or sums.dtype != torch.float64:
+                raise ValueError("pointing ic: float32 weight and float64 sums expected")
+            self._need(weight, self.N, 'pointing ic weight')
+            if rbin is not None and (rbin.dtype != torch.uint8 or rbin.numel() < self.N):
+                raise ValueError("pointing ic bin: uint8 device tensor of the grid's size expected")
+            self._need(sums, batch * nbins, 'pointing ic sums')
+            icd = C.IcSum(weight.data_ptr(), C._ptr(rbin), int(nbins), 0, sums.data_ptr())
+            keep.append((weight, rbin, sums))
         C.check(C.lib().pb3d_fft_pointing(self._plan, batch, G, arr, C._ptr(mask), float(scale),
-                                          C.byref_or_none(kop_out), out.data_ptr(), work.data_ptr(), self.stream()))
+                                          C.byref_or_none(kop_out), C.byref_or_none(icd), out.data_ptr(),
+                                          work.data_ptr(), self.stream()))
         self.n_FFTs_reverse += batch * G
         self.n_FFTs_forward += batch
         return out

@@ -14,6 +14,7 @@ import torch
 
 from . import _capi as C
 from .base import harmonics_tags
+from .constraints import Pointing
 
 
 def triangle_bins(k_bins, k_bins_squeeze, lmax, include_partial_triangles):
@@ -78,12 +79,12 @@ class BSpec:
             print("N_bins: %d" % self.N_bins)
         assert type(self.lmax) == int, "l-max must be an integer!"
         assert self.lmax <= 4, "Only l-max<=4 is currently supported!"
-        if add_GIC or add_RIC:
-            raise NotImplementedError("integral constraints are not implemented by backend='hip'")
         if precision not in self._PREC:
             raise ValueError("precision must be one of %s" % sorted(self._PREC))
         if mask is None:
             self.const_mask, self.mask_mean, self.cube_mask_mean = True, 1., 1.
+            assert not add_GIC, "Global integral constraint cannot be used without a mask!"
+            assert not add_RIC, "Radial integral constraint cannot be used without a mask!"
         else:
             m = mask.detach().cpu().numpy() if torch.is_tensor(mask) else np.asarray(mask)
             m = np.broadcast_to(np.asarray(m, dtype=np.float64), tuple(int(v) for v in b.gridsize))
@@ -93,6 +94,17 @@ class BSpec:
                 self.const_mask = False
             self.cube_mask_mean = float(np.mean(m ** 3))
             self.mask = b.as_real(np.ascontiguousarray(m))
+        # integral constraints (bspec.py:86-112): the pointing matrix of the varying mask
+        if add_GIC:
+            assert mask_IC is not None, "Need to supply mask_IC!"
+        if add_RIC:
+            assert not add_GIC, "Radial integral constraint imposes global integral constraint automatically!"
+            assert mask_IC is not None, "Need to supply mask_IC!"
+            assert len(radial_bins_RIC) > 0, "Radial bins need to be supplied for radial integral constraint!"
+        if mask is not None and not self.const_mask:
+            mic = None if mask_IC is None else (mask_IC.detach().cpu().numpy() if torch.is_tensor(mask_IC)
+                                               else np.asarray(mask_IC, dtype=np.float64))
+            self._P = Pointing(b, np.ascontiguousarray(m), mic, add_GIC, add_RIC, radial_bins_RIC)
         if mask_shot is not None:
             ms = mask_shot.detach().cpu().numpy() if torch.is_tensor(mask_shot) else np.asarray(mask_shot)
             self.mask_shot = b.as_real(ms.astype(np.float64))
@@ -439,8 +451,8 @@ class BSpec:
 
     # ------------------------------------------------------------------ linear-term simulations
     def apply_pointing(self, map_real, transpose=False):
-        """Multiply by the mask (bspec.py:412-426, no integral constraints)."""
-        return self.base.as_real(map_real) * self.mask
+        """The pointing matrix of the varying mask, with its integral constraint (bspec.py:412-426)."""
+        return self._P.apply(map_real, transpose)
 
     def _process_sim(self, sim, input_type='real'):
         """g_{b,l} maps of one simulation (bspec.py:281-326)."""
@@ -477,8 +489,7 @@ class BSpec:
                 return self._process_sim(sf, input_type='fourier')
             sr = b.generate_data(int(1e5) + ii, Pk_input=Pk_input, output_type='real', include_pixel_window=False)
             if b.pixel_window != 'none':
-                sf = b.r2c(None, rop=C.rop(C.ROP_MUL, a=sr, b=self.mask),
-                           kop=C.kop(C.KOP_WEIGHT, w=b.pixel_window_grid))
+                sf = b.r2c(self.apply_pointing(sr), kop=C.kop(C.KOP_WEIGHT, w=b.pixel_window_grid))
                 return self._process_sim(sf, input_type='fourier')
             return self._process_sim(self.apply_pointing(sr), input_type='real')
 
@@ -589,7 +600,13 @@ class BSpec:
                 if ylm:
                     subs += [(C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common), None, ell, m, 1.0)
                              for m in range(2 * ell + 1)]
-                b.pointing(subs, nb, QA[t0:t1], mask=self.mask, scale=1.0, kop_out=C.kop(C.KOP_PACK, **pack))
+                # S^-1 P on every Q map (bspec.py:662-666): P of the varying mask, with its integral
+                # constraint (the constant-per-bin term subtracted in packed mode space)
+                packed = lambda wz: b.r2c(wz, batch=wz.shape[0], kop=C.kop(C.KOP_PACK, **pack),  # noqa: E731
+                                          out=torch.empty((wz.shape[0], M), dtype=torch.complex64, device=b.device))
+                mask_eff, ic, finish = self._P.chain(False, nb, packed, 'packed')
+                b.pointing(subs, nb, QA[t0:t1], mask=mask_eff, scale=1.0, kop_out=C.kop(C.KOP_PACK, **pack), ic=ic)
+                finish(QA[t0:t1])
             t0 = t1
 
     # ------------------------------------------------------------------ shot noise
@@ -702,8 +719,8 @@ class BSpec:
             v1 = b.kmap(a1, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
             v2 = b.kmap(a2, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
         else:
-            v1 = b.r2c(None, rop=C.rop(C.ROP_MUL, a=b.c2r(a1), b=self.mask))
-            v2 = b.r2c(None, rop=C.rop(C.ROP_MUL, a=b.c2r(a2), b=self.mask))
+            v1 = b.r2c(self.apply_pointing(b.c2r(a1)))
+            v2 = b.r2c(self.apply_pointing(b.c2r(a2)))
         H1, H2 = self._g_maps(v1), self._g_maps(v2)
         LT = torch.empty((self.Nl, len(self._sinv_pairs), M), dtype=torch.complex64, device=b.device)
         self._ft_tables(H1, H2, self._sinv_pairs, LT)

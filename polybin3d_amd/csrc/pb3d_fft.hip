@@ -655,6 +655,15 @@ struct SubW {
   float poly[kMaxSub][kMaxMono];
 };
 
+// Per-bin weighted sums of the combined real map (pb3d_icsum), by value for the kernel.
+constexpr int kMaxIcBins = 64;
+struct IcSum {
+  const float* weight;  // nullptr: no sums
+  const uint8_t* bin;   // nullptr: every voxel in bin 0
+  int nbins;
+  double* sums;
+};
+
 template <int M> struct PointCfg {
   static constexpr int P = Decomp<M>::P;      // lanes per row
   static constexpr int RW = 64 / P;           // rows per wave
@@ -667,16 +676,17 @@ template <int M> struct PointCfg {
   static_assert(64 % P == 0, "lanes per row must divide the wave");
 };
 
-template <int M>
+template <int M, bool IC>
 __global__ void __launch_bounds__(PointCfg<M>::NT, PB3D_POINT_WAVES)
 pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, long long slot_out, int pitch_out,
-              long long rows_per_map, int batch, SubW sw,
+              long long rows_per_map, int batch, SubW sw, IcSum ic,
               const float* __restrict__ mask, float scale, int lz_in, int lz_out, const float2* __restrict__ tw,
               const float2* __restrict__ twN, Geo geo) {
   using PC = PointCfg<M>;
   constexpr int RW = PC::RW, LDT = PC::LDT, E = PC::E, P = PC::P;
   __shared__ float2 tiles[PC::WAVES * PC::SIZE];
   __shared__ float2 stw[M], stwN[M + 1];
+  __shared__ float icl[IC ? PC::WAVES : 1][IC ? kMaxIcBins : 1];
   const int tid = threadIdx.x;
   stage_twiddles<M>(stw, stwN, tw, twN, tid, PC::NT);
   __syncthreads();  // twiddles staged: the only block barrier
@@ -811,6 +821,25 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   wave_lds_sync();  // all reads of the tile by the last inverse stage are issued
   // mask and scale in registers, then the forward row transform (first stage from registers)
   const float norm = scale / (float)geo.N;
+  if constexpr (IC) {
+    // per-bin sums of m * weight over this wave's voxels: FP32 in LDS, then one FP64 atomic per bin
+    if (lane < ic.nbins) icl[wave][lane] = 0.f;
+    wave_lds_sync();
+    if (row < rows_per_map) {
+      // one voxel pair at a time (this path runs only for integral constraints; keeping the weights
+      // of all slots live next to the accumulators would spill)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const long long v0 = row * (2 * M) + 2 * (tp + e * P);
+        const float2 wv = *reinterpret_cast<const float2*>(ic.weight + v0);
+        const int z0 = ic.bin ? ic.bin[v0] : 0, z1 = ic.bin ? ic.bin[v0 + 1] : 0;
+        if (z0 < ic.nbins) atomicAdd(&icl[wave][z0], acc[e][0] * norm * wv.x);
+        if (z1 < ic.nbins) atomicAdd(&icl[wave][z1], acc[e][1] * norm * wv.y);
+      }
+    }
+    wave_lds_sync();
+    if (lane < ic.nbins) atomicAdd(&ic.sums[t * ic.nbins + lane], (double)icl[wave][lane]);
+  }
   if (mask && row < rows_per_map) {
     float2 mk[E];  // all mask loads in flight together
 #pragma unroll
@@ -955,15 +984,21 @@ static hipError_t launch_c2r_rows(int M, const float2* in, float* out, long long
 
 static hipError_t launch_pointing_rows(int M, const float2* work, long long slot_in, int pitch_in, float2* out,
                                        long long slot_out, int pitch_out, long long rpm, int batch, const SubW& sw,
+                                       const IcSum& ic,
                                        const float* mask, float scale, int lz_in, int lz_out, const float2* tw,
                                        const float2* twN, const Geo& geo, hipStream_t s) {
 #define PB3D_PROWS(MM)                                                                           \
   case MM: {                                                                                     \
     constexpr int RB = PointCfg<MM>::RB;                                                         \
     dim3 grid((unsigned)((rpm + RB - 1) / RB), batch);                                           \
-    pointing_rows<MM><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, pitch_out, rpm,  \
-                                                         batch, sw, mask, scale, lz_in,          \
-                                                         lz_out, tw, twN, geo);                  \
+    if (ic.weight)                                                                               \
+      pointing_rows<MM, true><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, \
+                                                         pitch_out, rpm, batch, sw, ic, mask, scale, \
+                                                         lz_in, lz_out, tw, twN, geo);           \
+    else                                                                                         \
+      pointing_rows<MM, false><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, \
+                                                         pitch_out, rpm, batch, sw, ic, mask, scale, \
+                                                         lz_in, lz_out, tw, twN, geo);           \
     return hipGetLastError();                                                                    \
   }
   switch (M) {
@@ -1300,12 +1335,19 @@ extern "C" int64_t pb3d_fft_pointing_work_size(const pb3d_grid* grid, int batch,
 }
 
 extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const pb3d_submap* subs, const float* mask,
-                                 float scale, const pb3d_kop* kop_out, void* out_, void* work_, void* stream) {
+                                 float scale, const pb3d_kop* kop_out, const pb3d_icsum* ic_, void* out_, void* work_,
+                                 void* stream) {
   if (!p || !subs || !out_ || !work_ || batch < 1 || batch > 65535)
     return fail(PB3D_ERR_ARG, "pointing: null pointer or batch out of [1, 65535]");
   if (nsub < 1 || nsub > kMaxSub) return fail(PB3D_ERR_ARG, "pointing: nsub out of range [1, 16]");
   if (kop_out && kop_out->kind == 4) return fail(PB3D_ERR_ARG, "pointing: BKQ is a prologue");
   if (int e = check_kop(kop_out)) return e;
+  IcSum ic{nullptr, nullptr, 1, nullptr};
+  if (ic_) {
+    if (!ic_->weight || !ic_->sums || ic_->nbins < 1 || ic_->nbins > kMaxIcBins)
+      return fail(PB3D_ERR_ARG, "pointing: ic needs weight, sums and 1 <= nbins <= 64");
+    ic = IcSum{ic_->weight, ic_->bin, ic_->nbins, ic_->sums};
+  }
   hipStream_t s = (hipStream_t)stream;
   const Geo& g = p->geo;
   float2* work = (float2*)work_;
@@ -1358,7 +1400,7 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   // 3. fused rows: inverse rows of every sub-map, weights, mask, forward row -> output map b
   const double rows1 = (double)g.n0 * g.n1;
   on = prof_begin(5, s, sm, (double)batch * (rows1 * 8.0 * (pin.lz * (double)nsub + pout.lz) + (mask ? g.N * 4.0 : 0.0)));
-  PB3D_CHECK_HIP(launch_pointing_rows(p->L[2], work, wsi, pzi, outw, wso, pzo, (long long)g.n0 * g.n1, batch, sw,
+  PB3D_CHECK_HIP(launch_pointing_rows(p->L[2], work, wsi, pzi, outw, wso, pzo, (long long)g.n0 * g.n1, batch, sw, ic,
                                       mask, scale, pin.lz, pout.lz, p->tw[2], p->twN, g, s));
   prof_end(on, sm, s);
   // 4. pruned forward y and x column passes, epilogue into out

@@ -16,6 +16,7 @@ import torch
 
 from . import _capi as C
 from .base import harmonics_tags
+from .constraints import Pointing
 
 
 class PSpec:
@@ -44,10 +45,10 @@ class PSpec:
             raise NotImplementedError("odd_l (wide-angle) multipoles are not implemented by backend='hip'")
         assert type(self.lmax) == int, "l-max must be an integer!"
         assert self.lmax <= 4, "Only l-max<=4 is currently supported!"
-        if add_GIC or add_RIC:
-            raise NotImplementedError("integral constraints are not implemented by backend='hip'")
         if mask is None:
             self.const_mask, self.mask_mean, self.sq_mask_mean = True, 1., 1.
+            assert not add_GIC, "Global integral constraint cannot be used without a mask!"
+            assert not add_RIC, "Radial integral constraint cannot be used without a mask!"
         else:
             m = mask.detach().cpu().numpy() if torch.is_tensor(mask) else np.asarray(mask)
             m = np.ascontiguousarray(np.broadcast_to(m.astype(np.float64), tuple(int(v) for v in base.gridsize)))
@@ -60,6 +61,17 @@ class PSpec:
         if mask_shot is not None:
             ms = mask_shot.detach().cpu().numpy() if torch.is_tensor(mask_shot) else np.asarray(mask_shot)
             self.mask_shot = base.as_real(ms.astype(np.float64))
+        # integral constraints (pspec.py:101-109): the pointing matrix of the varying mask
+        if add_GIC:
+            assert mask_IC is not None, "Need to supply mask_IC!"
+        if add_RIC:
+            assert not add_GIC, "Radial integral constraint imposes global integral constraint automatically!"
+            assert mask_IC is not None, "Need to supply mask_IC!"
+            assert len(radial_bins_RIC) > 0, "Radial bins need to be supplied for radial integral constraint!"
+        if mask is not None and not self.const_mask:
+            mic = None if mask_IC is None else (mask_IC.detach().cpu().numpy() if torch.is_tensor(mask_IC)
+                                               else np.asarray(mask_IC, dtype=np.float64))
+            self._P = Pointing(base, m, mic, add_GIC, add_RIC, radial_bins_RIC)
         # pspec.py:92-99
         self.user_Sinv = applySinv is not None
         if applySinv is None:
@@ -259,8 +271,18 @@ class PSpec:
         else:
             subs = [(C.kop(C.KOP_FILTER, ell=0, **common), leg[li], 0, 0, 0.5)]
             subs += [(C.kop(C.KOP_YLM, ell=ell, m=m, **common), SinvP, ell, m, 0.5) for m in range(2 * ell + 1)]
-        mask, scale = (None, self.mask_mean) if self.const_mask else (self.mask, 1.0)
-        b.pointing(subs, Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup))
+        mask, scale, ic, finish = self._chain_pointing(Nk)
+        b.pointing(subs, Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup), ic=ic)
+        finish(Q)
+
+    def _chain_pointing(self, batch):
+        """Masking of the fused Q-map chain: P^T of the varying mask (with its integral constraint;
+        pspec.py:291-304), or the constant mask's mean.  Returns (mask, scale, ic, finish)."""
+        if self.const_mask:
+            return None, self.mask_mean, None, (lambda out: out)
+        full = lambda wz: self.base.r2c(wz, batch=wz.shape[0])  # noqa: E731
+        mask, ic, finish = self._P.chain(True, batch, full, 'full')
+        return mask, 1.0, ic, finish
 
     def compute_fisher_contribution(self, seed, verb=False):
         """One MC realization of the Fisher matrix (pspec.py:141-147, 215-493; compute_cov=False).
@@ -283,8 +305,7 @@ class PSpec:
         if self.const_mask:
             SinvP = b.kmap(a, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
         else:
-            a_real = b.c2r(a)
-            SinvP = b.r2c(None, rop=C.rop(C.ROP_MUL, a=a_real, b=self.mask))
+            SinvP = b.r2c(self._P.apply(b.c2r(a)))
         local = b.sightline == 'local' and self.lmax > 0
         lefts = torch.empty((Nl,) + b.half_shape, dtype=torch.complex64, device=b.device)
         lefts[0].copy_(Ainv)
@@ -346,9 +367,7 @@ class PSpec:
         if self.const_mask:
             SinvP = b.kmap(a, C.kop(C.KOP_PLAIN, scale=self.mask_mean, x_batch_stride=H), batch=R)
         else:
-            a_real = b.c2r(a, batch=R)
-            SinvP = b.r2c(None, rop=C.rop(C.ROP_MUL, a=a_real, b=self.mask, sa=N, sb=0), batch=R)
-            del a_real
+            SinvP = b.r2c(self._P.apply(b.c2r(a, batch=R)), batch=R)
         local = b.sightline == 'local' and self.lmax > 0
         lefts = torch.empty((R, Nl) + b.half_shape, dtype=torch.complex64, device=b.device)
         lefts[:, 0].copy_(Ainv)
@@ -373,7 +392,6 @@ class PSpec:
         scaling = 0.5 * b.volume / b.gridsize.prod() ** 2
         sup = b.support(self.k_bins[-1])
         common = dict(bin_table=self.table, bins=bins, x_batch_stride=H, x_batch_group=Nk, support=sup)
-        mask, scale = (None, self.mask_mean) if self.const_mask else (self.mask, 1.0)
         for li in range(Nl):
             ell = 2 * li
             # the Q maps of _q_maps for every seed: batch entry r*Nk + bin reads seed r's maps
@@ -382,7 +400,9 @@ class PSpec:
             else:
                 subs = [(C.kop(C.KOP_FILTER, ell=0, **common), leg[li], 0, 0, 0.5)]
                 subs += [(C.kop(C.KOP_YLM, ell=ell, m=m, **common), SinvP, ell, m, 0.5) for m in range(2 * ell + 1)]
-            b.pointing(subs, R * Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup))
+            mask, scale, ic, finish = self._chain_pointing(R * Nk)
+            b.pointing(subs, R * Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup), ic=ic)
+            finish(Q)
             for r in range(R):
                 red = b.bin_reduce(Q[r * Nk:(r + 1) * Nk], self.k_bins, ells, y=lefts[r], y_qstride=0,
                                    y_ostride=H, nq=Nk)
@@ -401,14 +421,14 @@ class PSpec:
         def apply_filter(m):
             if self.const_mask:
                 return self.mask_mean * b.as_fourier(self.applySinv(m, input_type='fourier', output_type='fourier'))
-            return b.as_fourier(self.applySinv(b.as_real(m) * self.mask, input_type='real', output_type='fourier'))
+            return b.as_fourier(self.applySinv(self._P.apply(m), input_type='real', output_type='fourier'))
 
         def apply_filter_dagger(m, real=False):
             it = 'real' if real else 'fourier'
             if self.const_mask:
                 return self.mask_mean * b.as_fourier(self.applySinv_transpose(m, input_type=it, output_type='fourier'))
             r = b.as_real(self.applySinv_transpose(m, input_type=it, output_type='real'))
-            return b.r2c(None, rop=C.rop(C.ROP_MUL, a=r, b=self.mask))
+            return b.r2c(self._P.apply(r, transpose=True))
 
         w = b.ainv_weight()
         Ainv = b.kmap(a, C.kop(C.KOP_WEIGHT, w=w)) if w is not None else a

@@ -136,3 +136,37 @@ def test_bk_shot_contribution(hip, case):
         s1 = bs.compute_shot_contribution(seed, data=d, data2=d2)
         s2, sc = bo.compute_shot_contribution(seed, data=d, data2=d2, return_scale=True)
         close(s1, s2, scale=sc)
+
+
+@pytest.mark.parametrize("case", ["gic_global_l2", "ric_local_l2"])
+def test_bk_integral_constraints(hip, case):
+    """MC Fisher per seed and the linear-term numerator with the integral-constrained pointing
+    matrix (bspec.py:412-426; utils.pyx:405-513) against the oracle."""
+    import polybin3d_amd as pb
+    n = 16
+    edges = np.linspace(0.007, 0.05, 4)
+    if case.startswith("gic"):
+        kw, ic = {}, dict(add_GIC=True)
+    else:
+        kw, ic = dict(boxcenter=[10., 10., 10.], sightline='local'), dict(add_RIC=True,
+                                                                          radial_bins_RIC=[100., 250., 400., 600.])
+    mask = shell_mask(n, L, kw.get('boxcenter', [0., 0., 0.]))
+    mask_IC = mask * (1.2 + 0.1 * np.cos(np.arange(n) * 0.3))[None, None, :]
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    bs = pb.BSpec(b, edges, lmax=2, mask=mask, mask_IC=mask_IC, **ic)
+    bo = O.BSpec(o, edges, lmax=2, mask=mask, mask_IC=mask_IC, **ic)
+    for seed in (0, 3):
+        F1 = bs.compute_fisher_contribution(seed)
+        F2 = bo.compute_fisher_contribution(seed)
+        close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+    bs.generate_sims(2)
+    bo.generate_sims(2)
+    d = ng_field((n, n, n), 4) * mask
+    n1, n2 = bs.Bk_numerator(d), bo.Bk_numerator(d)
+    # scale: the same triple sums over |g| maps (local l = 2 bins cancel to ~1e-2 of it)
+    g = np.abs(bo._g_maps(o.to_fourier(np.asarray(d, dtype=np.float64))))
+    sc = O.assemble_b3(g, bo.all_bins)
+    for ii in range(bo.N_it):
+        sc = sc + O.assemble_b1(g, np.abs(bo.g_bl_maps[ii]), bo.all_bins) / bo.N_it
+    close(n1, n2, scale=sc * o.volume / o.gridsize.prod() / bo.sym_factor)

@@ -150,3 +150,29 @@ def test_pk_mc_fisher_batched_seeds(hip, case):
     sc = np.sqrt(np.outer(np.abs(np.diag(fish)), np.abs(np.diag(fish))))
     close(ps.compute_fisher(3, batch=3), fish, scale=sc)
     close(ps.compute_fisher(3, batch=2), fish, scale=sc)
+
+
+@pytest.mark.parametrize("case", ["gic_global", "ric_local"])
+def test_pk_mc_fisher_integral_constraints(hip, case):
+    """MC Fisher with the global / radial integral-constrained pointing matrix (pspec.py:199-213;
+    utils.pyx:405-513): per seed against the oracle, and the batched-seed mean."""
+    import polybin3d_amd as pb
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    if case == "gic_global":
+        kw, ic = {}, dict(add_GIC=True)
+    else:
+        kw, ic = dict(boxcenter=[10., 10., 10.], sightline='local'), dict(add_RIC=True,
+                                                                          radial_bins_RIC=[100., 250., 400., 600.])
+    mask = shell_mask(n, L, kw.get('boxcenter', [0., 0., 0.]))
+    mask_IC = mask * (1.2 + 0.1 * np.cos(np.arange(n) * 0.3))[None, None, :]
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    ps = pb.PSpec(b, edges, lmax=4, mask=mask, mask_IC=mask_IC, **ic)
+    po = O.PSpec(o, edges, lmax=4, mask=mask, mask_IC=mask_IC, **ic)
+    for seed in (0, 1):
+        F1 = ps.compute_fisher_contribution(seed)
+        F2 = po.compute_fisher_contribution(seed)
+        close(F1, F2, scale=np.sqrt(np.outer(np.abs(np.diag(F2)), np.abs(np.diag(F2)))))
+    fish = po.compute_fisher(2)
+    close(ps.compute_fisher(2, batch=2), fish, scale=np.sqrt(np.outer(np.abs(np.diag(fish)), np.abs(np.diag(fish)))))

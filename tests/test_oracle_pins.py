@@ -498,3 +498,20 @@ def test_bk_numerator_local_dense():
     for (b1, b2, b3, ell), sym in zip(bs.all_bins, bs.sym_factor):
         ref.append(np.sum(g[0, b1] * g[0, b2] * g[ell, b3]) * b.volume / n ** 3 / sym)
     np.testing.assert_allclose(num, ref, rtol=1e-9, atol=1e-12 * np.max(np.abs(ref)))
+
+
+def test_integral_constraint_transposes_are_adjoints():
+    """<P m1, m2> = <m1, P^T m2> for the global and radial integral-constraint pointing matrices
+    (the transposes are derived independently of the forward maps)."""
+    rng = np.random.RandomState(9)
+    n = (6, 5, 4)
+    m1, m2, mask, mic = rng.randn(*n), rng.randn(*n), rng.rand(*n), rng.rand(*n)
+    modr = rng.uniform(0., 10., n)
+    edges = np.array([1., 3.5, 6., 9.])
+    lhs = np.sum(O.pointing_GIC(m1, mask, mic) * m2)
+    assert abs(lhs - np.sum(m1 * O.pointing_GIC_transpose(m2, mask, mic))) < 1e-12 * abs(lhs) + 1e-12
+    lhs = np.sum(O.pointing_RIC(m1, mask, mic, edges, modr) * m2)
+    rhs = np.sum(m1 * O.pointing_RIC_transpose(m2, mask, mic, edges, modr))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs) + 1e-12
+    # a constant map is annihilated by the constraint
+    assert np.max(np.abs(O.pointing_GIC(np.ones(n), mask, mic))) < 1e-14

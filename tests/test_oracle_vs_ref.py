@@ -95,3 +95,20 @@ def test_triplet_assembly_matches_reference(setup):
     np.testing.assert_allclose(O.assemble_b1(g, s, bins), U.assemble_b1(g, s, bins, 2), rtol=1e-12)
     c = rng.randn(2, 3, *b.gridsize)
     np.testing.assert_allclose(O.assemble_bshot(g, s, c, bins), U.assemble_bshot(g, s, c, bins, 2), rtol=1e-12)
+
+
+def test_integral_constraint_pointing_matches_reference(setup):
+    """pointing_GIC/RIC and their transposes (utils.pyx:405-513) against the compiled kernels."""
+    U, b, I, M = setup
+    rng = np.random.RandomState(4)
+    n = tuple(int(v) for v in b.gridsize)
+    m, mask, mic = rng.randn(*n), rng.rand(*n), rng.rand(*n)
+    modr = rng.uniform(0., 10., n)
+    edges = np.array([1., 3.5, 6., 9.])
+    np.testing.assert_allclose(O.pointing_GIC(m, mask, mic), U.pointing_GIC(m, mask, mic, 2), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(O.pointing_GIC_transpose(m, mask, mic), U.pointing_GIC_transpose(m, mask, mic, 2),
+                               rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(O.pointing_RIC(m, mask, mic, edges, modr),
+                               U.pointing_RIC(m, mask, mic, edges, modr, 2), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(O.pointing_RIC_transpose(m, mask, mic, edges, modr),
+                               U.pointing_RIC_transpose(m, mask, mic, edges, modr, 2), rtol=1e-12, atol=1e-14)
