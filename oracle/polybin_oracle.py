@@ -24,7 +24,8 @@ How each part is pinned (tests/test_oracle_*.py):
   * Bk numerator: exact Fourier-space triple sum by brute force on 8^3 grids;
   * FFT counts of the whole call graphs: the notebook counters 1/0, 15/1, 2/96 and 452/682;
   * Pk MC Fisher (global, constant mask): per-seed closed form from the GRF amplitudes; local and
-    masked branches: dense-operator restatement on 6^3-8^3 grids via the addition theorem;
+    masked branches: dense-operator restatement on 6^3-8^3 grids via the addition theorem, also with
+    odd_l (odd-l legs and their -i / i phases) for the MC Fisher and the numerator;
   * Bk MC Fisher: in expectation over 200 seeds (unit mask, ideal weights) equal to the ideal Fisher
     with the discreteness correction, a separate code path, which it thereby also pins for l > 0;
   * Bk symmetry factors for l > 0: lattice triangle sums with the addition theorem (brute force);

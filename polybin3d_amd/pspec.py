@@ -42,7 +42,6 @@ class PSpec:
         if odd_l:
             assert self.lmax > 0, "lmax must be greater than 0 to use odd harmonics!"
             assert base.sightline == 'local', "Odd spherical harmonics should only be used for local line-of-sight!"
-            raise NotImplementedError("odd_l (wide-angle) multipoles are not implemented by backend='hip'")
         assert type(self.lmax) == int, "l-max must be an integer!"
         assert self.lmax <= 4, "Only l-max<=4 is currently supported!"
         if mask is None:
@@ -87,6 +86,16 @@ class PSpec:
         else:
             self.Ylm_real = self.Ylm_fourier = None
         self.table = base.bin_table(self.k_bins)
+
+    def _harm(self, real_map, ell, out=None):
+        """sum_m FFT[m(x) Y_lm(x)] Y_lm(k) (apply_fourier_harmonics), times -i for odd l: the phase
+        the reference gives odd-l legs (pspec.py:469-481, 542-543, 677-678); in a cross-integral
+        Re[Q conj(-i leg)] it is the reference's Re[leg conj(i Q)] (pspec.py:410-411)."""
+        b = self.base
+        out = b.apply_fourier_harmonics(real_map, self.Ylm_real[ell], self.Ylm_fourier[ell], out=out)
+        if ell % 2 == 1:
+            out.mul_(-1j)
+        return out
 
     def get_ks(self):
         """Bin centres (pspec.py:122-128)."""
@@ -140,7 +149,7 @@ class PSpec:
         S_real, then for each l>0 the harmonics sum (2l+1 fused R2Cs) and one cross-integral pass."""
         assert filtering in ['ideal', 'Sinv'], "Unknown filtering option supplied!"
         b = self.base
-        Nl = self.Nl if filtering == 'Sinv' else self.Nl_even
+        Nl, lf = (self.Nl, 2 - self.odd_l) if filtering == 'Sinv' else (self.Nl_even, 2)
         ells = [2 * li for li in range(Nl)]
         stack = torch.empty((Nl,) + b.half_shape, dtype=torch.complex64, device=b.device)
         S = self._filtered_data(data, filtering)
@@ -150,7 +159,7 @@ class PSpec:
         else:
             S_real = b.c2r(stack[0])
             for li in range(1, Nl):
-                b.apply_fourier_harmonics(S_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=stack[li])
+                self._harm(S_real, lf * li, out=stack[li])
             out = b.bin_reduce(stack[0], self.k_bins, [0] * Nl, y=stack, y_ostride=b.H)[0]
         return out.reshape(-1) * (0.5 * b.volume / float(b.gridsize.prod()) ** 2)
 
@@ -262,15 +271,16 @@ class PSpec:
         b = self.base
         Nk = self.Nk
         bins = self._bins_dev
-        SinvP, leg = src
-        ell = 2 * li
+        SinvP, leg, SinvP_mi = src
+        ell = (2 - self.odd_l) * li
         sup = b.support(self.k_bins[-1])
         common = dict(bin_table=self.table, bins=bins, x_batch_stride=0, support=sup)
         if b.sightline == 'global' or li == 0:
             subs = [(C.kop(C.KOP_FILTER, ell=ell, **common), SinvP, 0, 0, 1.0)]
         else:
             subs = [(C.kop(C.KOP_FILTER, ell=0, **common), leg[li], 0, 0, 0.5)]
-            subs += [(C.kop(C.KOP_YLM, ell=ell, m=m, **common), SinvP, ell, m, 0.5) for m in range(2 * ell + 1)]
+            src_y = SinvP_mi if ell % 2 == 1 else SinvP  # -i on the odd-l Y_lm-weighted maps
+            subs += [(C.kop(C.KOP_YLM, ell=ell, m=m, **common), src_y, ell, m, 0.5) for m in range(2 * ell + 1)]
         mask, scale, ic, finish = self._chain_pointing(Nk)
         b.pointing(subs, Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup), ic=ic)
         finish(Q)
@@ -313,18 +323,20 @@ class PSpec:
         if local:
             SinvP_real = b.c2r(SinvP)
             Ainv_real = b.c2r(Ainv)
+            lf = 2 - self.odd_l
             for li in range(1, Nl):
-                b.apply_fourier_harmonics(Ainv_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=lefts[li])
-                leg[li] = b.apply_fourier_harmonics(SinvP_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li])
+                self._harm(Ainv_real, lf * li, out=lefts[li])
+                leg[li] = self._harm(SinvP_real, lf * li)
         else:
             for li in range(1, Nl):
                 lefts[li].copy_(Ainv)
+        SinvP_mi = SinvP * (-1j) if self.odd_l else None
         ells = [0] * Nl if local else [2 * li for li in range(Nl)]
         Q = torch.empty((Nk,) + b.half_shape, dtype=torch.complex64, device=b.device)
         F = torch.zeros((self.N_bins, self.N_bins), dtype=torch.float64, device=b.device)
         scaling = 0.5 * b.volume / b.gridsize.prod() ** 2
         for li in range(Nl):
-            self._q_maps((SinvP, leg), li, Q)
+            self._q_maps((SinvP, leg, SinvP_mi), li, Q)
             r = b.bin_reduce(Q, self.k_bins, ells, y=lefts, y_qstride=0, y_ostride=b.H, nq=Nk)  # [Nk, Nl, Nk]
             F[li * Nk:(li + 1) * Nk, :] = r.reshape(Nk, Nl * Nk) * scaling
         return F
@@ -375,13 +387,12 @@ class PSpec:
         if local:
             SinvP_real = b.c2r(SinvP, batch=R)
             Ainv_real = b.c2r(Ainv, batch=R)
+            lf = 2 - self.odd_l
             for li in range(1, Nl):
                 leg[li] = torch.empty((R,) + b.half_shape, dtype=torch.complex64, device=b.device)
                 for r in range(R):
-                    b.apply_fourier_harmonics(Ainv_real[r], self.Ylm_real[2 * li], self.Ylm_fourier[2 * li],
-                                              out=lefts[r, li])
-                    b.apply_fourier_harmonics(SinvP_real[r], self.Ylm_real[2 * li], self.Ylm_fourier[2 * li],
-                                              out=leg[li][r])
+                    self._harm(Ainv_real[r], lf * li, out=lefts[r, li])
+                    self._harm(SinvP_real[r], lf * li, out=leg[li][r])
             del SinvP_real, Ainv_real
         else:
             for li in range(1, Nl):
@@ -392,14 +403,16 @@ class PSpec:
         scaling = 0.5 * b.volume / b.gridsize.prod() ** 2
         sup = b.support(self.k_bins[-1])
         common = dict(bin_table=self.table, bins=bins, x_batch_stride=H, x_batch_group=Nk, support=sup)
+        SinvP_mi = SinvP * (-1j) if self.odd_l else None
         for li in range(Nl):
-            ell = 2 * li
+            ell = (2 - self.odd_l) * li
             # the Q maps of _q_maps for every seed: batch entry r*Nk + bin reads seed r's maps
             if not local or li == 0:
                 subs = [(C.kop(C.KOP_FILTER, ell=ell, **common), SinvP, 0, 0, 1.0)]
             else:
+                src_y = SinvP_mi if ell % 2 == 1 else SinvP
                 subs = [(C.kop(C.KOP_FILTER, ell=0, **common), leg[li], 0, 0, 0.5)]
-                subs += [(C.kop(C.KOP_YLM, ell=ell, m=m, **common), SinvP, ell, m, 0.5) for m in range(2 * ell + 1)]
+                subs += [(C.kop(C.KOP_YLM, ell=ell, m=m, **common), src_y, ell, m, 0.5) for m in range(2 * ell + 1)]
             mask, scale, ic, finish = self._chain_pointing(R * Nk)
             b.pointing(subs, R * Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup), ic=ic)
             finish(Q)
@@ -440,7 +453,7 @@ class PSpec:
             SinvP_real = b.c2r(SinvP)
             Ainv_real = b.c2r(Ainv)
             for li in range(1, Nl):
-                b.apply_fourier_harmonics(Ainv_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=lefts[li])
+                self._harm(Ainv_real, (2 - self.odd_l) * li, out=lefts[li])
         else:
             for li in range(1, Nl):
                 lefts[li].copy_(Ainv)
@@ -452,20 +465,21 @@ class PSpec:
             F[row] = r.reshape(-1).cpu().numpy() * scaling
 
         for li in range(Nl):
-            ell = 2 * li
+            ell = (2 - self.odd_l) * li
             if not local or li == 0:
                 for ki in range(Nk):
                     km = b.map_utils.fourier_filter(SinvP, ell, self.k_bins[ki], self.k_bins[ki + 1])
                     add_row(li * Nk + ki, apply_filter_dagger(km))
             else:
-                legm = b.apply_fourier_harmonics(SinvP_real, self.Ylm_real[ell], self.Ylm_fourier[ell])
+                legm = self._harm(SinvP_real, ell)
+                src_y = SinvP * (-1j) if ell % 2 == 1 else SinvP
                 for ki in range(Nk):
                     t = b.bin_table([self.k_bins[ki], self.k_bins[ki + 1]])
                     km = b.kmap(legm, C.kop(C.KOP_FILTER, bin_table=t))
                     Qm = 0.5 * apply_filter_dagger(km)
                     real_map = torch.zeros(tuple(int(v) for v in b.gridsize), dtype=torch.float32, device=b.device)
                     for m in range(2 * ell + 1):
-                        b.c2r(SinvP, out=real_map, kop=C.kop(C.KOP_YLM, bin_table=t, ell=ell, m=m),
+                        b.c2r(src_y, out=real_map, kop=C.kop(C.KOP_YLM, bin_table=t, ell=ell, m=m),
                               rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=1))
                     Qm = Qm + 0.5 * apply_filter_dagger(real_map, real=True)
                     add_row(li * Nk + ki, Qm)
@@ -516,7 +530,7 @@ class PSpec:
         if local:
             Sa_real = b.c2r(a)
             for li in range(1, Nl):
-                b.apply_fourier_harmonics(Sa_real, self.Ylm_real[2 * li], self.Ylm_fourier[2 * li], out=stack[li])
+                self._harm(Sa_real, (2 - self.odd_l) * li, out=stack[li])
             r = b.bin_reduce(SNA, self.k_bins, [0] * Nl, y=stack, y_ostride=b.H)[0]
         else:
             r = b.bin_reduce(SNA, self.k_bins, [2 * li for li in range(Nl)], y=a, y_ostride=0)[0]

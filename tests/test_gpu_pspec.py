@@ -176,3 +176,34 @@ def test_pk_mc_fisher_integral_constraints(hip, case):
         close(F1, F2, scale=np.sqrt(np.outer(np.abs(np.diag(F2)), np.abs(np.diag(F2)))))
     fish = po.compute_fisher(2)
     close(ps.compute_fisher(2, batch=2), fish, scale=np.sqrt(np.outer(np.abs(np.diag(fish)), np.abs(np.diag(fish)))))
+
+
+def test_pk_odd_l(hip):
+    """odd_l (l = 0..3, local sightline, varying mask): numerator, MC Fisher per seed (single and
+    batched seeds), shot noise and Pk_unwindowed against the oracle (pspec.py:410-411, 469-481,
+    539-543, 607, 677-678)."""
+    import polybin3d_amd as pb
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    kw = dict(boxcenter=[10., 10., 10.], sightline='local')
+    mask = shell_mask(n, L, [10., 10., 10.])
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    ps = pb.PSpec(b, edges, lmax=3, mask=mask, odd_l=True, mask_shot=mask)
+    po = O.PSpec(o, edges, lmax=3, mask=mask, odd_l=True, mask_shot=mask)
+    assert ps.N_bins == po.N_bins == 4 * 8
+    d = powerlaw_field((n, n, n), seed=3) * mask
+    n1, n2 = ps.Pk_numerator(d), po.Pk_numerator(d)
+    close(n1, n2, scale=np.abs(n2[:ps.Nk]).max())
+    for seed in (0, 1):
+        F1, F2 = ps.compute_fisher_contribution(seed), po.compute_fisher_contribution(seed)
+        close(F1, F2, scale=np.sqrt(np.outer(np.abs(np.diag(F2)), np.abs(np.diag(F2)))))
+    s1, s2 = ps.compute_shot_contribution(5), po.compute_shot_contribution(5)
+    close(s1, s2, scale=np.abs(s2[:ps.Nk]).max())
+    fish = po.compute_fisher(2)
+    sc = np.sqrt(np.outer(np.abs(np.diag(fish)), np.abs(np.diag(fish))))
+    close(ps.compute_fisher(2, batch=2), fish, scale=sc)
+    got, ref = ps.Pk_unwindowed(d, fish=fish), po.Pk_unwindowed(d, fish=fish)
+    assert sorted(got) == sorted(ref) == ['p0', 'p1', 'p2', 'p3']
+    for k in ref:
+        close(got[k], ref[k], tol=2e-5, scale=np.abs(ref['p0']))

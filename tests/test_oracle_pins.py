@@ -515,3 +515,69 @@ def test_integral_constraint_transposes_are_adjoints():
     assert abs(lhs - rhs) < 1e-12 * abs(lhs) + 1e-12
     # a constant map is annihilated by the constraint
     assert np.max(np.abs(O.pointing_GIC(np.ones(n), mask, mic))) < 1e-14
+
+
+def test_pk_local_odd_l_dense():
+    """odd_l (l = 0, 1, 2), local sightline, varying mask: the per-seed MC Fisher and the numerator
+    against a dense-matrix restatement.  Odd-l legs carry the reference's factors: -i on the
+    filtered leg maps and on the Y_lm-weighted k-space maps (pspec.py:469-481), i on Q in the
+    cross-integral (pspec.py:410-411), -i on the numerator's leg map (pspec.py:677-678)."""
+    n = 8
+    b = O.PolyBin3D(200., [n, n, n], boxcenter=[300., 140., 90.], sightline='local')
+    mask = 0.5 + np.random.RandomState(6).rand(n, n, n)
+    edges = np.array([1.0, 2.0, 3.0]) * b.kF[0] + 1e-9
+    ps = O.PSpec(b, edges, lmax=2, mask=mask, odd_l=True)
+    F = ps.compute_fisher_contribution(2)
+    N = n ** 3
+    ph, kv, cosang, fr = _dense_ops(b)
+    ah = b.generate_data(seed=2 + int(1e7), output_type='fourier')
+    a = np.fft.fftn(b.to_real(ah)).reshape(N)
+    ar = np.fft.ifftn(a.reshape(n, n, n)).real.reshape(N)
+    modk = np.linalg.norm(kv, axis=1)
+    mk = mask.reshape(N)
+    SinvP = ph @ (mk * ar)
+    Ainv = a.copy()
+    Ainv_r = (ph.conj().T @ Ainv).real / N
+    ells = (0, 1, 2)
+    Lk = {l: O.legendre(l, cosang) for l in ells}
+    lefts = {0: Ainv, 1: (ph * Lk[1]) @ Ainv_r, 2: (ph * Lk[2]) @ Ainv_r}
+    Nk = 2
+    ref = np.zeros((3 * Nk, 3 * Nk))
+    SinvP_r = (ph.conj().T @ SinvP).real / N
+    for li, ell in enumerate(ells):
+        ph_odd = -1j if ell % 2 == 1 else 1.
+        for kb in range(Nk):
+            th = (modk >= edges[kb]) & (modk < edges[kb + 1])
+            if ell == 0:
+                Q = ph @ (mk * (ph.conj().T @ (th * SinvP)).real / N)
+            else:
+                leg = (ph * Lk[ell]) @ SinvP_r
+                t1 = ph @ (mk * (ph.conj().T @ (ph_odd * th * leg)).real / N)
+                real_map = ((ph.conj() * Lk[ell]).T @ (ph_odd * th * SinvP)).real / N
+                Q = 0.5 * t1 + 0.5 * (ph @ (mk * real_map))
+            for lj, ell2 in enumerate(ells):
+                Qx = 1j * Q if ell2 % 2 == 1 else Q
+                for kb2 in range(Nk):
+                    th2 = (modk >= edges[kb2]) & (modk < edges[kb2 + 1])
+                    ref[li * Nk + kb, lj * Nk + kb2] = 0.5 * b.volume / N ** 2 * np.sum(
+                        th2 * (lefts[ell2] * np.conj(Qx)).real)
+    np.testing.assert_allclose(F, ref, atol=1e-10 * np.abs(ref).max(), rtol=1e-8)
+    # numerator (S^-1 filtering: S = FFT[d]): 0.5 sum degk Re[S conj(-i lm)] over the half grid
+    d = np.random.RandomState(3).randn(n, n, n)
+    num = ps.Pk_numerator(d)
+    S = np.fft.fftn(d).reshape(N)
+    S_r = d.reshape(N)
+    half = fr[:, 2] >= 0
+    nyq = fr[:, 2] == -n // 2  # fftfreq puts the z Nyquist plane at -n/2: it belongs to the half grid
+    deg = np.where(fr[:, 2] > 0, 2., 1.)
+    deg = np.where(nyq, 2., deg)
+    sel = half | nyq
+    refn = np.zeros(3 * Nk)
+    for li, ell in enumerate(ells):
+        lm = S if ell == 0 else (ph * Lk[ell]) @ S_r
+        if ell % 2 == 1:
+            lm = -1j * lm
+        for kb in range(Nk):
+            th = (modk >= edges[kb]) & (modk < edges[kb + 1]) & sel
+            refn[li * Nk + kb] = 0.5 * b.volume / N ** 2 * np.sum(deg[th] * (S[th] * np.conj(lm[th])).real)
+    np.testing.assert_allclose(num, refn, atol=1e-10 * np.abs(refn).max(), rtol=1e-8)
