@@ -677,10 +677,49 @@ class BSpec:
         by the S^-1-weighted triplet, not symmetrised (as the reference)."""
         return self._fisher_contribution_dev(seed).cpu().numpy()
 
+    def _apply_filter_generic(self, a):
+        """S^-1 P a with the user's S^-1 (bspec.py:499-505): the mask mean times S^-1 of the Fourier
+        map for a constant mask, else S^-1 of the pointing matrix applied to the real map."""
+        b = self.base
+        if self.const_mask:
+            return self.mask_mean * b.as_fourier(self.applySinv(a, input_type='fourier', output_type='fourier'))
+        return b.as_fourier(self.applySinv(self.apply_pointing(b.c2r(a)), input_type='real', output_type='fourier'))
+
+    def _q_ainv_generic(self, FT, pair_tab, QA, chunk=8):
+        """QA rows with the user's S^-1 (bspec.py:615-666): the Q[A^-1 a] maps of a chunk of triplets
+        are assembled in real space (C2R with the BKQ prologue; for local l > 0 the external leg's
+        sum_m Y_lm(x) IFFT[Theta Y_lm(k) FT_0], bspec.py:629-636), then S^-1 P is the user's callable on
+        each map, and the in-bin modes are packed."""
+        b = self.base
+        pk = self._packed_layout()
+        M, Nq = pk['M'], self.Nk_squeeze
+        ab = self.all_bins[self._qperm]
+        local = b.sightline == 'local'
+        modes = pk['modes'].long()
+        common = dict(bin_table=self.table, pack_index=pk['pack_index'], packed_len=M, trip=self._trip_dev,
+                      pair_index=pair_tab, ft=FT, ft_lstride=FT.shape[1] * M, nk=Nq, support=self._sup)
+        t0 = 0
+        while t0 < self.N_bins:
+            ell = int(ab[t0, 3])
+            ylm = local and ell > 0
+            t1 = t0
+            while t1 < self.N_bins and t1 - t0 < chunk and ab[t1, 3] == ell:
+                t1 += 1
+            nb = t1 - t0
+            real = b.c2r(None, batch=nb, kop=C.kop(C.KOP_BKQ, terms=2 if ylm else 3, trip0=t0,
+                                                   leg3_legendre=int(not local), **common))
+            real = real.reshape((nb,) + tuple(int(v) for v in b.gridsize))
+            if ylm:
+                for m in range(2 * ell + 1):
+                    b.c2r(None, out=real, batch=nb, kop=C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common),
+                          rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=1))
+            for i in range(nb):
+                S = self._apply_filter_generic(b.r2c(real[i]))
+                QA[t0 + i] = S.reshape(-1)[modes]
+            t0 = t1
+
     def _fisher_contribution_dev(self, seed):
         b = self.base
-        if self.user_Sinv:
-            raise NotImplementedError("compute_fisher with a user S^-1 is not implemented by backend='hip'")
         if not hasattr(self, 'sym_factor'):
             self._compute_symmetry_factor()
         pk = self._packed_layout()
@@ -712,10 +751,15 @@ class BSpec:
         FT = torch.empty((self.Nl, len(self._ainv_pairs), M), dtype=torch.complex64, device=b.device)
         self._ft_tables(G1, G2, self._ainv_pairs, FT)
         del G1, G2
-        self._q_ainv(FT, self._ainv_tab, QA)
+        if self.user_Sinv:
+            self._q_ainv_generic(FT, self._ainv_tab, QA)
+        else:
+            self._q_ainv(FT, self._ainv_tab, QA)
         del FT
         # S^-1 P side (bspec.py:671, compute_Q('Sinv'))
-        if self.const_mask:
+        if self.user_Sinv:
+            v1, v2 = self._apply_filter_generic(a1), self._apply_filter_generic(a2)
+        elif self.const_mask:
             v1 = b.kmap(a1, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
             v2 = b.kmap(a2, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
         else:

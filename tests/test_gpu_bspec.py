@@ -170,3 +170,37 @@ def test_bk_integral_constraints(hip, case):
     for ii in range(bo.N_it):
         sc = sc + O.assemble_b1(g, np.abs(bo.g_bl_maps[ii]), bo.all_bins) / bo.N_it
     close(n1, n2, scale=sc * o.volume / o.gridsize.prod() / bo.sym_factor)
+
+
+@pytest.mark.parametrize("case", ["global_mask_l2", "local_const_l2"])
+def test_bk_mc_fisher_user_sinv(hip, case):
+    """MC Fisher per seed with a user S^-1 (a mode-space weight given as a callable, bspec.py:499-505):
+    the generic path (user callable on every Q map) against the oracle with the same S^-1."""
+    import polybin3d_amd as pb
+    import torch
+    n = 16
+    edges = np.linspace(0.007, 0.05, 4)
+    if case.startswith("global"):
+        kw, mask = {}, shell_mask(n, L, [0., 0., 0.])
+    else:
+        kw, mask = dict(boxcenter=[10., 10., 10.], sightline='local'), 0.6 * np.ones((n, n, n))
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    wk = 1. / (1. + (o.modk_grid / 0.02) ** 2)
+    wk_d = torch.from_numpy(wk.astype(np.float32)).cuda()
+
+    def sinv_o(m, input_type='real', output_type='fourier'):
+        f = o.to_fourier(m) if input_type == 'real' else m
+        f = f * wk
+        return f if output_type == 'fourier' else o.to_real(f)
+
+    def sinv_g(m, input_type='real', output_type='fourier'):
+        f = b.to_fourier(m) if input_type == 'real' else m
+        f = f * wk_d
+        return f if output_type == 'fourier' else b.to_real(f)
+    bs = pb.BSpec(b, edges, lmax=2, mask=mask, applySinv=sinv_g)
+    bo = O.BSpec(o, edges, lmax=2, mask=mask, applySinv=sinv_o)
+    for seed in (0, 3):
+        F1 = bs.compute_fisher_contribution(seed)
+        F2 = bo.compute_fisher_contribution(seed)
+        close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
