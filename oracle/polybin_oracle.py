@@ -29,7 +29,14 @@ How each part is pinned (tests/test_oracle_*.py):
   * Bk MC Fisher: in expectation over 200 seeds (unit mask, ideal weights) equal to the ideal Fisher
     with the discreteness correction, a separate code path, which it thereby also pins for l > 0;
   * Bk symmetry factors for l > 0: lattice triangle sums with the addition theorem (brute force);
-  * local-sightline Bk numerator: dense DFT-matrix restatement with L_l(k_hat . x_hat).
+  * local-sightline Bk numerator: dense DFT-matrix restatement with L_l(k_hat . x_hat);
+  * apply_xi (local): the dense symmetric operator sum_k e^{ik(x-y)} [P_0 + P_2 (L_2(k.x)+L_2(k.y))/2];
+  * Pk covariance contribution: per-seed closed form (global, constant mask, filter m^2 P + m) and the
+    dense restatement with the filter P xi P^T + diag(mask_shot) (local, varying mask);
+  * Pk theory binning matrix: equal to the Fisher matrix on the data bins, its l = 4 block equal to
+    the lmax = 4 Fisher matrix's, refined bins summing to coarse ones; the wide-angle terms by a dense
+    restatement; derivative_coefficient on hand-evaluated edges; cross_integrate_wcoeff against
+    the compiled reference kernel.
 Parity unpinned (no check beyond restatement and the compiled utils.pyx kernels they call):
 BSpec._compute_fisher per-seed values with a mask or a local sightline (pinned in expectation only
 for the unit mask); PSpec.compute_shot_contribution; BSpec.compute_shot_contribution as a whole (its
@@ -149,6 +156,12 @@ class Integrator:
         """sum_{k in b} degk Re[s1 conj(s2)] L_l(mu) (utils.pyx:51-75)."""
         w = self.degk * np.real(s1 * np.conj(s2)) * (legendre(l, self.muk) if l != 0 else 1.)
         return self._binsum(w, edges)
+
+    def cross_integrate_wcoeff(self, coeff, s1, s2, edges):
+        """sum_{k in b} degk Re[coeff_b s1 conj(s2)], one complex coefficient per bin (utils.pyx:139-157)."""
+        coeff = np.asarray(coeff, dtype=np.complex128)
+        prod = self.degk * s1 * np.conj(s2)
+        return coeff.real * self._binsum(prod.real, edges) - coeff.imag * self._binsum(prod.imag, edges)
 
     def integrate(self, s, edges, l):
         """sum_{k in b} degk |s|^2 L_l(mu) (utils.pyx:80-104)."""
@@ -480,14 +493,51 @@ class PolyBin3D:
             out += self.to_fourier(a_real * Ylm_real[m]) * Ylm_fourier[m]
         return out
 
+    def apply_xi(self, input_fourier, input_real=None, Ylm_real=None, Ylm_fourier=None, Pk_grid=None,
+                 output_type='real'):
+        """The correlation function xi(x, y) applied symmetrically (base.py:413-437).  Pk_grid: {0: P(k, mu)}
+        (global) or {0: P_0(k), 2: P_2(k), 4: P_4(k)} (local, multipoles beyond the monopole optional).
+        With input_real None and local multipoles the reference fails; here it is the C2R of the input."""
+        if self.sightline == 'global':
+            P = input_fourier * Pk_grid[0]
+            return P if output_type == 'fourier' else self.to_real(P)
+        P = input_fourier * Pk_grid[0]
+        ells = [l for l in Pk_grid if l != 0]
+        if not ells:
+            return P if output_type == 'fourier' else self.to_real(P)
+        if input_real is None:
+            input_real = self.to_real(input_fourier)
+        P_real = self.real_zeros()
+        for l in ells:
+            P = P + 0.5 * self.apply_fourier_harmonics(input_real, Ylm_real[l], Ylm_fourier[l]) * Pk_grid[l]
+            P_real = P_real + 0.5 * self.apply_real_harmonics(input_fourier * Pk_grid[l], Ylm_real[l], Ylm_fourier[l])
+        if output_type == 'fourier':
+            return P + self.to_fourier(P_real)
+        return self.to_real(P) + P_real
+
 
 # ----------------------------------------------------------------------------------------------
 # Power spectrum (PolyBin3D/pspec.py)
 # ----------------------------------------------------------------------------------------------
 
+def derivative_coefficient(k_bins_theory):
+    """Per-bin weights of the wide-angle k dP/dk term (pspec.py:369-379): the column sums of a
+    central-difference operator on the bin edges, with one-sided end rows."""
+    kb = np.asarray(k_bins_theory, dtype=np.float64)
+    n = len(kb) - 1
+    kfac = np.zeros(n)
+    for ki in range(n):
+        if 0 < ki - 1 < n - 1:
+            kfac[ki] += kb[ki - 1] / (kb[ki] - kb[ki - 2])
+        if 0 < ki + 1 < n - 1:
+            kfac[ki] -= kb[ki + 1] / (kb[ki + 2] - kb[ki])
+        kfac[ki] += kb[0] / (kb[1] - kb[0]) * ((ki == 1) - (ki == 0))
+        kfac[ki] -= kb[-1] / (kb[-1] - kb[-2]) * ((ki == n - 1) - (ki == n - 2))
+    return kfac
+
+
 class PSpec:
-    """FP64 restatement of PolyBin3D.PSpec (PolyBin3D/pspec.py:10-784), hot-path members only
-    (no covariance/theory matrices, no integral constraints)."""
+    """FP64 restatement of PolyBin3D.PSpec (PolyBin3D/pspec.py:10-784)."""
 
     def __init__(self, base, k_bins, lmax=2, mask=None, applySinv=None, odd_l=False,
                  applySinv_transpose=None, mask_shot=None, add_GIC=False, mask_IC=None, add_RIC=False,
@@ -609,22 +659,113 @@ class PSpec:
 
     # --- MC Fisher ------------------------------------------------------------------------
     def compute_fisher_contribution(self, seed, verb=False):
-        """pspec.py:141-147 -> _compute_fisher (pspec.py:215-493, compute_cov=False, compute_theory=False)."""
+        """pspec.py:141-147."""
+        return self._compute_fisher(seed)
+
+    def compute_covariance_contribution(self, seed, Pk_cov, verb=False):
+        """One realization of the Gaussian covariance for a theory spectrum Pk_cov = [k, P_0 (, P_2, P_4)]
+        without shot noise (pspec.py:149-163)."""
+        if not self.const_mask:
+            assert hasattr(self, 'mask_shot'), "Must supply mask_shot to compute shot-noise contribution"
+        assert len(Pk_cov) != 0, "Must provide fiducial power spectrum if computing covariance!"
+        Pk_cov = np.asarray(Pk_cov, dtype=np.float64)
+        assert Pk_cov.ndim == 2 and len(Pk_cov) in [2, 3, 4], "Pk should contain k and P_0, (and optionally P_2, P_4) columns"
+        assert (len(Pk_cov) - 2) * 2 <= self.lmax, "Can't use higher multipoles than lmax"
+        return self._compute_fisher(seed, compute_cov=True, Pk_cov=Pk_cov)
+
+    def compute_theory_contribution(self, seed, k_bins_theory, lmax_theory=None, include_wideangle=False,
+                                    verb=False):
+        """(Fisher contribution, rectangular theory binning matrix) from one realization (pspec.py:165-197)."""
+        b = self.base
+        k_bins_theory = np.asarray(k_bins_theory, dtype=np.float64)
+        assert np.max(k_bins_theory) < np.min(b.kNy), "k_max must be less than k_Nyquist!"
+        assert np.min(k_bins_theory) >= np.max(b.kF), "k_min must be at least the k_fundamental!"
+        assert np.max(k_bins_theory) <= b.Pfid[0][-1], "Fiducial power spectrum should extend up to k_max!"
+        assert np.min(k_bins_theory) >= b.Pfid[0][0], "Fiducial power spectrum should extend down to k_min!"
+        if lmax_theory is None:
+            lmax_theory = self.lmax
+        # the reference looks up the data's multipoles among the theory legs (KeyError otherwise)
+        assert lmax_theory >= self.lmax, "Can't use larger lmax for data than for theory!"
+        if include_wideangle:
+            assert b.sightline != 'global', "Wide-angle effects require local lines-of-sight!"
+            assert not self.odd_l, "Cannot include wide-angle effects for odd l!"
+            assert lmax_theory > 1, "There are no wide-angle effects for the monopole!"
+            assert lmax_theory <= 4, "Higher-order effects not yet implemented!"
+            if not hasattr(b, 'modr_grid'):
+                b.modr_grid = np.sqrt(b.r_grids[0] ** 2. + b.r_grids[1] ** 2. + b.r_grids[2] ** 2.)
+        elif not self.odd_l:
+            assert lmax_theory % 2 == 0, "Must use even lmax if not including wide-angle effects!"
+        return self._compute_fisher(seed, compute_theory=True, k_bins_theory=k_bins_theory,
+                                    lmax_theory=lmax_theory, include_wideangle=include_wideangle)
+
+    def _cov_pk_grid(self, Pk_cov):
+        """Gridded theory spectrum of the covariance (pspec.py:237-251)."""
+        b = self.base
+        interp = lambda col: scipy.interpolate.interp1d(Pk_cov[0], Pk_cov[col], bounds_error=False,  # noqa: E731
+                                                        fill_value=0.)(b.modk_grid)
+        if b.sightline == 'global':
+            P = interp(1)
+            for col, l in ((2, 2), (3, 4)):
+                if len(Pk_cov) > col:
+                    P = P + legendre(l, b.muk_grid) * interp(col)
+            return {0: P}
+        out = {0: interp(1)}
+        for col, l in ((2, 2), (3, 4)):
+            if len(Pk_cov) > col:
+                out[l] = interp(col)
+        return out
+
+    def _compute_fisher(self, seed, compute_cov=False, Pk_cov=(), compute_theory=False, k_bins_theory=(),
+                        lmax_theory=None, include_wideangle=False):
+        """pspec.py:215-493: the MC Fisher contribution, its covariance counterpart (the filter
+        S^-1 [P xi P^T + N] S^-dag in place of S^-1 P) or the theory binning matrix."""
         b = self.base
         Nk = self.Nk
         F = np.zeros((self.N_bins, self.N_bins))
         lf = 2 - self.odd_l
+        if compute_theory:                                                          # 221-231
+            odd_th = True if include_wideangle else self.odd_l
+            Nl_th = lmax_theory + 1 if (odd_th and not include_wideangle) else lmax_theory // 2 + 1
+            lf_th = 2 - odd_th
+            Nk_th = len(k_bins_theory) - 1
+            Bm = np.zeros((self.N_bins, Nl_th * Nk_th))
+        if compute_cov:                                                             # 234-251
+            assert not compute_theory and not include_wideangle
+            Pk_grid = self._cov_pk_grid(Pk_cov)
+        Ylm_real, Ylm_fourier = self.Ylm_real, self.Ylm_fourier
+        if compute_theory and b.sightline == 'local':                               # 254-264
+            if lmax_theory > self.lmax or odd_th != self.odd_l:
+                Yr_th = real_harmonics(np.asarray(b.r_grids), lmax_theory, odd_th)
+                Yf_th = real_harmonics(np.asarray(b.k_grids), lmax_theory, odd_th)
+            else:
+                Yr_th, Yf_th = Ylm_real, Ylm_fourier
         a = b.generate_data(seed=seed + int(1e7), output_type='fourier')            # 266
         if not self.const_mask:
             a_real = b.to_real(a)                                                   # 268
 
-        def apply_filter(m):                                                        # 270-276
+        def cov_filter(m, it):                                                      # 278-289, 312-325
+            if self.const_mask:
+                Sid = self.applySinv_transpose(m, input_type=it, output_type='fourier')
+                CSid = self.mask_mean ** 2 * b.apply_xi(Sid, None, Ylm_real, Ylm_fourier, Pk_grid, 'fourier') \
+                    + self.mask_mean * Sid
+                return self.applySinv(CSid, input_type='fourier', output_type='fourier')
+            Sid = self.applySinv_transpose(m, input_type=it, output_type='real')
+            PdSid_real = self.apply_pointing(Sid, transpose=True)
+            CPdSid = b.apply_xi(b.to_fourier(PdSid_real), PdSid_real, Ylm_real, Ylm_fourier, Pk_grid, 'real')
+            Cov = self.apply_pointing(CPdSid) + Sid * self.mask_shot
+            return self.applySinv(Cov, input_type='real', output_type='fourier')
+
+        def apply_filter(m):                                                        # 270-289
+            if compute_cov:
+                return cov_filter(m, 'fourier' if self.const_mask else 'real')
             if self.const_mask:
                 return self.mask_mean * self.applySinv(m, input_type='fourier', output_type='fourier')
             return self.applySinv(self.apply_pointing(m), input_type='real', output_type='fourier')
 
-        def apply_filter_dagger(m, real=False):                                     # 291-304
+        def apply_filter_dagger(m, real=False):                                     # 291-326
             it = 'real' if real else 'fourier'
+            if compute_cov:
+                return cov_filter(m, it)
             if self.const_mask:
                 return self.mask_mean * self.applySinv_transpose(m, input_type=it, output_type='fourier')
             return b.to_fourier(self.apply_pointing(self.applySinv_transpose(m, input_type=it, output_type='real'),
@@ -633,33 +774,72 @@ class PSpec:
         Ainv = b.applyAinv(a, input_type='fourier', output_type='fourier')          # 328
         SinvP = apply_filter(a if self.const_mask else a_real)                     # 329-332
         if b.sightline == 'local' and self.lmax > 0:
-            SinvP_real = b.to_real(SinvP)                                           # 336
-            Ainv_real = b.to_real(Ainv)                                             # 343
+            SinvP_real = b.to_real(SinvP)                                           # 335-336
+        if (b.sightline == 'local' and self.lmax > 0) or (compute_theory and b.sightline == 'local' and lmax_theory > 0):
+            Ainv_real = b.to_real(Ainv)                                             # 337-343
         leg_maps = {}
-        if b.sightline == 'local':                                                  # 349-352
-            for li in range(1, self.Nl):
-                leg_maps[li * lf] = b.apply_fourier_harmonics(Ainv_real, self.Ylm_real[lf * li], self.Ylm_fourier[lf * li])
+        if not compute_theory:
+            if b.sightline == 'local':                                              # 347-352
+                for li in range(1, self.Nl):
+                    leg_maps[li * lf] = b.apply_fourier_harmonics(Ainv_real, Ylm_real[lf * li], Ylm_fourier[lf * li])
+        elif b.sightline == 'local' and not include_wideangle:                      # 354-357
+            for li in range(1, Nl_th):
+                leg_maps[li * lf_th] = b.apply_fourier_harmonics(Ainv_real, Yr_th[li * lf_th], Yf_th[li * lf_th])
+        elif b.sightline == 'local':                                                # 359-367
+            inv_r_Ainv = Ainv_real / b.modr_grid
+            for l in range(1, lmax_theory + 1):
+                src = Ainv_real if l % 2 == 0 else inv_r_Ainv
+                leg_maps[l] = b.apply_fourier_harmonics(src, Yr_th[l], Yf_th[l])
+        if compute_theory and include_wideangle:
+            kfac = derivative_coefficient(k_bins_theory)                            # 369-379
         scaling = 0.5 * b.volume / b.gridsize.prod() ** 2                          # 390
 
-        def add_to_matrix(row, Q):                                                  # 380-413
+        def add_to_matrix(row, Q):                                                  # 381-436
             F[row, :Nk] += b.integrator.cross_integrate(Ainv, Q, self.k_bins, 0) * scaling
-            for li in range(1, self.Nl):
-                if b.sightline == 'global':
+            if compute_theory:
+                Bm[row, :Nk_th] += b.integrator.cross_integrate(Ainv, Q, k_bins_theory, 0) * scaling
+            if b.sightline == 'global':
+                for li in range(1, self.Nl):
                     F[row, li * Nk:(li + 1) * Nk] += b.integrator.cross_integrate(Ainv, Q, self.k_bins, 2 * li) * scaling
-                else:
-                    Qx = 1.0j * Q if (self.odd_l and li % 2 == 1) else Q
-                    F[row, li * Nk:(li + 1) * Nk] += b.integrator.cross_integrate(leg_maps[li * lf], Qx, self.k_bins, 0) * scaling
+                if compute_theory:
+                    for li in range(1, Nl_th):
+                        Bm[row, li * Nk_th:(li + 1) * Nk_th] += b.integrator.cross_integrate(
+                            Ainv, Q, k_bins_theory, 2 * li) * scaling
+                return
+            for li in range(1, self.Nl):
+                Qx = 1.0j * Q if (self.odd_l and li % 2 == 1) else Q
+                F[row, li * Nk:(li + 1) * Nk] += b.integrator.cross_integrate(leg_maps[li * lf], Qx, self.k_bins, 0) * scaling
+            if not compute_theory:
+                return
+            if not include_wideangle:
+                for li in range(1, Nl_th):
+                    Qx = 1.0j * Q if (odd_th and li % 2 == 1) else Q
+                    Bm[row, li * Nk_th:(li + 1) * Nk_th] += b.integrator.cross_integrate(
+                        leg_maps[li * lf_th], Qx, k_bins_theory, 0) * scaling
+                return
+            wc = b.integrator.cross_integrate_wcoeff
+            for l in range(1, lmax_theory + 1):
+                if l % 2 == 0:
+                    Bm[row, l // 2 * Nk_th:(l // 2 + 1) * Nk_th] += b.integrator.cross_integrate(
+                        leg_maps[l], Q, k_bins_theory, 0) * scaling
+                if l == 1:      # the l = 1 leg feeds l = 2
+                    Bm[row, Nk_th:2 * Nk_th] += wc(3.0j / 5. * (3 + kfac), leg_maps[l], Q, k_bins_theory) * scaling
+                if l == 3:      # the l = 3 leg feeds l = 2 and l = 4
+                    Bm[row, Nk_th:2 * Nk_th] += wc(3.0j / 5. * (2 - kfac), leg_maps[l], Q, k_bins_theory) * scaling
+                    if lmax_theory >= 4:
+                        Bm[row, 2 * Nk_th:3 * Nk_th] += wc(10.0j / 9. * (5 + kfac), leg_maps[l], Q,
+                                                           k_bins_theory) * scaling
 
-        for ki in range(Nk):                                                        # 444-446
+        for ki in range(Nk):                                                        # 439-442
             km = b.map_utils.fourier_filter(SinvP, 0, self.k_bins[ki], self.k_bins[ki + 1])
             add_to_matrix(ki, apply_filter_dagger(km))
-        for li in range(1, self.Nl):                                                # 449-488
+        for li in range(1, self.Nl):                                                # 444-488
             if b.sightline == 'global':
                 for ki in range(Nk):
                     km = b.map_utils.fourier_filter(SinvP, li * 2, self.k_bins[ki], self.k_bins[ki + 1])
                     add_to_matrix(li * Nk + ki, apply_filter_dagger(km))
             else:
-                Yr, Yf = self.Ylm_real[lf * li], self.Ylm_fourier[lf * li]
+                Yr, Yf = Ylm_real[lf * li], Ylm_fourier[lf * li]
                 leg_map = b.apply_fourier_harmonics(SinvP_real, Yr, Yf)
                 for ki in range(Nk):
                     km = b.map_utils.fourier_filter(leg_map, 0, self.k_bins[ki], self.k_bins[ki + 1])
@@ -674,7 +854,7 @@ class PSpec:
                         real_map += b.to_real(k_map) * Yr[m]
                     Q = Q + 0.5 * apply_filter_dagger(real_map, real=True)
                     add_to_matrix(li * Nk + ki, Q)
-        return F
+        return (F, Bm) if compute_theory else F
 
     def compute_fisher(self, N_it, verb=False):
         """pspec.py:553-569."""

@@ -581,3 +581,202 @@ def test_pk_local_odd_l_dense():
             th = (modk >= edges[kb]) & (modk < edges[kb + 1]) & sel
             refn[li * Nk + kb] = 0.5 * b.volume / N ** 2 * np.sum(deg[th] * (S[th] * np.conj(lm[th])).real)
     np.testing.assert_allclose(num, refn, atol=1e-10 * np.abs(refn).max(), rtol=1e-8)
+
+
+# ---------------------------------------------------------------- covariance and theory matrices
+def _dense_q_rows(b, Cr, edges, seed, ells):
+    """Dense restatement of the MC-Fisher Q maps (pspec.py:439-488) for trivial S^-1 and a real-space
+    filter matrix Cr in place of the pointing (S^-1 P -> FFT Cr IFFT): full-grid DFT matrices and
+    addition-theorem kernels.  Returns the Q maps in row order and the A^-1 a maps."""
+    n = b.gridsize
+    N = int(np.prod(n))
+    ph, kv, cosang, fr = _dense_ops(b)
+    ah = b.generate_data(seed=seed + int(1e7), output_type='fourier')
+    a = np.fft.fftn(b.to_real(ah)).reshape(N)
+    ar = np.fft.ifftn(a.reshape(*n)).real.reshape(N)
+    modk = np.linalg.norm(kv, axis=1)
+    SinvP = ph @ (Cr @ ar)
+    SinvP_r = (ph.conj().T @ SinvP).real / N
+    rows = []
+    for ell in ells:
+        Lk = O.legendre(ell, cosang)
+        for kb in range(len(edges) - 1):
+            th = (modk >= edges[kb]) & (modk < edges[kb + 1])
+            if ell == 0:
+                rows.append(ph @ (Cr @ ((ph.conj().T @ (th * SinvP)).real / N)))
+            else:
+                leg = (ph * Lk) @ SinvP_r
+                t1 = ph @ (Cr @ ((ph.conj().T @ (th * leg)).real / N))
+                real_map = ((ph.conj() * Lk).T @ (th * SinvP)).real / N
+                rows.append(0.5 * t1 + 0.5 * (ph @ (Cr @ real_map)))
+    Ainv = a.copy()  # unit fiducial spectrum
+    Ainv_r = (ph.conj().T @ Ainv).real / N
+    return rows, Ainv, Ainv_r, modk, ph, cosang
+
+
+def test_derivative_coefficient_known_answers():
+    """The wide-angle k dP/dk weights (pspec.py:369-379) on hand-evaluated edges, and -1 in the
+    interior of a linear binning (the column sum of a central difference of k P(k))."""
+    np.testing.assert_allclose(O.derivative_coefficient([1., 2., 3., 4., 5.]), [-2., -0.5, 6., -3.5])
+    kf = O.derivative_coefficient(0.01 + 0.005 * np.arange(12))
+    np.testing.assert_allclose(kf[2:-3], -1., rtol=1e-12)
+
+
+def test_apply_xi_local_dense():
+    """apply_xi (base.py:413-437), local sightline with P_0 and P_2: against the dense operator
+    xi(x, y) = (1/N) sum_k e^{ik(x-y)} [P_0(k) + P_2(k) (L_2(k.x) + L_2(k.y)) / 2]; both output types,
+    and the operator is symmetric."""
+    n = 8
+    N = n ** 3
+    b = O.PolyBin3D(200., [n, n, n], boxcenter=[300., 140., 90.], sightline='local')
+    Yr = O.real_harmonics(np.asarray(b.r_grids), 2, False)
+    Yf = O.real_harmonics(np.asarray(b.k_grids), 2, False)
+    kk = np.linspace(0.5 * b.kF[0], 0.85 * b.kNy[0], 60)  # vanishes at the Nyquist modes and k = 0
+    P0f = lambda k: 1. + 30. * k  # noqa: E731
+    P2f = lambda k: 0.7 - 11. * k ** 2  # noqa: E731
+    Pk_grid = {0: np.interp(b.modk_grid, kk, P0f(kk), left=0., right=0.) * (b.modk_grid <= kk[-1]) * (b.modk_grid >= kk[0]),
+               2: np.interp(b.modk_grid, kk, P2f(kk), left=0., right=0.) * (b.modk_grid <= kk[-1]) * (b.modk_grid >= kk[0])}
+    ph, kv, cosang, fr = _dense_ops(b)
+    modk = np.linalg.norm(kv, axis=1)
+    inr = (modk >= kk[0]) & (modk <= kk[-1])
+    P0 = np.where(inr, np.interp(modk, kk, P0f(kk)), 0.)
+    P2 = np.where(inr, np.interp(modk, kk, P2f(kk)), 0.)
+    L2 = O.legendre(2, cosang)
+    Xi = (ph.conj().T @ (P0[:, None] * ph) + 0.5 * ph.conj().T @ (P2[:, None] * (ph * L2))
+          + 0.5 * (ph * L2).conj().T @ (P2[:, None] * ph)) / N
+    assert np.abs(Xi.imag).max() < 1e-9 * np.abs(Xi).max()
+    Xi = Xi.real
+    np.testing.assert_allclose(Xi, Xi.T, atol=1e-10 * np.abs(Xi).max())
+    u = np.random.RandomState(4).randn(n, n, n)
+    uf = b.to_fourier(u)
+    out_r = b.apply_xi(uf, u, Yr, Yf, Pk_grid, 'real')
+    np.testing.assert_allclose(out_r.reshape(N), Xi @ u.reshape(N), atol=1e-10 * np.abs(out_r).max())
+    out_f = b.apply_xi(uf, u, Yr, Yf, Pk_grid, 'fourier')
+    np.testing.assert_allclose(out_f, b.to_fourier(out_r), atol=1e-10 * np.abs(out_f).max())
+
+
+def test_pk_covariance_global_closed_form():
+    """Covariance contribution, global sightline, constant mask m, trivial S^-1 (pspec.py:278-289): the
+    filter is c(k) = m^2 P(k, mu) + m, so per seed
+    C[(l,b),(l',b')] = delta_bb' V/(2N^2) sum_{k in b} degk c^2 invPk0 |a_k|^2 L_l L_l'."""
+    n = 16
+    Pk = [np.linspace(0, 2, 400), 1e3 * (1 + np.linspace(0, 2, 400))]
+    b = O.PolyBin3D(600., [n, n, n], Pk=Pk)
+    edges = np.linspace(0.012, 0.07, 5)
+    m = 0.7
+    ps = O.PSpec(b, edges, lmax=4, mask=m * np.ones((n, n, n)))
+    kc = np.linspace(0., 0.08, 50)
+    Pc = [kc, 2e3 / (1 + 50 * kc), 300. - 2e3 * kc, 40. + kc]
+    C = ps.compute_covariance_contribution(3, Pc)
+    a = b.generate_data(seed=3 + int(1e7), output_type='fourier')
+    k, mu = b.modk_grid, b.muk_grid
+    Pkmu = (np.interp(k, kc, Pc[1], right=0.) + O.legendre(2, mu) * np.interp(k, kc, Pc[2], right=0.)
+            + O.legendre(4, mu) * np.interp(k, kc, Pc[3], right=0.))
+    c = m ** 2 * Pkmu + m
+    ref = np.zeros_like(C)
+    Nk = ps.Nk
+    idx = O.bin_index(k, edges)
+    for l1 in range(3):
+        for l2 in range(3):
+            for kb in range(Nk):
+                sel = idx == kb
+                w = b.degeneracy_factor[sel] * b.invPk0_grid[sel] * np.abs(a[sel]) ** 2 * c[sel] ** 2
+                w = w * O.legendre(2 * l1, mu[sel]) * O.legendre(2 * l2, mu[sel])
+                ref[l1 * Nk + kb, l2 * Nk + kb] = b.volume / (2 * n ** 6) * w.sum()
+    np.testing.assert_allclose(C, ref, atol=1e-12 * np.abs(ref).max())
+
+
+def test_pk_covariance_local_masked_dense():
+    """Covariance contribution, local sightline, varying mask with a shot-noise mask (pspec.py:278-325):
+    the MC-Fisher rows with the real-space filter P xi P^T + diag(mask_shot), xi the dense local
+    correlation operator."""
+    n = 8
+    N = n ** 3
+    b = O.PolyBin3D(200., [n, n, n], boxcenter=[300., 140., 90.], sightline='local')
+    rng = np.random.RandomState(6)
+    mask = 0.5 + rng.rand(n, n, n)
+    mshot = 0.2 + rng.rand(n, n, n)
+    edges = np.array([1.0, 2.0, 3.0]) * b.kF[0] + 1e-9
+    ps = O.PSpec(b, edges, lmax=2, mask=mask, mask_shot=mshot)
+    kk = np.linspace(0.5 * b.kF[0], 0.85 * b.kNy[0], 60)
+    Pc = [kk, 1. + 30. * kk, 0.7 - 11. * kk ** 2]
+    C = ps.compute_covariance_contribution(2, Pc)
+    ph, kv, cosang, fr = _dense_ops(b)
+    modk = np.linalg.norm(kv, axis=1)
+    inr = (modk >= kk[0]) & (modk <= kk[-1])
+    P0 = np.where(inr, np.interp(modk, kk, Pc[1]), 0.)
+    P2 = np.where(inr, np.interp(modk, kk, Pc[2]), 0.)
+    L2 = O.legendre(2, cosang)
+    Xi = ((ph.conj().T @ (P0[:, None] * ph) + 0.5 * ph.conj().T @ (P2[:, None] * (ph * L2))
+           + 0.5 * (ph * L2).conj().T @ (P2[:, None] * ph)) / N).real
+    mk = mask.reshape(N)
+    Cr = mk[:, None] * Xi * mk[None, :] + np.diag(mshot.reshape(N))
+    rows, Ainv, Ainv_r, modk, ph, cosang = _dense_q_rows(b, Cr, edges, 2, (0, 2))
+    lefts = [Ainv, (ph * L2) @ Ainv_r]
+    Nk = 2
+    ref = np.zeros((2 * Nk, 2 * Nk))
+    for r, Q in enumerate(rows):
+        for lj in range(2):
+            for kb2 in range(Nk):
+                th2 = (modk >= edges[kb2]) & (modk < edges[kb2 + 1])
+                ref[r, lj * Nk + kb2] = 0.5 * b.volume / N ** 2 * np.sum(th2 * (lefts[lj] * np.conj(Q)).real)
+    np.testing.assert_allclose(C, ref, atol=1e-10 * np.abs(ref).max(), rtol=1e-8)
+
+
+def test_pk_theory_matrix_cross_pins():
+    """Theory binning matrix (pspec.py:165-197, 381-436), local sightline, varying mask: with theory bins
+    equal to the data bins it is the Fisher matrix; with lmax_theory = 4 its l = 4 block is the l = 4
+    column block of the lmax = 4 Fisher matrix's l <= 2 rows (a separate code path); with refined
+    theory bins that extend past the data's k-range, the sums of the refined bins are the coarse ones."""
+    n = 8
+    b = O.PolyBin3D(200., [n, n, n], boxcenter=[300., 140., 90.], sightline='local')
+    mask = 0.5 + np.random.RandomState(6).rand(n, n, n)
+    kF = b.kF[0]
+    edges = np.array([1.0, 2.0, 3.0]) * kF + 1e-9
+    ps2 = O.PSpec(b, edges, lmax=2, mask=mask)
+    ps4 = O.PSpec(b, edges, lmax=4, mask=mask)
+    F2 = ps2.compute_fisher_contribution(5)
+    F4 = ps4.compute_fisher_contribution(5)
+    Ft, Bm = ps2.compute_theory_contribution(5, edges, lmax_theory=4)
+    np.testing.assert_array_equal(Ft, F2)
+    np.testing.assert_allclose(Bm, F4[:4, :], atol=1e-12 * np.abs(F4).max())
+    fine = np.array([1.0, 1.5, 2.0, 2.4, 3.0, 3.5]) * kF + 1e-9
+    _, Bf = ps2.compute_theory_contribution(5, fine, lmax_theory=2)
+    Nf = len(fine) - 1
+    for blk in range(2):
+        coarse = np.stack([Bf[:, blk * Nf:blk * Nf + 2].sum(1), Bf[:, blk * Nf + 2:blk * Nf + 4].sum(1)], 1)
+        np.testing.assert_allclose(coarse, F2[:, blk * 2:blk * 2 + 2], atol=1e-12 * np.abs(F2).max())
+    # the bin past the data's k range sees Q maps spread by the mask
+    assert np.abs(Bf[:, Nf - 1]).max() > 1e-3 * np.abs(F2).max()
+
+
+def test_pk_theory_wideangle_dense():
+    """Wide-angle theory matrix (lmax_theory = 4, pspec.py:359-367, 414-436): the odd legs are the
+    harmonic transforms of A^-1 a / |r|, entering the l = 2 and l = 4 blocks with per-bin weights
+    3i/5 (3 + kfac), 3i/5 (2 - kfac) and 10i/9 (5 + kfac); dense restatement."""
+    n = 8
+    N = n ** 3
+    b = O.PolyBin3D(200., [n, n, n], boxcenter=[300., 140., 90.], sightline='local')
+    mask = 0.5 + np.random.RandomState(6).rand(n, n, n)
+    kF = b.kF[0]
+    edges = np.array([1.0, 2.0, 3.0]) * kF + 1e-9
+    th_edges = np.array([1.0, 1.6, 2.2, 3.1]) * kF + 1e-9
+    ps = O.PSpec(b, edges, lmax=2, mask=mask)
+    F, Bm = ps.compute_theory_contribution(4, th_edges, lmax_theory=4, include_wideangle=True)
+    rows, Ainv, Ainv_r, modk, ph, cosang = _dense_q_rows(b, np.diag(mask.reshape(N)), edges, 4, (0, 2))
+    xs = np.stack([g.reshape(N) for g in b.r_grids], -1)
+    inv_r = 1. / np.linalg.norm(xs, axis=1)
+    leg = {l: (ph * O.legendre(l, cosang)) @ (Ainv_r * (inv_r if l % 2 else 1.)) for l in range(1, 5)}
+    kf = O.derivative_coefficient(th_edges)
+    Nt = len(th_edges) - 1
+    s = 0.5 * b.volume / N ** 2
+    ref = np.zeros_like(Bm)
+    for r, Q in enumerate(rows):
+        for kb in range(Nt):
+            th = (modk >= th_edges[kb]) & (modk < th_edges[kb + 1])
+            X = lambda y: np.sum(th * y * np.conj(Q))  # noqa: E731
+            ref[r, kb] = s * X(Ainv).real
+            ref[r, Nt + kb] = s * (X(leg[2]).real + (3j / 5. * (3 + kf[kb]) * X(leg[1])).real
+                                   + (3j / 5. * (2 - kf[kb]) * X(leg[3])).real)
+            ref[r, 2 * Nt + kb] = s * (X(leg[4]).real + (10j / 9. * (5 + kf[kb]) * X(leg[3])).real)
+    np.testing.assert_allclose(Bm, ref, atol=1e-10 * np.abs(ref).max(), rtol=1e-8)

@@ -50,6 +50,9 @@ def test_integrals_match_reference(setup):
     arr = np.stack([s1, s2, s1 - s2])
     np.testing.assert_allclose(b.integrator.integrate_row(s2, arr, 0.03, 0.05), I.integrate_row(s2, arr, 0.03, 0.05),
                                rtol=1e-12)
+    coeff = rng.randn(3) + 1j * rng.randn(3)
+    np.testing.assert_allclose(b.integrator.cross_integrate_wcoeff(coeff, s1, s2, edges),
+                               I.cross_integrate_wcoeff(coeff, s1, s2, edges), rtol=1e-12, atol=1e-12)
 
 
 def test_maps_match_reference(setup):
