@@ -538,6 +538,58 @@ class PolyBin3D:
                      rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=m > 0))
         return out
 
+    def pk_grid(self, Pk_cov):
+        """The gridded theory spectrum of apply_xi (pspec.py:237-251) as FP32 device half grids:
+        global {0: P_0 + L_2(mu) P_2 + L_4(mu) P_4}, local {0: P_0, 2: P_2, 4: P_4} (columns present
+        in Pk_cov = [k, P_0 (, P_2, P_4)]), interpolated in FP64 with zero outside the table."""
+        Pk_cov = np.asarray(Pk_cov, dtype=np.float64)
+        modk = self.modk_grid
+        interp = lambda col: scipy.interpolate.interp1d(Pk_cov[0], Pk_cov[col], bounds_error=False,  # noqa: E731
+                                                        fill_value=0.)(modk)
+        if self.sightline == 'global':
+            P = interp(1)
+            mu = self.muk_grid
+            if len(Pk_cov) > 2:
+                P = P + 0.5 * (3 * mu ** 2 - 1) * interp(2)
+            if len(Pk_cov) > 3:
+                P = P + (3 - 30 * mu ** 2 + 35 * mu ** 4) / 8. * interp(3)
+            grids = {0: P}
+        else:
+            grids = {0: interp(1)}
+            for col, l in ((2, 2), (3, 4)):
+                if len(Pk_cov) > col:
+                    grids[l] = interp(col)
+        return {l: torch.from_numpy(g.astype(np.float32)).to(self.device) for l, g in grids.items()}
+
+    def apply_xi(self, input_map_fourier, input_map_real=None, Ylm_real=None, Ylm_fourier=None, Pk_grid={},
+                 output_type='real'):
+        """xi(x, y) applied symmetrically for a gridded spectrum (base.py:413-437), Pk_grid from pk_grid:
+        P_0 a + sum_l [P_l sum_m Y_lm(k) FFT[Y_lm a_real] + sum_m Y_lm(x) IFFT[Y_lm(k) P_l a]] / 2 on
+        the local sightline, P(k, mu) a on the global one.  The weights ride in the fused transforms'
+        mode ops.  Without input_map_real the local multipoles use the C2R of the input (the reference
+        requires it)."""
+        a = self.as_fourier(input_map_fourier)
+        P = self.kmap(a, C.kop(C.KOP_WEIGHT, w=Pk_grid[0]))
+        ells = [l for l in Pk_grid if l != 0]
+        if self.sightline == 'global' or not ells:
+            return P if output_type == 'fourier' else self.c2r(P)
+        ar = self.c2r(a) if input_map_real is None else self.as_real(input_map_real)
+        P_real = torch.empty(tuple(int(v) for v in self.gridsize), dtype=torch.float32, device=self.device)
+        h = torch.empty(self.half_shape, dtype=torch.complex64, device=self.device)
+        first = True
+        for l in ells:
+            self.apply_fourier_harmonics(ar, l, l, out=h)
+            self.kmap(h, C.kop(C.KOP_WEIGHT, w=Pk_grid[l], scale=0.5, accumulate=1), out=P)
+            for m in range(2 * l + 1):
+                self.c2r(a, out=P_real, kop=C.kop(C.KOP_YLM, ell=l, m=m, w=Pk_grid[l], x_batch_stride=0),
+                         rop=C.rop(C.ROP_YLM, ell=l, m=m, accumulate=not first, scale=0.5))
+                first = False
+        if output_type == 'fourier':
+            return self.r2c(P_real, out=P, kop=C.kop(C.KOP_PLAIN, accumulate=1))
+        out = self.c2r(P)
+        out += P_real
+        return out
+
     # ------------------------------------------------------------------ reductions
     def bin_reduce(self, x, edges, ells, y=None, ells2=None, real=False, y_ostride=None, y_qstride=0,
                    nq=None):

@@ -19,6 +19,27 @@ from .base import harmonics_tags
 from .constraints import Pointing
 
 
+def derivative_coefficient(edges):
+    """Per-bin weights of the wide-angle k dP/dk term on the theory bin edges e_0..e_n (pspec.py:369-379):
+    w_j = e_{j-1} / (e_j - e_{j-2}) [2 <= j]  -  e_{j+1} / (e_{j+2} - e_j) [j <= n-3]
+          + e_0 / (e_1 - e_0) ([j = 1] - [j = 0])  -  e_n / (e_n - e_{n-1}) ([j = n-1] - [j = n-2])."""
+    e = np.asarray(edges, dtype=np.float64)
+    n = len(e) - 1
+    j = np.arange(n)
+    w = np.zeros(n)
+    lo = j >= 2
+    w[lo] += e[j[lo] - 1] / (e[j[lo]] - e[j[lo] - 2])
+    hi = j <= n - 3
+    w[hi] -= e[j[hi] + 1] / (e[j[hi] + 2] - e[j[hi]])
+    first, last = e[0] / (e[1] - e[0]), e[n] / (e[n] - e[n - 1])
+    w[0] -= first
+    w[n - 1] -= last
+    if n >= 2:
+        w[1] += first
+        w[n - 2] += last
+    return w
+
+
 class PSpec:
     """Power spectrum estimators (PolyBin3D/pspec.py:10-784)."""
 
@@ -261,19 +282,21 @@ class PSpec:
         return res
 
     # ------------------------------------------------------------------ MC Fisher
-    def _q_maps(self, src, li, Q):
+    def _q_maps(self, src, li, Q, kmax_out=None):
         """Q_{l,b} = (S^-1 P)^dag Theta_b L_l S^-1 P a for all bins b of multipole index li, written into
         Q[Nk, H] (pspec.py:444-488) with trivial S^-1, as one fused pointing chain:
           global / l=0 : Q = R2C(P * C2R(Theta_b L_l SinvP))
           local  l>0   : Q = 0.5 R2C(P * [C2R(Theta_b leg) + sum_m Y_m(x) C2R(Theta_b Y_m(k) SinvP)])
         (the reference applies (S^-1 P)^dag to both local half-terms separately; the operator is linear,
-        so one R2C of the summed real map is the same).  Only modes inside the k-bins are written."""
+        so one R2C of the summed real map is the same).  Only modes with |k| below the last k-bin edge
+        (or kmax_out, if larger) are written."""
         b = self.base
         Nk = self.Nk
         bins = self._bins_dev
         SinvP, leg, SinvP_mi = src
         ell = (2 - self.odd_l) * li
         sup = b.support(self.k_bins[-1])
+        sup_out = b.support(max(self.k_bins[-1], kmax_out or 0.))
         common = dict(bin_table=self.table, bins=bins, x_batch_stride=0, support=sup)
         if b.sightline == 'global' or li == 0:
             subs = [(C.kop(C.KOP_FILTER, ell=ell, **common), SinvP, 0, 0, 1.0)]
@@ -282,7 +305,7 @@ class PSpec:
             src_y = SinvP_mi if ell % 2 == 1 else SinvP  # -i on the odd-l Y_lm-weighted maps
             subs += [(C.kop(C.KOP_YLM, ell=ell, m=m, **common), src_y, ell, m, 0.5) for m in range(2 * ell + 1)]
         mask, scale, ic, finish = self._chain_pointing(Nk)
-        b.pointing(subs, Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup), ic=ic)
+        b.pointing(subs, Nk, Q, mask=mask, scale=scale, kop_out=C.kop(C.KOP_PLAIN, support=sup_out), ic=ic)
         finish(Q)
 
     def _chain_pointing(self, batch):
@@ -299,10 +322,13 @@ class PSpec:
         Returns an FP64 numpy array [N_bins, N_bins]; row = Q-map (l, b), as in the reference."""
         return self._fisher_contribution_dev(seed).cpu().numpy()
 
-    def _fisher_contribution_dev(self, seed, a_map_fourier=None):
+    def _fisher_contribution_dev(self, seed, a_map_fourier=None, theory=None):
+        """The MC Fisher contribution of one seed as an FP64 device tensor; with `theory` (from
+        _theory_setup) also the theory binning matrix from the same Q maps."""
         b = self.base
         if self.user_Sinv:
-            return torch.from_numpy(self._fisher_contribution_generic(seed))
+            out = self._fisher_contribution_generic(seed, theory=theory)
+            return tuple(torch.from_numpy(x) for x in out) if theory is not None else torch.from_numpy(out)
         Nk, Nl = self.Nk, self.Nl
         if not hasattr(self, '_bins_dev'):
             self._bins_dev = torch.arange(Nk, dtype=torch.int32, device=b.device)
@@ -320,6 +346,7 @@ class PSpec:
         lefts = torch.empty((Nl,) + b.half_shape, dtype=torch.complex64, device=b.device)
         lefts[0].copy_(Ainv)
         leg = {}
+        Ainv_real = None
         if local:
             SinvP_real = b.c2r(SinvP)
             Ainv_real = b.c2r(Ainv)
@@ -335,11 +362,17 @@ class PSpec:
         Q = torch.empty((Nk,) + b.half_shape, dtype=torch.complex64, device=b.device)
         F = torch.zeros((self.N_bins, self.N_bins), dtype=torch.float64, device=b.device)
         scaling = 0.5 * b.volume / b.gridsize.prod() ** 2
+        if theory is not None:
+            th_lefts = self._theory_lefts(theory, Ainv, Ainv_real)
+            Bm = torch.zeros((self.N_bins, theory['N_bins']), dtype=torch.float64, device=b.device)
+        kmax_out = theory['edges'][-1] if theory is not None else None
         for li in range(Nl):
-            self._q_maps((SinvP, leg, SinvP_mi), li, Q)
+            self._q_maps((SinvP, leg, SinvP_mi), li, Q, kmax_out=kmax_out)
             r = b.bin_reduce(Q, self.k_bins, ells, y=lefts, y_qstride=0, y_ostride=b.H, nq=Nk)  # [Nk, Nl, Nk]
             F[li * Nk:(li + 1) * Nk, :] = r.reshape(Nk, Nl * Nk) * scaling
-        return F
+            if theory is not None:
+                self._theory_add(theory, th_lefts, Q, Bm[li * Nk:(li + 1) * Nk], scaling)
+        return (F, Bm) if theory is not None else F
 
     def _fisher_contributions_dev(self, seeds):
         """Sum over `seeds` of the MC Fisher contributions (pspec.py:215-493 for each seed), FP64 on the
@@ -422,22 +455,45 @@ class PSpec:
                 F[li * Nk:(li + 1) * Nk, :] += red.reshape(Nk, Nl * Nk) * scaling
         return F
 
-    def _fisher_contribution_generic(self, seed):
-        """Reference call sequence with a user-supplied S^-1 (pspec.py:215-493), maps on the GPU."""
+    def _fisher_contribution_generic(self, seed, cov=None, theory=None):
+        """Reference call sequence (pspec.py:215-493) with maps on the GPU, one Q map at a time: for a
+        user-supplied S^-1, for the covariance (cov: the gridded spectrum of base.pk_grid; the filter
+        S^-1 [P xi P^T + N] S^-dag, pspec.py:278-325) and, with `theory`, the theory binning matrix.
+        Returns FP64 numpy arrays: F, or (F, binning matrix)."""
         b = self.base
         Nk, Nl = self.Nk, self.Nl
         F = np.zeros((self.N_bins, self.N_bins))
+        Bm = np.zeros((self.N_bins, theory['N_bins'])) if theory is not None else None
         a = b.generate_data(seed=seed + int(1e7), output_type='fourier')
         if not self.const_mask:
             a_real = b.c2r(a)
+        Yr, Yf = self.Ylm_real, self.Ylm_fourier
+
+        def cov_filter(m, it):
+            # S^-1 [P xi P^T + N] S^-dag (pspec.py:278-289, 312-325)
+            if self.const_mask:
+                Sid = b.as_fourier(self.applySinv_transpose(m, input_type=it, output_type='fourier'))
+                CSid = b.apply_xi(Sid, None, Yr, Yf, cov, 'fourier')
+                CSid.mul_(self.mask_mean ** 2).add_(Sid, alpha=self.mask_mean)
+                return b.as_fourier(self.applySinv(CSid, input_type='fourier', output_type='fourier'))
+            Sid = b.as_real(self.applySinv_transpose(m, input_type=it, output_type='real'))
+            PdSid_real = self._P.apply(Sid, transpose=True)
+            CPdSid = b.apply_xi(b.r2c(PdSid_real), PdSid_real, Yr, Yf, cov, 'real')
+            Cov = self._P.apply(CPdSid)
+            Cov.addcmul_(Sid, self.mask_shot)
+            return b.as_fourier(self.applySinv(Cov, input_type='real', output_type='fourier'))
 
         def apply_filter(m):
+            if cov is not None:
+                return cov_filter(m, 'fourier' if self.const_mask else 'real')
             if self.const_mask:
                 return self.mask_mean * b.as_fourier(self.applySinv(m, input_type='fourier', output_type='fourier'))
             return b.as_fourier(self.applySinv(self._P.apply(m), input_type='real', output_type='fourier'))
 
         def apply_filter_dagger(m, real=False):
             it = 'real' if real else 'fourier'
+            if cov is not None:
+                return cov_filter(m, it)
             if self.const_mask:
                 return self.mask_mean * b.as_fourier(self.applySinv_transpose(m, input_type=it, output_type='fourier'))
             r = b.as_real(self.applySinv_transpose(m, input_type=it, output_type='real'))
@@ -449,6 +505,7 @@ class PSpec:
         local = b.sightline == 'local' and self.lmax > 0
         lefts = torch.empty((Nl,) + b.half_shape, dtype=torch.complex64, device=b.device)
         lefts[0].copy_(Ainv)
+        Ainv_real = None
         if local:
             SinvP_real = b.c2r(SinvP)
             Ainv_real = b.c2r(Ainv)
@@ -457,12 +514,18 @@ class PSpec:
         else:
             for li in range(1, Nl):
                 lefts[li].copy_(Ainv)
+        if theory is not None:
+            th_lefts = self._theory_lefts(theory, Ainv, Ainv_real)
         ells = [0] * Nl if local else [2 * li for li in range(Nl)]
         scaling = 0.5 * b.volume / b.gridsize.prod() ** 2
 
         def add_row(row, Qm):
             r = b.bin_reduce(Qm, self.k_bins, ells, y=lefts, y_qstride=0, y_ostride=b.H)[0]
             F[row] = r.reshape(-1).cpu().numpy() * scaling
+            if theory is not None:
+                acc = torch.zeros((1, theory['N_bins']), dtype=torch.float64, device=b.device)
+                self._theory_add(theory, th_lefts, Qm, acc, scaling)
+                Bm[row] = acc[0].cpu().numpy()
 
         for li in range(Nl):
             ell = (2 - self.odd_l) * li
@@ -483,7 +546,118 @@ class PSpec:
                               rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=1))
                     Qm = Qm + 0.5 * apply_filter_dagger(real_map, real=True)
                     add_row(li * Nk + ki, Qm)
-        return F
+        return (F, Bm) if theory is not None else F
+
+    # ------------------------------------------------------------------ covariance and theory matrices
+    def compute_covariance_contribution(self, seed, Pk_cov, verb=False):
+        """One realization of the Gaussian covariance of the estimator for the theory spectrum
+        Pk_cov = [k, P_0 (, P_2, P_4)] without shot noise (pspec.py:149-163): the MC-Fisher rows with
+        S^-1 [P xi P^T + N] S^-dag as the filter, xi = base.apply_xi and N = mask_shot."""
+        if not self.const_mask:
+            assert hasattr(self, 'mask_shot'), "Must supply mask_shot to compute shot-noise contribution"
+        assert len(Pk_cov) != 0, "Must provide fiducial power spectrum if computing covariance!"
+        Pk_cov = np.asarray(Pk_cov, dtype=np.float64)
+        assert Pk_cov.ndim == 2, "Pk should contain k and P_0, (and optionally P_2, P_4) columns"
+        assert len(Pk_cov) in [2, 3, 4], "Pk should contain k and P_0, (and optionally P_2, P_4) columns"
+        assert (len(Pk_cov) - 2) * 2 <= self.lmax, "Can't use higher multipoles than lmax"
+        key = Pk_cov.tobytes()
+        if getattr(self, '_cov_key', None) != key:
+            self._cov_grid, self._cov_key = self.base.pk_grid(Pk_cov), key
+        return self._fisher_contribution_generic(seed, cov=self._cov_grid)
+
+    def compute_theory_contribution(self, seed, k_bins_theory, lmax_theory=None, include_wideangle=False,
+                                    verb=False):
+        """(Fisher contribution, rectangular theory binning matrix [N_bins, Nl_theory * Nk_theory]) from
+        one realization (pspec.py:165-197), optionally with the wide-angle (odd-l) terms.  Both come from
+        the same Q maps; the binning matrix is one more bin reduction per multipole against the theory
+        legs (pspec.py:381-436)."""
+        b = self.base
+        k_bins_theory = np.asarray(k_bins_theory, dtype=np.float64)
+        assert np.max(k_bins_theory) < np.min(b.kNy), "k_max must be less than k_Nyquist!"
+        assert np.min(k_bins_theory) >= np.max(b.kF), "k_min must be at least the k_fundamental!"
+        assert np.max(k_bins_theory) <= b.Pfid[0][-1], "Fiducial power spectrum should extend up to k_max!"
+        assert np.min(k_bins_theory) >= b.Pfid[0][0], "Fiducial power spectrum should extend down to k_min!"
+        if lmax_theory is None:
+            lmax_theory = self.lmax
+        # the data's multipoles are looked up among the theory legs (the reference fails otherwise)
+        assert lmax_theory >= self.lmax, "Can't use larger lmax for data than for theory!"
+        if include_wideangle:
+            assert b.sightline != 'global', "Wide-angle effects require local lines-of-sight!"
+            assert not self.odd_l, "Cannot include wide-angle effects for odd l!"
+            assert lmax_theory > 1, "There are no wide-angle effects for the monopole!"
+            assert lmax_theory <= 4, "Higher-order effects not yet implemented!"
+        elif not self.odd_l:
+            assert lmax_theory % 2 == 0, "Must use even lmax if not including wide-angle effects!"
+        th = self._theory_setup(k_bins_theory, lmax_theory, include_wideangle)
+        F, Bm = self._fisher_contribution_dev(seed, theory=th)
+        return F.cpu().numpy(), Bm.cpu().numpy()
+
+    def _theory_setup(self, edges, lmax_theory, include_wideangle):
+        """The binning matrix as a list of terms (pspec.py:221-231, 354-436): each is a left map, its
+        Legendre order in the bin reduction, the column block it adds to and an optional per-bin weight.
+        Left maps: 'ainv' (A^-1 a), ('harm', l, phase) = phase sum_m FFT[Y_lm A^-1 a] Y_lm(k), and
+        ('harm_r', l, phase) the same of A^-1 a / |r|.  The reference's phases: i Q on the odd-l
+        cross-integrals (a -i on the leg) and i kfac-weights on the wide-angle legs (an i on the leg)."""
+        b = self.base
+        odd_th = True if include_wideangle else self.odd_l
+        Nl_th = lmax_theory + 1 if (odd_th and not include_wideangle) else lmax_theory // 2 + 1
+        Nk_th = len(edges) - 1
+        terms = [('ainv', 0, 0, None)]
+        if b.sightline == 'global':
+            terms += [('ainv', 2 * li, li, None) for li in range(1, Nl_th)]
+        elif not include_wideangle:
+            lf = 2 - odd_th
+            terms += [(('harm', lf * li, -1j if (lf * li) % 2 else 1.), 0, li, None) for li in range(1, Nl_th)]
+        else:
+            kfac = derivative_coefficient(edges)
+            terms += [(('harm', l, 1.), 0, l // 2, None) for l in range(2, lmax_theory + 1, 2)]
+            terms += [(('harm_r', 1, 1j), 0, 1, 3. / 5. * (3 + kfac))]
+            if lmax_theory >= 3:
+                terms += [(('harm_r', 3, 1j), 0, 1, 3. / 5. * (2 - kfac))]
+            if lmax_theory >= 4:
+                terms += [(('harm_r', 3, 1j), 0, 2, 10. / 9. * (5 + kfac))]
+        dev = lambda c: None if c is None else torch.as_tensor(c, dtype=torch.float64, device=b.device)  # noqa: E731
+        return dict(edges=edges, Nk=Nk_th, Nl=Nl_th, N_bins=Nl_th * Nk_th,
+                    terms=[(src, lg, blk, dev(c)) for src, lg, blk, c in terms])
+
+    def _theory_lefts(self, th, Ainv, Ainv_real):
+        """The left maps of the theory terms, stacked [T, H] (equal sources share one transform)."""
+        b = self.base
+        T = len(th['terms'])
+        L = torch.empty((T,) + b.half_shape, dtype=torch.complex64, device=b.device)
+        done = {}
+        for t, (src, _, _, _) in enumerate(th['terms']):
+            if src in done:
+                L[t].copy_(L[done[src]])
+                continue
+            done[src] = t
+            if src == 'ainv':
+                L[t].copy_(Ainv)
+                continue
+            kind, ell, phase = src
+            if Ainv_real is None:
+                Ainv_real = b.c2r(Ainv)
+            r = Ainv_real
+            if kind == 'harm_r':
+                if not hasattr(self, '_inv_r'):
+                    rg = b.r_grids
+                    self._inv_r = b.as_real(1. / np.sqrt(rg[0] ** 2. + rg[1] ** 2. + rg[2] ** 2.))
+                r = Ainv_real * self._inv_r
+            b.apply_fourier_harmonics(r, ell, ell, out=L[t])
+            if phase != 1.:
+                L[t].mul_(phase)
+        return L
+
+    def _theory_add(self, th, lefts, Q, out, scaling):
+        """out[q, blk * Nk_th + b] += scaling * w_b * sum_{k in b} degk Re[Q_q conj(left)] L_l(mu) over the
+        terms, for the nq = out.shape[0] Q maps in Q."""
+        b = self.base
+        nq, Nk_th = out.shape[0], th['Nk']
+        red = b.bin_reduce(Q, th['edges'], [lg for _, lg, _, _ in th['terms']], y=lefts, y_qstride=0,
+                           y_ostride=b.H, nq=nq)  # [nq, T, Nk_th]
+        for t, (_, _, blk, wt) in enumerate(th['terms']):
+            r = red[:, t, :] * scaling
+            out[:, blk * Nk_th:(blk + 1) * Nk_th] += r * wt if wt is not None else r
 
     def compute_fisher(self, N_it, verb=False, batch=1):
         """Mean of N_it realizations, accumulated in FP64 on the GPU, then inverted (pspec.py:553-569).

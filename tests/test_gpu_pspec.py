@@ -96,7 +96,7 @@ def shell_mask(n, L, center, seed=1, r_lo=300., r_hi=480., nholes=200, smooth=5.
     k = [np.fft.fftfreq(n, L / n) * 2 * np.pi] * 2 + [np.fft.rfftfreq(n, L / n) * 2 * np.pi]
     K = np.meshgrid(*k, indexing='ij')
     g = np.exp(-0.5 * smooth ** 2 * (K[0] ** 2 + K[1] ** 2 + K[2] ** 2))
-    return np.fft.irfftn(np.fft.rfftn(m) * g, s=(n, n, n))
+    return np.fft.irfftn(np.fft.rfftn(m) * g, s=(n, n, n), axes=(0, 1, 2))
 
 
 @pytest.mark.parametrize("case", ["global_const", "local_mask"])
@@ -207,3 +207,108 @@ def test_pk_odd_l(hip):
     assert sorted(got) == sorted(ref) == ['p0', 'p1', 'p2', 'p3']
     for k in ref:
         close(got[k], ref[k], tol=2e-5, scale=np.abs(ref['p0']))
+
+
+def _diag_scale(F):
+    d = np.abs(np.diag(F))
+    return np.sqrt(np.outer(d, d))
+
+
+@pytest.mark.parametrize("case", ["global_const_l4", "local_mask_fine", "local_wideangle", "local_odd_l",
+                                  "local_user_sinv"])
+def test_pk_theory_contribution(hip, case):
+    """compute_theory_contribution (pspec.py:165-197, 381-436): the Fisher contribution and the
+    rectangular theory binning matrix against the oracle per seed; theory bins finer than the data's
+    and extending past them (the Q maps are written up to the theory k_max), higher theory lmax,
+    the wide-angle odd legs with their per-bin weights, odd_l phases, and a user S^-1 (the
+    one-Q-map-at-a-time path)."""
+    import polybin3d_amd as pb
+    import torch
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    local = dict(boxcenter=[10., 10., 10.], sightline='local')
+    kw, mask, lmax, opts, th = local, shell_mask(n, L, [10., 10., 10.]), 2, {}, {}
+    tb = np.linspace(0.01, 0.1, 19)
+    if case == "global_const_l4":
+        kw, mask, th = {}, 0.8 * np.ones((n, n, n)), dict(lmax_theory=4)
+    elif case == "local_mask_fine":
+        th = dict(lmax_theory=4)
+    elif case == "local_wideangle":
+        th = dict(lmax_theory=4, include_wideangle=True)
+    elif case == "local_odd_l":
+        lmax, opts, th = 1, dict(odd_l=True), dict(lmax_theory=3)
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    if case == "local_user_sinv":
+        wk = 1. / (1. + (o.modk_grid / 0.03) ** 2)
+        wk_d = torch.from_numpy(wk.astype(np.float32)).cuda()
+
+        def sinv_o(m, input_type='real', output_type='fourier'):
+            f = (o.to_fourier(m) if input_type == 'real' else m) * wk
+            return f if output_type == 'fourier' else o.to_real(f)
+
+        def sinv_g(m, input_type='real', output_type='fourier'):
+            f = (b.to_fourier(m) if input_type == 'real' else m) * wk_d
+            return f if output_type == 'fourier' else b.to_real(f)
+        ps = pb.PSpec(b, edges, lmax=lmax, mask=mask, applySinv=sinv_g)
+        po = O.PSpec(o, edges, lmax=lmax, mask=mask, applySinv=sinv_o)
+        th = dict(lmax_theory=2)
+    else:
+        ps = pb.PSpec(b, edges, lmax=lmax, mask=mask, **opts)
+        po = O.PSpec(o, edges, lmax=lmax, mask=mask, **opts)
+    for seed in (0, 2):
+        F1, B1 = ps.compute_theory_contribution(seed, tb, **th)
+        F2, B2 = po.compute_theory_contribution(seed, tb, **th)
+        assert B1.shape == B2.shape
+        sc = _diag_scale(F2)
+        close(F1, F2, scale=sc)
+        # per row: the row's diagonal Fisher scale (binning entries of a row share its Q map)
+        close(B1, B2, scale=np.abs(np.diag(F2))[:, None] * np.ones_like(B2))
+
+
+@pytest.mark.parametrize("case", ["global_const_l4", "local_mask_shot", "local_gic"])
+def test_pk_covariance_contribution(hip, case):
+    """compute_covariance_contribution (pspec.py:149-163, 278-325): the MC-Fisher rows with the filter
+    S^-1 [P xi P^T + N] S^-dag, xi = apply_xi of the theory spectrum (global P(k, mu); local P_0 and
+    P_2 through the fused harmonic transforms), N = mask_shot; per seed against the oracle."""
+    import polybin3d_amd as pb
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    kc = np.linspace(0, 1.5, 300)
+    Pc = [kc, 2e4 / (1 + (kc / 0.04) ** 2), 5e3 / (1 + (kc / 0.05) ** 2), 1e3 / (1 + (kc / 0.05) ** 2)]
+    ic = {}
+    if case == "global_const_l4":
+        kw, mask, lmax = {}, 0.8 * np.ones((n, n, n)), 4
+    else:
+        kw, mask, lmax, Pc = dict(boxcenter=[10., 10., 10.], sightline='local'), None, 2, Pc[:3]
+        mask = shell_mask(n, L, [10., 10., 10.])
+        if case == "local_gic":
+            ic = dict(add_GIC=True, mask_IC=mask * 1.1)
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    ps = pb.PSpec(b, edges, lmax=lmax, mask=mask, mask_shot=0.5 * mask, **ic)
+    po = O.PSpec(o, edges, lmax=lmax, mask=mask, mask_shot=0.5 * mask, **ic)
+    for seed in (0, 1):
+        C1 = ps.compute_covariance_contribution(seed, Pc)
+        C2 = po.compute_covariance_contribution(seed, Pc)
+        close(C1, C2, scale=_diag_scale(C2))
+
+
+def test_apply_xi_matches_oracle(hip):
+    """base.apply_xi on device maps (base.py:413-437), local sightline with P_0, P_2, P_4, both output
+    types, against the oracle."""
+    import polybin3d_amd as pb
+    n = 32
+    kw = dict(boxcenter=[10., 10., 10.], sightline='local')
+    b, o = pb.PolyBin3D(L, [n] * 3, **kw), O.PolyBin3D(L, [n] * 3, **kw)
+    kc = np.linspace(0, 1.5, 300)
+    Pc = [kc, 2e4 / (1 + (kc / 0.04) ** 2), 5e3 / (1 + (kc / 0.05) ** 2), 1e3 / (1 + (kc / 0.05) ** 2)]
+    po = O.PSpec(o, np.linspace(0.01, 0.09, 9), lmax=4)
+    grid_o = po._cov_pk_grid(np.asarray(Pc))
+    grid_g = b.pk_grid(Pc)
+    u = np.random.RandomState(2).randn(n, n, n)
+    uf = o.to_fourier(u)
+    for ot in ('real', 'fourier'):
+        ref = o.apply_xi(uf, u, po.Ylm_real, po.Ylm_fourier, grid_o, ot)
+        got = b.apply_xi(uf.astype(np.complex64), u.astype(np.float32), None, None, grid_g, ot).cpu().numpy()
+        close(got, ref, scale=np.abs(ref).max())
