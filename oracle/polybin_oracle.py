@@ -536,6 +536,42 @@ def derivative_coefficient(k_bins_theory):
     return kfac
 
 
+def covariance_filter(spec, m, input_type, Pk_grid, noise_mask, Ylm_real, Ylm_fourier):
+    """S^-1 [P xi P^T + N] S^-dag of a map, the covariance counterpart of S^-1 P (pspec.py:278-289,
+    312-325; bspec.py:507-519): mask_mean^2 xi + mask_mean for a constant mask (Fourier output), else
+    the pointing matrix around xi plus the noise mask N, with xi = apply_xi."""
+    b = spec.base
+    if spec.const_mask:
+        Sid = spec.applySinv_transpose(m, input_type=input_type, output_type='fourier')
+        CSid = spec.mask_mean ** 2 * b.apply_xi(Sid, None, Ylm_real, Ylm_fourier, Pk_grid, 'fourier') \
+            + spec.mask_mean * Sid
+        return spec.applySinv(CSid, input_type='fourier', output_type='fourier')
+    Sid = spec.applySinv_transpose(m, input_type=input_type, output_type='real')
+    PdSid_real = spec.apply_pointing(Sid, transpose=True)
+    CPdSid = b.apply_xi(b.to_fourier(PdSid_real), PdSid_real, Ylm_real, Ylm_fourier, Pk_grid, 'real')
+    Cov = spec.apply_pointing(CPdSid) + Sid * noise_mask
+    return spec.applySinv(Cov, input_type='real', output_type='fourier')
+
+
+def cov_pk_grid(base, Pk_cov):
+    """Gridded theory spectrum of the covariance (pspec.py:237-251; bspec.py:484-497): global
+    {0: P_0 + L_2(mu) P_2 + L_4(mu) P_4}, local {0: P_0, 2: P_2, 4: P_4}, zero outside the table."""
+    b = base
+    interp = lambda col: scipy.interpolate.interp1d(Pk_cov[0], Pk_cov[col], bounds_error=False,  # noqa: E731
+                                                    fill_value=0.)(b.modk_grid)
+    if b.sightline == 'global':
+        P = interp(1)
+        for col, l in ((2, 2), (3, 4)):
+            if len(Pk_cov) > col:
+                P = P + legendre(l, b.muk_grid) * interp(col)
+        return {0: P}
+    out = {0: interp(1)}
+    for col, l in ((2, 2), (3, 4)):
+        if len(Pk_cov) > col:
+            out[l] = interp(col)
+    return out
+
+
 class PSpec:
     """FP64 restatement of PolyBin3D.PSpec (PolyBin3D/pspec.py:10-784)."""
 
@@ -699,21 +735,8 @@ class PSpec:
                                     lmax_theory=lmax_theory, include_wideangle=include_wideangle)
 
     def _cov_pk_grid(self, Pk_cov):
-        """Gridded theory spectrum of the covariance (pspec.py:237-251)."""
-        b = self.base
-        interp = lambda col: scipy.interpolate.interp1d(Pk_cov[0], Pk_cov[col], bounds_error=False,  # noqa: E731
-                                                        fill_value=0.)(b.modk_grid)
-        if b.sightline == 'global':
-            P = interp(1)
-            for col, l in ((2, 2), (3, 4)):
-                if len(Pk_cov) > col:
-                    P = P + legendre(l, b.muk_grid) * interp(col)
-            return {0: P}
-        out = {0: interp(1)}
-        for col, l in ((2, 2), (3, 4)):
-            if len(Pk_cov) > col:
-                out[l] = interp(col)
-        return out
+        """pspec.py:237-251."""
+        return cov_pk_grid(self.base, Pk_cov)
 
     def _compute_fisher(self, seed, compute_cov=False, Pk_cov=(), compute_theory=False, k_bins_theory=(),
                         lmax_theory=None, include_wideangle=False):
@@ -744,16 +767,7 @@ class PSpec:
             a_real = b.to_real(a)                                                   # 268
 
         def cov_filter(m, it):                                                      # 278-289, 312-325
-            if self.const_mask:
-                Sid = self.applySinv_transpose(m, input_type=it, output_type='fourier')
-                CSid = self.mask_mean ** 2 * b.apply_xi(Sid, None, Ylm_real, Ylm_fourier, Pk_grid, 'fourier') \
-                    + self.mask_mean * Sid
-                return self.applySinv(CSid, input_type='fourier', output_type='fourier')
-            Sid = self.applySinv_transpose(m, input_type=it, output_type='real')
-            PdSid_real = self.apply_pointing(Sid, transpose=True)
-            CPdSid = b.apply_xi(b.to_fourier(PdSid_real), PdSid_real, Ylm_real, Ylm_fourier, Pk_grid, 'real')
-            Cov = self.apply_pointing(CPdSid) + Sid * self.mask_shot
-            return self.applySinv(Cov, input_type='real', output_type='fourier')
+            return covariance_filter(self, m, it, Pk_grid, getattr(self, 'mask_shot', None), Ylm_real, Ylm_fourier)
 
         def apply_filter(m):                                                        # 270-289
             if compute_cov:
@@ -1228,7 +1242,33 @@ class BSpec:
 
     # --- MC Fisher ------------------------------------------------------------------------
     def compute_fisher_contribution(self, seed, verb=False):
-        """bspec.py:441-445 -> _compute_fisher (bspec.py:473-676, compute_cov=False)."""
+        """bspec.py:441-445."""
+        return self._compute_fisher(seed)
+
+    def compute_covariance_contribution(self, seed, Pk_cov, verb=False, mask_shot_2pt=None):
+        """One realization of the Gaussian covariance for Pk_cov = [k, P_0 (, P_2, P_4)] without shot
+        noise (bspec.py:447-471); mask_shot_2pt defaults to mask_shot.  (The reference fails for a
+        constant mask without mask_shot_2pt, which the constant-mask filter does not use.)"""
+        b = self.base
+        if not self.const_mask and mask_shot_2pt is None:
+            assert getattr(self, 'mask_shot', None) is not None, \
+                "Must supply a shot-noise mask to compute the shot-noise contribution!"
+            noise = self.mask_shot
+        else:
+            noise = None if mask_shot_2pt is None else np.asarray(mask_shot_2pt, dtype=np.float64)
+        assert len(Pk_cov) != 0, "Must provide fiducial power spectrum if computing covariance!"
+        Pk_cov = np.asarray(Pk_cov, dtype=np.float64)
+        assert Pk_cov.ndim == 2 and len(Pk_cov) in [2, 3, 4], "Pk should contain k and P_0 (and optionally P_2, P_4) columns"
+        Yr, Yf = self.Ylm_real, self.Ylm_fourier
+        lcov = (len(Pk_cov) - 2) * 2
+        if b.sightline == 'local' and lcov > self.lmax:                            # 461-467
+            Yr = real_harmonics(np.asarray(b.r_grids), lcov, False)
+            Yf = real_harmonics(np.asarray(b.k_grids), lcov, False)
+        return self._compute_fisher(seed, cov=(cov_pk_grid(b, Pk_cov), noise, Yr, Yf))
+
+    def _compute_fisher(self, seed, cov=None):
+        """bspec.py:473-676; cov = (gridded spectrum, noise mask, Ylm tables) selects the covariance
+        filter S^-1 [P xi P^T + N] S^-dag in place of S^-1 P."""
         b = self.base
         if not hasattr(self, 'sym_factor'):
             self._compute_symmetry_factor()
@@ -1236,7 +1276,10 @@ class BSpec:
         ks = self.k_bins_squeeze
         ab = self.all_bins
 
-        def apply_filter(m):                                                       # 499-505
+        def apply_filter(m):                                                       # 499-519
+            it = 'fourier' if self.const_mask else 'real'
+            if cov is not None:
+                return covariance_filter(self, m, it, cov[0], cov[1], cov[2], cov[3])
             if self.const_mask:
                 return self.mask_mean * self.applySinv(m, input_type='fourier', output_type='fourier')
             return self.applySinv(self.apply_pointing(m), input_type='real', output_type='fourier')

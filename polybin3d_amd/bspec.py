@@ -14,7 +14,7 @@ import torch
 
 from . import _capi as C
 from .base import harmonics_tags
-from .constraints import Pointing
+from .constraints import Pointing, covariance_filter
 
 
 def triangle_bins(k_bins, k_bins_squeeze, lmax, include_partial_triangles):
@@ -685,11 +685,11 @@ class BSpec:
             return self.mask_mean * b.as_fourier(self.applySinv(a, input_type='fourier', output_type='fourier'))
         return b.as_fourier(self.applySinv(self.apply_pointing(b.c2r(a)), input_type='real', output_type='fourier'))
 
-    def _q_ainv_generic(self, FT, pair_tab, QA, chunk=8):
+    def _q_ainv_generic(self, FT, pair_tab, QA, chunk=8, filt=None):
         """QA rows with the user's S^-1 (bspec.py:615-666): the Q[A^-1 a] maps of a chunk of triplets
         are assembled in real space (C2R with the BKQ prologue; for local l > 0 the external leg's
         sum_m Y_lm(x) IFFT[Theta Y_lm(k) FT_0], bspec.py:629-636), then S^-1 P is the user's callable on
-        each map, and the in-bin modes are packed."""
+        each map (or filt(real map, 'real'), the covariance filter), and the in-bin modes are packed."""
         b = self.base
         pk = self._packed_layout()
         M, Nq = pk['M'], self.Nk_squeeze
@@ -714,11 +714,30 @@ class BSpec:
                     b.c2r(None, out=real, batch=nb, kop=C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common),
                           rop=C.rop(C.ROP_YLM, ell=ell, m=m, accumulate=1))
             for i in range(nb):
-                S = self._apply_filter_generic(b.r2c(real[i]))
+                S = self._apply_filter_generic(b.r2c(real[i])) if filt is None else filt(real[i], 'real')
                 QA[t0 + i] = S.reshape(-1)[modes]
             t0 = t1
 
-    def _fisher_contribution_dev(self, seed):
+    def compute_covariance_contribution(self, seed, Pk_cov, verb=False, mask_shot_2pt=None):
+        """One realization of the Gaussian covariance for the theory spectrum Pk_cov = [k, P_0 (, P_2, P_4)]
+        without shot noise (bspec.py:447-471): the MC-Fisher contribution with S^-1 [P xi P^T + N] S^-dag
+        in place of S^-1 P on both sides, N = mask_shot_2pt (default mask_shot).  Every Q map passes
+        through the filter one at a time (no fused chain).  A constant mask needs no noise mask."""
+        b = self.base
+        if not self.const_mask and mask_shot_2pt is None:
+            assert getattr(self, 'mask_shot', None) is not None, \
+                "Must supply a shot-noise mask to compute the shot-noise contribution!"
+            print("## Caution: assuming the same shot-noise mask for the power spectrum and bispectrum!")
+            noise = self.mask_shot
+        else:
+            noise = None if mask_shot_2pt is None else b.as_real(np.asarray(mask_shot_2pt, dtype=np.float64))
+        assert len(Pk_cov) != 0, "Must provide fiducial power spectrum if computing covariance!"
+        Pk_cov = np.asarray(Pk_cov, dtype=np.float64)
+        assert Pk_cov.ndim == 2, "Pk should contain k and P_0 (and optionally P_2, P_4) columns"
+        assert len(Pk_cov) in [2, 3, 4], "Pk should contain k and P_0 (and optionally P_2, P_4) columns"
+        return self._fisher_contribution_dev(seed, cov=(b.pk_grid(Pk_cov), noise)).cpu().numpy()
+
+    def _fisher_contribution_dev(self, seed, cov=None):
         b = self.base
         if not hasattr(self, 'sym_factor'):
             self._compute_symmetry_factor()
@@ -751,13 +770,16 @@ class BSpec:
         FT = torch.empty((self.Nl, len(self._ainv_pairs), M), dtype=torch.complex64, device=b.device)
         self._ft_tables(G1, G2, self._ainv_pairs, FT)
         del G1, G2
-        if self.user_Sinv:
-            self._q_ainv_generic(FT, self._ainv_tab, QA)
+        filt = None if cov is None else (lambda m, it: covariance_filter(self, m, it, cov[0], cov[1]))  # noqa: E731
+        if filt is not None or self.user_Sinv:
+            self._q_ainv_generic(FT, self._ainv_tab, QA, filt=filt)
         else:
             self._q_ainv(FT, self._ainv_tab, QA)
         del FT
         # S^-1 P side (bspec.py:671, compute_Q('Sinv'))
-        if self.user_Sinv:
+        if filt is not None:
+            v1, v2 = filt(a1, 'fourier'), filt(a2, 'fourier')
+        elif self.user_Sinv:
             v1, v2 = self._apply_filter_generic(a1), self._apply_filter_generic(a2)
         elif self.const_mask:
             v1 = b.kmap(a1, C.kop(C.KOP_PLAIN, scale=self.mask_mean))

@@ -108,3 +108,23 @@ class Pointing:
             o -= coef @ c
             return out
         return self.mask_eff, (weight, self.rbin, self.nbins, sums), finish
+
+
+def covariance_filter(spec, m, input_type, Pk_grid, noise_mask):
+    """S^-1 [P xi P^T + N] S^-dag of a device map (real or Fourier, per input_type), the covariance
+    counterpart of the S^-1 P filter (PolyBin3D/pspec.py:278-289, 312-325; PolyBin3D/bspec.py:507-519):
+    mask_mean^2 xi + mask_mean for a constant mask, else the pointing matrix (with its integral
+    constraint) around xi plus the noise mask N.  xi = base.apply_xi of the gridded spectrum Pk_grid
+    (base.pk_grid).  Returns a complex64 half-grid map."""
+    b = spec.base
+    if spec.const_mask:
+        Sid = b.as_fourier(spec.applySinv_transpose(m, input_type=input_type, output_type='fourier'))
+        CSid = b.apply_xi(Sid, None, None, None, Pk_grid, 'fourier')
+        CSid.mul_(spec.mask_mean ** 2).add_(Sid, alpha=spec.mask_mean)
+        return b.as_fourier(spec.applySinv(CSid, input_type='fourier', output_type='fourier'))
+    Sid = b.as_real(spec.applySinv_transpose(m, input_type=input_type, output_type='real'))
+    PdSid_real = spec._P.apply(Sid, transpose=True)
+    CPdSid = b.apply_xi(b.r2c(PdSid_real), PdSid_real, None, None, Pk_grid, 'real')
+    Cov = spec._P.apply(CPdSid)
+    Cov.addcmul_(Sid, noise_mask)
+    return b.as_fourier(spec.applySinv(Cov, input_type='real', output_type='fourier'))

@@ -16,7 +16,7 @@ import torch
 
 from . import _capi as C
 from .base import harmonics_tags
-from .constraints import Pointing
+from .constraints import Pointing, covariance_filter
 
 
 def derivative_coefficient(edges):
@@ -467,21 +467,9 @@ class PSpec:
         a = b.generate_data(seed=seed + int(1e7), output_type='fourier')
         if not self.const_mask:
             a_real = b.c2r(a)
-        Yr, Yf = self.Ylm_real, self.Ylm_fourier
-
         def cov_filter(m, it):
             # S^-1 [P xi P^T + N] S^-dag (pspec.py:278-289, 312-325)
-            if self.const_mask:
-                Sid = b.as_fourier(self.applySinv_transpose(m, input_type=it, output_type='fourier'))
-                CSid = b.apply_xi(Sid, None, Yr, Yf, cov, 'fourier')
-                CSid.mul_(self.mask_mean ** 2).add_(Sid, alpha=self.mask_mean)
-                return b.as_fourier(self.applySinv(CSid, input_type='fourier', output_type='fourier'))
-            Sid = b.as_real(self.applySinv_transpose(m, input_type=it, output_type='real'))
-            PdSid_real = self._P.apply(Sid, transpose=True)
-            CPdSid = b.apply_xi(b.r2c(PdSid_real), PdSid_real, Yr, Yf, cov, 'real')
-            Cov = self._P.apply(CPdSid)
-            Cov.addcmul_(Sid, self.mask_shot)
-            return b.as_fourier(self.applySinv(Cov, input_type='real', output_type='fourier'))
+            return covariance_filter(self, m, it, cov, getattr(self, 'mask_shot', None))
 
         def apply_filter(m):
             if cov is not None:

@@ -204,3 +204,32 @@ def test_bk_mc_fisher_user_sinv(hip, case):
         F1 = bs.compute_fisher_contribution(seed)
         F2 = bo.compute_fisher_contribution(seed)
         close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+
+
+@pytest.mark.parametrize("case", ["global_const_l2", "local_mask_l2", "local_gic_l2"])
+def test_bk_covariance_contribution(hip, case):
+    """compute_covariance_contribution (bspec.py:447-519) per seed against the oracle: the MC Fisher with
+    S^-1 [P xi P^T + N] S^-dag on both sides; global P(k, mu) from P_0, P_2, P_4; local P_0 and P_2
+    through the harmonic transforms, the shot-noise mask, and the integral-constrained pointing."""
+    import polybin3d_amd as pb
+    n = 16
+    edges = np.linspace(0.007, 0.05, 4)
+    kc = np.linspace(0, 2, 300)
+    Pc = [kc, 3e3 / (1 + (kc / 0.05) ** 2), 800. / (1 + (kc / 0.05) ** 2), 200. / (1 + (kc / 0.04) ** 2)]
+    ic, opt = {}, {}
+    if case == "global_const_l2":
+        kw, mask = {}, 0.7 * np.ones((n, n, n))
+    else:
+        kw, mask, Pc = dict(boxcenter=[10., 10., 10.], sightline='local'), shell_mask(n, L, [10., 10., 10.]), Pc[:3]
+        opt = dict(mask_shot=0.5 * mask)
+        if case == "local_gic_l2":
+            ic = dict(add_GIC=True, mask_IC=1.1 * mask)
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    bs = pb.BSpec(b, edges, lmax=2, mask=mask, **opt, **ic)
+    bo = O.BSpec(o, edges, lmax=2, mask=mask, **opt, **ic)
+    for seed in (0, 3):
+        ms2 = 0.3 * mask if seed == 3 and case != "global_const_l2" else None
+        C1 = bs.compute_covariance_contribution(seed, Pc, mask_shot_2pt=ms2)
+        C2 = bo.compute_covariance_contribution(seed, Pc, mask_shot_2pt=ms2)
+        close(C1, C2, tol=2e-5, scale=fisher_scale(C2))

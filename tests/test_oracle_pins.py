@@ -780,3 +780,39 @@ def test_pk_theory_wideangle_dense():
                                    + (3j / 5. * (2 - kf[kb]) * X(leg[3])).real)
             ref[r, 2 * Nt + kb] = s * (X(leg[4]).real + (10j / 9. * (5 + kf[kb]) * X(leg[3])).real)
     np.testing.assert_allclose(Bm, ref, atol=1e-10 * np.abs(ref).max(), rtol=1e-8)
+
+
+@pytest.mark.parametrize("sightline", ["global", "local"])
+def test_bk_covariance_equals_fisher_with_filter(sightline):
+    """BSpec covariance contribution (bspec.py:447-519) with a constant mask m and trivial S^-1: its
+    filter S^-1 [m^2 xi + m] S^-dag is the mode-space multiplier c = m^2 P(k, mu) + m, so it equals the
+    MC Fisher contribution (unmasked) with the user filter S^-1 = c: the covariance code path against
+    the Fisher one, in both weightings (the Q[A^-1 a] maps and the S^-1-weighted g maps).  Global:
+    P_0, P_2 and P_4 (a mu-dependent c); local: P_0 (c is isotropic, the Yamamoto legs are exercised
+    by lmax = 2)."""
+    L, n = 1000., 16
+    kw = {} if sightline == 'global' else dict(boxcenter=[300., 200., 500.], sightline='local')
+    b = O.PolyBin3D(L, [n] * 3, **kw)
+    edges = (2 * np.pi / L) * np.array([1., 2., 3., 4., 5.])
+    m = 0.6
+    kc = np.linspace(0., 0.2, 100)
+    Pc = [kc, 3e3 / (1 + 40 * kc), 500. - 2e3 * kc, 80. + kc]
+    if sightline == 'local':
+        Pc = Pc[:2]
+    bc = O.BSpec(b, edges, lmax=2, mask=m * np.ones((n, n, n)))
+    C = bc.compute_covariance_contribution(2, Pc, mask_shot_2pt=np.ones((n, n, n)))
+    P = np.interp(b.modk_grid, kc, Pc[1], right=0.)
+    if sightline == 'global':
+        mu = b.muk_grid
+        P = P + O.legendre(2, mu) * np.interp(b.modk_grid, kc, Pc[2], right=0.) \
+            + O.legendre(4, mu) * np.interp(b.modk_grid, kc, Pc[3], right=0.)
+    c = m ** 2 * P + m
+
+    def Sinv(x, input_type='real', output_type='real'):
+        f = b.to_fourier(x) if input_type == 'real' else x
+        f = f * c
+        return b.to_real(f) if output_type == 'real' else f
+    bf = O.BSpec(b, edges, lmax=2, applySinv=Sinv)
+    F = bf.compute_fisher_contribution(2)
+    np.testing.assert_allclose(C, F, atol=1e-11 * np.abs(F).max(), rtol=1e-9)
+    assert np.abs(F).max() > 0
