@@ -561,14 +561,35 @@ class PolyBin3D:
                     grids[l] = interp(col)
         return {l: torch.from_numpy(g.astype(np.float32)).to(self.device) for l, g in grids.items()}
 
+    def _grid_weight(self, g):
+        """A half-grid weight map (numpy array, as the reference's callers build them on modk_grid, or a
+        tensor) as an FP32 device tensor.  A numpy grid is uploaded once and reused while the same
+        array is passed (a few grids are kept)."""
+        if torch.is_tensor(g):
+            t = g.to(device=self.device, dtype=torch.float32).contiguous()
+        else:
+            cache = self._bufs.setdefault('grid_weights', [])
+            for arr, t in cache:
+                if arr is g:
+                    break
+            else:
+                t = torch.from_numpy(np.ascontiguousarray(g, dtype=np.float32)).to(self.device)
+                cache.append((g, t))
+                del cache[:-4]
+        if tuple(t.shape) != self.half_shape:
+            raise ValueError("a gridded spectrum must have the half-grid shape %s" % (self.half_shape,))
+        return t
+
     def apply_xi(self, input_map_fourier, input_map_real=None, Ylm_real=None, Ylm_fourier=None, Pk_grid={},
                  output_type='real'):
-        """xi(x, y) applied symmetrically for a gridded spectrum (base.py:413-437), Pk_grid from pk_grid:
+        """xi(x, y) applied symmetrically for a gridded spectrum (base.py:413-437), Pk_grid a dict of half
+        grids (numpy, as the reference's callers build them, or device tensors, e.g. from pk_grid):
         P_0 a + sum_l [P_l sum_m Y_lm(k) FFT[Y_lm a_real] + sum_m Y_lm(x) IFFT[Y_lm(k) P_l a]] / 2 on
         the local sightline, P(k, mu) a on the global one.  The weights ride in the fused transforms'
         mode ops.  Without input_map_real the local multipoles use the C2R of the input (the reference
         requires it)."""
         a = self.as_fourier(input_map_fourier)
+        Pk_grid = {l: self._grid_weight(g) for l, g in Pk_grid.items()}
         P = self.kmap(a, C.kop(C.KOP_WEIGHT, w=Pk_grid[0]))
         ells = [l for l in Pk_grid if l != 0]
         if self.sightline == 'global' or not ells:

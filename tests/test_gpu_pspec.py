@@ -312,3 +312,71 @@ def test_apply_xi_matches_oracle(hip):
         ref = o.apply_xi(uf, u, po.Ylm_real, po.Ylm_fourier, grid_o, ot)
         got = b.apply_xi(uf.astype(np.complex64), u.astype(np.float32), None, None, grid_g, ot).cpu().numpy()
         close(got, ref, scale=np.abs(ref).max())
+
+
+def _cg_filters(base, smooth_mask, beta, Pkl, Yr, Yf, dot, n_it):
+    """The conjugate-gradient S^-1 and S^-dag of the reference's BOSS tutorial (Tutorial 3, cells 45 and
+    49): solve [W xi + beta] x = u (and [xi W + beta] x = u, nulled outside the mask) with apply_xi of a
+    multipole spectrum; a fixed number of unpreconditioned iterations (converged to ~1e-7 here)."""
+    def fwd(m):
+        return smooth_mask * base.apply_xi(base.to_fourier(m), m, Yr, Yf, Pk_grid=Pkl, output_type='real') + beta * m
+
+    def tr(m):
+        mm = smooth_mask * m
+        return (smooth_mask != 0) * base.apply_xi(base.to_fourier(mm), mm, Yr, Yf, Pk_grid=Pkl,
+                                                  output_type='real') + beta * m
+
+    def cg(u, A):
+        x = u * 1.
+        r = u - A(x)
+        p = r * 1.
+        Cp = A(p)
+        old = dot(r, r)
+        alpha = old / dot(p, Cp)
+        for _ in range(n_it):
+            x = x + alpha * p
+            r = r - alpha * Cp
+            new = dot(r, r)
+            p = r + (new / old) * p
+            old = new
+            Cp = A(p)
+            alpha = old / dot(p, Cp)
+        return x
+
+    def wrap(A):
+        def f(d, input_type='real', output_type='real'):
+            u = base.to_real(d) if input_type == 'fourier' else d
+            x = cg(u, A)
+            return x if output_type == 'real' else base.to_fourier(x)
+        return f
+    return wrap(fwd), wrap(tr)
+
+
+def test_pk_fisher_with_cg_sinv(hip):
+    """MC Fisher with the tutorial's conjugate-gradient S^-1 (a user callable built on base.apply_xi with
+    numpy spectrum grids, as the reference's callers write it) on the GPU against the same CG on the
+    oracle.  Tolerance 2e-3 of the diagonal scale: this CG amplifies rounding, so that the oracle's own
+    CG with its transforms and iterates rounded to FP32 already moves an entry by 2.2e-4 (the GPU by
+    7.9e-4); a wrong term in apply_xi or the filter path moves entries by O(1)."""
+    import polybin3d_amd as pb
+    import torch
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    kw = dict(boxcenter=[10., 10., 10.], sightline='local')
+    mask = shell_mask(n, L, [10., 10., 10.])
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    kk = np.linspace(0., 1., 400)
+    Pkl = {0: np.interp(o.modk_grid, kk, 8e3 / (1 + (kk / 0.05) ** 2)),
+           2: np.interp(o.modk_grid, kk, 3e3 / (1 + (kk / 0.05) ** 2))}
+    beta = 2e4
+    po_tmp = O.PSpec(o, edges, lmax=2)
+    so, sto = _cg_filters(o, mask, beta, Pkl, po_tmp.Ylm_real, po_tmp.Ylm_fourier,
+                          lambda a, c: float(np.sum(a * c)), 10)
+    mask_d = torch.from_numpy(mask.astype(np.float32)).cuda()
+    sg, stg = _cg_filters(b, mask_d, beta, Pkl, None, None,
+                          lambda a, c: float((a.double() * c.double()).sum()), 10)
+    ps = pb.PSpec(b, edges, lmax=2, mask=mask, applySinv=sg, applySinv_transpose=stg)
+    po = O.PSpec(o, edges, lmax=2, mask=mask, applySinv=so, applySinv_transpose=sto)
+    F1, F2 = ps.compute_fisher_contribution(1), po.compute_fisher_contribution(1)
+    close(F1, F2, tol=2e-3, scale=_diag_scale(F2))
