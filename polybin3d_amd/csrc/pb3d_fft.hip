@@ -250,6 +250,17 @@ template <> struct ColCfg<2>    { static constexpr int T = 256; };
 template <> struct ColCfg<4>    { static constexpr int T = 256; };
 template <> struct ColCfg<8>    { static constexpr int T = 128; };
 template <> struct ColCfg<16>   { static constexpr int T = 128; };
+// Columns per tile by pass: the first C2R pass (mode-op prologue) and the last R2C pass (mode-op
+// epilogue) gather from mode tables and are latency bound; half-width tiles halve their LDS, so two
+// blocks share a CU and one's table gathers overlap the other's butterflies (measured at 256^3:
+// 8 % faster for both).  The plain passes stream whole planes and keep the wide tiles (32-column
+// tiles there cost 35 %).
+#ifndef PB3D_COL_NARROW
+#define PB3D_COL_NARROW 1
+#endif
+template <int L, int MODE> struct ColTile {
+  static constexpr int T = (PB3D_COL_NARROW && MODE != 0 && ColCfg<L>::T / 2 >= 16) ? ColCfg<L>::T / 2 : ColCfg<L>::T;
+};
 
 // ------------------------------------------------------------------------------------------
 // Column pass along axis 0 or 1 of the half-complex grid, optionally pruned to a band limit.
@@ -277,11 +288,75 @@ __device__ __forceinline__ bool in_radius(int p, int L, int rad) {
   return rad < 0 || p <= rad || p >= L - rad;
 }
 
+template <int L, int R, int P, int DIR, typename Get, typename Dst>
+__device__ __forceinline__ void first_stage_from_regs(int t, Get get, Dst dst);
+template <int L, int R, int NS, int P, int DIR, typename Src, typename Sink>
+__device__ __forceinline__ void last_stage_to_regs(int t, Src src, Sink sink, const float2* tw);
+
+#ifndef PB3D_BKQ_BATCH
+#define PB3D_BKQ_BATCH 1
+#endif
+
+// The BKQ prologue of one pencil (kop_value case 4 for the E = L/P positions p = t + e*P of this
+// thread) with its table reads batched: per group of 8 positions, first the bin and pack-index
+// loads, then the packed-row gathers, then the weights, so that the dependent loads of the
+// positions overlap instead of running one chain at a time.
+template <int L, int E>
+__device__ __forceinline__ void bkq_gather(const KOp& op, const Geo& g, long long bt, long long base, long long S,
+                                           int t, bool valid, int rad_in, int mj, int ml, float2 (&vals)[E]) {
+  constexpr int P = L / E;
+  constexpr int CH = E < 4 ? E : 4;
+  const int* tr = op.trip + 4 * (long long)(op.trip0 + bt);
+  const int b1 = tr[0], b2 = tr[1], b3 = tr[2], ell = tr[3];
+  const int pr3 = (op.terms & 5) ? op.pair_index[b1 * op.nk + b2] : -1;
+  const int pr1 = (op.terms & 2) ? op.pair_index[b2 * op.nk + b3] : -1;
+  const int pr2 = (op.terms & 2) ? op.pair_index[b1 * op.nk + b3] : -1;
+  const float2* fl = op.ft + (long long)(ell / 2) * op.ft_lstride;
+#pragma unroll
+  for (int c0 = 0; c0 < E; c0 += CH) {
+    int bn[CH], pk[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int p = t + (c0 + u) * P;
+      const bool ok = valid && in_radius(p, L, rad_in);
+      const long long md = base + (long long)p * S;
+      bn[u] = ok ? op.bin_table[md] : PB3D_NO_BIN;
+      pk[u] = ok ? op.pack_index[md] : -1;
+    }
+    float2 f3[CH], f1[CH], f2[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const bool in = bn[u] != PB3D_NO_BIN && pk[u] >= 0;
+      f3[u] = (in && bn[u] == b3 && pr3 >= 0) ? op.ft[(long long)pr3 * op.packed_len + pk[u]] : make_float2(0.f, 0.f);
+      f1[u] = (in && bn[u] == b1 && pr1 >= 0) ? fl[(long long)pr1 * op.packed_len + pk[u]] : make_float2(0.f, 0.f);
+      f2[u] = (in && bn[u] == b2 && pr2 >= 0) ? fl[(long long)pr2 * op.packed_len + pk[u]] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int p = t + (c0 + u) * P;
+      float2 acc = make_float2(0.f, 0.f);
+      if (bn[u] == b3 && pr3 >= 0 && pk[u] >= 0) {
+        float kx, ky, kz; kvec(g, p, mj, ml, kx, ky, kz);
+        const float k2 = kx * kx + ky * ky + kz * kz;
+        const float inv = k2 > 0.f ? rsqrtf(k2) : 0.f;
+        if (op.terms & 1) {
+          float w = 1.f;
+          if (op.leg3_legendre && ell != 0) w = legendre_f(ell, (g.s0 * kx + g.s1 * ky + g.s2 * kz) * inv);
+          acc = cadd(acc, cscale(f3[u], w));
+        }
+        if (op.terms & 4) acc = cadd(acc, cscale(f3[u], ylm_f(op.ell, op.m, kx * inv, ky * inv, kz * inv)));
+      }
+      acc = cadd(cadd(acc, f1[u]), f2[u]);
+      vals[c0 + u] = acc;
+    }
+  }
+}
+
 template <int L, int DIR, int MODE>
-__global__ void __launch_bounds__(Decomp<L>::P * ColCfg<L>::T)
+__global__ void __launch_bounds__((Decomp<L>::P * ColTile<L, MODE>::T))
 fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long long out_bstride,
          const float2* __restrict__ tw, Geo geo, KOp kop) {
-  constexpr int T = ColCfg<L>::T;
+  constexpr int T = ColTile<L, MODE>::T;
   constexpr int LD = T + 1;
   __shared__ float2 lds[(Decomp<L>::R1 == 1) ? 1 : L * LD];
   __shared__ float2 stw[L];
@@ -351,6 +426,64 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
     __device__ float2 operator()(int p) const { return l[p * LDv + colv]; }
     __device__ void operator()(int p, float2 v) const { l[p * LDv + colv] = v; }
   } lrw{lds, LD, col};
+  if constexpr (MODE == 1 && PB3D_BKQ_BATCH) {
+    if (kop.kind == 4) {
+      using D = Decomp<L>;
+      constexpr int E = L / D::P;
+      float2 vals[E];
+      bkq_gather<L, E>(kop, geo, bt, base, cm.S, t, valid, cm.rad_in, mj, ml, vals);
+      auto get = [&](int e) { return vals[e]; };
+      // pencil_fft with the first stage fed from registers (slot e holds position t + e*P)
+      if constexpr (D::R1 == 1) {
+        first_stage_from_regs<L, D::R0, D::P, DIR>(t, get, store);
+      } else if constexpr (D::R2 == 1) {
+        first_stage_from_regs<L, D::R0, D::P, DIR>(t, get, lrw);
+        __syncthreads();
+        stage<L, D::R1, D::R0, D::P, DIR>(t, lrw, store, stw, true);
+      } else {
+        first_stage_from_regs<L, D::R0, D::P, DIR>(t, get, lrw);
+        __syncthreads();
+        stage<L, D::R1, D::R0, D::P, DIR>(t, lrw, lrw, stw, true);
+        __syncthreads();
+        stage<L, D::R2, D::R0 * D::R1, D::P, DIR>(t, lrw, store, stw, true);
+      }
+      return;
+    }
+  }
+  if constexpr (MODE == 2 && PB3D_BKQ_BATCH) {
+    if (kop.kind == 5) {
+      // packed output: the pack indices of this thread's outputs do not depend on the data, so
+      // they are loaded up front and the last stage stores straight from registers
+      using D = Decomp<L>;
+      constexpr int E = L / D::P;
+      int pk[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int p = t + e * D::P;
+        pk[e] = (valid && in_radius(p, L, cm.rad_out)) ? kop.pack_index[base + (long long)p * cm.S] : -1;
+      }
+      auto sink = [&](int e, int, float2 v) {
+        if (pk[e] < 0) return;
+        v = cscale(v, kop.scale);
+        if (kop.accumulate) v = cadd(v, dst[pk[e]]);
+        dst[pk[e]] = v;
+      };
+      if constexpr (D::R1 == 1) {
+        last_stage_to_regs<L, D::R0, 1, D::P, DIR>(t, load, sink, stw);
+      } else if constexpr (D::R2 == 1) {
+        stage<L, D::R0, 1, D::P, DIR>(t, load, lrw, stw, false);
+        __syncthreads();
+        last_stage_to_regs<L, D::R1, D::R0, D::P, DIR>(t, lrw, sink, stw);
+      } else {
+        stage<L, D::R0, 1, D::P, DIR>(t, load, lrw, stw, false);
+        __syncthreads();
+        stage<L, D::R1, D::R0, D::P, DIR>(t, lrw, lrw, stw, true);
+        __syncthreads();
+        last_stage_to_regs<L, D::R2, D::R0 * D::R1, D::P, DIR>(t, lrw, sink, stw);
+      }
+      return;
+    }
+  }
   pencil_fft<L, DIR>(t, load, lrw, store, stw, false);
 }
 
@@ -929,7 +1062,7 @@ static hipError_t launch_cols(int L, const float2* in, float2* out, const ColMap
   // a straddling tile would read a few columns of the next plane at a stride.
 #define PB3D_COLS(LL)                                                                          \
   case LL: {                                                                                   \
-    constexpr int T = ColCfg<LL>::T;                                                           \
+    constexpr int T = ColTile<LL, MODE>::T;                                                    \
     ColMap c2 = cm;                                                                            \
     c2.tpp = 0;                                                                                \
     if (PB3D_COL_ALIGN && cm.tpp && cm.cpp % T >= T / 2) c2.tpp = (cm.cpp + T - 1) / T;       \
