@@ -174,6 +174,9 @@ def main():
                          "PSpec MC Fisher realizations at 128^3 and 256^3 (--realizations per call)")
     ap.add_argument("--realizations", type=int, default=1,
                     help="pk_mc_fisher: realizations per compute_fisher batch")
+    ap.add_argument("--mixed", action="store_true",
+                    help="pk_mc_fisher: issue the 128^3 and 256^3 realizations together, on a stream per grid "
+                         "(compute_fisher_mixed), instead of one grid after the other")
     ap.add_argument("--bk-bins", type=int, default=16)
     ap.add_argument("--bk-kmax", type=float, default=0.17)
     ap.add_argument("--lmax", type=int, default=0)
@@ -273,10 +276,24 @@ def pk_fisher_workload(args, torch, pb):
     F = {n: torch.zeros((specs[n].N_bins,) * 2, dtype=torch.float64, device=specs[n].base.device) for n in grids}
     per = {n: 0. for n in grids}
 
+    streams = {n: torch.cuda.Stream() for n in grids}  # default priority
+
     def step(seed0, timed):
+        seeds = list(range(seed0, seed0 + R))
+        if args.mixed:
+            # every grid's realizations on its own stream (compute_fisher_mixed's schedule)
+            main = torch.cuda.current_stream()
+            for n in grids:
+                streams[n].wait_stream(main)
+                with torch.cuda.stream(streams[n]):
+                    F[n].add_(specs[n]._fisher_contributions_dev(seeds))
+            for n in grids:
+                main.wait_stream(streams[n])
+            torch.cuda.synchronize()
+            return
         for n in grids:
             t = time.perf_counter()
-            F[n].add_(specs[n]._fisher_contributions_dev(list(range(seed0, seed0 + R))))
+            F[n].add_(specs[n]._fisher_contributions_dev(seeds))
             torch.cuda.synchronize()
             if timed:
                 per[n] += time.perf_counter() - t
@@ -294,8 +311,9 @@ def pk_fisher_workload(args, torch, pb):
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "pk_mc_fisher_128_256", "grids": list(grids), "boxsize": L, "sightline": "local",
                       "lmax": 4, "nbins": 30, "kmax": 0.3, "mask": "shell", "realizations_per_batch": R,
-                      "fisher_accumulation": "fp64"},
-           "ms_per_realization": {str(n): round(1e3 * per[n] / (args.steps * R), 2) for n in grids}}
+                      "fisher_accumulation": "fp64", "schedule": "mixed streams" if args.mixed else "grid by grid"}}
+    if not args.mixed:
+        res["ms_per_realization"] = {str(n): round(1e3 * per[n] / (args.steps * R), 2) for n in grids}
     print(json.dumps(res), flush=True)
 
 

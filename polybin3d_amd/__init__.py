@@ -5,8 +5,8 @@ backend='hip'.  The compute path is libpb3d.so (hand-written HIP kernels for gfx
 ctypes in _capi.py; it must be built in-tree (python -m polybin3d_amd.build) before use.
 """
 from .base import PolyBin3D
-from .pspec import PSpec
+from .pspec import PSpec, compute_fisher_mixed
 from .bspec import BSpec
 from . import autograd
 
-__all__ = ["PolyBin3D", "PSpec", "BSpec", "autograd"]
+__all__ = ["PolyBin3D", "PSpec", "BSpec", "autograd", "compute_fisher_mixed"]

@@ -719,3 +719,44 @@ class PSpec:
             res['p%s' % l] = out[idx:idx + self.Nk]
             idx += self.Nk
         return res
+
+
+def compute_fisher_mixed(specs, N_it, batch=1, verb=False):
+    """MC Fisher matrices of several PSpec objects on different grids (e.g. 128^3 and 256^3)
+    computed together: every spec's realizations (batch seeds per pass, as compute_fisher) are
+    issued on a HIP stream of its own, so the GPU overlaps the small grid's launches with the large
+    grid's.  Returns the list of Fisher matrices, each equal to that spec's compute_fisher(N_it,
+    batch=batch) up to FP64 summation order, and sets fish / inv_fish on each spec.  No counterpart
+    in the reference, whose compute_fisher (pspec.py:553-569) runs one spec's seeds serially.  The
+    specs must have distinct base objects (a base's work buffers serve one stream)."""
+    specs = list(specs)
+    if not specs:
+        return []
+    if len({id(ps.base) for ps in specs}) != len(specs):
+        raise ValueError("compute_fisher_mixed: every spec needs its own PolyBin3D base")
+    dev = specs[0].base.device
+    if any(ps.base.device != dev for ps in specs):
+        raise ValueError("compute_fisher_mixed: all specs must be on one device")
+    main = torch.cuda.current_stream(dev)
+    streams = [torch.cuda.Stream(device=dev) for _ in specs]  # default priority
+    for st in streams:
+        st.wait_stream(main)
+    fish = [None] * len(specs)
+    batch = max(1, int(batch))
+    for s0 in range(0, N_it, batch):
+        if verb:
+            print("Computing Fisher contributions %d-%d of %d" % (s0 + 1, min(N_it, s0 + batch), N_it))
+        seeds = list(range(s0, min(N_it, s0 + batch)))
+        for i, (ps, st) in enumerate(zip(specs, streams)):
+            with torch.cuda.stream(st):
+                f = ps._fisher_contributions_dev(seeds) if len(seeds) > 1 else ps._fisher_contribution_dev(seeds[0])
+                f = f.to(dev) / N_it
+                fish[i] = f if fish[i] is None else fish[i] + f
+    for st in streams:
+        main.wait_stream(st)
+    out = []
+    for ps, f in zip(specs, fish):
+        ps.fish = f.cpu().numpy()
+        ps.inv_fish = torch.linalg.inv(f).cpu().numpy()
+        out.append(ps.fish)
+    return out

@@ -380,3 +380,22 @@ def test_pk_fisher_with_cg_sinv(hip):
     po = O.PSpec(o, edges, lmax=2, mask=mask, applySinv=so, applySinv_transpose=sto)
     F1, F2 = ps.compute_fisher_contribution(1), po.compute_fisher_contribution(1)
     close(F1, F2, tol=2e-3, scale=_diag_scale(F2))
+
+
+def test_pk_fisher_mixed_grids(hip):
+    """compute_fisher_mixed: the MC Fishers of a 32^3 and a 64^3 spec, issued together on a stream
+    each, equal each spec's own compute_fisher (same seeds; FP64 summation order aside)."""
+    import polybin3d_amd as pb
+    edges = np.linspace(0.01, 0.09, 9)
+    kw = dict(boxcenter=[10., 10., 10.], sightline='local')
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    specs = []
+    for n in (32, 64):
+        b = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+        specs.append(pb.PSpec(b, edges, lmax=4, mask=shell_mask(n, L, [10., 10., 10.])))
+    mixed = pb.compute_fisher_mixed(specs, 5, batch=2)
+    for ps, fm in zip(specs, mixed):
+        ref = ps.compute_fisher(5, batch=2)
+        np.testing.assert_allclose(fm, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+    with pytest.raises(ValueError):
+        pb.compute_fisher_mixed([specs[0], pb.PSpec(specs[0].base, edges, lmax=2)], 1)
