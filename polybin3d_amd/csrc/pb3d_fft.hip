@@ -293,6 +293,9 @@ __device__ __forceinline__ void first_stage_from_regs(int t, Get get, Dst dst);
 template <int L, int R, int NS, int P, int DIR, typename Src, typename Sink>
 __device__ __forceinline__ void last_stage_to_regs(int t, Src src, Sink sink, const float2* tw);
 
+#ifndef PB3D_ROW_GROUP
+#define PB3D_ROW_GROUP 16
+#endif
 #ifndef PB3D_BKQ_BATCH
 #define PB3D_BKQ_BATCH 1
 #endif
@@ -679,8 +682,50 @@ r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows
   const long long row0 = (long long)blockIdx.x * T;
   // 1. coalesced load of T rows (T*M float2) with the voxel prologue (T*M is a multiple of NT)
   static_assert((T * M) % NT == 0, "row tile");
+  constexpr int IT = (T * M) / NT;
+  const float* abase = rop.a ? rop.a : in;
+  const long long astride = rop.a ? rop.sa : (long long)rows_per_batch * 2 * M;
+  const bool prod = (rop.kind == 1 || rop.kind == 2) &&
+                    ((reinterpret_cast<unsigned long long>(abase) | reinterpret_cast<unsigned long long>(rop.b) |
+                      (rop.kind == 2 ? (reinterpret_cast<unsigned long long>(rop.c) |
+                                        reinterpret_cast<unsigned long long>(rop.d)) : 0ull)) & 7ull) == 0 &&
+                    ((astride | rop.sb | (rop.kind == 2 ? (rop.sc | rop.sd) : 0ll)) & 1ll) == 0;
+  if (prod) {
+    // Products of maps (a*b, a*b - c*d): every operand read as a float2, and the loads of
+    // PB3D_ROW_GROUP iterations (by default all of them) issued before any is used, so that they
+    // overlap instead of waiting one by one (measured 0.98 -> 0.71 ms per launch for the 15-entry
+    // FT-table rows at 256^3).
+    constexpr int G4 = IT < PB3D_ROW_GROUP ? IT : PB3D_ROW_GROUP;
 #pragma unroll
-  for (int it = 0; it < (T * M) / NT; ++it) {
+    for (int g0 = 0; g0 < IT; g0 += G4) {
+      float2 va[G4], vb[G4], vc[G4], vd[G4];
+#pragma unroll
+      for (int u = 0; u < G4; ++u) {
+        const int f = tid + (g0 + u) * NT;
+        const int r = f / M, m = f % M;
+        const long long row = row0 + r;
+        const bool ok = row < nrows;
+        const long long rowc = ok ? row : 0;
+        const long long bt = rowc / rows_per_batch, rr = rowc % rows_per_batch;
+        const long long vox = rr * (2 * M) + 2 * m;
+        const float2 z = make_float2(0.f, 0.f);
+        va[u] = ok ? *reinterpret_cast<const float2*>(abase + bt * astride + vox) : z;
+        vb[u] = ok ? *reinterpret_cast<const float2*>(rop.b + bt * rop.sb + vox) : z;
+        vc[u] = (ok && rop.kind == 2) ? *reinterpret_cast<const float2*>(rop.c + bt * rop.sc + vox) : z;
+        vd[u] = (ok && rop.kind == 2) ? *reinterpret_cast<const float2*>(rop.d + bt * rop.sd + vox) : z;
+      }
+#pragma unroll
+      for (int u = 0; u < G4; ++u) {
+        const int f = tid + (g0 + u) * NT;
+        const int r = f / M, m = f % M;
+        const float x0 = va[u].x * vb[u].x - vc[u].x * vd[u].x;
+        const float x1 = va[u].y * vb[u].y - vc[u].y * vd[u].y;
+        buf[m * LDT + r] = make_float2(x0 * rop.scale, x1 * rop.scale);
+      }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < (prod ? 0 : IT); ++it) {
     const int f = tid + it * NT;
     const int r = f / M, m = f % M;
     const long long row = row0 + r;
