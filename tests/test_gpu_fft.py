@@ -192,3 +192,57 @@ def test_fused_pointing_chain(hip, n, L):
             r += 0.25 * Yr[m] * o.to_real(o.map_utils.prod_fourier_filter(A, Yk[m], edges[q], edges[q + 1]))
         ref = o.to_fourier(1.5 * mask * r)
         assert rel(got[q], ref) < 2e-5
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512], ids=lambda n: "L%d" % n)
+def test_bkq_prologue_and_packed_epilogue(hip, n):
+    """The first C2R pass with the bispectrum Q-map prologue (table reads batched per thread) equals the
+    element-wise evaluation of the same mode op (pb3d_kmap_packed) followed by a plain C2R, for every
+    pencil length L = n0 of the x pass and every term set (the Theta terms of bspec.py:615-640 with
+    and without the external-leg Legendre factor, and the Y_lm-weighted leg); the last R2C pass with
+    packed output equals a plain R2C gathered at the in-bin modes."""
+    import polybin3d_amd as pb
+    from polybin3d_amd import _capi as C
+    shape = [n, 8, 16] if n == 512 else [n] * 3
+    L = 1000.
+    b = pb.PolyBin3D(L, shape, boxcenter=[300., 200., 500.], sightline='local')
+    kF = 2 * np.pi / L
+    edges = kF * (np.array([1.0, 2.0, 3.0, 3.9]) if n == 512 else (n / 2) * np.array([0.15, 0.35, 0.6, 0.85]))
+    bs = pb.BSpec(b, edges, lmax=2)
+    pk = bs._packed_layout()
+    M, Nq = pk['M'], bs.Nk_squeeze
+    ab = bs.all_bins
+    NB = len(ab)
+    trip = torch.from_numpy(ab.astype(np.int32).reshape(-1)).cuda()
+    pairs = sorted({(int(x), int(y)) for t in ab for x, y in ((t[0], t[1]), (t[1], t[2]), (t[0], t[2]))})
+    tab = np.full(Nq * Nq, -1, dtype=np.int32)
+    for r, (x, y) in enumerate(pairs):
+        tab[x * Nq + y] = r
+    tab = torch.from_numpy(tab).cuda()
+    rng = np.random.RandomState(n)
+    FT = torch.from_numpy((rng.randn(bs.Nl, len(pairs), M) + 1j * rng.randn(bs.Nl, len(pairs), M))
+                          .astype(np.complex64)).cuda()
+    sup = b.support(edges[-1])
+    common = dict(bin_table=bs.table, pack_index=pk['pack_index'], packed_len=M, trip=trip, pair_index=tab,
+                  ft=FT, ft_lstride=len(pairs) * M, nk=Nq, support=sup)
+    modes = pk['modes'].long()
+    ops = [dict(terms=3, leg3_legendre=1), dict(terms=3, leg3_legendre=0), dict(terms=2, leg3_legendre=0),
+           dict(terms=4, ell=2, m=1), dict(terms=4, ell=2, m=4)]
+    for op in ops:
+        kop = C.kop(C.KOP_BKQ, trip0=0, **op, **common)
+        got = b.c2r(None, batch=NB, kop=kop).reshape((NB,) + tuple(shape))
+        packed = torch.empty((NB, M), dtype=torch.complex64, device='cuda')
+        C.check(C.lib().pb3d_kmap_packed(C.ctypes.byref(b.cgrid), None, pk['modes'].data_ptr(), M, packed.data_ptr(),
+                                         NB, C.ctypes.byref(kop), b.stream()))
+        full = torch.zeros((NB, b.H), dtype=torch.complex64, device='cuda')
+        full[:, modes] = packed
+        ref = b.c2r(full.reshape((NB,) + b.half_shape), batch=NB).reshape((NB,) + tuple(shape))
+        err = (torch.linalg.vector_norm((got - ref).reshape(NB, -1), dim=1) /
+               torch.linalg.vector_norm(ref.reshape(NB, -1), dim=1).clamp_min(1e-30)).max().item()
+        assert err < 1e-5, (op, err)
+    # packed epilogue: R2C of real maps, kept at the in-bin modes only
+    x = torch.from_numpy(rng.randn(3, *shape).astype(np.float32)).cuda()
+    out = torch.empty((3, M), dtype=torch.complex64, device='cuda')
+    b.r2c(x, out=out, kop=C.kop(C.KOP_PACK, pack_index=pk['pack_index'], packed_len=M, support=sup))
+    ref = b.r2c(x).reshape(3, -1)[:, modes]
+    assert rel(out.cpu().numpy(), ref.cpu().numpy()) < 1e-6
