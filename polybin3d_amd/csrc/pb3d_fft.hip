@@ -299,6 +299,11 @@ __device__ __forceinline__ void last_stage_to_regs(int t, Src src, Sink sink, co
 #ifndef PB3D_BKQ_BATCH
 #define PB3D_BKQ_BATCH 1
 #endif
+// positions per batch of table reads in the Q-map prologue: 4 keeps the pass at 121 VGPRs (two
+// 512-thread blocks per CU); 8 and 16 need 129 and 143 and measured 30 % slower at 256^3
+#ifndef PB3D_BKQ_GROUP
+#define PB3D_BKQ_GROUP 4
+#endif
 
 // The BKQ prologue of one pencil (kop_value case 4 for the E = L/P positions p = t + e*P of this
 // thread) with its table reads batched: per group of 8 positions, first the bin and pack-index
@@ -308,7 +313,7 @@ template <int L, int E>
 __device__ __forceinline__ void bkq_gather(const KOp& op, const Geo& g, long long bt, long long base, long long S,
                                            int t, bool valid, int rad_in, int mj, int ml, float2 (&vals)[E]) {
   constexpr int P = L / E;
-  constexpr int CH = E < 4 ? E : 4;
+  constexpr int CH = E < PB3D_BKQ_GROUP ? E : PB3D_BKQ_GROUP;
   const int* tr = op.trip + 4 * (long long)(op.trip0 + bt);
   const int b1 = tr[0], b2 = tr[1], b3 = tr[2], ell = tr[3];
   const int pr3 = (op.terms & 5) ? op.pair_index[b1 * op.nk + b2] : -1;
