@@ -46,6 +46,10 @@ bin_reduce_kernel(const void* __restrict__ xv, long long xs, const float2* __res
     const long long e = base + threadIdx.x;
     const bool valid = e < end;
     const int key = valid ? table[e] : PB3D_NO_BIN;
+    // Most of the half grid lies outside every bin (the bins end well below the Nyquist modes): a
+    // wave whose 64 modes are all outside skips the loads and the scan (wave-uniform branch; the
+    // loop holds no block barrier).
+    if (!__any(key != PB3D_NO_BIN)) continue;
     const int row = valid ? (int)(e / g.n2h) : -1;
     float v[kMaxOut];
 #pragma unroll
