@@ -261,6 +261,16 @@ int pb3d_bin_reduce(const pb3d_grid* grid, const void* x, int64_t x_stride, cons
                     const int32_t* ells2, int mode, const uint8_t* bin_table, int nbins,
                     double* out, void* work, void* stream);
 
+/* The same sums (utils.pyx:51-134) visiting the listed modes only: modes = ascending linear
+ * half-grid indices, nmodes <= H, that must
+ * include every mode whose bin_table entry is not PB3D_NO_BIN (modes outside every bin may be
+ * listed; they add nothing).  Work size as pb3d_bin_reduce_work_size().  Same result, without the
+ * pass over the (mostly out-of-bin) rest of the grid. */
+int pb3d_bin_reduce_modes(const pb3d_grid* grid, const void* x, int64_t x_stride, const void* y,
+                          int64_t y_qstride, int64_t y_ostride, int nq, int nout, const int32_t* ells,
+                          const int32_t* ells2, int mode, const uint8_t* bin_table, int nbins,
+                          const int32_t* modes, int64_t nmodes, double* out, void* work, void* stream);
+
 /* ---------------------------------------------------------------- triplet contraction
  * T[t] = sum_x g[i1[t]](x) g[i2[t]](x) g[i3[t]](x) for ntrip index triples into the
  * float32 map stack g[nmaps][nvox] (assemble_b3 / sum_triplet, utils.pyx:357-366,614-636).

@@ -155,6 +155,7 @@ class PolyBin3D:
         self.n_FFTs_forward = 0
         self.n_FFTs_reverse = 0
         self._bin_tables = {}
+        self._bin_modes = {}
         self._bufs = {}
         self.integrator = IntegrationUtils(self)
         self.map_utils = MapUtils(self)
@@ -222,7 +223,15 @@ class PolyBin3D:
             idx = np.searchsorted(edges, self.modk_grid, side='right') - 1
             idx[(idx < 0) | (idx >= nb)] = C.PB3D_NO_BIN
             self._bin_tables[key] = torch.from_numpy(idx.astype(np.uint8)).to(self.device)
+            # the in-bin modes (ascending linear index), for reductions that visit only them
+            inb = np.flatnonzero(idx.reshape(-1) != C.PB3D_NO_BIN).astype(np.int32)
+            self._bin_modes[key] = torch.from_numpy(inb).to(self.device)
         return self._bin_tables[key]
+
+    def bin_modes(self, edges):
+        """int32 device list of the modes inside the bins of `edges` (ascending linear index)."""
+        self.bin_table(edges)
+        return self._bin_modes[tuple(np.asarray(edges, dtype=np.float64).tolist())]
 
     # ------------------------------------------------------------------ buffers
     def real_zeros(self, size=None, numpy=False):
@@ -632,9 +641,11 @@ class PolyBin3D:
         self._need(x, nq * self.H if not real else nq * self.H, 'bin_reduce x')
         if y is not None:
             self._need(y, (nq - 1) * y_qstride + (nout - 1) * y_ostride + self.H, 'bin_reduce y')
-        C.check(C.lib().pb3d_bin_reduce(C.ctypes.byref(self.cgrid), x.data_ptr(), xs, C._ptr(y), y_qstride,
-                                        y_ostride, nq, nout, l1, l2, 1 if real else 0, table.data_ptr(), nb,
-                                        out.data_ptr(), work.data_ptr(), self.stream()))
+        modes = self.bin_modes(edges)
+        C.check(C.lib().pb3d_bin_reduce_modes(C.ctypes.byref(self.cgrid), x.data_ptr(), xs, C._ptr(y), y_qstride,
+                                              y_ostride, nq, nout, l1, l2, 1 if real else 0, table.data_ptr(), nb,
+                                              modes.data_ptr(), int(modes.numel()), out.data_ptr(),
+                                              work.data_ptr(), self.stream()))
         return out
 
 

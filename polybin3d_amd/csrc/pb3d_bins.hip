@@ -25,7 +25,7 @@ template <int MODE>
 __global__ void __launch_bounds__(256)
 bin_reduce_kernel(const void* __restrict__ xv, long long xs, const float2* __restrict__ y, long long yq,
                   long long yo, Ells el, const uint8_t* __restrict__ table, int nbins, Geo g, long long chunk,
-                  double* __restrict__ partial) {
+                  const int32_t* __restrict__ modes, long long count, double* __restrict__ partial) {
   extern __shared__ double hist[];
   const int nout = el.n;
   const int nh = nout * nbins;
@@ -34,7 +34,7 @@ bin_reduce_kernel(const void* __restrict__ xv, long long xs, const float2* __res
   const long long q = blockIdx.y;
   const long long start = (long long)blockIdx.x * chunk;
   long long end = start + chunk;
-  if (end > g.H) end = g.H;
+  if (end > count) end = count;  // count = H, or the length of the mode list
   const int lane = threadIdx.x & 63;
   const long long n12 = (long long)g.n1 * g.n2h;
   bool need_mu = false;
@@ -43,8 +43,11 @@ bin_reduce_kernel(const void* __restrict__ xv, long long xs, const float2* __res
   const float* xr = MODE == 1 ? (const float*)xv + q * xs : nullptr;
   const float2* yb = y ? y + q * yq : nullptr;
   for (long long base = start; base < end; base += blockDim.x) {
-    const long long e = base + threadIdx.x;
-    const bool valid = e < end;
+    const long long idx = base + threadIdx.x;
+    const bool valid = idx < end;
+    // with a mode list the lanes walk the listed (ascending) modes: runs of one (row, bin) stay
+    // contiguous, so the segmented scan below is unchanged
+    const long long e = modes ? (valid ? (long long)modes[idx] : 0) : idx;
     const int key = valid ? table[e] : PB3D_NO_BIN;
     // Most of the half grid lies outside every bin (the bins end well below the Nyquist modes): a
     // wave whose 64 modes are all outside skips the loads and the scan (wave-uniform branch; the
@@ -130,10 +133,10 @@ extern "C" size_t pb3d_bin_reduce_work_size(const pb3d_grid* grid, int nq, int n
   return (size_t)bin_blocks(g.H, nq) * nq * nout * nbins * sizeof(double);
 }
 
-extern "C" int pb3d_bin_reduce(const pb3d_grid* grid, const void* x, int64_t x_stride, const void* y,
-                               int64_t y_qstride, int64_t y_ostride, int nq, int nout, const int32_t* ells,
-                               const int32_t* ells2, int mode, const uint8_t* bin_table, int nbins,
-                               double* out, void* work, void* stream) {
+static int bin_reduce_impl(const pb3d_grid* grid, const void* x, int64_t x_stride, const void* y,
+                           int64_t y_qstride, int64_t y_ostride, int nq, int nout, const int32_t* ells,
+                           const int32_t* ells2, int mode, const uint8_t* bin_table, int nbins,
+                           const int32_t* modes, int64_t nmodes, double* out, void* work, void* stream) {
   if (!grid || !x || !bin_table || !out || !work) return fail(PB3D_ERR_ARG, "bin_reduce: null pointer");
   if (nq < 1 || nq > 65535) return fail(PB3D_ERR_ARG, "bin_reduce: nq out of range [1, 65535]");
   if (nout < 1 || nout > kMaxOut) return fail(PB3D_ERR_ARG, "bin_reduce: nout out of range [1, 16]");
@@ -149,8 +152,10 @@ extern "C" int pb3d_bin_reduce(const pb3d_grid* grid, const void* x, int64_t x_s
       return fail(PB3D_ERR_ARG, "bin_reduce: ell out of range [0, 6]");
   }
   Geo g = make_geo(*grid);
-  const long long gx = bin_blocks(g.H, nq);
-  long long chunk = (g.H + gx - 1) / gx;
+  if (modes && (nmodes < 0 || nmodes > g.H)) return fail(PB3D_ERR_ARG, "bin_reduce: nmodes out of range [0, H]");
+  const long long count = modes ? (long long)nmodes : g.H;
+  const long long gx = bin_blocks(count > 0 ? count : 1, nq);
+  long long chunk = (count + gx - 1) / gx;
   chunk = (chunk + 255) / 256 * 256;
   const int nh = nout * nbins;
   dim3 gr((unsigned)gx, (unsigned)nq);
@@ -158,13 +163,30 @@ extern "C" int pb3d_bin_reduce(const pb3d_grid* grid, const void* x, int64_t x_s
   const size_t shm = (size_t)nh * sizeof(double);
   if (mode == 0)
     bin_reduce_kernel<0><<<gr, 256, shm, s>>>(x, x_stride, (const float2*)y, y_qstride, y_ostride, el, bin_table,
-                                              nbins, g, chunk, (double*)work);
+                                              nbins, g, chunk, modes, count, (double*)work);
   else
     bin_reduce_kernel<1><<<gr, 256, shm, s>>>(x, x_stride, nullptr, 0, 0, el, bin_table, nbins, g, chunk,
-                                              (double*)work);
+                                              modes, count, (double*)work);
   PB3D_CHECK_LAUNCH();
   const long long tot = (long long)nq * nh;
   bin_finalize<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>((const double*)work, gx, nq, nh, out);
   PB3D_CHECK_LAUNCH();
   return 0;
+}
+
+extern "C" int pb3d_bin_reduce(const pb3d_grid* grid, const void* x, int64_t x_stride, const void* y,
+                               int64_t y_qstride, int64_t y_ostride, int nq, int nout, const int32_t* ells,
+                               const int32_t* ells2, int mode, const uint8_t* bin_table, int nbins,
+                               double* out, void* work, void* stream) {
+  return bin_reduce_impl(grid, x, x_stride, y, y_qstride, y_ostride, nq, nout, ells, ells2, mode, bin_table, nbins,
+                         nullptr, 0, out, work, stream);
+}
+
+extern "C" int pb3d_bin_reduce_modes(const pb3d_grid* grid, const void* x, int64_t x_stride, const void* y,
+                                     int64_t y_qstride, int64_t y_ostride, int nq, int nout, const int32_t* ells,
+                                     const int32_t* ells2, int mode, const uint8_t* bin_table, int nbins,
+                                     const int32_t* modes, int64_t nmodes, double* out, void* work, void* stream) {
+  if (!modes) return fail(PB3D_ERR_ARG, "bin_reduce_modes: null mode list");
+  return bin_reduce_impl(grid, x, x_stride, y, y_qstride, y_ostride, nq, nout, ells, ells2, mode, bin_table, nbins,
+                         modes, nmodes, out, work, stream);
 }
