@@ -399,3 +399,40 @@ def test_pk_fisher_mixed_grids(hip):
         np.testing.assert_allclose(fm, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
     with pytest.raises(ValueError):
         pb.compute_fisher_mixed([specs[0], pb.PSpec(specs[0].base, edges, lmax=2)], 1)
+
+
+def test_bin_reduce_full_grid_and_mode_list_agree(hip):
+    """pb3d_bin_reduce (a pass over the whole half grid) and pb3d_bin_reduce_modes (the list of in-bin
+    modes) give the same sums, complex and real modes, with Legendre weights.  Sums are FP32 within a
+    wave's run of one bin and FP64 across runs; the two walks group the modes into different runs,
+    so they agree to FP32 rounding (1e-6 of the largest sum)."""
+    import polybin3d_amd as pb
+    from polybin3d_amd import _capi as C
+    n = 64
+    b = pb.PolyBin3D(L, [n] * 3, boxcenter=[300., 200., 800.], sightline='local')
+    rng = np.random.RandomState(1)
+    x = torch.from_numpy((rng.randn(3, *b.half_shape) + 1j * rng.randn(3, *b.half_shape)).astype(np.complex64)).cuda()
+    y = torch.from_numpy((rng.randn(2, *b.half_shape) + 1j * rng.randn(2, *b.half_shape)).astype(np.complex64)).cuda()
+    xr = torch.from_numpy(rng.rand(3, *b.half_shape).astype(np.float32)).cuda()
+    edges = np.linspace(0.01, 0.12, 12)
+    table, modes = b.bin_table(edges), b.bin_modes(edges)
+    nb = len(edges) - 1
+    for real in (0, 1):
+        xx = xr if real else x
+        yy = None if real else y
+        ells = C.int_array([0, 2]) if not real else C.int_array([2])
+        nout = 1 if real else 2
+        ws = C.lib().pb3d_bin_reduce_work_size(C.ctypes.byref(b.cgrid), 3, nout, nb)
+        outs = []
+        for use_list in (0, 1):
+            work = torch.empty(ws // 8 + 1, dtype=torch.float64, device='cuda')
+            out = torch.empty((3, nout, nb), dtype=torch.float64, device='cuda')
+            args = [C.ctypes.byref(b.cgrid), xx.data_ptr(), b.H, C._ptr(yy), 0, b.H, 3, nout, ells, None, real,
+                    table.data_ptr(), nb]
+            if use_list:
+                C.check(C.lib().pb3d_bin_reduce_modes(*args, modes.data_ptr(), int(modes.numel()), out.data_ptr(),
+                                                      work.data_ptr(), b.stream()))
+            else:
+                C.check(C.lib().pb3d_bin_reduce(*args, out.data_ptr(), work.data_ptr(), b.stream()))
+            outs.append(out.cpu().numpy())
+        np.testing.assert_allclose(outs[1], outs[0], rtol=1e-6, atol=1e-6 * np.abs(outs[0]).max())
