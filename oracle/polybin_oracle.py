@@ -37,10 +37,12 @@ How each part is pinned (tests/test_oracle_*.py):
     the lmax = 4 Fisher matrix's, refined bins summing to coarse ones; the wide-angle terms by a dense
     restatement; derivative_coefficient on hand-evaluated edges; cross_integrate_wcoeff against
     the compiled reference kernel.
+  * PSpec.compute_shot_contribution: per-seed closed form (global, constant mask) and a dense
+    restatement (local, varying and shot-noise masks).
 Parity unpinned (no check beyond restatement and the compiled utils.pyx kernels they call):
 BSpec._compute_fisher per-seed values with a mask or a local sightline (pinned in expectation only
-for the unit mask); PSpec.compute_shot_contribution; BSpec.compute_shot_contribution as a whole (its
-parts, the 1/P_fid GRF draws, A^-1, the g-maps and assemble_bshot, are each pinned above).
+for the unit mask); BSpec.compute_shot_contribution as a whole (its parts, the 1/P_fid GRF draws,
+A^-1, the g-maps and assemble_bshot, are each pinned above).
 """
 import numpy as np
 import scipy.fft

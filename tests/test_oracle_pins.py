@@ -816,3 +816,57 @@ def test_bk_covariance_equals_fisher_with_filter(sightline):
     F = bf.compute_fisher_contribution(2)
     np.testing.assert_allclose(C, F, atol=1e-11 * np.abs(F).max(), rtol=1e-9)
     assert np.abs(F).max() > 0
+
+
+def test_pk_shot_contribution_closed_form_and_dense():
+    """PSpec.compute_shot_contribution (pspec.py:495-551).  Global sightline, constant mask m, trivial
+    S^-1: the GRF a is drawn with P = 1/P_fid and the numerator is
+      shot[(l, b)] = V/(2N^2) sum_{k in b} degk m |a_k|^2 / P_A(k) L_l(mu),  P_A = 1/P_fid (interpolated,
+    held at the table ends);
+    local sightline with a shot-noise mask: the dense restatement
+      shot[(l, b)] = V/(2N^2) sum_{k in b} Re[leg_l(k) conj(FFT[N IFFT(a / P_A)](k))],
+    leg_0 = a, leg_l = sum_x e^{-ik.x} L_l(k.x) IFFT(a)(x)."""
+    n = 16
+    kk = np.linspace(0, 2, 400)
+    Pk = [kk, 1e3 * (1 + kk)]
+    b = O.PolyBin3D(600., [n, n, n], Pk=Pk)
+    edges = np.linspace(0.012, 0.07, 5)
+    m = 0.7
+    ps = O.PSpec(b, edges, lmax=4, mask=m * np.ones((n, n, n)))
+    shot = ps.compute_shot_contribution(4)
+    PkA = [b.Pfid[0], 1. / b.Pfid[1]]
+    a = b.generate_data(seed=4 + int(1e7), Pk_input=PkA, output_type='fourier')
+    PA = np.interp(b.modk_grid, PkA[0], PkA[1])
+    idx = O.bin_index(b.modk_grid, edges)
+    ref = np.zeros_like(shot)
+    for li in range(3):
+        for kb in range(ps.Nk):
+            sel = idx == kb
+            w = b.degeneracy_factor[sel] * m * np.abs(a[sel]) ** 2 / PA[sel] * O.legendre(2 * li, b.muk_grid[sel])
+            ref[li * ps.Nk + kb] = b.volume / (2 * n ** 6) * w.sum()
+    np.testing.assert_allclose(shot, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    # local sightline, varying mask and shot-noise mask: dense restatement
+    n = 8
+    N = n ** 3
+    bl = O.PolyBin3D(200., [n, n, n], boxcenter=[300., 140., 90.], sightline='local')
+    rng = np.random.RandomState(8)
+    mask, mshot = 0.5 + rng.rand(n, n, n), 0.2 + rng.rand(n, n, n)
+    e2 = np.array([1.0, 2.0, 3.0]) * bl.kF[0] + 1e-9
+    psl = O.PSpec(bl, e2, lmax=2, mask=mask, mask_shot=mshot)
+    shot = psl.compute_shot_contribution(3)
+    ph, kv, cosang, fr = _dense_ops(bl)
+    modk = np.linalg.norm(kv, axis=1)
+    PkA = [bl.Pfid[0], 1. / bl.Pfid[1]]
+    ah = bl.generate_data(seed=3 + int(1e7), Pk_input=PkA, output_type='fourier')
+    a = np.fft.fftn(bl.to_real(ah)).reshape(N)
+    ar = (ph.conj().T @ a).real / N
+    PA = np.interp(modk, PkA[0], PkA[1])
+    AiA_r = (ph.conj().T @ (a / PA)).real / N
+    SNA = ph @ (mshot.reshape(N) * AiA_r)
+    legs = {0: a, 2: (ph * O.legendre(2, cosang)) @ ar}
+    ref = np.zeros(2 * 2)
+    for li, ell in enumerate((0, 2)):
+        for kb in range(2):
+            th = (modk >= e2[kb]) & (modk < e2[kb + 1])
+            ref[li * 2 + kb] = 0.5 * bl.volume / N ** 2 * np.sum(th * (legs[ell] * np.conj(SNA)).real)
+    np.testing.assert_allclose(shot, ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max())
