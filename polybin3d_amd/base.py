@@ -199,6 +199,15 @@ class PolyBin3D:
              for i in range(3)]
         return np.meshgrid(*r, indexing='ij')
 
+    @property
+    def modr_grid(self):
+        """|r| of every voxel in FP64 (the attribute pspec.py:110 and bspec.py:135 set for the radial
+        integral constraint and the wide-angle terms), computed once."""
+        if getattr(self, '_modr', None) is None:
+            r = self.r_grids
+            self._modr = np.sqrt(r[0] ** 2. + r[1] ** 2. + r[2] ** 2.)
+        return self._modr
+
     def bin_index_host(self, edges):
         """int64 host array: b with edges[b] <= |k| < edges[b+1] in FP64 (utils.pyx:63), else -1."""
         edges = np.asarray(edges, dtype=np.float64)

@@ -39,8 +39,7 @@ class Pointing:
             raise ValueError("radial_bins_RIC: 1 to %d radial bins supported" % MAX_RADIAL_BINS)
         if b.sightline != 'local':
             raise ValueError("the radial integral constraint needs the local sightline's coordinate grid")
-        r = b.r_grids
-        modr = np.sqrt(r[0] ** 2. + r[1] ** 2. + r[2] ** 2.)
+        modr = b.modr_grid
         z = np.full(modr.shape, nb, dtype=np.int64)
         for iz in range(nb):
             z[(modr >= edges[iz]) & (modr < edges[iz + 1])] = iz

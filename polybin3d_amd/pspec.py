@@ -628,8 +628,7 @@ class PSpec:
             r = Ainv_real
             if kind == 'harm_r':
                 if not hasattr(self, '_inv_r'):
-                    rg = b.r_grids
-                    self._inv_r = b.as_real(1. / np.sqrt(rg[0] ** 2. + rg[1] ** 2. + rg[2] ** 2.))
+                    self._inv_r = b.as_real(1. / b.modr_grid)
                 r = Ainv_real * self._inv_r
             b.apply_fourier_harmonics(r, ell, ell, out=L[t])
             if phase != 1.:
