@@ -259,3 +259,47 @@ def test_bk_pixel_windows(hip, window):
     bsm, bom = pb.BSpec(b, edges, lmax=2, mask=mask), O.BSpec(o, edges, lmax=2, mask=mask)
     F1, F2 = bsm.compute_fisher_contribution(1), bom.compute_fisher_contribution(1)
     close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+
+
+def test_bk_mc_fisher_squeezed_partial(hip):
+    """MC Fisher per seed with squeezed bins (k_bins_squeeze) and partial triangles
+    (bspec.py:31-170, 473-676), masked, global sightline, against the oracle."""
+    import polybin3d_amd as pb
+    n = 16
+    edges = np.linspace(0.007, 0.035, 3)
+    sq = np.append(edges, [0.042, 0.049])
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk), O.PolyBin3D(L, [n] * 3, Pk=Pk)
+    mask = shell_mask(n, L, [0., 0., 0.])
+    for partial in (False, True):
+        bs = pb.BSpec(b, edges, lmax=2, mask=mask, k_bins_squeeze=sq, include_partial_triangles=partial)
+        bo = O.BSpec(o, edges, lmax=2, mask=mask, k_bins_squeeze=sq, include_partial_triangles=partial)
+        assert bs.N_bins == bo.N_bins and np.array_equal(bs.all_bins, bo.all_bins)
+        F1, F2 = bs.compute_fisher_contribution(2), bo.compute_fisher_contribution(2)
+        close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+
+
+def test_non_cubic_grid_and_box(hip):
+    """Anisotropic grid and box (kF and kNy differ per axis; base.py:63-94): Pk_ideal, a masked Pk MC
+    Fisher realization (local sightline) and Bk_ideal against the oracle."""
+    import polybin3d_amd as pb
+    shape, box = [32, 64, 16], [1000., 2000., 500.]
+    kw = dict(boxcenter=[10., 20., 30.], sightline='local')
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(box, shape, Pk=Pk, **kw), O.PolyBin3D(box, shape, Pk=Pk, **kw)
+    rng = np.random.RandomState(3)
+    d = rng.randn(*shape)
+    d = d + 0.3 * (d ** 2 - 1)
+    edges = np.linspace(0.013, 0.09, 6)
+    got, ref = pb.PSpec(b, edges, lmax=2).Pk_ideal(d), O.PSpec(o, edges, lmax=2).Pk_ideal(d)
+    for k in ref:
+        close(got[k], ref[k], scale=np.abs(ref['p0']))
+    mask = 0.3 + rng.rand(*shape)
+    F1 = pb.PSpec(b, edges, lmax=2, mask=mask).compute_fisher_contribution(0)
+    F2 = O.PSpec(o, edges, lmax=2, mask=mask).compute_fisher_contribution(0)
+    close(F1, F2, scale=fisher_scale(F2))
+    e3 = np.linspace(0.013, 0.09, 4)
+    gb, rb = pb.BSpec(b, e3, lmax=2).Bk_ideal(d), O.BSpec(o, e3, lmax=2).Bk_ideal(d)
+    for k in rb:
+        s = np.abs(rb['b0'])
+        close(gb[k], rb[k], tol=2e-5, scale=np.maximum(s, 1e-3 * s.max()))
