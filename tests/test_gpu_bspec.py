@@ -303,3 +303,31 @@ def test_non_cubic_grid_and_box(hip):
     for k in rb:
         s = np.abs(rb['b0'])
         close(gb[k], rb[k], tol=2e-5, scale=np.maximum(s, 1e-3 * s.max()))
+
+
+def test_bk_linear_term_sims(hip):
+    """The linear term of the Bk numerator (bspec.py:281-410, 963-978), local sightline, mask, pixel
+    window: sims generated with an input spectrum (streamed, preload=False) and sims handed in by a
+    loader (load_sims, preloaded and streamed), against the oracle with the same sims."""
+    import polybin3d_amd as pb
+    n = 16
+    edges = np.linspace(0.007, 0.05, 4)
+    kw = dict(boxcenter=[10., 10., 10.], sightline='local', pixel_window='cic')
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    PkS = [np.linspace(0, 2, 500), 2e3 / (1 + (np.linspace(0, 2, 500) / 0.04) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    mask = shell_mask(n, L, [10., 10., 10.])
+    d = ng_field((n, n, n), 5) * mask
+    bs, bo = pb.BSpec(b, edges, lmax=2, mask=mask), O.BSpec(o, edges, lmax=2, mask=mask)
+    bs.generate_sims(3, Pk_input=PkS, preload=False)
+    bo.generate_sims(3, Pk_input=PkS)
+    n1, n2 = bs.Bk_numerator(d), bo.Bk_numerator(d)
+    close(n1, n2, tol=2e-5, scale=np.maximum(np.abs(n2), 1e-3 * np.abs(n2).max()))
+    sims = [ng_field((n, n, n), 10 + i) * mask for i in range(3)]
+    bo.g_bl_maps = np.asarray([bo._process_sim(s, input_type='real') for s in sims])
+    bo.N_it = 3
+    ref = bo.Bk_numerator(d)
+    for preload in (True, False):
+        bl = pb.BSpec(b, edges, lmax=2, mask=mask)
+        bl.load_sims(lambda ii: sims[ii], 3, input_type='real', preload=preload)
+        close(bl.Bk_numerator(d), ref, tol=2e-5, scale=np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max()))
