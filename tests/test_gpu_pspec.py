@@ -461,3 +461,31 @@ def test_pk_pixel_windows(hip, window):
     close(n1, n2, scale=np.abs(n2[:ps.Nk]).max())
     F1, F2 = ps.compute_fisher_contribution(2), po.compute_fisher_contribution(2)
     close(F1, F2, scale=_diag_scale(F2))
+
+
+def test_pk_unwindowed_shot_subtraction(hip):
+    """Pk_unwindowed with a shot-noise numerator and subtract_shotnoise (pspec.py:571-611), masked,
+    local sightline, against the oracle; without a Fisher matrix it raises as the reference does."""
+    import polybin3d_amd as pb
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    kw = dict(boxcenter=[10., 10., 10.], sightline='local')
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    mask = shell_mask(n, L, [10., 10., 10.])
+    ps = pb.PSpec(b, edges, lmax=2, mask=mask, mask_shot=mask)
+    po = O.PSpec(o, edges, lmax=2, mask=mask, mask_shot=mask)
+    d = powerlaw_field((n, n, n), seed=6) * mask
+    with pytest.raises(Exception):
+        ps.Pk_unwindowed(d)
+    fish = po.compute_fisher(2)
+    shot = po.compute_shot_contribution(7)
+    got = ps.Pk_unwindowed(d, fish=fish, shot_num=shot, subtract_shotnoise=True)
+    ref = po.Pk_unwindowed(d, fish=fish, shot_num=shot, subtract_shotnoise=True)
+    for k in ref:
+        close(got[k], ref[k], tol=2e-5, scale=np.abs(ref['p0']))
+    # the shot term is stored and reused by a later call that only asks for the subtraction
+    again = ps.Pk_unwindowed(d, subtract_shotnoise=False)
+    plain = po.Pk_unwindowed(d, fish=fish)
+    for k in plain:
+        close(again[k], plain[k], tol=2e-5, scale=np.abs(plain['p0']))
