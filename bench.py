@@ -168,10 +168,12 @@ def main():
     # odd sampling period: launches of one kind come in patterns of even length (sub-maps, y passes of
     # both directions), which an even period would sample at one phase only
     ap.add_argument("--prof-period", type=int, default=7)
-    ap.add_argument("--workload", default="mc_fisher", choices=["mc_fisher", "bk_ideal", "pk_mc_fisher"],
+    ap.add_argument("--workload", default="mc_fisher", choices=["mc_fisher", "bk_ideal", "pk_mc_fisher", "extras"],
                     help="mc_fisher: the Pk+Bk MC Fisher metric (default); bk_ideal: Bk_ideal evaluations "
                          "and the MFMA triplet contraction (--n, --bk-bins, --lmax, --precision); pk_mc_fisher: "
-                         "PSpec MC Fisher realizations at 128^3 and 256^3 (--realizations per call)")
+                         "PSpec MC Fisher realizations at 128^3 and 256^3 (--realizations per call); extras: "
+                         "seconds per call of the members around the MC Fisher (shot noise, linear term, "
+                         "integral constraints, covariance and theory matrices) at 256^3")
     ap.add_argument("--realizations", type=int, default=1,
                     help="pk_mc_fisher: realizations per compute_fisher batch")
     ap.add_argument("--mixed", action="store_true",
@@ -194,6 +196,8 @@ def main():
         return bk_ideal_workload(args, torch, pb)
     if args.workload == "pk_mc_fisher":
         return pk_fisher_workload(args, torch, pb)
+    if args.workload == "extras":
+        return extras_workload(args, torch, pb)
 
     n, L, center = args.n, 1000., [10., 10., 10.]
     kny = n * np.pi / L
@@ -315,6 +319,70 @@ def pk_fisher_workload(args, torch, pb):
     if not args.mixed:
         res["ms_per_realization"] = {str(n): round(1e3 * per[n] / (args.steps * R), 2) for n in grids}
     print(json.dumps(res), flush=True)
+
+
+def extras_workload(args, torch, pb):
+    """Seconds per call (median of args.steps after args.warmup) of the estimator members around the
+    MC Fisher, on the metric's 256^3 configuration (local sightline, shell mask, device GRFs):
+    PSpec (lmax 4, 30 bins to 0.30): shot-noise numerator, MC Fisher with the global and the radial
+    integral constraint, covariance contribution (P_0, P_2), theory contribution (60 theory bins,
+    lmax_theory 4, wide-angle); BSpec (lmax 2, 40 bins to 0.40, 12 700 triangles): numerator with the
+    linear term of 4 sims, shot-noise numerator (cubic and quadratic terms); BSpec covariance on a
+    smaller case (64^3, 8 bins to 0.16, lmax 2), whose Q maps pass through the filter one at a time."""
+    n, L, center = args.n, 1000., [10., 10., 10.]
+    kw = dict(boxcenter=center, sightline='local', rng='device')
+    b = pb.PolyBin3D(L, [n] * 3, Pk=powerlaw_pk(2 * n * np.pi / L * 1.8), **kw)
+    mask = shell_mask(n, L, center)
+    kk = np.linspace(0., 2. * n * np.pi / L, 4000)
+    Pc = [kk, 2e4 / (1 + (kk / 0.04) ** 2), 5e3 / (1 + (kk / 0.05) ** 2)]
+    res = {}
+
+    def timeit(name, f):
+        for w in range(args.warmup):
+            f(10 ** 6 + w)
+        ts = []
+        for s in range(args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            f(s)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        res[name] = round(float(np.median(ts)), 4)
+        print(json.dumps({name: res[name]}), flush=True)
+
+    pk_bins = np.linspace(0.01, 0.30, 31)
+    ps = pb.PSpec(b, pk_bins, lmax=4, mask=mask, mask_shot=mask)
+    timeit("pk_shot_contribution", lambda s: ps.compute_shot_contribution(s))
+    timeit("pk_covariance_contribution", lambda s: ps.compute_covariance_contribution(s, Pc))
+    timeit("pk_theory_contribution_wideangle", lambda s: ps.compute_theory_contribution(
+        s, np.linspace(0.01, 0.30, 61), lmax_theory=4, include_wideangle=True))
+    x = np.fft.fftshift(np.arange(-n // 2, n // 2)) * L / n + center[0]
+    X = np.meshgrid(x, x, x, indexing='ij')
+    modr = np.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
+    del X
+    pg = pb.PSpec(b, pk_bins, lmax=4, mask=mask, add_GIC=True, mask_IC=mask)
+    timeit("pk_mc_fisher_gic", lambda s: pg._fisher_contribution_dev(s))
+    pr = pb.PSpec(b, pk_bins, lmax=4, mask=mask, add_RIC=True, mask_IC=mask,
+                  radial_bins_RIC=np.linspace(modr[mask > 0.5].min(), modr[mask > 0.5].max() + 1., 9))
+    timeit("pk_mc_fisher_ric", lambda s: pr._fisher_contribution_dev(s))
+    del ps, pg, pr
+    bs = pb.BSpec(b, np.linspace(0.01, 0.40, 41), lmax=2, mask=mask, mask_shot=mask)
+    data = b.generate_data(seed=5, output_type='real') * b.as_real(mask)
+    bs.generate_sims(4)
+    timeit("bk_numerator_linear_term_4sims", lambda s: bs.Bk_numerator(data))
+    timeit("bk_shot_contribution", lambda s: bs.compute_shot_contribution(s, data=data))
+    del bs
+    nb = 64
+    b64 = pb.PolyBin3D(L, [nb] * 3, Pk=powerlaw_pk(2 * nb * np.pi / L * 1.8), **kw)
+    m64 = shell_mask(nb, L, center)
+    bc = pb.BSpec(b64, np.linspace(0.01, 0.16, 9), lmax=2, mask=m64, mask_shot=m64)
+    timeit("bk_covariance_contribution_64cubed_%dbins" % bc.N_bins, lambda s: bc.compute_covariance_contribution(s, Pc))
+    out = {"metric": "seconds per call of the members around the MC Fisher", "unit": "s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "data": "synthetic",
+           "dtype": "f32", "config": {"workload": "extras_256", "grid": [n] * 3, "boxsize": L, "sightline": "local",
+                                      "mask": "shell", "rng": "device"},
+           "seconds": res}
+    print(json.dumps(out), flush=True)
 
 
 def bk_ideal_workload(args, torch, pb):

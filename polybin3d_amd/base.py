@@ -511,17 +511,23 @@ class PolyBin3D:
         (applyAinv, base.py:364-389)."""
         if invPk0_grid is None and self.pixel_window == 'none':
             return None if self.invPk0_is_one else self.invPk0_grid
-        key = ('ainv', id(invPk0_grid))
-        if key not in self._bufs:
-            inv = self.invPk0_grid_host if invPk0_grid is None else np.asarray(invPk0_grid)
-            w = inv.copy()
-            if self.pixel_window != 'none':
-                pw = self.pixel_window_grid_host
-                nz = pw != 0
-                w[nz] = w[nz] / pw[nz] ** 2
-                w[~nz] = 0.
-            self._bufs[key] = torch.from_numpy(w.astype(np.float32)).to(self.device)
-        return self._bufs[key]
+        # cached per array: the entry keeps the array it was made from, so that an id reused by another
+        # (temporary) array is never served a stale table; a few arrays are kept
+        cache = self._bufs.setdefault('ainv_weights', [])
+        for arr, t in cache:
+            if arr is invPk0_grid:
+                return t
+        inv = self.invPk0_grid_host if invPk0_grid is None else np.asarray(invPk0_grid)
+        w = inv.copy()
+        if self.pixel_window != 'none':
+            pw = self.pixel_window_grid_host
+            nz = pw != 0
+            w[nz] = w[nz] / pw[nz] ** 2
+            w[~nz] = 0.
+        t = torch.from_numpy(w.astype(np.float32)).to(self.device)
+        cache.append((invPk0_grid, t))
+        del cache[:-4]
+        return t
 
     def applyAinv(self, input_data, invPk0_grid=None, input_type='real', output_type='real'):
         """base.py:364-389: multiply by invPk0 and divide by the pixel window twice."""

@@ -270,15 +270,7 @@ class BSpec:
     def _filtered_data(self, data, filtering):
         b = self.base
         if filtering == 'ideal':
-            w = self.invPk0
-            if b.pixel_window != 'none':
-                key = ('bk_ideal_pw', id(self))
-                if key not in b._bufs:
-                    pw = b.pixel_window_grid_host
-                    ww = np.zeros(pw.shape)
-                    ww[pw != 0] = self.invPk0_host[pw != 0] / pw[pw != 0]
-                    b._bufs[key] = torch.from_numpy(ww.astype(np.float32)).to(b.device)
-                w = b._bufs[key]
+            w = self.invPk0 if b.pixel_window == 'none' else self._filtered_weight_pw()
             return b.r2c(b.as_real(data), kop=C.kop(C.KOP_WEIGHT, w=w, scale=1. / self.cube_mask_mean ** (1. / 3.)))
         if b.pixel_window != 'none':
             pw = b.pixel_window_grid_host
@@ -440,14 +432,14 @@ class BSpec:
         return b.c2r(Z)
 
     def _filtered_weight_pw(self):
-        b = self.base
-        key = ('bk_ideal_pw', id(self))
-        if key not in b._bufs:
+        """invPk0 / W as a device FP32 half grid (the ideal filter with a pixel window), made once."""
+        if getattr(self, '_w_pw', None) is None:
+            b = self.base
             pw = b.pixel_window_grid_host
             ww = np.zeros(pw.shape)
             ww[pw != 0] = self.invPk0_host[pw != 0] / pw[pw != 0]
-            b._bufs[key] = torch.from_numpy(ww.astype(np.float32)).to(b.device)
-        return b._bufs[key]
+            self._w_pw = torch.from_numpy(ww.astype(np.float32)).to(b.device)
+        return self._w_pw
 
     # ------------------------------------------------------------------ linear-term simulations
     def apply_pointing(self, map_real, transpose=False):

@@ -47,6 +47,13 @@ class Pointing:
         self.den = np.bincount(z.reshape(-1), weights=mic.reshape(-1), minlength=nb + 1)[:nb]
         self.rbin = torch.from_numpy(np.where(z < nb, z, 255).astype(np.uint8)).to(b.device).contiguous()
         self.rbin_long = torch.from_numpy(z.reshape(-1)).to(b.device)
+        # the voxels of each radial bin as one segment of a fixed permutation: per-bin sums are segment
+        # sums of the permuted map (an atomic histogram into a few bins serialises on the GPU)
+        zf = z.reshape(-1)
+        order = np.argsort(zf, kind='stable')
+        counts = np.bincount(zf, minlength=nb + 1)[:nb]
+        self.rperm = torch.from_numpy(order[:int(counts.sum())].astype(np.int64)).to(b.device)
+        self.rcounts = torch.from_numpy(counts.astype(np.int64)).to(b.device)
         self.mask_eff = b.as_real(np.ascontiguousarray(mask * (z < nb)))
 
     # ---------------------------------------------------------------- voxel-space application
@@ -56,7 +63,7 @@ class Pointing:
         if self.kind == 'GIC':
             s = mw.sum().reshape(1)
         else:
-            s = torch.bincount(self.rbin_long, weights=mw, minlength=self.nbins + 1)[:self.nbins]
+            s = torch.segment_reduce(mw[self.rperm], 'sum', lengths=self.rcounts)
         return s / torch.from_numpy(self.den).to(s.device)
 
     def _per_voxel(self, av):

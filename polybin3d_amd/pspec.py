@@ -188,8 +188,7 @@ class PSpec:
         """a(k)^2 for the ideal filter S = a(k) d(k) (a = invPk0 [/W] / sqrt(<mask^2>)), FP32 device
         half grid, or None when a is the constant 1/sqrt(<mask^2>)."""
         b = self.base
-        key = ('pk_ideal_w2', id(self))
-        if key not in b._bufs:
+        if not hasattr(self, '_w2'):
             if b.pixel_window != 'none':
                 pw = b.pixel_window_grid_host
                 w = np.zeros(pw.shape)
@@ -198,9 +197,9 @@ class PSpec:
                 w = None
             else:
                 w = b.invPk0_grid_host
-            b._bufs[key] = None if w is None else \
+            self._w2 = None if w is None else \
                 torch.from_numpy((w ** 2 / self.sq_mask_mean).astype(np.float32)).to(b.device)
-        return b._bufs[key]
+        return self._w2
 
     def _pk_ideal_adjoint(self, data, grad_pk):
         """d(grad_pk . Pk_ideal)/d(data) for a real map `data` (global sightline): with v = F^-T grad_pk,
@@ -671,13 +670,16 @@ class PSpec:
         if self.user_Sinv:
             raise NotImplementedError("compute_shot_contribution with a user S^-1 is not implemented by backend='hip'")
         PkA = [b.Pfid[0], 1. / b.Pfid[1]]
-        modk = b.modk_grid
-        PkA_grid = scipy.interpolate.interp1d(PkA[0], PkA[1], bounds_error=False,
-                                              fill_value=(PkA[1][0], PkA[1][-1]))(modk)
-        invPkA = np.zeros_like(PkA_grid)
-        invPkA[modk != 0] = 1. / PkA_grid[modk != 0]
+        if getattr(self, '_invPkA', None) is None:
+            # 1/P_A on the grid, interpolated once (the weight is cached by the array's identity)
+            modk = b.modk_grid
+            PkA_grid = scipy.interpolate.interp1d(PkA[0], PkA[1], bounds_error=False,
+                                                  fill_value=(PkA[1][0], PkA[1][-1]))(modk)
+            invPkA = np.zeros_like(PkA_grid)
+            invPkA[modk != 0] = 1. / PkA_grid[modk != 0]
+            self._invPkA = invPkA
         a = b.generate_data(seed=seed + int(1e7), Pk_input=PkA, output_type='fourier')
-        wA = b.ainv_weight(invPkA)
+        wA = b.ainv_weight(self._invPkA)
         AiA = b.kmap(a, C.kop(C.KOP_WEIGHT, w=wA))
         if self.const_mask:
             SNA = b.kmap(AiA, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
