@@ -61,30 +61,41 @@ __device__ __forceinline__ f32x16 mma_split(const bf16x8* a, const bf16x8* b, f3
 
 // ------------------------------------------------------------------------------------------
 // Forward.  Block (blockIdx.x = group of kPgWaves N-tiles of 32 pairs, blockIdx.y = voxel chunk):
-// stages kPgStage voxels of every map of the stack in LDS (the next stage's loads are issued into
+// stages kPgStage = ST (32, 64 or 128) voxels of every map of the stack in LDS (the next stage's loads go into
 // registers before the current stage is used), splits the A rows once into bf16 parts shared by
 // the waves, then each wave forms the products of its 32 pairs and runs NP-part MFMAs for every
 // M-tile, kPgStage/kPgKS k-steps per stage.  Partial T of the chunk -> part[chunk][a][p] (FP64),
 // accumulated in two FP32 levels (MFMA accumulator over 1024 voxels, then a second accumulator).
-// LDS (dynamic): sg[nmaps][kPgStage] floats, then sa[NP][MT*32][kPgStage/8] bf16x8.
+// LDS (dynamic): sg[nmaps][kPgStage + 4] floats, then sa[NP][MT*32][kPgStage/8 + SAP] bf16x8.  The
+// row pads put consecutive rows 16 bytes apart in the bank space: the lanes of a k-step read 8
+// consecutive floats of 32 different rows (the pair products, consecutive pairs share b and take
+// consecutive c) and 16 bytes of 32 consecutive A rows, which an unpadded power-of-two row length
+// maps onto the same banks.
 // ------------------------------------------------------------------------------------------
-constexpr int kPgStage = 32;                          // voxels per LDS stage
-constexpr int kPgFlush = 32;                          // stages (1024 voxels) per FP64 flush
+constexpr int kPgFlushVox = 1024;                     // voxels per first-level accumulation
+constexpr size_t kPgLdsBudget = 64 * 1024;            // dynamic LDS per block (two blocks per CU)
+constexpr int kPgSgPad = 4;                           // floats of pad per staged map row
 
-static size_t pg_fwd_lds(int nmaps, int np_parts, int mt) {
-  return (size_t)nmaps * kPgStage * sizeof(float) + (size_t)np_parts * mt * 32 * (kPgStage / 8) * 16;
+static size_t pg_fwd_lds(int nmaps, int np_parts, int mt, int stage, int sap) {
+  return (size_t)nmaps * (stage + kPgSgPad) * sizeof(float) + (size_t)np_parts * mt * 32 * (stage / 8 + sap) * 16;
 }
 
-// PF = prefetch registers per thread: nmaps * kPgStage <= PF * kPgThreads
-template <int NP, int MT, int PF>
+// ST = voxels per LDS stage (a stage costs three block barriers, its global loads and the A split,
+// so small stacks take long stages); PF = prefetch registers per thread: nmaps * ST <= PF * kPgThreads;
+// SAP = pad (bf16x8) per split A row (0 or 1)
+template <int NP, int MT, int ST, int PF, int SAP>
 __global__ void __launch_bounds__(kPgThreads)
 pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int* __restrict__ amap, int na,
               const int* __restrict__ pb, const int* __restrict__ pc, int np, long long chunk,
               double* __restrict__ part) {
+  constexpr int kPgStage = ST;
+  constexpr int kPgFlush = kPgFlushVox / ST;  // stages per flush of the first-level accumulator
   extern __shared__ __align__(16) char smem[];
-  float* sg = reinterpret_cast<float*>(smem);                                         // [nmaps][kPgStage]
-  bf16x8* sa = reinterpret_cast<bf16x8*>(smem + (size_t)nmaps * kPgStage * sizeof(float));
+  constexpr int SGS = kPgStage + kPgSgPad;                                             // sg row stride
+  float* sg = reinterpret_cast<float*>(smem);                                         // [nmaps][SGS]
+  bf16x8* sa = reinterpret_cast<bf16x8*>(smem + (size_t)nmaps * SGS * sizeof(float));
   constexpr int K8 = kPgStage / 8;                                                     // bf16x8 per row
+  constexpr int K8S = K8 + SAP;                                                        // sa row stride
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const long long x0 = (long long)blockIdx.y * chunk;
   long long x1 = x0 + chunk;
@@ -126,7 +137,7 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int e = tid + i * kPgThreads;
-      if (e < nel) sg[e] = pf[i];
+      if (e < nel) sg[(e / kPgStage) * SGS + e % kPgStage] = pf[i];
     }
     if (xs + kPgStage < x1) fetch(xs + kPgStage);  // next stage in flight while this one computes
     __syncthreads();
@@ -139,28 +150,34 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
       if (abad) am = 0;
 #pragma unroll
       for (int r = 0; r < 8; ++r)
-        v[r] = row >= na ? 0.f : (abad ? __builtin_nanf("") : sg[am * kPgStage + kb * 8 + r]);
+        v[r] = row >= na ? 0.f : (abad ? __builtin_nanf("") : sg[am * SGS + kb * 8 + r]);
       bf16x8 parts[NP];
       split8<NP>(v, parts);
 #pragma unroll
-      for (int q = 0; q < NP; ++q) sa[((long long)q * MT * 32 + row) * K8 + kb] = parts[q];
+      for (int q = 0; q < NP; ++q) sa[(q * MT * 32 + row) * K8S + kb] = parts[q];
     }
     __syncthreads();  // sa ready
-#pragma unroll
+#pragma unroll 2  // (full unrolling of the longer stages costs an occupancy step, no speed)
     for (int ks = 0; ks < kPgStage / kPgKS; ++ks) {
       // B operand: this lane's 8 pair products of k-step ks
       float pr[8];
       const int k0 = ks * kPgKS + kh;
+      {  // two 16-byte LDS reads per operand (rows and k0 are multiples of 4 floats)
+        const float4* rb = reinterpret_cast<const float4*>(sg + mb * SGS + k0);
+        const float4* rc = reinterpret_cast<const float4*>(sg + mc * SGS + k0);
+        const float4 b0 = rb[0], b1 = rb[1], c0 = rc[0], c1 = rc[1];
+        const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+        const float cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
-        pr[r] = !pvalid ? 0.f : (pbad ? __builtin_nanf("") : sg[mb * kPgStage + k0 + r] * sg[mc * kPgStage + k0 + r]);
+        for (int r = 0; r < 8; ++r) pr[r] = !pvalid ? 0.f : (pbad ? __builtin_nanf("") : bv[r] * cv[r]);
+      }
       bf16x8 bparts[NP];
       split8<NP>(pr, bparts);
 #pragma unroll
       for (int t = 0; t < MT; ++t) {
         bf16x8 aparts[NP];
 #pragma unroll
-        for (int q = 0; q < NP; ++q) aparts[q] = sa[((long long)q * MT * 32 + t * 32 + lane % 32) * K8 + k0 / 8];
+        for (int q = 0; q < NP; ++q) aparts[q] = sa[(q * MT * 32 + t * 32 + lane % 32) * K8S + k0 / 8];
         acc[t] = mma_split<NP>(aparts, bparts, acc[t]);
       }
     }
@@ -316,20 +333,33 @@ extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int
   hipStream_t s = (hipStream_t)stream;
   const int ngroups = (np + 32 * kPgWaves - 1) / (32 * kPgWaves);
   const long long nc = pg_chunks(nvox, ngroups);
+  const int mt = pg_mtiles(na);
+  // stack-size bucket (maps <= 32, 64, 96, 128, 256) and stage length: the long stage of a bucket
+  // (128 voxels for <= 32 maps, 64 for <= 128) when its LDS fits kPgLdsBudget, else 32 voxels;
+  // prefetch registers = bucket maps * stage / threads.  The largest stacks keep the A rows unpadded.
+  const int pfb = nmaps <= 32 ? 0 : nmaps <= 64 ? 1 : nmaps <= 96 ? 2 : nmaps <= 128 ? 3 : 4;
+  const int nparts = precision == 2 ? 3 : 2, mtl = mt < 4 ? mt : 4, sap = pfb == 4 ? 0 : 1;
+  int stage = pfb == 0 ? 128 : pfb <= 3 ? 64 : 32;
+  if (pg_fwd_lds(nmaps, nparts, mtl, stage, sap) > kPgLdsBudget) stage = 32;
+  const size_t lds = pg_fwd_lds(nmaps, nparts, mtl, stage, sap);
+  if (lds > kPgLdsBudget) return fail(PB3D_ERR_ARG, "pair_gemm: stack does not fit in LDS");
   long long chunk = (nvox + nc - 1) / nc;
-  chunk = (chunk + kPgStage - 1) / kPgStage * kPgStage;
+  chunk = (chunk + stage - 1) / stage * stage;
   dim3 grid((unsigned)ngroups, (unsigned)nc);
   double* part = (double*)work;
-  const int mt = pg_mtiles(na);
-  const size_t lds = pg_fwd_lds(nmaps, precision == 2 ? 3 : 2, mt < 4 ? mt : 4);
-  const int pfb = nmaps <= 32 ? 0 : nmaps <= 64 ? 1 : nmaps <= 128 ? 2 : 3;  // prefetch depth bucket
-#define PB3D_PG3(NP, MT, PF) pair_gemm_fwd<NP, MT, PF><<<grid, kPgThreads, lds, s>>>(g, nmaps, nvox, amap, na, pb, pc, np, chunk, part)
-#define PB3D_PG(NP, MT)                                                   \
-  switch (pfb) {                                                          \
-    case 0: PB3D_PG3(NP, MT, 32 * kPgStage / kPgThreads); break;          \
-    case 1: PB3D_PG3(NP, MT, 64 * kPgStage / kPgThreads); break;          \
-    case 2: PB3D_PG3(NP, MT, 128 * kPgStage / kPgThreads); break;         \
-    default: PB3D_PG3(NP, MT, 256 * kPgStage / kPgThreads); break;        \
+#define PB3D_PG3(NP, MT, ST, PF, SAP) \
+  pair_gemm_fwd<NP, MT, ST, PF, SAP><<<grid, kPgThreads, lds, s>>>(g, nmaps, nvox, amap, na, pb, pc, np, chunk, part)
+#define PB3D_PG(NP, MT)                                                     \
+  switch (pfb * 2 + (stage == 32)) {                                        \
+    case 0: PB3D_PG3(NP, MT, 128, 32 * 128 / kPgThreads, 1); break;         \
+    case 1: PB3D_PG3(NP, MT, 32, 32 * 32 / kPgThreads, 1); break;           \
+    case 2: PB3D_PG3(NP, MT, 64, 64 * 64 / kPgThreads, 1); break;           \
+    case 3: PB3D_PG3(NP, MT, 32, 64 * 32 / kPgThreads, 1); break;           \
+    case 4: PB3D_PG3(NP, MT, 64, 96 * 64 / kPgThreads, 1); break;           \
+    case 5: PB3D_PG3(NP, MT, 32, 96 * 32 / kPgThreads, 1); break;           \
+    case 6: PB3D_PG3(NP, MT, 64, 128 * 64 / kPgThreads, 1); break;          \
+    case 7: PB3D_PG3(NP, MT, 32, 128 * 32 / kPgThreads, 1); break;          \
+    default: PB3D_PG3(NP, MT, 32, 256 * 32 / kPgThreads, 0); break;         \
   }
   if (precision == 1) {
     switch (mt) { case 1: PB3D_PG(2, 1); break; case 2: PB3D_PG(2, 2); break; case 3: PB3D_PG(2, 3); break; default: PB3D_PG(2, 4); }
