@@ -103,6 +103,7 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
   // this lane's pair (column j = lane%32 of the wave's N-tile)
   const int p = (blockIdx.x * kPgWaves + wave) * 32 + lane % 32;
   const bool pvalid = p < np;
+  const bool wave_on = (blockIdx.x * kPgWaves + wave) * 32 < np;  // wave-uniform
   int mb = pvalid ? pb[p] : 0, mc = pvalid ? pc[p] : 0;
   // an index outside the stack gives NaN for that pair (no stray LDS read)
   const bool pbad = mb < 0 || mb >= nmaps || mc < 0 || mc >= nmaps;
@@ -157,8 +158,9 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
       for (int q = 0; q < NP; ++q) sa[(q * MT * 32 + row) * K8S + kb] = parts[q];
     }
     __syncthreads();  // sa ready
+    // a wave whose 32 pairs all lie beyond np only helps stage (uniform skip of its MFMAs)
 #pragma unroll 2  // (full unrolling of the longer stages costs an occupancy step, no speed)
-    for (int ks = 0; ks < kPgStage / kPgKS; ++ks) {
+    for (int ks = 0; ks < (wave_on ? kPgStage / kPgKS : 0); ++ks) {
       // B operand: this lane's 8 pair products of k-step ks
       float pr[8];
       const int k0 = ks * kPgKS + kh;
