@@ -190,29 +190,43 @@ class BSpec:
             return self._triplet_mfma(stack, np.asarray(i1), np.asarray(i2), np.asarray(i3), prec)
         return self._triplet_vec(stack, i1, i2, i3)
 
+    def _triplet_plan(self, nm, i1, i2, i3):
+        """Launch list of the MFMA triplet sums: per block of at most 128 distinct third maps, the
+        device index arrays (A maps, pair maps b <= c) and where each triplet reads its T entry."""
+        b = self.base
+        lo, hi = np.minimum(i1, i2), np.maximum(i1, i2)
+        ua, ia = np.unique(i3, return_inverse=True)
+        dev = lambda a, dt=np.int32: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(b.device)  # noqa: E731
+        plan = []
+        for c0 in range(0, len(ua), 128):
+            sel = np.nonzero((ia >= c0) & (ia < c0 + 128))[0]
+            up, ip = np.unique(lo[sel] * nm + hi[sel], return_inverse=True)
+            amap = ua[c0:c0 + 128]
+            plan.append((dev(amap), dev(up // nm), dev(up % nm), len(amap), len(up), dev(sel, np.int64),
+                         dev(ia[sel] - c0, np.int64), dev(ip, np.int64)))
+        return plan
+
     def _triplet_mfma(self, stack, i1, i2, i3, prec):
         b = self.base
         nm = stack.shape[0]
         nvox = int(np.prod(stack.shape[1:]))
-        lo, hi = np.minimum(i1, i2), np.maximum(i1, i2)
-        ua, ia = np.unique(i3, return_inverse=True)
-        out = np.zeros(len(i1))
-        for c0 in range(0, len(ua), 128):
-            sel = np.nonzero((ia >= c0) & (ia < c0 + 128))[0]
-            amap = ua[c0:c0 + 128]
-            up, ip = np.unique(lo[sel] * nm + hi[sel], return_inverse=True)
-            dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(b.device)
-            amap_d, pb_d, pc_d = dev(amap), dev(up // nm), dev(up % nm)
-            na, npair = len(amap), len(up)
+        # launch lists are kept for read-only index arrays (the estimator's own triplet list), looked
+        # up by identity; other index arrays get a fresh list
+        plans = getattr(self, '_tplans', [])
+        plan = next((e[4] for e in plans if e[0] is i1 and e[1] is i2 and e[2] is i3 and e[3] == nm), None)
+        if plan is None:
+            plan = self._triplet_plan(nm, i1, i2, i3)
+            if not (i1.flags.writeable or i2.flags.writeable or i3.flags.writeable):
+                self._tplans = (plans + [(i1, i2, i3, nm, plan)])[-4:]
+        out = torch.empty(len(i1), dtype=torch.float64, device=b.device)
+        for amap_d, pb_d, pc_d, na, npair, sel_d, rows_d, cols_d in plan:
             T = torch.empty((na, npair), dtype=torch.float64, device=b.device)
             ws = C.lib().pb3d_pair_gemm_work_size(nvox, na, npair)
             work = b.work('pair_gemm', ws)
             C.check(C.lib().pb3d_pair_gemm(stack.data_ptr(), nm, nvox, amap_d.data_ptr(), na, pb_d.data_ptr(),
                                            pc_d.data_ptr(), npair, prec, T.data_ptr(), work.data_ptr(), b.stream()))
-            rows = torch.from_numpy(ia[sel] - c0).to(b.device)
-            cols = torch.from_numpy(ip).to(b.device)
-            out[sel] = T[rows, cols].cpu().numpy()
-        return out
+            out[sel_d] = T[rows_d, cols_d]
+        return out.cpu().numpy()
 
     def _triplet_vec(self, stack, i1, i2, i3):
         b = self.base
@@ -281,9 +295,15 @@ class BSpec:
         return b.as_fourier(self.applySinv(b.as_real(data), input_type='real', output_type='fourier'))
 
     def _b3_indices(self):
-        Nq = self.Nk_squeeze
-        ab = self.all_bins
-        return ab[:, 0], ab[:, 1], (ab[:, 3] // 2) * Nq + ab[:, 2]
+        """(b1, b2, stack row of b3 with its multipole) of every bin, computed once (read-only)."""
+        if getattr(self, '_b3_idx', None) is None:
+            Nq = self.Nk_squeeze
+            ab = self.all_bins
+            idx = (ab[:, 0].copy(), ab[:, 1].copy(), (ab[:, 3] // 2) * Nq + ab[:, 2])
+            for v in idx:
+                v.setflags(write=False)
+            self._b3_idx = idx
+        return self._b3_idx
 
     def _compute_bk_numerator(self, data, include_linear_term=True, filtering='ideal', verb=False):
         """bspec.py:895-983."""
