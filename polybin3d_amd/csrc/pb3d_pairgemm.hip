@@ -204,12 +204,28 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
   }
 }
 
-__global__ void pair_gemm_reduce(const double* __restrict__ part, int nchunks, long long n, double* __restrict__ out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// out[i] = sum_c part[c][i].  A block takes 16 consecutive outputs with 16 chunk lanes each (lane q
+// sums chunks q, q+16, ...), then adds the 16 lane sums in a fixed order: deterministic, and enough
+// blocks and independent loads in flight for a few outputs over a thousand chunks.
+constexpr int kRedOut = 16, kRedLanes = 16;
+__global__ void __launch_bounds__(kRedOut * kRedLanes)
+pair_gemm_reduce(const double* __restrict__ part, int nchunks, long long n, double* __restrict__ out) {
+  __shared__ double red[kRedLanes][kRedOut + 1];
+  const int j = threadIdx.x % kRedOut, q = threadIdx.x / kRedOut;
+  const long long i = (long long)blockIdx.x * kRedOut + j;
   double s = 0.;
-  for (int c = 0; c < nchunks; ++c) s += part[(long long)c * n + i];
-  out[i] = s;
+  if (i < n) {
+#pragma unroll 4
+    for (int c = q; c < nchunks; c += kRedLanes) s += part[(long long)c * n + i];
+  }
+  red[q][j] = s;
+  __syncthreads();
+  if (q == 0 && i < n) {
+    double t = 0.;
+#pragma unroll
+    for (int l = 0; l < kRedLanes; ++l) t += red[l][j];
+    out[i] = t;
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -379,7 +395,8 @@ extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int
 #undef PB3D_PG3
     PB3D_CHECK_LAUNCH();
     const long long n = (long long)nar * np;
-    pair_gemm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, (int)nc, n, out + (long long)r0 * np);
+    pair_gemm_reduce<<<(unsigned)((n + kRedOut - 1) / kRedOut), kRedOut * kRedLanes, 0, s>>>(part, (int)nc, n,
+                                                                                            out + (long long)r0 * np);
     PB3D_CHECK_LAUNCH();
   }
   return 0;
