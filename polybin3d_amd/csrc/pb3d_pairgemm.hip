@@ -73,7 +73,7 @@ __device__ __forceinline__ f32x16 mma_split(const bf16x8* a, const bf16x8* b, f3
 // maps onto the same banks.
 // ------------------------------------------------------------------------------------------
 constexpr int kPgFlushVox = 1024;                     // voxels per first-level accumulation
-constexpr size_t kPgLdsBudget = 64 * 1024;            // dynamic LDS per block (two blocks per CU)
+constexpr size_t kPgLdsBudget = 64 * 1024 - 512;      // dynamic LDS per block (+ <= 384 B static: two blocks per CU)
 constexpr int kPgSgPad = 4;                           // floats of pad per staged map row
 
 static size_t pg_fwd_lds(int nmaps, int np_parts, int mt, int stage, int sap) {
@@ -116,6 +116,17 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
   for (int t = 0; t < MT; ++t)
 #pragma unroll
     for (int v = 0; v < 16; ++v) { acc[t][v] = 0.f; acc2[t][v] = 0.f; }
+  // sg offset of each A row's map, read from LDS in every stage's split (-1: row beyond na, zero;
+  // -2: map index outside the stack, NaN); visible after the first barrier of the stage loop
+  __shared__ int s_aoff[MT * 32];
+  for (int r = tid; r < MT * 32; r += kPgThreads) {
+    int o = -1;
+    if (r < na) {
+      const int am = amap[r];
+      o = am < 0 || am >= nmaps ? -2 : am * SGS;
+    }
+    s_aoff[r] = o;
+  }
   // staged element e = m * kPgStage + k, k = e % kPgStage: thread tid loads e = tid + i * kPgThreads
   const int nel = nmaps * kPgStage;
   float pf[PF];
@@ -145,13 +156,16 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
     // split the A rows (rows beyond na are zero)
     for (int e = tid; e < MT * 32 * K8; e += kPgThreads) {
       const int row = e / K8, kb = e % K8;
+      const int o = s_aoff[row];
       float v[8];
-      int am = row < na ? amap[row] : 0;
-      const bool abad = am < 0 || am >= nmaps;
-      if (abad) am = 0;
+      if (o >= 0) {  // two 16-byte LDS reads (offsets are multiples of 4 floats)
+        const float4 u0 = *reinterpret_cast<const float4*>(sg + o + kb * 8);
+        const float4 u1 = *reinterpret_cast<const float4*>(sg + o + kb * 8 + 4);
+        v[0] = u0.x; v[1] = u0.y; v[2] = u0.z; v[3] = u0.w; v[4] = u1.x; v[5] = u1.y; v[6] = u1.z; v[7] = u1.w;
+      } else {
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
-        v[r] = row >= na ? 0.f : (abad ? __builtin_nanf("") : sg[am * SGS + kb * 8 + r]);
+        for (int r = 0; r < 8; ++r) v[r] = o == -1 ? 0.f : __builtin_nanf("");
+      }
       bf16x8 parts[NP];
       split8<NP>(v, parts);
 #pragma unroll
