@@ -9,8 +9,9 @@
 // the GEMM U . P with voxels as the N axis.
 //
 // Split BF16: an FP32 operand x = hi + mid (+ lo), each part bf16 (x - hi - mid exact in FP32).
-// precision 1 = 3-term product (hi*hi + hi*mid + mid*hi: ~2^-16 relative per product),
-// precision 2 = 6-term product (adds hi*lo + lo*hi + mid*mid: ~2^-24, FP32-like).
+// precision 1 = 3-term product (hi*hi + hi*mid + mid*hi: ~2^-16 relative per product; parts rounded
+// to nearest), precision 2 = 6-term product (adds hi*lo + lo*hi + mid*mid: ~2^-24, FP32-like; parts
+// truncated, which makes hi + mid + lo exactly x).
 //
 // MFMA 32x32x16 lane layout (checked on the device): A[i][k] from lane l: i = l%32,
 // k = 8*(l/32) + r (r = 0..7 of an 8 x bf16 operand); B[k][j]: j = l%32, same k; accumulator
@@ -28,9 +29,34 @@ constexpr int kPgThreads = 64 * kPgWaves;
 constexpr int kPgMaxM = 128;      // A rows (maps) per call: up to 4 M-tiles of 32
 constexpr int kPgMaxMaps = 256;   // maps in the stack the forward kernel stages
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // x = p[0] + p[1] (+ p[2]) in bf16 parts; NP = 2 or 3
 template <int NP>
 __device__ __forceinline__ void split8(const float* x, bf16x8* p) {
+  if constexpr (NP == 3) {
+    // truncating split, exact: hi = the top 8 significand bits of x, mid = the top 8 of x - hi,
+    // lo = x - hi - mid, which has at most 8 significant bits (a bf16 is the high half of a float);
+    // masks, subtractions and one byte permute per pair of values
+    uint32_t w[3][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const uint32_t h = __float_as_uint(x[r]) & 0xffff0000u;
+      const float r1 = x[r] - __uint_as_float(h);
+      const uint32_t m = __float_as_uint(r1) & 0xffff0000u;
+      w[0][r] = h;
+      w[1][r] = m;
+      w[2][r] = __float_as_uint(r1 - __uint_as_float(m));
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      u32x4 v;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = __builtin_amdgcn_perm(w[q][2 * j + 1], w[q][2 * j], 0x07060302u);
+      p[q] = __builtin_bit_cast(bf16x8, v);
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     float v = x[r];
@@ -66,7 +92,7 @@ __device__ __forceinline__ f32x16 mma_split(const bf16x8* a, const bf16x8* b, f3
 // the waves, then each wave forms the products of its 32 pairs and runs NP-part MFMAs for every
 // M-tile, kPgStage/kPgKS k-steps per stage.  Partial T of the chunk -> part[chunk][a][p] (FP64),
 // accumulated in two FP32 levels (MFMA accumulator over 1024 voxels, then a second accumulator).
-// LDS (dynamic): sg[nmaps][kPgStage + 4] floats, then sa[NP][MT*32][kPgStage/8 + SAP] bf16x8.  The
+// LDS (dynamic): sg[nmaps + 2][kPgStage + 4] floats, then sa[NP][MT*32][kPgStage/8 + SAP] bf16x8.  The
 // row pads put consecutive rows 16 bytes apart in the bank space: the lanes of a k-step read 8
 // consecutive floats of 32 different rows (the pair products, consecutive pairs share b and take
 // consecutive c) and 16 bytes of 32 consecutive A rows, which an unpadded power-of-two row length
@@ -77,7 +103,7 @@ constexpr size_t kPgLdsBudget = 64 * 1024 - 512;      // dynamic LDS per block (
 constexpr int kPgSgPad = 4;                           // floats of pad per staged map row
 
 static size_t pg_fwd_lds(int nmaps, int np_parts, int mt, int stage, int sap) {
-  return (size_t)nmaps * (stage + kPgSgPad) * sizeof(float) + (size_t)np_parts * mt * 32 * (stage / 8 + sap) * 16;
+  return (size_t)(nmaps + 2) * (stage + kPgSgPad) * sizeof(float) + (size_t)np_parts * mt * 32 * (stage / 8 + sap) * 16;
 }
 
 // ST = voxels per LDS stage (a stage costs three block barriers, its global loads and the A split,
@@ -93,7 +119,7 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
   extern __shared__ __align__(16) char smem[];
   constexpr int SGS = kPgStage + kPgSgPad;                                             // sg row stride
   float* sg = reinterpret_cast<float*>(smem);                                         // [nmaps][SGS]
-  bf16x8* sa = reinterpret_cast<bf16x8*>(smem + (size_t)nmaps * SGS * sizeof(float));
+  bf16x8* sa = reinterpret_cast<bf16x8*>(smem + (size_t)(nmaps + 2) * SGS * sizeof(float));
   constexpr int K8 = kPgStage / 8;                                                     // bf16x8 per row
   constexpr int K8S = K8 + SAP;                                                        // sa row stride
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
@@ -105,9 +131,12 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
   const bool pvalid = p < np;
   const bool wave_on = (blockIdx.x * kPgWaves + wave) * 32 < np;  // wave-uniform
   int mb = pvalid ? pb[p] : 0, mc = pvalid ? pc[p] : 0;
-  // an index outside the stack gives NaN for that pair (no stray LDS read)
-  const bool pbad = mb < 0 || mb >= nmaps || mc < 0 || mc >= nmaps;
-  if (pbad) { mb = 0; mc = 0; }
+  // two constant rows follow the staged maps in sg: row nmaps is zero (the products of a pair
+  // beyond np), row nmaps + 1 is NaN (an index outside the stack gives NaN for that pair, with no
+  // stray LDS read), so the products need no selects
+  if (mb < 0 || mb >= nmaps || mc < 0 || mc >= nmaps) { mb = nmaps + 1; mc = nmaps + 1; }
+  if (!pvalid) { mb = nmaps; mc = nmaps; }
+  for (int k = tid; k < 2 * SGS; k += kPgThreads) sg[nmaps * SGS + k] = k < SGS ? 0.f : __builtin_nanf("");
   const int kh = 8 * (lane / 32);  // this lane's k offset inside a k-step
   // two-level FP32 accumulation: the MFMA accumulator acc covers at most kPgFlush stages (1024
   // voxels), then is added into acc2 (rounding grows like sqrt(steps per level), not sqrt(steps))
@@ -185,7 +214,7 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
         const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
         const float cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
 #pragma unroll
-        for (int r = 0; r < 8; ++r) pr[r] = !pvalid ? 0.f : (pbad ? __builtin_nanf("") : bv[r] * cv[r]);
+        for (int r = 0; r < 8; ++r) pr[r] = bv[r] * cv[r];
       }
       bf16x8 bparts[NP];
       split8<NP>(pr, bparts);
