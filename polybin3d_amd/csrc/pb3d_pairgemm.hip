@@ -86,7 +86,8 @@ __device__ __forceinline__ f32x16 mma_split(const bf16x8* a, const bf16x8* b, f3
 }
 
 // ------------------------------------------------------------------------------------------
-// Forward.  Block (blockIdx.x = group of kPgWaves N-tiles of 32 pairs, blockIdx.y = voxel chunk):
+// Forward.  Block (blockIdx.x = group of nw N-tiles of 32 pairs, one per wave, blockIdx.y = voxel
+// chunk; nw = kPgWaves, or 4..8 waves for a single M-tile so that up to 256 pairs share one staging):
 // stages kPgStage = ST (32, 64 or 128) voxels of every map of the stack in LDS (the next stage's loads go into
 // registers before the current stage is used), splits the A rows once into bf16 parts shared by
 // the waves, then each wave forms the products of its 32 pairs and runs NP-part MFMAs for every
@@ -107,10 +108,11 @@ static size_t pg_fwd_lds(int nmaps, int np_parts, int mt, int stage, int sap) {
 }
 
 // ST = voxels per LDS stage (a stage costs three block barriers, its global loads and the A split,
-// so small stacks take long stages); PF = prefetch registers per thread: nmaps * ST <= PF * kPgThreads;
+// so small stacks take long stages); PF = prefetch registers per thread: nmaps * ST <= PF * kPgThreads
+// (and blocks have at least kPgThreads threads);
 // SAP = pad (bf16x8) per split A row (0 or 1)
 template <int NP, int MT, int ST, int PF, int SAP>
-__global__ void __launch_bounds__(kPgThreads)
+__global__ void __launch_bounds__(MT == 1 && PF <= 24 ? 2 * kPgThreads : kPgThreads)  // kWide below
 pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int* __restrict__ amap, int na,
               const int* __restrict__ pb, const int* __restrict__ pc, int np, long long chunk,
               double* __restrict__ part) {
@@ -122,21 +124,25 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
   bf16x8* sa = reinterpret_cast<bf16x8*>(smem + (size_t)(nmaps + 2) * SGS * sizeof(float));
   constexpr int K8 = kPgStage / 8;                                                     // bf16x8 per row
   constexpr int K8S = K8 + SAP;                                                        // sa row stride
+  // kPgWaves waves per block (a compile-time stride), or 4..8 for a single M-tile over a stack of
+  // at most 96 maps (one pair group holds every pair)
+  constexpr bool kWide = MT == 1 && PF <= 24;
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int nthr = kWide ? (int)blockDim.x : kPgThreads, nw = nthr / 64;
   const long long x0 = (long long)blockIdx.y * chunk;
   long long x1 = x0 + chunk;
   if (x1 > nvox) x1 = nvox;
   // this lane's pair (column j = lane%32 of the wave's N-tile)
-  const int p = (blockIdx.x * kPgWaves + wave) * 32 + lane % 32;
+  const int p = (blockIdx.x * nw + wave) * 32 + lane % 32;
   const bool pvalid = p < np;
-  const bool wave_on = (blockIdx.x * kPgWaves + wave) * 32 < np;  // wave-uniform
+  const bool wave_on = (blockIdx.x * nw + wave) * 32 < np;  // wave-uniform
   int mb = pvalid ? pb[p] : 0, mc = pvalid ? pc[p] : 0;
   // two constant rows follow the staged maps in sg: row nmaps is zero (the products of a pair
   // beyond np), row nmaps + 1 is NaN (an index outside the stack gives NaN for that pair, with no
   // stray LDS read), so the products need no selects
   if (mb < 0 || mb >= nmaps || mc < 0 || mc >= nmaps) { mb = nmaps + 1; mc = nmaps + 1; }
   if (!pvalid) { mb = nmaps; mc = nmaps; }
-  for (int k = tid; k < 2 * SGS; k += kPgThreads) sg[nmaps * SGS + k] = k < SGS ? 0.f : __builtin_nanf("");
+  for (int k = tid; k < 2 * SGS; k += nthr) sg[nmaps * SGS + k] = k < SGS ? 0.f : __builtin_nanf("");
   const int kh = 8 * (lane / 32);  // this lane's k offset inside a k-step
   // two-level FP32 accumulation: the MFMA accumulator acc covers at most kPgFlush stages (1024
   // voxels), then is added into acc2 (rounding grows like sqrt(steps per level), not sqrt(steps))
@@ -148,7 +154,7 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
   // sg offset of each A row's map, read from LDS in every stage's split (-1: row beyond na, zero;
   // -2: map index outside the stack, NaN); visible after the first barrier of the stage loop
   __shared__ int s_aoff[MT * 32];
-  for (int r = tid; r < MT * 32; r += kPgThreads) {
+  for (int r = tid; r < MT * 32; r += nthr) {
     int o = -1;
     if (r < na) {
       const int am = amap[r];
@@ -156,13 +162,13 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
     }
     s_aoff[r] = o;
   }
-  // staged element e = m * kPgStage + k, k = e % kPgStage: thread tid loads e = tid + i * kPgThreads
+  // staged element e = m * kPgStage + k, k = e % kPgStage: thread tid loads e = tid + i * nthr
   const int nel = nmaps * kPgStage;
   float pf[PF];
   auto fetch = [&](long long xs) {
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
-      const int e = tid + i * kPgThreads;
+      const int e = tid + i * nthr;
       float v = 0.f;
       if (e < nel) {
         const long long x = xs + (e % kPgStage);
@@ -177,13 +183,13 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
     __syncthreads();  // previous stage's readers of sg / sa are done
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
-      const int e = tid + i * kPgThreads;
+      const int e = tid + i * nthr;
       if (e < nel) sg[(e / kPgStage) * SGS + e % kPgStage] = pf[i];
     }
     if (xs + kPgStage < x1) fetch(xs + kPgStage);  // next stage in flight while this one computes
     __syncthreads();
     // split the A rows (rows beyond na are zero)
-    for (int e = tid; e < MT * 32 * K8; e += kPgThreads) {
+    for (int e = tid; e < MT * 32 * K8; e += nthr) {
       const int row = e / K8, kb = e % K8;
       const int o = s_aoff[row];
       float v[8];
@@ -374,6 +380,14 @@ static long long pg_chunks(long long nvox, int ngroups) {
   return c;
 }
 static int pad16(int n) { return (n + kPgKS - 1) / kPgKS * kPgKS; }
+// waves per forward block: up to 256 pairs of a single M-tile (<= 32 A rows) over a stack of at
+// most 96 maps in one pair group of 4..8 waves (one staging of the stack for all of them; those
+// kernels are built for 512 threads), else kPgWaves
+static int pg_waves(int nmaps, int na, int np) {
+  if (nmaps > 96 || na > 32 || np > 8 * 32) return kPgWaves;
+  const int w = (np + 31) / 32;
+  return w < kPgWaves ? kPgWaves : w;
+}
 
 }  // namespace pb3d
 
@@ -381,7 +395,8 @@ using namespace pb3d;
 
 extern "C" size_t pb3d_pair_gemm_work_size(int64_t nvox, int na, int np) {
   if (nvox < 1 || na < 1 || np < 1) return 0;
-  const int ngroups = (np + 32 * kPgWaves - 1) / (32 * kPgWaves);
+  // the widest block any stack allows: fewest pair groups, most chunks (an upper bound for every nmaps)
+  const int nw = pg_waves(1, na, np), ngroups = (np + 32 * nw - 1) / (32 * nw);
   return (size_t)pg_chunks(nvox, ngroups) * na * np * sizeof(double);
 }
 
@@ -392,7 +407,7 @@ extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int
   if (nvox < 1 || na < 1 || na > kPgMaxM || np < 1) return fail(PB3D_ERR_ARG, "pair_gemm: na out of [1, 128] or empty input");
   if (precision != 1 && precision != 2) return fail(PB3D_ERR_ARG, "pair_gemm: precision must be 1 (bf16 x3) or 2 (bf16 x6)");
   hipStream_t s = (hipStream_t)stream;
-  const int ngroups = (np + 32 * kPgWaves - 1) / (32 * kPgWaves);
+  const int nw = pg_waves(nmaps, na, np), ngroups = (np + 32 * nw - 1) / (32 * nw);  // nc <= the work size's
   const long long nc = pg_chunks(nvox, ngroups);
   // stack-size bucket (maps <= 32, 64, 96, 128, 256) and stage length: the long stage of a bucket
   // (128 voxels for <= 32 maps, 64 for <= 128) when its LDS fits kPgLdsBudget, else 32 voxels;
@@ -416,7 +431,7 @@ extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int
     const int32_t* am = amap + r0;
     double* part = (double*)work + nc * r0 * np;  // [nc][nar][np] region of this row block
 #define PB3D_PG3(NP, MT, ST, PF, SAP) \
-  pair_gemm_fwd<NP, MT, ST, PF, SAP><<<grid, kPgThreads, lds, s>>>(g, nmaps, nvox, am, nar, pb, pc, np, chunk, part)
+  pair_gemm_fwd<NP, MT, ST, PF, SAP><<<grid, 64 * nw, lds, s>>>(g, nmaps, nvox, am, nar, pb, pc, np, chunk, part)
 #define PB3D_PG(NP, MT)                                                       \
     switch (pfb * 2 + (stage == 32)) {                                        \
       case 0: PB3D_PG3(NP, MT, 128, 32 * 128 / kPgThreads, 1); break;         \

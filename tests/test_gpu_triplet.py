@@ -22,7 +22,8 @@ def stack(nm, nvox, seed):
 
 @pytest.mark.parametrize("prec", [1, 2])
 @pytest.mark.parametrize("nm,nvox,na,npair", [(6, 5000, 3, 7), (40, 40000, 40, 300), (130, 4099, 100, 50),
-                                             (128, 70001, 128, 260), (90, 30000, 120, 140), (24, 9000, 30, 70)])
+                                             (128, 70001, 128, 260), (90, 30000, 120, 140), (24, 9000, 30, 70),
+                                             (16, 30001, 16, 136), (40, 50000, 20, 200), (96, 20000, 32, 256)])
 def test_pair_gemm_matches_fp64(hip, prec, nm, nvox, na, npair):
     C = hip
     g = stack(nm, nvox, 1)
