@@ -177,16 +177,21 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
       pf[i] = v;
     }
   };
-  if (x0 < x1) fetch(x0);
+  // PF <= 24: the next stage's loads are in flight in registers while this one computes; the largest
+  // prefetch (32 registers) is loaded at the start of its own stage instead, so that those registers
+  // are free during the MFMA loop (another block on the CU covers the load latency)
+  constexpr bool kPre = PF <= 24;
+  if (kPre && x0 < x1) fetch(x0);
   int nstage = 0;
   for (long long xs = x0; xs < x1; xs += kPgStage) {
     __syncthreads();  // previous stage's readers of sg / sa are done
+    if constexpr (!kPre) fetch(xs);
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int e = tid + i * nthr;
       if (e < nel) sg[(e / kPgStage) * SGS + e % kPgStage] = pf[i];
     }
-    if (xs + kPgStage < x1) fetch(xs + kPgStage);  // next stage in flight while this one computes
+    if (kPre && xs + kPgStage < x1) fetch(xs + kPgStage);  // next stage in flight while this one computes
     __syncthreads();
     // split the A rows (rows beyond na are zero)
     for (int e = tid; e < MT * 32 * K8; e += nthr) {
@@ -208,7 +213,7 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
     }
     __syncthreads();  // sa ready
     // a wave whose 32 pairs all lie beyond np only helps stage (uniform skip of its MFMAs)
-#pragma unroll 2  // (full unrolling of the longer stages costs an occupancy step, no speed)
+#pragma unroll 1  // (unrolled k-steps hold more registers: an occupancy step for the 3-M-tile kernels)
     for (int ks = 0; ks < (wave_on ? kPgStage / kPgKS : 0); ++ks) {
       // B operand: this lane's 8 pair products of k-step ks
       float pr[8];
