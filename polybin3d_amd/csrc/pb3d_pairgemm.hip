@@ -88,8 +88,9 @@ __device__ __forceinline__ f32x16 mma_split(const bf16x8* a, const bf16x8* b, f3
 // ------------------------------------------------------------------------------------------
 // Forward.  Block (blockIdx.x = group of nw N-tiles of 32 pairs, one per wave, blockIdx.y = voxel
 // chunk; nw = kPgWaves, or 4..8 waves for a single M-tile so that up to 256 pairs share one staging):
-// stages kPgStage = ST (32, 64 or 128) voxels of every map of the stack in LDS (the next stage's loads go into
-// registers before the current stage is used), splits the A rows once into bf16 parts shared by
+// stages kPgStage = ST (32, 64 or 128) voxels of every map of the stack in LDS (for stacks up to 96
+// maps the next stage's loads go into registers before the current stage is used; larger stacks load
+// at the start of each stage), splits the A rows once into bf16 parts shared by
 // the waves, then each wave forms the products of its 32 pairs and runs NP-part MFMAs for every
 // M-tile, kPgStage/kPgKS k-steps per stage.  Partial T of the chunk -> part[chunk][a][p] (FP64),
 // accumulated in two FP32 levels (MFMA accumulator over 1024 voxels, then a second accumulator).
@@ -120,7 +121,7 @@ pair_gemm_fwd(const float* __restrict__ g, int nmaps, long long nvox, const int*
   constexpr int kPgFlush = kPgFlushVox / ST;  // stages per flush of the first-level accumulator
   extern __shared__ __align__(16) char smem[];
   constexpr int SGS = kPgStage + kPgSgPad;                                             // sg row stride
-  float* sg = reinterpret_cast<float*>(smem);                                         // [nmaps][SGS]
+  float* sg = reinterpret_cast<float*>(smem);                                         // [nmaps + 2][SGS]
   bf16x8* sa = reinterpret_cast<bf16x8*>(smem + (size_t)(nmaps + 2) * SGS * sizeof(float));
   constexpr int K8 = kPgStage / 8;                                                     // bf16x8 per row
   constexpr int K8S = K8 + SAP;                                                        // sa row stride
