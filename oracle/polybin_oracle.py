@@ -28,6 +28,11 @@ How each part is pinned (tests/test_oracle_*.py):
     odd_l (odd-l legs and their -i / i phases) for the MC Fisher and the numerator;
   * Bk MC Fisher: in expectation over 200 seeds (unit mask, ideal weights) equal to the ideal Fisher
     with the discreteness correction, a separate code path, which it thereby also pins for l > 0;
+    per seed, with a varying mask (global and local sightline) and with the global integral
+    constraint, against a dense-operator restatement (explicit filter matrices, the cubic form's
+    gradient through their transposes, the pointing matrix as a matrix; tests/test_oracle_pins_bk.py),
+    which rejects planted mistakes (a dropped Legendre leg, a missing or transposed pointing matrix, a
+    conjugated row, an unweighted A^-1, transposed harmonics);
   * Bk symmetry factors for l > 0: lattice triangle sums with the addition theorem (brute force);
   * local-sightline Bk numerator: dense DFT-matrix restatement with L_l(k_hat . x_hat);
   * apply_xi (local): the dense symmetric operator sum_k e^{ik(x-y)} [P_0 + P_2 (L_2(k.x)+L_2(k.y))/2];
@@ -39,10 +44,12 @@ How each part is pinned (tests/test_oracle_*.py):
     the compiled reference kernel.
   * PSpec.compute_shot_contribution: per-seed closed form (global, constant mask) and a dense
     restatement (local, varying and shot-noise masks).
-Parity unpinned (no check beyond restatement and the compiled utils.pyx kernels they call):
-BSpec._compute_fisher per-seed values with a mask or a local sightline (pinned in expectation only
-for the unit mask); BSpec.compute_shot_contribution as a whole (its parts, the 1/P_fid GRF draws,
-A^-1, the g-maps and assemble_bshot, are each pinned above).
+  * BSpec.compute_shot_contribution: per seed as exact Fourier-space lattice sums with the
+    real-space products as circular convolutions (global, constant masks), against dense local
+    filter matrices (local, varying mask and shot-noise mask), and the cubic term's expectation
+    against its closed form c T_i / (V sym_i); a dropped leg permutation or a Legendre factor on the
+    wrong leg fails them.
+Every function above has an independent pin; none is left as restatement only.
 """
 import numpy as np
 import scipy.fft
