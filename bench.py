@@ -1,6 +1,10 @@
 """Benchmark: Monte Carlo Fisher realizations per second for the power spectrum + bispectrum at 256^3.
 
-One step = one Monte Carlo realization of both Fisher matrices on one GPU:
+The metric compares two legs measured in the same run.  The timed (reported) leg: one step = R
+Monte Carlo realizations (--realizations, default 4) of both Fisher matrices on one GPU, the R Pk
+seeds sharing every pass and the Bk Fisher contraction on the matrix cores in split BF16
+(pb3d_fisher_contract, accumulating into the FP64 sum); the plain leg: one realization at a time with
+the Bk contraction as an FP32 GEMM (hipBLASLt).  vs_baseline = timed / plain.  Each realization:
   * PSpec MC Fisher: 256^3, box 1000 Mpc/h, local (Yamamoto) sightline, lmax=4, 30 bins 0.01-0.30 h/Mpc;
   * BSpec MC Fisher: same grid, lmax=2, 40 bins 0.01-0.40 h/Mpc (12700 triangle bins);
   * mask: shell 300 <= r <= 480 Mpc/h minus 200 random holes (r = 10-30 Mpc/h), smoothed 5 Mpc/h;
@@ -9,7 +13,7 @@ Prints one JSON line with the throughput, the roofline of the dominant kernel (s
 timing over the timed region) and a CPU baseline (the FP64 oracle's unit operations timed on this
 host and scaled by the reference's exact per-realization operation counts).
 
-usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 256] [--no-cpu]
+usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 256] [--realizations R] [--plain-steps P] [--no-cpu]
 """
 import argparse
 import ctypes
@@ -29,6 +33,11 @@ HBM_PEAK_GBS = 8000.0
 # by a 1 GiB device copy), averaged over the launches of one realization of this workload, for the
 # kernel they were taken on (profiles/hbm_traffic_pk_bk_mc_fisher_256.md); other kernels report null.
 MEASURED_TRAFFIC = {"pointing_rows<128>": 11.03e9}
+# VALU instructions (wave64, summed over the waves) per launch, rocprofv3 --pmc SQ_INSTS_VALU averaged
+# over the launches of one realization of this workload (profiles/pointing_rows_sq_counters_pk_bk_mc_fisher_256.md)
+MEASURED_VALU = {}
+SIMDS = 256 * 4          # MI355X: 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9         # peak engine clock
 KIND_NAMES = {0: "r2c_rows", 1: "c2r_rows", 2: "fft_cols (plain C2R)", 3: "fft_cols (C2R first pass, fused prologue)",
               4: "fft_cols (R2C last pass, fused epilogue)", 5: "pointing_rows (fused Q-map rows)",
               6: "fft_cols (plain R2C)"}
@@ -174,8 +183,11 @@ def main():
                          "PSpec MC Fisher realizations at 128^3 and 256^3 (--realizations per call); extras: "
                          "seconds per call of the members around the MC Fisher (shot noise, linear term, "
                          "integral constraints, covariance and theory matrices) at 256^3")
-    ap.add_argument("--realizations", type=int, default=1,
-                    help="pk_mc_fisher: realizations per compute_fisher batch")
+    ap.add_argument("--realizations", type=int, default=None,
+                    help="realizations per step: mc_fisher (default 4; the Pk seeds of a step share every pass) "
+                         "and pk_mc_fisher (default 1, seeds per compute_fisher batch)")
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="mc_fisher: realizations timed on the plain FP32 single-realization path (0 = skip)")
     ap.add_argument("--mixed", action="store_true",
                     help="pk_mc_fisher: issue the 128^3 and 256^3 realizations together, on a stream per grid "
                          "(compute_fisher_mixed), instead of one grid after the other")
@@ -184,6 +196,8 @@ def main():
     ap.add_argument("--lmax", type=int, default=0)
     ap.add_argument("--precision", default="bf16x6", choices=["fp32", "bf16x3", "bf16x6"])
     args = ap.parse_args()
+    if args.realizations is None:
+        args.realizations = 4 if args.workload == "mc_fisher" else 1
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the Fisher realizations of one device and has no "
               "multi-GPU path (realizations would shard across GPUs with one FP64 all-reduce of the Fisher "
@@ -203,29 +217,58 @@ def main():
     kny = n * np.pi / L
     pk_bins = np.linspace(0.01, min(0.30, 0.7 * kny), 31)
     bk_bins = np.linspace(0.01, min(0.40, 0.99 * kny), 41)
+    R = max(1, args.realizations)
     t_setup = time.perf_counter()
     base = pb.PolyBin3D(L, [n] * 3, Pk=powerlaw_pk(2 * kny * 1.8), boxcenter=center, sightline='local', rng='device')
     mask = shell_mask(n, L, center)
     ps = pb.PSpec(base, pk_bins, lmax=4, mask=mask)
-    bs = pb.BSpec(base, bk_bins, lmax=2, mask=mask)
+    bs = pb.BSpec(base, bk_bins, lmax=2, mask=mask, precision=args.precision)
     bs._compute_symmetry_factor()
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
     Fp = torch.zeros((ps.N_bins, ps.N_bins), dtype=torch.float64, device=base.device)
     Fb = torch.zeros((bs.N_bins, bs.N_bins), dtype=torch.float64, device=base.device)
 
-    def step(seed):
+    def plain_step(seed):
+        """The plain FP32 single-realization path: one Pk seed, then one Bk seed whose Fisher
+        contraction is an FP32 GEMM (hipBLASLt), each Fisher formed on its own and then accumulated."""
+        prec = bs.precision
+        bs.precision = 'fp32'
         Fp.add_(ps._fisher_contribution_dev(seed))
         Fb.add_(bs._fisher_contribution_dev(seed))
+        bs.precision = prec
 
+    def batched_step(s0):
+        """R realizations: the R Pk seeds share every pass (batched), the Bk seeds run back to back with
+        the split-BF16 MFMA contraction accumulating straight into the FP64 Fisher sum."""
+        seeds = list(range(s0, s0 + R))
+        Fp.add_(ps._fisher_contributions_dev(seeds) if R > 1 else ps._fisher_contribution_dev(seeds[0]))
+        for sd in seeds:
+            bs._fisher_contribution_dev(sd, out=Fb)
+
+    # leg 1: the plain FP32 single-realization path (the metric's comparison point)
+    plain = None
+    if args.plain_steps > 0:
+        plain_step(10 ** 7)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(args.plain_steps):
+            plain_step(2 * 10 ** 7 + s)
+        torch.cuda.synchronize()
+        dtp = time.perf_counter() - t0
+        plain = {"value": round(args.plain_steps / dtp, 6), "unit": "realizations/s", "steps": args.plain_steps,
+                 "ms_per_realization": round(1e3 * dtp / args.plain_steps, 2),
+                 "path": "one Pk seed per pass, Bk Fisher contraction as FP32 GEMM (hipBLASLt), no batching"}
+        print("plain FP32 leg: %.1f ms per realization" % (1e3 * dtp / args.plain_steps), file=sys.stderr, flush=True)
+    # leg 2 (timed): the batched / split-BF16 path
     for w in range(args.warmup):
-        step(10 ** 6 + w)
+        batched_step(10 ** 6 + w * R)
         print("warmup step %d done" % w, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     C.check(C.lib().pb3d_prof_enable(args.prof_period))
     t0 = time.perf_counter()
     for s in range(args.steps):
-        step(s)
+        batched_step(s * R)
         torch.cuda.synchronize()
         print("timed step %d done (%.2f s)" % (s, time.perf_counter() - t0), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
@@ -242,23 +285,41 @@ def main():
     if dom is not None:
         cnt, ms, by = kinds[dom]
         achieved = by / (ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get(kind_kernel(dom, n)),
-                "kernel": kind_kernel(dom, n), "pass": KIND_NAMES[dom],
+        kname = kind_kernel(dom, n)
+        hbm_frac = achieved / HBM_PEAK_GBS
+        valu = MEASURED_VALU.get(kname)
+        valu_frac = None
+        if valu is not None:
+            # VALU-issue fraction: wave64 VALU instructions per launch x 4 cycles over the SIMD cycles of
+            # the measured launch duration
+            valu_frac = valu * 4. / (SIMDS * CLOCK_HZ * (ms / cnt) * 1e-3)
+        bound = "valu" if (valu_frac is not None and valu_frac > hbm_frac) else "hbm"
+        roof = {"bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(hbm_frac, 4), "traffic": MEASURED_TRAFFIC.get(kname),
+                "valu_issue_frac": None if valu_frac is None else round(valu_frac, 3),
+                "bound_from": "rocprofv3 SQ_INSTS_VALU and FETCH_SIZE/WRITE_SIZE of %s on this workload "
+                              "(profiles/) with this run's launch durations" % kname,
+                "kernel": kname, "pass": KIND_NAMES[dom],
                 "avg_launch_ms": round(ms / cnt, 4), "bytes_per_launch": by / cnt,
                 "sampled_launches": cnt, "share_of_sampled_fft_time": round(ms / sum(v[1] for v in kinds.values()), 3),
                 "passes": {KIND_NAMES[k]: {"sampled_launches": v[0], "avg_launch_ms": round(v[1] / v[0], 4),
                                            "GB_per_s": round(v[2] / (v[1] * 1e-3) / 1e9, 1)}
                            for k, v in sorted(kinds.items())}}
-    value = args.steps / dt
+    value = args.steps * R / dt
     res = {"metric": METRIC, "value": round(value, 6), "unit": "realizations/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 2), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "scaling": "weak", "vs_baseline": None if plain is None else round(value / plain["value"], 4),
+           "dtype": "f32", "data": "synthetic",
            "config": {"workload": "pk_bk_mc_fisher_%d" % n, "grid": [n] * 3, "boxsize": L, "sightline": "local",
                       "pk": {"lmax": 4, "nbins": len(pk_bins) - 1, "kmax": float(pk_bins[-1])},
                       "bk": {"lmax": 2, "nbins": len(bk_bins) - 1, "kmax": float(bk_bins[-1]), "N_bins": bs.N_bins},
                       "mask": "shell 300-480 Mpc/h, 200 holes, 5 Mpc/h smoothing", "rng": "device",
+                      "realizations_per_step": R, "pk_seeds_per_pass": R,
+                      "bk_fisher_contraction": args.precision + " MFMA (split BF16, FP32 runs, FP64 store)",
                       "fisher_accumulation": "fp64", "setup_s": round(t_setup, 1)},
+           "vs_baseline_is": "value / plain FP32 single-realization leg measured in this run (the metric's "
+                             "comparison; no published reference throughput exists)",
+           "plain_fp32_leg": plain,
            "roofline": roof}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(n, pk_bins, bk_bins, 4, 2, bs.N_bins)

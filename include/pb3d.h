@@ -302,6 +302,39 @@ size_t pb3d_pair_adjoint_work_size(int na, int np, int precision);
 int pb3d_pair_adjoint(const float* g, int nmaps, int64_t nvox, const int32_t* pb, const int32_t* pc, int np,
                       const float* U, int na, int precision, float* out, void* work, void* stream);
 
+/* ---------------------------------------------------------------- Fisher contraction
+ * The rows of the bispectrum Monte Carlo Fisher matrix (integrate_row, PolyBin3D/cython/utils.pyx:162-179,
+ * as called at PolyBin3D/bspec.py:597: fish[i][j] = sum_{k in b2(i)} degk Re[conj(m_i(k)) Q_j(k)]) with
+ * the in-bin modes stored bin-major, as one GEMM per bin segment.  A segment describes one middle
+ * bin: `rows` left rows of `k` reals each (re, im interleaved, degk folded in) stored row-major at
+ * left + a_off, contracted with reals [q_off, q_off + k) of every Q row (q + n * q_ld, n < nq). */
+typedef struct {
+  int64_t a_off;  /* float offset of the segment's [rows][k] left block */
+  int64_t q_off;  /* float offset of the segment inside every Q row */
+  int32_t k;      /* reals per row (even) */
+  int32_t rows;   /* left rows */
+  int32_t row0;   /* first entry of row_map for this segment */
+  int32_t pad_;
+} pb3d_fisher_seg;
+
+/* Host-only: the output-tile list of pb3d_fisher_contract for these segments (host array) and nq Q
+ * rows, written to tiles[3 * t .. 3 * t + 2] = (segment, first row, first column), largest k first.
+ * Returns the number of tiles (tiles == NULL: only count), or a negative error. */
+int pb3d_fisher_tiles(const pb3d_fisher_seg* segs, int nseg, int nq, int32_t* tiles, int max_tiles);
+
+/* For every segment s, r < s.rows and n < nq (segs, tiles, row_map, col_map, scale on the device):
+ *   out[row_map[s.row0 + r] * out_ld + col_map[n]] +=
+ *       alpha * scale[row_map[..]] * scale[col_map[n]] * sum_{t < s.k} left[s.a_off + r s.k + t] q[n q_ld + s.q_off + t]
+ * (scale == NULL: 1).  FP64 out, accumulated (callers zero it for a single realization).  One
+ * persistent launch over the tile list; split-BF16 products on the matrix cores, precision 1 = 3
+ * products (~2^-16 per product), 2 = 6 products (~2^-24); FP32 accumulation in runs of 512 reals,
+ * then across runs, then FP64.  Every (row_map, col_map) target must be distinct: each output is
+ * written by one thread, without atomics, in a fixed order. */
+int pb3d_fisher_contract(const float* left, const float* q, int64_t q_ld, int nq, const pb3d_fisher_seg* segs,
+                         int nseg, const int32_t* tiles, int ntiles, const int32_t* row_map, const int32_t* col_map,
+                         const double* scale, double alpha, int precision, double* out, int64_t out_ld,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

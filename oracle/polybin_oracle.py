@@ -1369,6 +1369,91 @@ class BSpec:
         compute_Q('Sinv', a1, ar1, a2, ar2)                                         # 671
         return 0.5 * fish * b.volume / b.gridsize.prod() ** 2 / np.outer(self.sym_factor, self.sym_factor)
 
+    def fisher_subblock(self, seed, rows, cols, return_scale=False):
+        """fish[rows][:, cols] of _compute_fisher(seed) (bspec.py:473-676, trivial or user S^-1, no
+        covariance), computing only the Q[A^-1 a] maps of the listed columns and the S^-1-weighted
+        left maps of the listed rows, so that it runs on grids where the full [N_bins, H] Q-map array
+        cannot be held.  Per triplet (b1, b2, b3, l) the Q map is the sum of the three leg terms the
+        reference places through add_Q_element (bspec.py:615-640): Theta_b3 W_l FT[g_0,b1 g_0,b2]
+        (W_l = L_l(mu), or the Y_lm sum of apply_real_harmonics for the local sightline),
+        Theta_b1 FT[g_0,b2 g_l,b3] and Theta_b2 FT[g_0,b1 g_l,b3], each with the GRF-pair difference
+        (prod_real_diff), then S^-1 P (bspec.py:662-666); a row is integrate_row of
+        FT[g_0,b1 g_l,b3] (S^-1 P weighting) over bin b2 (bspec.py:579-597).  return_scale (for tests):
+        also the same sums over |FT| and |Q|, sum_{k in b2} degk |m(k)| |Q_j(k)| with the normalisation,
+        the magnitude that FP32 rounding of the maps acts on (an entry that cancels to far below it is
+        ill-conditioned in FP32)."""
+        b = self.base
+        if not hasattr(self, 'sym_factor'):
+            self._compute_symmetry_factor()
+        ab = self.all_bins
+        ks = self.k_bins_squeeze
+        rows, cols = np.asarray(rows, dtype=int), np.asarray(cols, dtype=int)
+        local = b.sightline == 'local' and self.lmax > 0
+
+        def apply_filter(m):                                                       # 499-519
+            if self.const_mask:
+                return self.mask_mean * self.applySinv(m, input_type='fourier', output_type='fourier')
+            return self.applySinv(self.apply_pointing(m), input_type='real', output_type='fourier')
+
+        a1 = b.generate_data(seed=seed + int(1e7), output_type='fourier')            # 648-649
+        a2 = b.generate_data(seed=seed + int(2e7), output_type='fourier')
+
+        def gmaps(w1, w2, need):
+            """{(l, q): (g1, g2)} for the (multipole, bin) pairs in `need` (bspec.py:546-576)."""
+            out, legs = {}, {}
+            for l, q in sorted(need):
+                if l == 0 or b.sightline == 'global':
+                    out[l, q] = tuple(b.to_real(self._filt(w, l, q)) for w in (w1, w2))
+                else:
+                    if l not in legs:
+                        legs[l] = tuple(b.apply_fourier_harmonics(b.to_real(w), self.Ylm_real[l], self.Ylm_fourier[l])
+                                        for w in (w1, w2))
+                    out[l, q] = tuple(b.to_real(self._filt(lg, 0, q)) for lg in legs[l])
+            return out
+
+        def diff(g, A, B):
+            return g[A][0] * g[B][0] - g[A][1] * g[B][1]
+
+        # columns: Q[A^-1 a] (compute_Q('Ainv'), bspec.py:659)
+        wA = [b.applyAinv(a, input_type='fourier', output_type='fourier') for a in (a1, a2)]
+        needA = set()
+        for b1, b2, b3, l in ab[cols]:
+            needA |= {(0, b1), (0, b2), (0, b3), (l, b3)}
+        gA = gmaps(wA[0], wA[1], needA)
+        Q = np.zeros((len(cols),) + b.modk_grid.shape, dtype=np.complex128)
+        for c, (b1, b2, b3, l) in enumerate(ab[cols]):
+            ft0 = b.to_fourier(diff(gA, (0, b1), (0, b2)))
+            if l == 0:
+                Q[c] += self._filt(ft0, 0, b3)
+            elif b.sightline == 'global':
+                Q[c] += self._filt(ft0, l, b3)
+            else:
+                Q[c] += b.to_fourier(b.apply_real_harmonics(self._filt(ft0, 0, b3), self.Ylm_real[l], self.Ylm_fourier[l]))
+            Q[c] += self._filt(b.to_fourier(diff(gA, (0, b2), (l, b3))), 0, b1)
+            Q[c] += self._filt(b.to_fourier(diff(gA, (0, b1), (l, b3))), 0, b2)
+            Q[c] = apply_filter(Q[c]) if self.const_mask else apply_filter(b.to_real(Q[c]))
+        del gA
+        # rows: S^-1 P weighting (compute_Q('Sinv'), bspec.py:671)
+        if self.const_mask:
+            wS = [apply_filter(a) for a in (a1, a2)]
+        else:
+            wS = [apply_filter(b.to_real(a)) for a in (a1, a2)]
+        needS = set()
+        for b1, b2, b3, l in ab[rows]:
+            needS |= {(0, b1), (l, b3)}
+        gS = gmaps(wS[0], wS[1], needS)
+        fish = np.zeros((len(rows), len(cols)))
+        mag = np.zeros((len(rows), len(cols)))
+        Qabs = np.abs(Q) if return_scale else None
+        for r, (b1, b2, b3, l) in enumerate(ab[rows]):
+            ft = b.to_fourier(diff(gS, (0, b1), (l, b3)))
+            fish[r] = b.integrator.integrate_row(ft, Q, ks[b2], ks[b2 + 1])
+            if return_scale:
+                mag[r] = b.integrator.integrate_row(np.abs(ft), Qabs, ks[b2], ks[b2 + 1])
+        sym = self.sym_factor
+        norm = 0.5 * b.volume / b.gridsize.prod() ** 2 / np.outer(sym[rows], sym[cols])
+        return (fish * norm, np.abs(mag * norm)) if return_scale else fish * norm
+
     def compute_fisher(self, N_it, verb=False):
         """bspec.py:678-693."""
         fish = np.zeros((self.N_bins, self.N_bins))

@@ -52,6 +52,10 @@ EXPORTS = {
     "pb3d_pair_adjoint_work_size": (c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pb3d_pair_adjoint": (ctypes.c_int, [c_void_p, ctypes.c_int, c_int64, c_void_p, c_void_p, ctypes.c_int, c_void_p,
                                          ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
+    "pb3d_fisher_tiles": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, ctypes.c_int]),
+    "pb3d_fisher_contract": (ctypes.c_int, [c_void_p, c_void_p, c_int64, ctypes.c_int, c_void_p, ctypes.c_int,
+                                            c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_double,
+                                            ctypes.c_int, c_void_p, c_int64, c_void_p]),
     "pb3d_triplet_work_size": (c_size_t, [c_int64, ctypes.c_int, ctypes.c_int]),
     "pb3d_triplet_sum": (ctypes.c_int, [c_void_p, ctypes.c_int, c_int64, c_void_p, c_void_p, c_void_p, ctypes.c_int,
                                         ctypes.c_int, c_void_p, c_void_p, c_void_p]),
@@ -85,6 +89,12 @@ class KOp(ctypes.Structure):
 class SubMap(ctypes.Structure):
     """pb3d_submap: one weighted sub-map of pb3d_fft_pointing."""
     _fields_ = [("kop", KOp), ("inp", c_void_p), ("ell", c_int32), ("m", c_int32), ("coef", c_float),
+                ("pad_", c_int32)]
+
+
+class FisherSeg(ctypes.Structure):
+    """pb3d_fisher_seg: one bin segment of the Fisher contraction."""
+    _fields_ = [("a_off", c_int64), ("q_off", c_int64), ("k", c_int32), ("rows", c_int32), ("row0", c_int32),
                 ("pad_", c_int32)]
 
 

@@ -749,7 +749,9 @@ class BSpec:
         assert len(Pk_cov) in [2, 3, 4], "Pk should contain k and P_0 (and optionally P_2, P_4) columns"
         return self._fisher_contribution_dev(seed, cov=(b.pk_grid(Pk_cov), noise)).cpu().numpy()
 
-    def _fisher_contribution_dev(self, seed, cov=None):
+    def _fisher_contribution_dev(self, seed, cov=None, out=None):
+        """One realization's Fisher contribution as an FP64 device tensor [N_bins, N_bins] (added to
+        `out` when given)."""
         b = self.base
         if not hasattr(self, 'sym_factor'):
             self._compute_symmetry_factor()
@@ -804,6 +806,10 @@ class BSpec:
         self._ft_tables(H1, H2, self._sinv_pairs, LT)
         del H1, H2
         # contraction: F[i, j] = sum_{k in b2(i)} degk Re[conj(L_i) QA_j]  (integrate_row, bspec.py:597)
+        if self.precision != 'fp32':
+            F = torch.zeros((NB, NB), dtype=torch.float64, device=b.device) if out is None else out
+            self._contract_mfma(LT, QA, F)
+            return F
         F = torch.zeros((NB, NB), dtype=torch.float64, device=b.device)
         QAr = torch.view_as_real(QA)
         nsp = len(self._sinv_pairs)
@@ -827,7 +833,70 @@ class BSpec:
         sym = torch.from_numpy(self.sym_factor).to(b.device)
         F *= 0.5 * b.volume / b.gridsize.prod() ** 2
         F /= sym[:, None] * sym[None, :]
+        if out is not None:
+            out += F
+            return out
         return F
+
+    def _contract_setup(self):
+        """Launch description of the MFMA Fisher contraction, built once per spec: one segment per
+        middle bin b2 (its rows: the triangles with that b2, in triplet order; its left rows: their
+        S^-1 FT-table rows), the tile list, the row and column maps (columns of the Q-map order go back
+        to triplet order in the store) and the 1/sym scales."""
+        if getattr(self, '_cplan', None) is not None:
+            return self._cplan
+        b = self.base
+        pk = self._packed_layout()
+        ab = self.all_bins
+        nsp = len(self._sinv_pairs)
+        segs, srcs, rows_all = [], [], []
+        a_off = 0
+        for q in range(self.Nk_squeeze):
+            rows = np.nonzero(ab[:, 1] == q)[0]
+            s0, s1 = int(pk['seg'][q]), int(pk['seg'][q + 1])
+            if len(rows) == 0 or s1 == s0:
+                continue
+            src = np.asarray([(ab[i, 3] // 2) * nsp + self._sinv_rows[(int(ab[i, 0]), int(ab[i, 2]))] for i in rows],
+                             dtype=np.int32)
+            seg = C.FisherSeg()
+            seg.a_off, seg.q_off, seg.k, seg.rows, seg.row0 = a_off, 2 * s0, 2 * (s1 - s0), len(rows), len(rows_all)
+            segs.append((seg, s0, s1, torch.from_numpy(src).to(b.device)))
+            rows_all.extend(rows.tolist())
+            a_off += 2 * (s1 - s0) * len(rows)
+        arr = (C.FisherSeg * len(segs))(*[x[0] for x in segs])
+        nt = C.lib().pb3d_fisher_tiles(arr, len(segs), self.N_bins, None, 0)
+        if nt < 0:
+            C.check(nt)
+        tiles = np.zeros(3 * nt, dtype=np.int32)
+        C.check(min(0, C.lib().pb3d_fisher_tiles(arr, len(segs), self.N_bins, tiles.ctypes.data, nt)))
+        seg_bytes = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(b.device)  # noqa: E731
+        self._cplan = dict(segs=segs, segs_d=dev(seg_bytes), nseg=len(segs), tiles=dev(tiles), ntiles=int(nt),
+                           row_map=dev(np.asarray(rows_all, dtype=np.int32)),
+                           col_map=dev(self._qperm.astype(np.int32)), left_floats=a_off,
+                           scale=dev(1. / self.sym_factor),
+                           alpha=0.5 * b.volume / float(b.gridsize.prod()) ** 2)
+        return self._cplan
+
+    def _contract_mfma(self, LT, QA, F):
+        """F[i, j] += 0.5 V/N^2 / (sym_i sym_j) sum_{k in b2(i)} degk Re[conj(L_i) QA_j] for every triangle
+        pair: the left rows of each segment are gathered (degk folded in) into one buffer, then one
+        persistent split-BF16 MFMA launch contracts every segment (pb3d_fisher_contract)."""
+        b = self.base
+        pk = self._packed_layout()
+        cp = self._contract_setup()
+        M = pk['M']
+        left = b.work('fisher_left', 4 * cp['left_floats'])
+        nsrc = LT.shape[0] * LT.shape[1]
+        for seg, s0, s1, src in cp['segs']:
+            C.check(C.lib().pb3d_gather_rows(LT.data_ptr(), M, nsrc, src.data_ptr(), seg.rows, s0, s1 - s0,
+                                             pk['degk'].data_ptr(), left.data_ptr() + 4 * seg.a_off, b.stream()))
+        prec = self._PREC[self.precision]
+        C.check(C.lib().pb3d_fisher_contract(left.data_ptr(), QA.data_ptr(), 2 * M, self.N_bins, cp['segs_d'].data_ptr(),
+                                             cp['nseg'], cp['tiles'].data_ptr(), cp['ntiles'],
+                                             cp['row_map'].data_ptr(), cp['col_map'].data_ptr(),
+                                             cp['scale'].data_ptr(), cp['alpha'], prec, F.data_ptr(), self.N_bins,
+                                             b.stream()))
 
     def compute_fisher(self, N_it, verb=False):
         """Mean over N_it realizations, FP64 on the GPU, then inverted (bspec.py:678-693)."""

@@ -1074,11 +1074,30 @@ static hipEvent_t prof_event() {
   return e;
 }
 
-// returns true and records the start event when this launch is sampled
+// folds the completed samples at the front of the pending list into the totals (non-blocking)
+static void prof_drain() {
+  size_t done = 0;
+  for (; done < g_prof_pending.size(); ++done) {
+    ProfSample& smp = g_prof_pending[done];
+    if (hipEventQuery(smp.b) != hipSuccess) break;
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, smp.a, smp.b) == hipSuccess) {
+      g_prof_n[smp.kind] += 1; g_prof_ms[smp.kind] += t; g_prof_bytes[smp.kind] += smp.bytes;
+    }
+    g_prof_free.push_back(smp.a); g_prof_free.push_back(smp.b);
+  }
+  if (done) g_prof_pending.erase(g_prof_pending.begin(), g_prof_pending.begin() + done);
+}
+
+// returns true and records the start event when this launch is sampled (never while the stream is
+// being captured into a graph: the events would only be recorded by the replays)
 static bool prof_begin(int kind, hipStream_t s, ProfSample& smp, double bytes) {
   if (g_prof_period <= 0) return false;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
   if ((g_prof_count[kind]++ % g_prof_period) != 0) return false;
-  if (g_prof_pending.size() > 4096) return false;
+  if (g_prof_pending.size() >= 512) prof_drain();           // keeps the whole timed region sampled
+  if (g_prof_pending.size() > (1u << 16)) return false;     // the host runs that far ahead: skip
   smp.kind = kind; smp.bytes = bytes; smp.a = prof_event(); smp.b = prof_event();
   if (!smp.a || !smp.b) return false;
   (void)hipEventRecord(smp.a, s);

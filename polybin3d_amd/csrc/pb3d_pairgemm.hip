@@ -17,73 +17,15 @@
 // k = 8*(l/32) + r (r = 0..7 of an 8 x bf16 operand); B[k][j]: j = l%32, same k; accumulator
 // register v of lane l holds C[8*(v/4) + 4*(l/32) + v%4][l%32].
 #include "pb3d_ops.h"
+#include "pb3d_mfma.h"
 
 namespace pb3d {
-
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 constexpr int kPgKS = 16;         // voxels (forward) / pairs (adjoint) per MFMA k-step
 constexpr int kPgWaves = 4;       // waves per block
 constexpr int kPgThreads = 64 * kPgWaves;
 constexpr int kPgMaxM = 128;      // A rows (maps) per call: up to 4 M-tiles of 32
 constexpr int kPgMaxMaps = 256;   // maps in the stack the forward kernel stages
-
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-// x = p[0] + p[1] (+ p[2]) in bf16 parts; NP = 2 or 3
-template <int NP>
-__device__ __forceinline__ void split8(const float* x, bf16x8* p) {
-  if constexpr (NP == 3) {
-    // truncating split, exact: hi = the top 8 significand bits of x, mid = the top 8 of x - hi,
-    // lo = x - hi - mid, which has at most 8 significant bits (a bf16 is the high half of a float);
-    // masks, subtractions and one byte permute per pair of values
-    uint32_t w[3][8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const uint32_t h = __float_as_uint(x[r]) & 0xffff0000u;
-      const float r1 = x[r] - __uint_as_float(h);
-      const uint32_t m = __float_as_uint(r1) & 0xffff0000u;
-      w[0][r] = h;
-      w[1][r] = m;
-      w[2][r] = __float_as_uint(r1 - __uint_as_float(m));
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      u32x4 v;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = __builtin_amdgcn_perm(w[q][2 * j + 1], w[q][2 * j], 0x07060302u);
-      p[q] = __builtin_bit_cast(bf16x8, v);
-    }
-    return;
-  }
-#pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    float v = x[r];
-    const __bf16 h = (__bf16)v;
-    v -= (float)h;
-    const __bf16 m = (__bf16)v;
-    p[0][r] = h;
-    p[1][r] = m;
-    if constexpr (NP == 3) {
-      v -= (float)m;
-      p[2][r] = (__bf16)v;
-    }
-  }
-}
-
-template <int NP>
-__device__ __forceinline__ f32x16 mma_split(const bf16x8* a, const bf16x8* b, f32x16 c) {
-  if constexpr (NP == 3) {  // small terms first
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[2], c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2], b[0], c, 0, 0, 0);
-  }
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[0], c, 0, 0, 0);
-  return c;
-}
 
 // ------------------------------------------------------------------------------------------
 // Forward.  Block (blockIdx.x = group of nw N-tiles of 32 pairs, one per wave, blockIdx.y = voxel

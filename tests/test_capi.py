@@ -110,3 +110,29 @@ def test_struct_layouts_match_c_compiler(tmp_path):
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert got == [ctypes.sizeof(C.Grid), ctypes.sizeof(C.ROp), ctypes.sizeof(C.KOp), ctypes.sizeof(C.SubMap),
                    C.KOp.support.offset, C.KOp.ft.offset, C.SubMap.coef.offset]
+
+
+def test_fisher_tiles_host_list():
+    """pb3d_fisher_tiles (host only): tile count, largest segment first, row tiles of one column tile
+    consecutive, odd or negative sizes rejected."""
+    lib = C.lib()
+    segs = []
+    for k, rows in ((64, 130), (1000, 5), (0, 7), (200, 300)):
+        s = C.FisherSeg()
+        s.k, s.rows = k, rows
+        segs.append(s)
+    arr = (C.FisherSeg * 4)(*segs)
+    nq = 300
+    n = lib.pb3d_fisher_tiles(arr, 4, nq, None, 0)
+    assert n == 3 * (2 + 1 + 3)
+    tiles = (ctypes.c_int32 * (3 * n))()
+    assert lib.pb3d_fisher_tiles(arr, 4, nq, tiles, n - 1) == -1
+    assert lib.pb3d_fisher_tiles(arr, 4, nq, tiles, n) == n
+    t = [tuple(tiles[3 * i:3 * i + 3]) for i in range(n)]
+    assert [x[0] for x in t] == [1] * 3 + [3] * 9 + [0] * 6
+    assert t[3:6] == [(3, 0, 0), (3, 128, 0), (3, 256, 0)]
+    segs[0].k = 63
+    arr = (C.FisherSeg * 4)(*segs)
+    assert lib.pb3d_fisher_tiles(arr, 4, nq, None, 0) == -1
+    assert b"odd" in lib.pb3d_last_error()
+

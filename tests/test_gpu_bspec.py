@@ -1,10 +1,15 @@
 """GPU parity of the bispectrum path against the FP64 oracle on identical seeded inputs.
-Tolerance: 1e-5 of the per-bin scale (|B| of the oracle numerator; sqrt(F_ii F_jj) for Fisher
-entries, which cross zero off the diagonal)."""
+Tolerance: 1e-5 of the per-bin scale (each multipole's own |B|, floored at 1e-3 of its largest bin;
+sqrt(F_ii F_jj) for Fisher entries, which cross zero off the diagonal); every check prints its error."""
+import inspect
+
 import numpy as np
 import pytest
 
 from oracle import polybin_oracle as O
+
+import parity
+from parity import own_scale
 
 pytestmark = pytest.mark.gpu
 
@@ -21,11 +26,11 @@ def ng_field(n, seed=0):
     return d + 0.5 * (d ** 2 - np.mean(d ** 2))
 
 
-def close(got, ref, tol=1e-5, scale=None):
-    got, ref = np.asarray(got), np.asarray(ref)
-    s = np.abs(ref) if scale is None else scale
-    err = np.max(np.abs(got - ref) / np.maximum(s, 1e-300))
-    assert err < tol, "max scaled error %.3e" % err
+def close(got, ref, tol=1e-5, scale=None, what=None):
+    """parity.check labelled by the calling test and line."""
+    f = inspect.currentframe().f_back
+    label = "%s:%d" % (f.f_code.co_name, f.f_lineno)
+    parity.check(label if what is None else "%s %s" % (label, what), got, ref, scale=scale, tol=tol)
 
 
 def fisher_scale(F):
@@ -60,8 +65,7 @@ def test_bk_ideal(hip, case):
         sc = np.abs(bo.Bk_numerator_ideal(d))
         close(bs.Bk_numerator_ideal(d), bo.Bk_numerator_ideal(d), scale=np.maximum(sc, sc.max() * 1e-3))
         for k in ref:
-            s = np.abs(ref['b0'])
-            close(got[k], ref[k], tol=2e-5, scale=np.maximum(s, 1e-3 * s.max()))
+            close(got[k], ref[k], scale=own_scale(ref[k]), what=k)
 
 
 def shell_mask(n, L, center):
@@ -88,7 +92,7 @@ def test_bk_mc_fisher_per_seed(hip, case):
     for seed in (0, 3):
         F1 = bs.compute_fisher_contribution(seed)
         F2 = bo.compute_fisher_contribution(seed)
-        close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+        close(F1, F2, scale=fisher_scale(F2))
 
 
 def test_bk_unwindowed_with_linear_term(hip):
@@ -102,14 +106,13 @@ def test_bk_unwindowed_with_linear_term(hip):
     bs.generate_sims(3)
     bo.generate_sims(3)
     fish = bo.compute_fisher(2)
-    close(bs.compute_fisher(2), fish, tol=2e-5, scale=fisher_scale(fish))
+    close(bs.compute_fisher(2), fish, scale=fisher_scale(fish))
     d = ng_field((n, n, n), 4) * mask
     n1, n2 = bs.Bk_numerator(d), bo.Bk_numerator(d)
     close(n1, n2, scale=np.maximum(np.abs(n2), 1e-3 * np.abs(n2).max()))
     got, ref = bs.Bk_unwindowed(d, fish=fish), bo.Bk_unwindowed(d, fish=fish)
     for k in ref:
-        s = np.abs(ref['b0'])
-        close(got[k], ref[k], tol=5e-5, scale=np.maximum(s, 1e-3 * s.max()))
+        close(got[k], ref[k], scale=own_scale(ref[k]), what=k)
 
 
 @pytest.mark.parametrize("case", ["global_mask_l2", "local_mask_l2"])
@@ -159,7 +162,7 @@ def test_bk_integral_constraints(hip, case):
     for seed in (0, 3):
         F1 = bs.compute_fisher_contribution(seed)
         F2 = bo.compute_fisher_contribution(seed)
-        close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+        close(F1, F2, scale=fisher_scale(F2))
     bs.generate_sims(2)
     bo.generate_sims(2)
     d = ng_field((n, n, n), 4) * mask
@@ -203,7 +206,7 @@ def test_bk_mc_fisher_user_sinv(hip, case):
     for seed in (0, 3):
         F1 = bs.compute_fisher_contribution(seed)
         F2 = bo.compute_fisher_contribution(seed)
-        close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+        close(F1, F2, scale=fisher_scale(F2))
 
 
 @pytest.mark.parametrize("case", ["global_const_l2", "local_mask_l2", "local_gic_l2"])
@@ -232,7 +235,7 @@ def test_bk_covariance_contribution(hip, case):
         ms2 = 0.3 * mask if seed == 3 and case != "global_const_l2" else None
         C1 = bs.compute_covariance_contribution(seed, Pc, mask_shot_2pt=ms2)
         C2 = bo.compute_covariance_contribution(seed, Pc, mask_shot_2pt=ms2)
-        close(C1, C2, tol=2e-5, scale=fisher_scale(C2))
+        close(C1, C2, scale=fisher_scale(C2))
 
 
 @pytest.mark.parametrize("window", ["tsc", "interlaced-cic"])
@@ -250,15 +253,14 @@ def test_bk_pixel_windows(hip, window):
     bs, bo = pb.BSpec(b, edges, lmax=2), O.BSpec(o, edges, lmax=2)
     got, ref = bs.Bk_ideal(d), bo.Bk_ideal(d)
     for k in ref:
-        s = np.abs(ref['b0'])
-        close(got[k], ref[k], tol=2e-5, scale=np.maximum(s, 1e-3 * s.max()))
+        close(got[k], ref[k], scale=own_scale(ref[k]), what=k)
     n1, n2 = bs.Bk_numerator(d, include_linear_term=False), bo.Bk_numerator(d, include_linear_term=False)
     s = np.abs(n2)
-    close(n1, n2, tol=2e-5, scale=np.maximum(s, 1e-3 * s.max()))
+    close(n1, n2, scale=np.maximum(s, 1e-3 * s.max()))
     mask = shell_mask(n, L, [10., 10., 10.])
     bsm, bom = pb.BSpec(b, edges, lmax=2, mask=mask), O.BSpec(o, edges, lmax=2, mask=mask)
     F1, F2 = bsm.compute_fisher_contribution(1), bom.compute_fisher_contribution(1)
-    close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+    close(F1, F2, scale=fisher_scale(F2))
 
 
 def test_bk_mc_fisher_squeezed_partial(hip):
@@ -276,7 +278,7 @@ def test_bk_mc_fisher_squeezed_partial(hip):
         bo = O.BSpec(o, edges, lmax=2, mask=mask, k_bins_squeeze=sq, include_partial_triangles=partial)
         assert bs.N_bins == bo.N_bins and np.array_equal(bs.all_bins, bo.all_bins)
         F1, F2 = bs.compute_fisher_contribution(2), bo.compute_fisher_contribution(2)
-        close(F1, F2, tol=2e-5, scale=fisher_scale(F2))
+        close(F1, F2, scale=fisher_scale(F2))
 
 
 def test_non_cubic_grid_and_box(hip):
@@ -291,9 +293,10 @@ def test_non_cubic_grid_and_box(hip):
     d = rng.randn(*shape)
     d = d + 0.3 * (d ** 2 - 1)
     edges = np.linspace(0.013, 0.09, 6)
-    got, ref = pb.PSpec(b, edges, lmax=2).Pk_ideal(d), O.PSpec(o, edges, lmax=2).Pk_ideal(d)
-    for k in ref:
-        close(got[k], ref[k], scale=np.abs(ref['p0']))
+    pg, po = pb.PSpec(b, edges, lmax=2), O.PSpec(o, edges, lmax=2)
+    got, ref = pg.Pk_ideal(d), po.Pk_ideal(d)
+    parity.check_solve("anisotropic Pk_ideal", got, ref, ['p0', 'p2'], po.fish_ideal, pg.fish_ideal,
+                       po.Pk_numerator_ideal(d), pg.Pk_numerator_ideal(d))
     mask = 0.3 + rng.rand(*shape)
     F1 = pb.PSpec(b, edges, lmax=2, mask=mask).compute_fisher_contribution(0)
     F2 = O.PSpec(o, edges, lmax=2, mask=mask).compute_fisher_contribution(0)
@@ -301,8 +304,7 @@ def test_non_cubic_grid_and_box(hip):
     e3 = np.linspace(0.013, 0.09, 4)
     gb, rb = pb.BSpec(b, e3, lmax=2).Bk_ideal(d), O.BSpec(o, e3, lmax=2).Bk_ideal(d)
     for k in rb:
-        s = np.abs(rb['b0'])
-        close(gb[k], rb[k], tol=2e-5, scale=np.maximum(s, 1e-3 * s.max()))
+        close(gb[k], rb[k], scale=own_scale(rb[k]), what=k)
 
 
 def test_bk_linear_term_sims(hip):
@@ -322,7 +324,7 @@ def test_bk_linear_term_sims(hip):
     bs.generate_sims(3, Pk_input=PkS, preload=False)
     bo.generate_sims(3, Pk_input=PkS)
     n1, n2 = bs.Bk_numerator(d), bo.Bk_numerator(d)
-    close(n1, n2, tol=2e-5, scale=np.maximum(np.abs(n2), 1e-3 * np.abs(n2).max()))
+    close(n1, n2, scale=np.maximum(np.abs(n2), 1e-3 * np.abs(n2).max()))
     sims = [ng_field((n, n, n), 10 + i) * mask for i in range(3)]
     bo.g_bl_maps = np.asarray([bo._process_sim(s, input_type='real') for s in sims])
     bo.N_it = 3
@@ -330,4 +332,4 @@ def test_bk_linear_term_sims(hip):
     for preload in (True, False):
         bl = pb.BSpec(b, edges, lmax=2, mask=mask)
         bl.load_sims(lambda ii: sims[ii], 3, input_type='real', preload=preload)
-        close(bl.Bk_numerator(d), ref, tol=2e-5, scale=np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max()))
+        close(bl.Bk_numerator(d), ref, scale=np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max()))

@@ -438,3 +438,20 @@ def test_bk_shot_pins_reject_planted_mistakes(which, monkeypatch):
     eg = np.max(np.abs(totg - gt)) / np.max(np.abs(gt))
     print("planted %s: local %.2e, global %.2e" % (which, el, eg))
     assert el > 1e-3 and eg > 1e-3
+
+
+@pytest.mark.parametrize("kind", ["global_mask", "local_mask", "local_gic"])
+def test_bk_fisher_subblock_equals_full(kind):
+    """BSpec.fisher_subblock (the restricted oracle used at 256^3, where the full Q-map array does not
+    fit) equals the rows x columns of the full per-seed oracle Fisher, for scattered rows and columns
+    of both multipoles."""
+    bs, _ = _bk_case(kind)
+    F = bs.compute_fisher_contribution(5)
+    rng = np.random.RandomState(2)
+    rows = np.sort(rng.choice(bs.N_bins, 7, replace=False))
+    cols = np.sort(rng.choice(bs.N_bins, 9, replace=False))
+    sub = bs.fisher_subblock(5, rows, cols)
+    ref = F[np.ix_(rows, cols)]
+    err = np.max(np.abs(sub - ref)) / np.max(np.abs(ref))
+    print("Bk Fisher sub-block (%s): %.2e" % (kind, err))
+    assert err < 1e-12
