@@ -33,9 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # by a 1 GiB device copy), averaged over the launches of one realization of this workload, for the
 # kernel they were taken on (profiles/hbm_traffic_pk_bk_mc_fisher_256.md); other kernels report null.
 MEASURED_TRAFFIC = {"pointing_rows<128>": 11.03e9}
-# VALU instructions (wave64, summed over the waves) per launch, rocprofv3 --pmc SQ_INSTS_VALU averaged
-# over the launches of one realization of this workload (profiles/pointing_rows_sq_counters_pk_bk_mc_fisher_256.md)
-MEASURED_VALU = {}
+# wave64 VALU instructions per algorithmic byte, rocprofv3 --pmc SQ_INSTS_VALU summed over the launches
+# of one realization of this workload divided by their algorithmic bytes (the sampler's, period 1)
+# (profiles/pointing_rows_sq_counters_pk_bk_mc_fisher_256.md)
+MEASURED_VALU_PER_BYTE = {"pointing_rows<128>": 0.1760}
 SIMDS = 256 * 4          # MI355X: 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9         # peak engine clock
 KIND_NAMES = {0: "r2c_rows", 1: "c2r_rows", 2: "fft_cols (plain C2R)", 3: "fft_cols (C2R first pass, fused prologue)",
@@ -186,6 +187,10 @@ def main():
     ap.add_argument("--realizations", type=int, default=None,
                     help="realizations per step: mc_fisher (default 4; the Pk seeds of a step share every pass) "
                          "and pk_mc_fisher (default 1, seeds per compute_fisher batch)")
+    ap.add_argument("--graph", action="store_true",
+                    help="mc_fisher, pk_mc_fisher: replay every pass as a captured HIP graph (FisherGraph)")
+    ap.add_argument("--grids", default="128,256",
+                    help="pk_mc_fisher: comma-separated grid sizes")
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="mc_fisher: realizations timed on the plain FP32 single-realization path (0 = skip)")
     ap.add_argument("--mixed", action="store_true",
@@ -238,9 +243,16 @@ def main():
         Fb.add_(bs._fisher_contribution_dev(seed))
         bs.precision = prec
 
+    graphs = None
+
     def batched_step(s0):
         """R realizations: the R Pk seeds share every pass (batched), the Bk seeds run back to back with
-        the split-BF16 MFMA contraction accumulating straight into the FP64 Fisher sum."""
+        the split-BF16 MFMA contraction accumulating straight into the FP64 Fisher sum; with --graph each
+        of the two passes is one HIP-graph replay."""
+        if graphs is not None:
+            graphs[0].replay(s0)
+            graphs[1].replay(s0)
+            return
         seeds = list(range(s0, s0 + R))
         Fp.add_(ps._fisher_contributions_dev(seeds) if R > 1 else ps._fisher_contribution_dev(seeds[0]))
         for sd in seeds:
@@ -261,6 +273,11 @@ def main():
                  "path": "one Pk seed per pass, Bk Fisher contraction as FP32 GEMM (hipBLASLt), no batching"}
         print("plain FP32 leg: %.1f ms per realization" % (1e3 * dtp / args.plain_steps), file=sys.stderr, flush=True)
     # leg 2 (timed): the batched / split-BF16 path
+    if args.graph:
+        t = time.perf_counter()
+        graphs = (pb.FisherGraph(ps, Fp, seeds_per_replay=R), pb.FisherGraph(bs, Fb, seeds_per_replay=R))
+        torch.cuda.synchronize()
+        print("captured the Pk and Bk passes (%.1f s)" % (time.perf_counter() - t), file=sys.stderr, flush=True)
     for w in range(args.warmup):
         batched_step(10 ** 6 + w * R)
         print("warmup step %d done" % w, file=sys.stderr, flush=True)
@@ -287,12 +304,12 @@ def main():
         achieved = by / (ms * 1e-3) / 1e9
         kname = kind_kernel(dom, n)
         hbm_frac = achieved / HBM_PEAK_GBS
-        valu = MEASURED_VALU.get(kname)
+        vpb = MEASURED_VALU_PER_BYTE.get(kname)
         valu_frac = None
-        if valu is not None:
-            # VALU-issue fraction: wave64 VALU instructions per launch x 4 cycles over the SIMD cycles of
-            # the measured launch duration
-            valu_frac = valu * 4. / (SIMDS * CLOCK_HZ * (ms / cnt) * 1e-3)
+        if vpb is not None:
+            # VALU-issue fraction: wave64 VALU instructions of the launch (per-byte count x its algorithmic
+            # bytes) x 4 cycles, over the SIMD cycles of the measured launch duration
+            valu_frac = vpb * (by / cnt) * 4. / (SIMDS * CLOCK_HZ * (ms / cnt) * 1e-3)
         bound = "valu" if (valu_frac is not None and valu_frac > hbm_frac) else "hbm"
         roof = {"bound": bound, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(hbm_frac, 4), "traffic": MEASURED_TRAFFIC.get(kname),
@@ -314,7 +331,7 @@ def main():
                       "pk": {"lmax": 4, "nbins": len(pk_bins) - 1, "kmax": float(pk_bins[-1])},
                       "bk": {"lmax": 2, "nbins": len(bk_bins) - 1, "kmax": float(bk_bins[-1]), "N_bins": bs.N_bins},
                       "mask": "shell 300-480 Mpc/h, 200 holes, 5 Mpc/h smoothing", "rng": "device",
-                      "realizations_per_step": R, "pk_seeds_per_pass": R,
+                      "realizations_per_step": R, "pk_seeds_per_pass": R, "hip_graph": bool(args.graph),
                       "bk_fisher_contraction": args.precision + " MFMA (split BF16, FP32 runs, FP64 store)",
                       "fisher_accumulation": "fp64", "setup_s": round(t_setup, 1)},
            "vs_baseline_is": "value / plain FP32 single-realization leg measured in this run (the metric's "
@@ -331,20 +348,31 @@ def pk_fisher_workload(args, torch, pb):
     accumulation) on a 128^3 and a 256^3 grid.  One step = args.realizations realizations on each grid;
     value = realizations per second over both grids."""
     L, center = 1000., [10., 10., 10.]
-    grids = (128, 256)
+    grids = tuple(int(v) for v in args.grids.split(','))
     specs = {}
     for n in grids:
         b = pb.PolyBin3D(L, [n] * 3, Pk=powerlaw_pk(2 * n * np.pi / L * 1.8), boxcenter=center, sightline='local',
                          rng='device')
-        specs[n] = pb.PSpec(b, np.linspace(0.01, 0.30, 31), lmax=4, mask=shell_mask(n, L, center))
+        # 30 bins to 0.30 h/Mpc, or to 0.9 k_Nyquist on grids too small for that
+        specs[n] = pb.PSpec(b, np.linspace(0.01, min(0.30, 0.9 * n * np.pi / L), 31), lmax=4,
+                            mask=shell_mask(n, L, center))
     R = max(1, args.realizations)
     F = {n: torch.zeros((specs[n].N_bins,) * 2, dtype=torch.float64, device=specs[n].base.device) for n in grids}
     per = {n: 0. for n in grids}
 
     streams = {n: torch.cuda.Stream() for n in grids}  # default priority
+    graphs = {n: pb.FisherGraph(specs[n], F[n], seeds_per_replay=R) for n in grids} if args.graph else None
 
     def step(seed0, timed):
         seeds = list(range(seed0, seed0 + R))
+        if graphs is not None:
+            for n in grids:
+                t = time.perf_counter()
+                graphs[n].replay(seed0)
+                torch.cuda.synchronize()
+                if timed:
+                    per[n] += time.perf_counter() - t
+            return
         if args.mixed:
             # every grid's realizations on its own stream (compute_fisher_mixed's schedule)
             main = torch.cuda.current_stream()
@@ -370,13 +398,15 @@ def pk_fisher_workload(args, torch, pb):
         step(s * R, True)
     dt = time.perf_counter() - t0
     nreal = args.steps * R * len(grids)
-    res = {"metric": "Pk MC Fisher realizations/s (128^3 + 256^3)", "value": round(nreal / dt, 4),
+    res = {"metric": "Pk MC Fisher realizations/s (%s)" % " + ".join("%d^3" % n for n in grids),
+           "value": round(nreal / dt, 4),
            "unit": "realizations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(1e3 * dt / args.steps, 2), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "pk_mc_fisher_128_256", "grids": list(grids), "boxsize": L, "sightline": "local",
-                      "lmax": 4, "nbins": 30, "kmax": 0.3, "mask": "shell", "realizations_per_batch": R,
-                      "fisher_accumulation": "fp64", "schedule": "mixed streams" if args.mixed else "grid by grid"}}
+           "config": {"workload": "pk_mc_fisher_" + "_".join(str(n) for n in grids), "grids": list(grids),
+                      "boxsize": L, "sightline": "local", "lmax": 4, "nbins": 30, "kmax": 0.3, "mask": "shell",
+                      "realizations_per_batch": R, "hip_graph": bool(args.graph), "fisher_accumulation": "fp64",
+                      "schedule": "mixed streams" if args.mixed else "grid by grid"}}
     if not args.mixed:
         res["ms_per_realization"] = {str(n): round(1e3 * per[n] / (args.steps * R), 2) for n in grids}
     print(json.dumps(res), flush=True)

@@ -234,6 +234,11 @@ int pb3d_grf_hermitian(const pb3d_grid* grid, const double* re, const double* im
  * instead of the numpy stream.  For throughput runs; parity runs use pb3d_grf_hermitian. */
 int pb3d_grf_device(const pb3d_grid* grid, uint64_t seed, const float* sqrtP_w, float norm, void* out,
                     void* stream);
+/* pb3d_grf_device with the seed read on the device when the kernel runs: seed = *seed_dev + offset
+ * (seed_dev: one uint64 in device memory).  A HIP graph that captured this call draws a new field on
+ * every replay after the caller rewrites *seed_dev (no counterpart in the reference). */
+int pb3d_grf_device_at(const pb3d_grid* grid, const uint64_t* seed_dev, uint64_t offset, const float* sqrtP_w,
+                       float norm, void* out, void* stream);
 
 /* Packed-mode assembly without a transform: out[batch][p] (+)= scale * kop(mode = modes[p]) for
  * p < packed_len (kop kinds FILTER, YLM, WEIGHT, BKQ; for BKQ the input x is ignored).  Used

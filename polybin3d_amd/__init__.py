@@ -7,6 +7,7 @@ ctypes in _capi.py; it must be built in-tree (python -m polybin3d_amd.build) bef
 from .base import PolyBin3D
 from .pspec import PSpec, compute_fisher_mixed
 from .bspec import BSpec
+from .graph import FisherGraph
 from . import autograd
 
-__all__ = ["PolyBin3D", "PSpec", "BSpec", "autograd", "compute_fisher_mixed"]
+__all__ = ["PolyBin3D", "PSpec", "BSpec", "FisherGraph", "autograd", "compute_fisher_mixed"]

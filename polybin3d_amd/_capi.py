@@ -38,6 +38,8 @@ EXPORTS = {
     "pb3d_gather_rows": (ctypes.c_int, [c_void_p, c_int64, ctypes.c_int, c_void_p, ctypes.c_int, c_int64, c_int64,
                                         c_void_p, c_void_p, c_void_p]),
     "pb3d_grf_device": (ctypes.c_int, [c_void_p, ctypes.c_uint64, c_void_p, c_float, c_void_p, c_void_p]),
+    "pb3d_grf_device_at": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_uint64, c_void_p, c_float, c_void_p,
+                                          c_void_p]),
     "pb3d_grf_hermitian": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
     "pb3d_bin_reduce_work_size": (c_size_t, [c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pb3d_bin_reduce": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, ctypes.c_int,

@@ -485,8 +485,14 @@ class PolyBin3D:
         norm = float(self.N / np.sqrt(self.volume))
         if self.rng == 'device':
             s = int(seed) if seed is not None else int(np.random.randint(0, 2 ** 62))
-            C.check(C.lib().pb3d_grf_device(C.ctypes.byref(self.cgrid), s, sqrtPw.data_ptr(), norm,
-                                            out.data_ptr(), self.stream()))
+            seed_dev = getattr(self, '_seed_dev', None)
+            if seed_dev is not None:
+                # inside a graph capture: seed = (device seed base) + s, set before every replay
+                C.check(C.lib().pb3d_grf_device_at(C.ctypes.byref(self.cgrid), seed_dev.data_ptr(), s,
+                                                   sqrtPw.data_ptr(), norm, out.data_ptr(), self.stream()))
+            else:
+                C.check(C.lib().pb3d_grf_device(C.ctypes.byref(self.cgrid), s, sqrtPw.data_ptr(), norm,
+                                                out.data_ptr(), self.stream()))
         else:
             rs = np.random.RandomState(seed) if seed is not None else np.random.mtrand._rand
             re = torch.from_numpy(rs.randn(*n)).to(self.device)

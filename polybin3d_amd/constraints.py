@@ -64,7 +64,13 @@ class Pointing:
             s = mw.sum().reshape(1)
         else:
             s = torch.segment_reduce(mw[self.rperm], 'sum', lengths=self.rcounts)
-        return s / torch.from_numpy(self.den).to(s.device)
+        return s / self._den_dev()
+
+    def _den_dev(self):
+        """The per-bin denominators as an FP64 device tensor, uploaded once."""
+        if getattr(self, '_den_t', None) is None:
+            self._den_t = torch.from_numpy(np.asarray(self.den, dtype=np.float64)).to(self.base.device)
+        return self._den_t
 
     def _per_voxel(self, av):
         if self.kind == 'GIC':
@@ -105,7 +111,7 @@ class Pointing:
             self._corr[key] = corr_maps(wz)
         corr = self._corr[key]
         sums = torch.zeros((batch, self.nbins), dtype=torch.float64, device=b.device)
-        den = torch.from_numpy(self.den).to(b.device)
+        den = self._den_dev()
 
         def finish(out):
             coef = (sums / den).to(torch.float32)  # [batch, nbins]

@@ -144,10 +144,15 @@ __global__ __launch_bounds__(kFthreads, 2) void fisher_gemm_kernel(
   }
 }
 
+// compute units of the current device, queried once per device (no runtime query inside a graph capture
+// after the first eager call)
 static int device_cus() {
+  static int cached[64] = {};
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 256;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) return 256;
+  if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  if (dev >= 0 && dev < 64) cached[dev] = cus;
   return cus;
 }
 

@@ -64,8 +64,10 @@ __device__ __forceinline__ double2 normal_pair(unsigned long long seed, long lon
   return make_double2(rr * cs, rr * sn);
 }
 
-__global__ void grf_device_kernel(unsigned long long seed, const float* __restrict__ sqrtPw, float norm,
-                                  float2* __restrict__ out, Geo g) {
+// seed_at != nullptr: the seed is *seed_at + seed, read on the device (graph replays with new seeds)
+__global__ void grf_device_kernel(unsigned long long seed, const unsigned long long* __restrict__ seed_at,
+                                  const float* __restrict__ sqrtPw, float norm, float2* __restrict__ out, Geo g) {
+  if (seed_at) seed += *seed_at;
   const long long n12 = (long long)g.n1 * g.n2h;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < g.H;
        e += (long long)gridDim.x * blockDim.x) {
@@ -157,7 +159,19 @@ extern "C" int pb3d_grf_device(const pb3d_grid* grid, uint64_t seed, const float
   if (int e = check_grid(grid)) return e;
   if (!out) return fail(PB3D_ERR_ARG, "grf_device: null output");
   Geo g = make_geo(*grid);
-  grf_device_kernel<<<grid_for(g.H, 256), 256, 0, (hipStream_t)stream>>>(seed, sqrtP_w, norm, (float2*)out, g);
+  grf_device_kernel<<<grid_for(g.H, 256), 256, 0, (hipStream_t)stream>>>(seed, nullptr, sqrtP_w, norm, (float2*)out,
+                                                                         g);
+  PB3D_CHECK_LAUNCH();
+  return 0;
+}
+
+extern "C" int pb3d_grf_device_at(const pb3d_grid* grid, const uint64_t* seed_dev, uint64_t offset,
+                                  const float* sqrtP_w, float norm, void* out, void* stream) {
+  if (int e = check_grid(grid)) return e;
+  if (!out || !seed_dev) return fail(PB3D_ERR_ARG, "grf_device_at: null output or seed pointer");
+  Geo g = make_geo(*grid);
+  grf_device_kernel<<<grid_for(g.H, 256), 256, 0, (hipStream_t)stream>>>(
+      offset, (const unsigned long long*)seed_dev, sqrtP_w, norm, (float2*)out, g);
   PB3D_CHECK_LAUNCH();
   return 0;
 }
