@@ -147,13 +147,17 @@ typedef struct {
  *   sums[b*nbins + z] += sum over voxels x with bin[x] == z of m_b(x) * weight[x],
  * where m_b = scale * sum_g coef_g Y_g(x_hat) C2R(kop_g(in_g))[b] is the combined real map before the
  * mask.  bin == NULL puts every voxel in bin 0 (global constraint, nbins = 1); entries >= nbins are
- * in no bin.  1 <= nbins <= 64.  The caller zeroes `sums`; FP64 accumulation. */
+ * in no bin.  1 <= nbins <= 64.  The caller zeroes `sums`; FP64 accumulation.  `partial`: device
+ * scratch of pb3d_icsum_partial_size() doubles, where every wave of the row kernel stores its own
+ * partial sums (no atomics); they are then added to `sums` in a fixed order, so the sums are
+ * bitwise reproducible. */
 typedef struct {
   const float* weight;
   const uint8_t* bin;
   int32_t nbins;
   int32_t pad_;
   double* sums;
+  double* partial;
 } pb3d_icsum;
 
 /* ---------------------------------------------------------------- library */
@@ -215,6 +219,9 @@ int pb3d_fft_pointing(const pb3d_plan* plan, int batch, int nsub, const pb3d_sub
  * pointing allocates full maps per step.) */
 int64_t pb3d_fft_pointing_work_size(const pb3d_grid* grid, int batch, int nsub, const pb3d_submap* subs,
                                     const pb3d_kop* kop_out);
+/* doubles of pb3d_icsum.partial for a pointing call of `batch` maps and `nbins` bins (host-only; -1 on
+ * a bad argument). */
+int64_t pb3d_icsum_partial_size(const pb3d_grid* grid, int batch, int nbins);
 
 /* ---------------------------------------------------------------- mode-space maps */
 /* out[batch] = kop(x) over the half grid, without a transform (fourier_filter,

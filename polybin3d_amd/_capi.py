@@ -32,6 +32,7 @@ EXPORTS = {
     "pb3d_fft_pointing": (ctypes.c_int, [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_float, c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_void_p]),
     "pb3d_fft_pointing_work_size": (c_int64, [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p]),
+    "pb3d_icsum_partial_size": (c_int64, [c_void_p, ctypes.c_int, ctypes.c_int]),
     "pb3d_kmap": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p]),
     "pb3d_kmap_packed": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, ctypes.c_int, c_void_p,
                                         c_void_p]),
@@ -102,7 +103,8 @@ class FisherSeg(ctypes.Structure):
 
 class IcSum(ctypes.Structure):
     """pb3d_icsum: per-bin weighted sums of the pointing chain's combined real map."""
-    _fields_ = [("weight", c_void_p), ("bin", c_void_p), ("nbins", c_int32), ("pad_", c_int32), ("sums", c_void_p)]
+    _fields_ = [("weight", c_void_p), ("bin", c_void_p), ("nbins", c_int32), ("pad_", c_int32), ("sums", c_void_p),
+                ("partial", c_void_p)]
 
 
 ROP_PLAIN, ROP_MUL, ROP_DIFF, ROP_YLM, ROP_MULYLM, ROP_SQUARE = range(6)

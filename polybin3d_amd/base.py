@@ -429,7 +429,11 @@ class PolyBin3D:
             if rbin is not None and (rbin.dtype != torch.uint8 or rbin.numel() < self.N):
                 raise ValueError("pointing ic bin: uint8 device tensor of the grid's size expected")
             self._need(sums, batch * nbins, 'pointing ic sums')
-            icd = C.IcSum(weight.data_ptr(), C._ptr(rbin), int(nbins), 0, sums.data_ptr())
+            npart = C.lib().pb3d_icsum_partial_size(C.ctypes.byref(self.cgrid), batch, int(nbins))
+            if npart < 0:
+                C.check(-1)
+            part = self.work('icsum_partial', 8 * npart)
+            icd = C.IcSum(weight.data_ptr(), C._ptr(rbin), int(nbins), 0, sums.data_ptr(), part.data_ptr())
             keep.append((weight, rbin, sums))
         C.check(C.lib().pb3d_fft_pointing(self._plan, batch, G, arr, C._ptr(mask), float(scale),
                                           C.byref_or_none(kop_out), C.byref_or_none(icd), out.data_ptr(),

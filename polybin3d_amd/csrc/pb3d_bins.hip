@@ -1,11 +1,13 @@
 // k-bin reductions: one coalesced pass over the half grid, wavefront-shuffle segmented sums keyed
-// by (row, bin), FP64 LDS accumulation per block, fixed-order FP64 reduction across blocks.
+// by (row, bin), FP64 LDS accumulation per one-wave block, fixed-order FP64 reduction across blocks.
 // Replaces IntegrationUtils.integrate / cross_integrate / real_integrate_double
 // (PolyBin3D/cython/utils.pyx:51-134), which sweep the whole grid once per bin.
 //
 // Along a row (fixed kx, ky) |k| grows with kz, so the bin index is non-decreasing along the
 // contiguous axis: equal-bin modes form contiguous runs and a 64-lane segmented scan reduces each
-// run to one LDS atomic.
+// run to one value.  The run totals of a wave step are added to the block's LDS histogram one run
+// at a time in lane order (a ballot over the run tails), by the lanes o < nout in parallel: every
+// sum is formed in the same order on every call, so the results are bitwise reproducible.
 #include "pb3d_ops.h"
 
 namespace pb3d {
@@ -13,16 +15,18 @@ namespace pb3d {
 constexpr int kMaxOut = 16;
 struct Ells { int n; int l1[kMaxOut]; int l2[kMaxOut]; };
 
+constexpr int kBinThreads = 64;  // one wave per block
+
 static long long bin_blocks(long long H, int nq) {
-  long long gx = 2048 / (nq > 0 ? nq : 1);
+  long long gx = 8192 / (nq > 0 ? nq : 1);
   if (gx < 1) gx = 1;
-  long long maxb = (H + 255) / 256;
+  long long maxb = (H + kBinThreads - 1) / kBinThreads;
   if (gx > maxb) gx = maxb;
   return gx;
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kBinThreads)
 bin_reduce_kernel(const void* __restrict__ xv, long long xs, const float2* __restrict__ y, long long yq,
                   long long yo, Ells el, const uint8_t* __restrict__ table, int nbins, Geo g, long long chunk,
                   const int32_t* __restrict__ modes, long long count, double* __restrict__ partial) {
@@ -101,10 +105,21 @@ bin_reduce_kernel(const void* __restrict__ xv, long long xs, const float2* __res
     }
     const int sn = __shfl_down(seg, 1);
     const bool tail = (lane == 63) || sn != seg;
-    if (tail && key != PB3D_NO_BIN) {
+    // run totals into the histogram in lane order: lane o adds output o of each run in turn
+    unsigned long long runs = __ballot(tail && key != PB3D_NO_BIN);
+    while (runs) {
+      const int src = __ffsll((long long)runs) - 1;
+      runs &= runs - 1;
+      const int k = __shfl(key, src);
+      float mine = 0.f;
 #pragma unroll
-      for (int o = 0; o < kMaxOut; ++o)
-        if (o < nout) atomicAdd(&hist[o * nbins + key], (double)v[o]);
+      for (int o = 0; o < kMaxOut; ++o) {
+        if (o < nout) {
+          const float vo = __shfl(v[o], src);
+          if (lane == o) mine = vo;
+        }
+      }
+      if (lane < nout) hist[lane * nbins + k] += (double)mine;
     }
   }
   __syncthreads();
@@ -156,16 +171,16 @@ static int bin_reduce_impl(const pb3d_grid* grid, const void* x, int64_t x_strid
   const long long count = modes ? (long long)nmodes : g.H;
   const long long gx = bin_blocks(count > 0 ? count : 1, nq);
   long long chunk = (count + gx - 1) / gx;
-  chunk = (chunk + 255) / 256 * 256;
+  chunk = (chunk + kBinThreads - 1) / kBinThreads * kBinThreads;
   const int nh = nout * nbins;
   dim3 gr((unsigned)gx, (unsigned)nq);
   hipStream_t s = (hipStream_t)stream;
   const size_t shm = (size_t)nh * sizeof(double);
   if (mode == 0)
-    bin_reduce_kernel<0><<<gr, 256, shm, s>>>(x, x_stride, (const float2*)y, y_qstride, y_ostride, el, bin_table,
+    bin_reduce_kernel<0><<<gr, kBinThreads, shm, s>>>(x, x_stride, (const float2*)y, y_qstride, y_ostride, el, bin_table,
                                               nbins, g, chunk, modes, count, (double*)work);
   else
-    bin_reduce_kernel<1><<<gr, 256, shm, s>>>(x, x_stride, nullptr, 0, 0, el, bin_table, nbins, g, chunk,
+    bin_reduce_kernel<1><<<gr, kBinThreads, shm, s>>>(x, x_stride, nullptr, 0, 0, el, bin_table, nbins, g, chunk,
                                               modes, count, (double*)work);
   PB3D_CHECK_LAUNCH();
   const long long tot = (long long)nq * nh;

@@ -838,13 +838,17 @@ struct SubW {
   float poly[kMaxSub][kMaxMono];
 };
 
-// Per-bin weighted sums of the combined real map (pb3d_icsum), by value for the kernel.
+// Per-bin weighted sums of the combined real map (pb3d_icsum), by value for the kernel: each wave
+// stores its FP64 partial sums in partial[(map * nwm + wave) * nbins + bin] (nwm = waves per map),
+// and ic_finalize adds them to sums in wave order.
 constexpr int kMaxIcBins = 64;
 struct IcSum {
   const float* weight;  // nullptr: no sums
   const uint8_t* bin;   // nullptr: every voxel in bin 0
   int nbins;
   double* sums;
+  double* partial;
+  long long nwm;
 };
 
 template <int M> struct PointCfg {
@@ -1005,23 +1009,42 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   // mask and scale in registers, then the forward row transform (first stage from registers)
   const float norm = scale / (float)geo.N;
   if constexpr (IC) {
-    // per-bin sums of m * weight over this wave's voxels: FP32 in LDS, then one FP64 atomic per bin
+    // per-bin sums of m * weight over this wave's voxels, in a fixed order: for each voxel slot and
+    // each bin present in it, a butterfly sum over the wave, added by lane 0 to the wave's FP32 LDS
+    // sums; then the wave's FP64 partials go to their own slot (no atomics)
     if (lane < ic.nbins) icl[wave][lane] = 0.f;
     wave_lds_sync();
-    if (row < rows_per_map) {
-      // one voxel pair at a time (this path runs only for integral constraints; keeping the weights
-      // of all slots live next to the accumulators would spill)
+    const bool rv = row < rows_per_map;
+    auto add_keyed = [&](int z, float c) {
+      unsigned long long act = __ballot(z < ic.nbins);
+      while (act) {
+        const int k = __shfl(z, __ffsll((long long)act) - 1);
+        float v = z == k ? c : 0.f;
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) icl[wave][k] += v;
+        act &= ~__ballot(z == k);
+      }
+    };
+    // one voxel pair at a time (this path runs only for integral constraints; keeping the weights
+    // of all slots live next to the accumulators would spill)
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      float c0 = 0.f, c1 = 0.f;
+      int z0 = kMaxIcBins, z1 = kMaxIcBins;
+      if (rv) {
         const long long v0 = row * (2 * M) + 2 * (tp + e * P);
         const float2 wv = *reinterpret_cast<const float2*>(ic.weight + v0);
-        const int z0 = ic.bin ? ic.bin[v0] : 0, z1 = ic.bin ? ic.bin[v0 + 1] : 0;
-        if (z0 < ic.nbins) atomicAdd(&icl[wave][z0], acc[e][0] * norm * wv.x);
-        if (z1 < ic.nbins) atomicAdd(&icl[wave][z1], acc[e][1] * norm * wv.y);
+        z0 = ic.bin ? ic.bin[v0] : 0;
+        z1 = ic.bin ? ic.bin[v0 + 1] : 0;
+        c0 = acc[e][0] * norm * wv.x;
+        c1 = acc[e][1] * norm * wv.y;
       }
+      add_keyed(z0, c0);
+      add_keyed(z1, c1);
     }
     wave_lds_sync();
-    if (lane < ic.nbins) atomicAdd(&ic.sums[t * ic.nbins + lane], (double)icl[wave][lane]);
+    if (lane < ic.nbins) ic.partial[(t * ic.nwm + row0 / RW) * ic.nbins + lane] = (double)icl[wave][lane];
   }
   if (mask && row < rows_per_map) {
     float2 mk[E];  // all mask loads in flight together
@@ -1046,6 +1069,18 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
     if (f >= RW * (M + 1) || row0 + rr >= rows_per_map || k >= lz_out) continue;
     o[(long long)rr * pitch_out + k] = r2c_split<M>(buf, LDT, rr, k, stwN);
   }
+}
+
+// sums[t * nbins + z] += sum over the waves of map t of their partials, in wave order
+__global__ void ic_finalize(const double* __restrict__ partial, long long nwm, int batch, int nbins,
+                            double* __restrict__ sums) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)batch * nbins) return;
+  const long long t = i / nbins;
+  const int z = (int)(i % nbins);
+  double s = 0.;
+  for (long long w = 0; w < nwm; ++w) s += partial[(t * nwm + w) * nbins + z];
+  sums[i] += s;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1193,11 +1228,17 @@ static hipError_t launch_pointing_rows(int M, const float2* work, long long slot
   case MM: {                                                                                     \
     constexpr int RB = PointCfg<MM>::RB;                                                         \
     dim3 grid((unsigned)((rpm + RB - 1) / RB), batch);                                           \
-    if (ic.weight)                                                                               \
+    if (ic.weight) {                                                                             \
+      IcSum icw = ic;                                                                            \
+      icw.nwm = (rpm + PointCfg<MM>::RW - 1) / PointCfg<MM>::RW;                                 \
       pointing_rows<MM, true><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, \
-                                                         pitch_out, rpm, batch, sw, ic, mask, scale, \
+                                                         pitch_out, rpm, batch, sw, icw, mask, scale, \
                                                          lz_in, lz_out, tw, twN, geo);           \
-    else                                                                                         \
+      hipError_t e = hipGetLastError();                                                          \
+      if (e != hipSuccess) return e;                                                             \
+      const long long nf = (long long)batch * ic.nbins;                                          \
+      ic_finalize<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(icw.partial, icw.nwm, batch, ic.nbins, ic.sums); \
+    } else                                                                                       \
       pointing_rows<MM, false><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, \
                                                          pitch_out, rpm, batch, sw, ic, mask, scale, \
                                                          lz_in, lz_out, tw, twN, geo);           \
@@ -1536,6 +1577,28 @@ extern "C" int64_t pb3d_fft_pointing_work_size(const pb3d_grid* grid, int batch,
   return (int64_t)batch * g.n0 * g.n1 * ((int64_t)nsub * compact_pitch(g, pin.lz) + compact_pitch(g, pout.lz));
 }
 
+extern "C" int64_t pb3d_icsum_partial_size(const pb3d_grid* grid, int batch, int nbins) {
+  if (!grid || batch < 1 || nbins < 1 || nbins > kMaxIcBins) {
+    fail(PB3D_ERR_ARG, "icsum_partial_size: null grid, batch < 1 or nbins out of [1, 64]");
+    return -1;
+  }
+  const Geo g = make_geo(*grid);
+  long long rw = 64;
+  switch (g.n2 / 2) {
+    case 2: rw = PointCfg<2>::RW; break;
+    case 4: rw = PointCfg<4>::RW; break;
+    case 8: rw = PointCfg<8>::RW; break;
+    case 16: rw = PointCfg<16>::RW; break;
+    case 32: rw = PointCfg<32>::RW; break;
+    case 64: rw = PointCfg<64>::RW; break;
+    case 128: rw = PointCfg<128>::RW; break;
+    case 256: rw = PointCfg<256>::RW; break;
+    default: rw = 1; break;
+  }
+  const long long rpm = (long long)g.n0 * g.n1;
+  return (int64_t)batch * ((rpm + rw - 1) / rw) * nbins;
+}
+
 extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const pb3d_submap* subs, const float* mask,
                                  float scale, const pb3d_kop* kop_out, const pb3d_icsum* ic_, void* out_, void* work_,
                                  void* stream) {
@@ -1544,11 +1607,11 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   if (nsub < 1 || nsub > kMaxSub) return fail(PB3D_ERR_ARG, "pointing: nsub out of range [1, 16]");
   if (kop_out && kop_out->kind == 4) return fail(PB3D_ERR_ARG, "pointing: BKQ is a prologue");
   if (int e = check_kop(kop_out)) return e;
-  IcSum ic{nullptr, nullptr, 1, nullptr};
+  IcSum ic{nullptr, nullptr, 1, nullptr, nullptr, 0};
   if (ic_) {
-    if (!ic_->weight || !ic_->sums || ic_->nbins < 1 || ic_->nbins > kMaxIcBins)
-      return fail(PB3D_ERR_ARG, "pointing: ic needs weight, sums and 1 <= nbins <= 64");
-    ic = IcSum{ic_->weight, ic_->bin, ic_->nbins, ic_->sums};
+    if (!ic_->weight || !ic_->sums || !ic_->partial || ic_->nbins < 1 || ic_->nbins > kMaxIcBins)
+      return fail(PB3D_ERR_ARG, "pointing: ic needs weight, sums, partial and 1 <= nbins <= 64");
+    ic = IcSum{ic_->weight, ic_->bin, ic_->nbins, ic_->sums, ic_->partial, 0};
   }
   hipStream_t s = (hipStream_t)stream;
   const Geo& g = p->geo;

@@ -1,0 +1,59 @@
+"""Bitwise reproducibility: the reductions are formed in a fixed order (bin sums: run totals added in
+lane order per one-wave block, block partials summed in block order; integral-constraint sums: per-wave
+butterfly sums stored per wave and added in wave order; Fisher contraction: one writer per output), so
+a realization computed twice gives identical bits."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+L = 1000.
+
+
+def shell(n, c):
+    x = [np.fft.fftshift(np.arange(-n // 2, n // 2)) * L / n + c[i] for i in range(3)]
+    X = np.meshgrid(*x, indexing='ij')
+    r = np.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
+    return 0.2 + ((r >= 150.) & (r <= 480.)).astype(np.float64)
+
+
+@pytest.mark.parametrize("ic", ["none", "gic", "ric"])
+def test_pk_fisher_realization_bitwise(hip, ic):
+    import polybin3d_amd as pb
+    n, c = 64, [10., 10., 10.]
+    b = pb.PolyBin3D(L, [n] * 3, boxcenter=c, sightline='local')
+    mask = shell(n, c)
+    kw = {} if ic == "none" else (dict(add_GIC=True, mask_IC=mask * 1.1) if ic == "gic" else
+                                   dict(add_RIC=True, mask_IC=mask * 1.1, radial_bins_RIC=[100., 250., 400., 600.]))
+    ps = pb.PSpec(b, np.linspace(0.01, 0.15, 11), lmax=4, mask=mask, **kw)
+    F1 = ps.compute_fisher_contribution(3)
+    F2 = ps.compute_fisher_contribution(3)
+    assert np.abs(F1).max() > 0
+    assert np.array_equal(F1, F2)
+
+
+@pytest.mark.parametrize("ic", ["none", "gic"])
+def test_bk_fisher_realization_bitwise(hip, ic):
+    import polybin3d_amd as pb
+    n, c = 32, [10., 10., 10.]
+    b = pb.PolyBin3D(L, [n] * 3, boxcenter=c, sightline='local')
+    mask = shell(n, c)
+    kw = {} if ic == "none" else dict(add_GIC=True, mask_IC=mask * 1.1)
+    bs = pb.BSpec(b, np.linspace(0.01, 0.07, 5), lmax=2, mask=mask, **kw)
+    F1 = bs.compute_fisher_contribution(2)
+    F2 = bs.compute_fisher_contribution(2)
+    assert np.abs(F1).max() > 0
+    assert np.array_equal(F1, F2)
+
+
+def test_bin_sums_bitwise(hip):
+    import polybin3d_amd as pb
+    import torch
+    n = 64
+    b = pb.PolyBin3D(L, [n] * 3)
+    rng = np.random.RandomState(5)
+    x = torch.from_numpy((rng.randn(4, *b.half_shape) + 1j * rng.randn(4, *b.half_shape)).astype(np.complex64)).cuda()
+    edges = np.linspace(0.01, 0.19, 19)
+    r1 = b.bin_reduce(x, edges, [0, 2, 4], nq=4).cpu().numpy()
+    r2 = b.bin_reduce(x, edges, [0, 2, 4], nq=4).cpu().numpy()
+    assert np.array_equal(r1, r2)
