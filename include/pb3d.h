@@ -283,6 +283,20 @@ int pb3d_bin_reduce_modes(const pb3d_grid* grid, const void* x, int64_t x_stride
                           const int32_t* ells2, int mode, const uint8_t* bin_table, int nbins,
                           const int32_t* modes, int64_t nmodes, double* out, void* work, void* stream);
 
+/* pb3d_bin_reduce_modes without a workspace argument: allocates nothing (no hipMalloc, hipFree or
+ * allocating library call), for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  No counterpart in the reference.  The block
+ * partials go to a fixed 16 MiB device buffer of the library (2^21 doubles per GPU, held until the
+ * process ends); a call that would need more (nq * nout * nbins * blocks, blocks <= 8192 / nq) is
+ * rejected with PB3D_ERR_ARG before any launch.  Every call on one GPU shares that GPU's buffer: calls
+ * that overlap on one GPU (two streams, two host threads) race; stream-ordered calls do not.  The
+ * buffer needs no clearing: each call overwrites the entries it reads.  Same results as
+ * pb3d_bin_reduce_modes, bit for bit. */
+int pb3d_bin_reduce_modes_noalloc(const pb3d_grid* grid, const void* x, int64_t x_stride, const void* y,
+                                  int64_t y_qstride, int64_t y_ostride, int nq, int nout, const int32_t* ells,
+                                  const int32_t* ells2, int mode, const uint8_t* bin_table, int nbins,
+                                  const int32_t* modes, int64_t nmodes, double* out, void* stream);
+
 /* ---------------------------------------------------------------- triplet contraction
  * T[t] = sum_x g[i1[t]](x) g[i2[t]](x) g[i3[t]](x) for ntrip index triples into the
  * float32 map stack g[nmaps][nvox] (assemble_b3 / sum_triplet, utils.pyx:357-366,614-636).

@@ -49,6 +49,9 @@ EXPORTS = {
     "pb3d_bin_reduce_modes": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, ctypes.c_int,
                                              ctypes.c_int, c_void_p, c_void_p, ctypes.c_int, c_void_p, ctypes.c_int,
                                              c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "pb3d_bin_reduce_modes_noalloc": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64,
+                                                     ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, ctypes.c_int,
+                                                     c_void_p, ctypes.c_int, c_void_p, c_int64, c_void_p, c_void_p]),
     "pb3d_pair_gemm_work_size": (c_size_t, [c_int64, ctypes.c_int, ctypes.c_int]),
     "pb3d_pair_gemm": (ctypes.c_int, [c_void_p, ctypes.c_int, c_int64, c_void_p, ctypes.c_int, c_void_p, c_void_p,
                                       ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p]),

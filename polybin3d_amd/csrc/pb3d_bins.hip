@@ -10,6 +10,8 @@
 // sum is formed in the same order on every call, so the results are bitwise reproducible.
 #include "pb3d_ops.h"
 
+#include <mutex>
+
 namespace pb3d {
 
 constexpr int kMaxOut = 16;
@@ -138,6 +140,33 @@ __global__ void bin_finalize(const double* __restrict__ partial, long long gx, l
   out[t] = s;
 }
 
+// Fixed device scratch of pb3d_bin_reduce_modes_noalloc (the block partials of one call): 16 MiB per
+// GPU, held until the process ends.  Host code reaches it only through the per-device address from
+// hipGetSymbolAddress, never through the symbol itself.
+constexpr long long kNoallocDoubles = 2LL << 20;
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's address of `scratch`, looked up on the first call on that device
+static int scratch_address(double** ptr) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(PB3D_ERR_HIP, "bin_reduce_modes_noalloc: no current device (or device index >= 64)");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PB3D_ERR_HIP, std::string("bin_reduce_modes_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = (double*)g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace pb3d
 
 using namespace pb3d;
@@ -204,4 +233,26 @@ extern "C" int pb3d_bin_reduce_modes(const pb3d_grid* grid, const void* x, int64
   if (!modes) return fail(PB3D_ERR_ARG, "bin_reduce_modes: null mode list");
   return bin_reduce_impl(grid, x, x_stride, y, y_qstride, y_ostride, nq, nout, ells, ells2, mode, bin_table, nbins,
                          modes, nmodes, out, work, stream);
+}
+
+extern "C" int pb3d_bin_reduce_modes_noalloc(const pb3d_grid* grid, const void* x, int64_t x_stride, const void* y,
+                                             int64_t y_qstride, int64_t y_ostride, int nq, int nout,
+                                             const int32_t* ells, const int32_t* ells2, int mode,
+                                             const uint8_t* bin_table, int nbins, const int32_t* modes, int64_t nmodes,
+                                             double* out, void* stream) {
+  if (!grid || !modes) return fail(PB3D_ERR_ARG, "bin_reduce_modes_noalloc: null grid or mode list");
+  if (nq < 1 || nq > 65535 || nout < 1 || nout > kMaxOut || nbins < 1 || nbins > 254)
+    return fail(PB3D_ERR_ARG, "bin_reduce_modes_noalloc: nq, nout or nbins out of range");
+  const Geo g = make_geo(*grid);
+  if (nmodes < 0 || nmodes > g.H) return fail(PB3D_ERR_ARG, "bin_reduce_modes_noalloc: nmodes out of range [0, H]");
+  const long long gx = bin_blocks(nmodes > 0 ? nmodes : 1, nq);
+  const long long need = gx * nq * (long long)nout * nbins;
+  if (need > kNoallocDoubles)
+    return fail(PB3D_ERR_ARG, "bin_reduce_modes_noalloc: this call needs " + std::to_string(need) +
+                                  " doubles of scratch; the fixed buffer holds " + std::to_string(kNoallocDoubles) +
+                                  " (use pb3d_bin_reduce_modes with a workspace)");
+  double* ptr = nullptr;
+  if (int e = scratch_address(&ptr)) return e;
+  return bin_reduce_impl(grid, x, x_stride, y, y_qstride, y_ostride, nq, nout, ells, ells2, mode, bin_table, nbins,
+                         modes, nmodes, out, ptr, stream);
 }
