@@ -320,6 +320,15 @@ size_t pb3d_pair_gemm_work_size(int64_t nvox, int na, int np);
 int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int32_t* amap, int na, const int32_t* pb,
                    const int32_t* pc, int np, int precision, double* out, void* work, void* stream);
 
+/* pb3d_pair_gemm with its launch shape chosen by the caller (the autotuner; no counterpart in the
+ * reference): stage_cfg = voxels per LDS stage (0 = the built-in choice, else 32 or the long stage of
+ * the stack's size class: 128 for <= 32 maps, 64 for <= 128), rows_cfg = A rows per launch (0 = the
+ * built-in choice, else 32, 64 or 96).  Any accepted shape gives the same sums up to FP32 rounding
+ * order; a shape that does not fit is rejected with PB3D_ERR_ARG before any launch. */
+int pb3d_pair_gemm_cfg(const float* g, int nmaps, int64_t nvox, const int32_t* amap, int na, const int32_t* pb,
+                       const int32_t* pc, int np, int precision, int stage_cfg, int rows_cfg, double* out, void* work,
+                       void* stream);
+
 /* Adjoint of the triplet sums (no counterpart in the reference; used by the Bk_ideal gradient):
  *   out[a][x] = sum_p U[a][p] g[pb[p]](x) g[pc[p]](x)           (float32 out[na][nvox])
  * U: float32 [na <= 128][np] on the device; g: float32 stack [nmaps <= 128][nvox].  precision as

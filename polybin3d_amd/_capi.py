@@ -55,6 +55,9 @@ EXPORTS = {
     "pb3d_pair_gemm_work_size": (c_size_t, [c_int64, ctypes.c_int, ctypes.c_int]),
     "pb3d_pair_gemm": (ctypes.c_int, [c_void_p, ctypes.c_int, c_int64, c_void_p, ctypes.c_int, c_void_p, c_void_p,
                                       ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p]),
+    "pb3d_pair_gemm_cfg": (ctypes.c_int, [c_void_p, ctypes.c_int, c_int64, c_void_p, ctypes.c_int, c_void_p, c_void_p,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p,
+                                          c_void_p]),
     "pb3d_pair_adjoint_work_size": (c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pb3d_pair_adjoint": (ctypes.c_int, [c_void_p, ctypes.c_int, c_int64, c_void_p, c_void_p, ctypes.c_int, c_void_p,
                                          ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p]),

@@ -13,6 +13,7 @@ import scipy.interpolate
 import torch
 
 from . import _capi as C
+from . import tune
 from .base import harmonics_tags
 from .constraints import Pointing, covariance_filter
 
@@ -223,10 +224,28 @@ class BSpec:
             T = torch.empty((na, npair), dtype=torch.float64, device=b.device)
             ws = C.lib().pb3d_pair_gemm_work_size(nvox, na, npair)
             work = b.work('pair_gemm', ws)
-            C.check(C.lib().pb3d_pair_gemm(stack.data_ptr(), nm, nvox, amap_d.data_ptr(), na, pb_d.data_ptr(),
-                                           pc_d.data_ptr(), npair, prec, T.data_ptr(), work.data_ptr(), b.stream()))
+
+            def launch(cfg, T=T, work=work, amap_d=amap_d, pb_d=pb_d, pc_d=pc_d, na=na, npair=npair):
+                rc = C.lib().pb3d_pair_gemm_cfg(stack.data_ptr(), nm, nvox, amap_d.data_ptr(), na, pb_d.data_ptr(),
+                                                pc_d.data_ptr(), npair, prec, cfg[0], cfg[1], T.data_ptr(),
+                                                work.data_ptr(), b.stream())
+                if rc == -1:
+                    raise ValueError(C.lib().pb3d_last_error().decode())
+                C.check(rc)
+            cfg = tune.choose('pair_gemm', (nm, nvox, na, npair, prec), self._pair_gemm_shapes(nm, na), launch,
+                              b.device, default=(0, 0))
+            launch(cfg)
             out[sel_d] = T[rows_d, cols_d]
         return out.cpu().numpy()
+
+    @staticmethod
+    def _pair_gemm_shapes(nm, na):
+        """Candidate (LDS stage, A rows per launch) shapes of pb3d_pair_gemm_cfg for a stack of nm maps
+        and na A rows (0 = the built-in choice)."""
+        long_stage = 128 if nm <= 32 else (64 if nm <= 128 else 32)
+        stages = sorted({0, 32, long_stage})
+        rows = [0] + [r for r in (32, 64, 96) if r < na]
+        return [(st, r) for st in stages for r in rows]
 
     def _triplet_vec(self, stack, i1, i2, i3):
         b = self.base
@@ -567,59 +586,85 @@ class BSpec:
         triplets of one multipole: the map is assembled from the packed FT rows inside the C2R prologue
         (kop BKQ) and masked / re-transformed by the fused pointing chain; for local l>0 the external-leg
         Legendre factor is the real-space sum_m Y_lm(x) IFFT[Theta_b3 Y_lm(k) FT_0(b1,b2)]
-        (apply_real_harmonics, bspec.py:629-636), added as 2l+1 weighted sub-maps."""
-        b = self.base
-        pk = self._packed_layout()
-        M, Nq = pk['M'], self.Nk_squeeze
+        (apply_real_harmonics, bspec.py:629-636), added as 2l+1 weighted sub-maps.  The number of maps
+        per chain comes from the autotuner (knob bk_qmap_chunk), by default as many as fit 12 GiB of
+        work maps, at most 64."""
         ab = self.all_bins[self._qperm]  # rows of QA follow this order (sorted by l, then b3)
-        local = b.sightline == 'local'
-        pack = dict(pack_index=pk['pack_index'], packed_len=M, support=self._sup)
         t0 = 0
         while t0 < self.N_bins:
             ell = int(ab[t0, 3])
-            ylm = local and ell > 0
-            G = 2 * ell + 2 if ylm else 1
-            # chunk of Q maps per fused chain: its work maps (compact, z planes kz <= support) within
-            # 12 GiB; larger chunks amortise the pass tails (measured: 60 maps 8 % faster than 15)
-            per_map = int(b.gridsize[0]) * int(b.gridsize[1]) * min(b.half_shape[2], self._sup[2] + 1)
-            CH = int(max(1, min(64, (12 << 30) // ((G + 1) * 8 * per_map))))
+            CH = self._qmap_chunk(FT, pair_tab, QA, ell)
             t1 = t0
             while t1 < self.N_bins and t1 - t0 < CH and ab[t1, 3] == ell:
                 t1 += 1
-            nb = t1 - t0
-            # before the mask, the Q maps of triplets (b1 <= b2 <= b3) live on modes below edge[b3+1]:
-            # the inverse passes are pruned to the chunk's largest b3 (the forward passes keep k_max)
-            sup_in = b.support(self.k_bins_squeeze[int(ab[t0:t1, 2].max()) + 1])
-            common = dict(bin_table=self.table, pack_index=pk['pack_index'], packed_len=M, trip=self._trip_dev,
-                          pair_index=pair_tab, ft=FT, ft_lstride=FT.shape[1] * M, nk=Nq, support=sup_in)
-            terms = 2 if ylm else 3
-            if self.const_mask:
-                # P = mask_mean: the Theta terms need no real-space step (bspec.py:662-664)
-                C.check(C.lib().pb3d_kmap_packed(C.ctypes.byref(b.cgrid), None, pk['modes'].data_ptr(), M,
-                                                 QA[t0:t1].data_ptr(), nb,
-                                                 C.ctypes.byref(C.kop(C.KOP_BKQ, terms=terms, trip0=t0,
-                                                                      leg3_legendre=int(not local),
-                                                                      scale=self.mask_mean, **common)),
-                                                 b.stream()))
-                if ylm:
-                    subs = [(C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common), None, ell, m, 1.0)
-                            for m in range(2 * ell + 1)]
-                    b.pointing(subs, nb, QA[t0:t1], mask=None, scale=self.mask_mean,
-                               kop_out=C.kop(C.KOP_PACK, accumulate=1, **pack))
-            else:
-                subs = [(C.kop(C.KOP_BKQ, terms=terms, trip0=t0, leg3_legendre=int(not local), **common), None,
-                         0, 0, 1.0)]
-                if ylm:
-                    subs += [(C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common), None, ell, m, 1.0)
-                             for m in range(2 * ell + 1)]
-                # S^-1 P on every Q map (bspec.py:662-666): P of the varying mask, with its integral
-                # constraint (the constant-per-bin term subtracted in packed mode space)
-                packed = lambda wz: b.r2c(wz, batch=wz.shape[0], kop=C.kop(C.KOP_PACK, **pack),  # noqa: E731
-                                          out=torch.empty((wz.shape[0], M), dtype=torch.complex64, device=b.device))
-                mask_eff, ic, finish = self._P.chain(False, nb, packed, 'packed')
-                b.pointing(subs, nb, QA[t0:t1], mask=mask_eff, scale=1.0, kop_out=C.kop(C.KOP_PACK, **pack), ic=ic)
-                finish(QA[t0:t1])
+            self._q_chunk(FT, pair_tab, QA, t0, t1)
             t0 = t1
+
+    def _qmap_chunk(self, FT, pair_tab, QA, ell):
+        """Q maps per fused chain for multipole ell: the tuned choice (timed per map on the first maps
+        of the multipole, which the caller then recomputes) or the default."""
+        b = self.base
+        local = b.sightline == 'local'
+        ylm = local and ell > 0
+        G = 2 * ell + 2 if ylm else 1
+        per_map = int(b.gridsize[0]) * int(b.gridsize[1]) * min(b.half_shape[2], self._sup[2] + 1)
+        fit = lambda budget: int(max(1, (budget << 30) // ((G + 1) * 8 * per_map)))  # noqa: E731
+        default = int(max(1, min(64, fit(12))))
+        ab = self.all_bins[self._qperm]
+        first = int(np.nonzero(ab[:, 3] == ell)[0][0])
+        nl = int(np.sum(ab[:, 3] == ell))
+        cands = sorted({c for c in (16, 32, 64, 128, 256) if c <= min(nl, fit(48))} | {default})
+        if len(cands) == 1:
+            return default
+        shape = (tuple(int(v) for v in b.gridsize), self.N_bins, int(self._packed_layout()['M']), ell, G,
+                 bool(self.const_mask), getattr(self, '_P', None) is not None and self._P.kind is not None)
+        return tune.choose('bk_qmap_chunk', shape, cands,
+                           lambda c: self._q_chunk(FT, pair_tab, QA, first, first + min(c, nl)), b.device,
+                           default=default, work=lambda c: min(c, nl))
+
+    def _q_chunk(self, FT, pair_tab, QA, t0, t1):
+        """The QA rows t0 .. t1 - 1 (one multipole) through one fused chain."""
+        b = self.base
+        pk = self._packed_layout()
+        M, Nq = pk['M'], self.Nk_squeeze
+        ab = self.all_bins[self._qperm]
+        local = b.sightline == 'local'
+        pack = dict(pack_index=pk['pack_index'], packed_len=M, support=self._sup)
+        ell = int(ab[t0, 3])
+        ylm = local and ell > 0
+        nb = t1 - t0
+        # before the mask, the Q maps of triplets (b1 <= b2 <= b3) live on modes below edge[b3+1]:
+        # the inverse passes are pruned to the chunk's largest b3 (the forward passes keep k_max)
+        sup_in = b.support(self.k_bins_squeeze[int(ab[t0:t1, 2].max()) + 1])
+        common = dict(bin_table=self.table, pack_index=pk['pack_index'], packed_len=M, trip=self._trip_dev,
+                      pair_index=pair_tab, ft=FT, ft_lstride=FT.shape[1] * M, nk=Nq, support=sup_in)
+        terms = 2 if ylm else 3
+        if self.const_mask:
+            # P = mask_mean: the Theta terms need no real-space step (bspec.py:662-664)
+            C.check(C.lib().pb3d_kmap_packed(C.ctypes.byref(b.cgrid), None, pk['modes'].data_ptr(), M,
+                                             QA[t0:t1].data_ptr(), nb,
+                                             C.ctypes.byref(C.kop(C.KOP_BKQ, terms=terms, trip0=t0,
+                                                                  leg3_legendre=int(not local),
+                                                                  scale=self.mask_mean, **common)),
+                                             b.stream()))
+            if ylm:
+                subs = [(C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common), None, ell, m, 1.0)
+                        for m in range(2 * ell + 1)]
+                b.pointing(subs, nb, QA[t0:t1], mask=None, scale=self.mask_mean,
+                           kop_out=C.kop(C.KOP_PACK, accumulate=1, **pack))
+            return
+        subs = [(C.kop(C.KOP_BKQ, terms=terms, trip0=t0, leg3_legendre=int(not local), **common), None,
+                 0, 0, 1.0)]
+        if ylm:
+            subs += [(C.kop(C.KOP_BKQ, terms=4, trip0=t0, ell=ell, m=m, **common), None, ell, m, 1.0)
+                     for m in range(2 * ell + 1)]
+        # S^-1 P on every Q map (bspec.py:662-666): P of the varying mask, with its integral
+        # constraint (the constant-per-bin term subtracted in packed mode space)
+        packed = lambda wz: b.r2c(wz, batch=wz.shape[0], kop=C.kop(C.KOP_PACK, **pack),  # noqa: E731
+                                  out=torch.empty((wz.shape[0], M), dtype=torch.complex64, device=b.device))
+        mask_eff, ic, finish = self._P.chain(False, nb, packed, 'packed')
+        b.pointing(subs, nb, QA[t0:t1], mask=mask_eff, scale=1.0, kop_out=C.kop(C.KOP_PACK, **pack), ic=ic)
+        finish(QA[t0:t1])
 
     # ------------------------------------------------------------------ shot noise
     def _bshot(self, gA, gB, gC):

@@ -348,8 +348,9 @@ extern "C" size_t pb3d_pair_gemm_work_size(int64_t nvox, int na, int np) {
   return (size_t)pg_chunks(nvox, ngroups) * na * np * sizeof(double);
 }
 
-extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int32_t* amap, int na, const int32_t* pb,
-                              const int32_t* pc, int np, int precision, double* out, void* work, void* stream) {
+extern "C" int pb3d_pair_gemm_cfg(const float* g, int nmaps, int64_t nvox, const int32_t* amap, int na, const int32_t* pb,
+                                  const int32_t* pc, int np, int precision, int stage_cfg, int rows_cfg, double* out,
+                                  void* work, void* stream) {
   if (!g || !amap || !pb || !pc || !out || !work) return fail(PB3D_ERR_ARG, "pair_gemm: null pointer");
   if (nmaps < 1 || nmaps > kPgMaxMaps) return fail(PB3D_ERR_ARG, "pair_gemm: nmaps out of [1, 256]");
   if (nvox < 1 || na < 1 || na > kPgMaxM || np < 1) return fail(PB3D_ERR_ARG, "pair_gemm: na out of [1, 128] or empty input");
@@ -365,12 +366,18 @@ extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int
   // More than 96 A rows run as launches of 64 rows (2 M-tiles each): a 4-M-tile wave holds 128
   // accumulator registers and runs at one wave per SIMD, and two 2-M-tile launches, which form the
   // pair products twice, took 20 % less time (128 rows x 2080 pairs at 256^3).
-  const int rows_per_launch = na > 3 * 32 ? 2 * 32 : na;
+  const int long_stage = pfb == 0 ? 128 : pfb <= 3 ? 64 : 32;
+  if (stage_cfg != 0 && stage_cfg != 32 && stage_cfg != long_stage)
+    return fail(PB3D_ERR_ARG, "pair_gemm: stage must be 0 (auto), 32 or " + std::to_string(long_stage) +
+                                  " for a stack of this size");
+  if (rows_cfg != 0 && rows_cfg != 32 && rows_cfg != 64 && rows_cfg != 96)
+    return fail(PB3D_ERR_ARG, "pair_gemm: rows per launch must be 0 (auto), 32, 64 or 96");
+  const int rows_per_launch = rows_cfg ? rows_cfg : (na > 3 * 32 ? 2 * 32 : na);
   for (int r0 = 0; r0 < na; r0 += rows_per_launch) {
     const int nar = na - r0 < rows_per_launch ? na - r0 : rows_per_launch;
     const int mt = pg_mtiles(nar);  // 1..3
-    int stage = pfb == 0 ? 128 : pfb <= 3 ? 64 : 32;
-    if (pg_fwd_lds(nmaps, nparts, mt, stage, sap) > kPgLdsBudget) stage = 32;
+    int stage = stage_cfg ? stage_cfg : long_stage;
+    if (!stage_cfg && pg_fwd_lds(nmaps, nparts, mt, stage, sap) > kPgLdsBudget) stage = 32;
     const size_t lds = pg_fwd_lds(nmaps, nparts, mt, stage, sap);
     if (lds > kPgLdsBudget) return fail(PB3D_ERR_ARG, "pair_gemm: stack does not fit in LDS");
     long long chunk = (nvox + nc - 1) / nc;
@@ -406,6 +413,11 @@ extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int
     PB3D_CHECK_LAUNCH();
   }
   return 0;
+}
+
+extern "C" int pb3d_pair_gemm(const float* g, int nmaps, int64_t nvox, const int32_t* amap, int na, const int32_t* pb,
+                              const int32_t* pc, int np, int precision, double* out, void* work, void* stream) {
+  return pb3d_pair_gemm_cfg(g, nmaps, nvox, amap, na, pb, pc, np, precision, 0, 0, out, work, stream);
 }
 
 extern "C" size_t pb3d_pair_adjoint_work_size(int na, int np, int precision) {

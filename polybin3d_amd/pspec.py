@@ -15,6 +15,7 @@ import scipy.interpolate
 import torch
 
 from . import _capi as C
+from . import tune
 from .base import harmonics_tags
 from .constraints import Pointing, covariance_filter
 
@@ -645,11 +646,28 @@ class PSpec:
             r = red[:, t, :] * scaling
             out[:, blk * Nk_th:(blk + 1) * Nk_th] += r * wt if wt is not None else r
 
+    def seeds_per_pass(self):
+        """Seeds per batched MC Fisher pass from the autotuner (knob pk_batch: 1, 2, 4 or 8, timed per
+        seed), or 1 when nothing is cached and tuning is off."""
+        b = self.base
+        if self.user_Sinv:
+            return 1
+        shape = (tuple(int(v) for v in b.gridsize), self.Nk, self.lmax, bool(self.odd_l), b.sightline,
+                 bool(self.const_mask), getattr(self, '_P', None) is not None and self._P.kind is not None)
+
+        def run(R):
+            f = self._fisher_contributions_dev(list(range(10 ** 6, 10 ** 6 + R))) if R > 1 else \
+                self._fisher_contribution_dev(10 ** 6)
+            del f
+        return tune.choose('pk_batch', shape, [1, 2, 4, 8], run, b.device, default=1, work=lambda R: R)
+
     def compute_fisher(self, N_it, verb=False, batch=1):
         """Mean of N_it realizations, accumulated in FP64 on the GPU, then inverted (pspec.py:553-569).
         batch > 1 (no counterpart in the reference) processes that many seeds per pass (same result up
-        to FP64 summation order)."""
+        to FP64 summation order); batch='auto' takes the autotuner's choice (seeds_per_pass)."""
         fish = None
+        if isinstance(batch, str) and batch == 'auto':
+            batch = self.seeds_per_pass()
         batch = max(1, int(batch))
         for s0 in range(0, N_it, batch):
             if verb and (batch > 1 or s0 % 5 == 0):
