@@ -952,18 +952,32 @@ class BSpec:
             f = self._fisher_contribution_dev(seed)
             fish = f / N_it if fish is None else fish + f / N_it
         self.fish = fish.cpu().numpy()
-        self.inv_fish = torch.linalg.inv(fish).cpu().numpy()
+        self._inv_fish_dev = torch.linalg.inv(fish)
+        self.inv_fish = self._inv_fish_dev.cpu().numpy()
+        self._inv_fish_src = self.inv_fish
         return self.fish
 
     # ------------------------------------------------------------------ unwindowed
     def Bk_unwindowed(self, data, fish=[], include_linear_term=True, verb=False):
-        """bspec.py:863-893."""
+        """bspec.py:863-893 (a given Fisher matrix is inverted in FP64 on the GPU, and the inverse is
+        applied there)."""
+        b = self.base
         if len(fish) != 0:
             self.fish = np.asarray(fish)
-            self.inv_fish = np.linalg.inv(self.fish)
+            Fd = torch.as_tensor(np.ascontiguousarray(self.fish, dtype=np.float64), device=b.device)
+            self._inv_fish_dev = torch.linalg.inv(Fd)
+            del Fd
+            self.inv_fish = self._inv_fish_dev.cpu().numpy()
+            self._inv_fish_src = self.inv_fish
         if not hasattr(self, 'inv_fish'):
             raise Exception("Need to compute Fisher matrix first!")
-        out = np.matmul(self.inv_fish, self.Bk_numerator(data, include_linear_term=include_linear_term, verb=verb))
+        num = self.Bk_numerator(data, include_linear_term=include_linear_term, verb=verb)
+        # the device copy of inv_fish, re-uploaded if inv_fish was replaced since
+        inv = getattr(self, '_inv_fish_dev', None)
+        if inv is None or getattr(self, '_inv_fish_src', None) is not self.inv_fish:
+            inv = torch.as_tensor(np.ascontiguousarray(self.inv_fish, dtype=np.float64), device=b.device)
+            self._inv_fish_dev, self._inv_fish_src = inv, self.inv_fish
+        out = (inv @ torch.as_tensor(num, dtype=torch.float64, device=b.device)).cpu().numpy()
         res, idx = {}, 0
         for l in range(0, self.lmax + 1, 2):
             res['b%s' % l] = out[idx:idx + self.N3]

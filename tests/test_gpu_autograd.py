@@ -147,3 +147,58 @@ def test_chi2_gradient_512_directional(hip):
         err = abs(dirder - rich) / abs(rich)
         msgs.append("%s: <grad, v> %.9e vs finite differences %.9e (rel %.2e)" % (name, dirder, rich, err))
         assert err < 1e-4, "; ".join(msgs)
+
+
+def test_adjoints_equal_the_reference_q_maps(hip):
+    """The hand-written adjoints against the Q-derivative maps the reference forms for its MC Fisher
+    (pspec.py:444-446: Q = (S^-1 P)^dag Theta_b L_l S^-1 P a; bspec.py:615-640: the three-leg
+    placement of add_Q_element), built here with the oracle's filters for the ideal weighting
+    (S^-1 = invPk0, P = 1, a = the data):
+      d n_(l,b) / d d = (V/N) C2R(Q_(l,b)),   d num_i / d d = V/(N sym_i) C2R(invPk0 Q_i).
+    The adjoint of grad . estimator with grad = F^T e_j is d n_j / d d (F the ideal Fisher).  Every
+    power-spectrum bin and a set of bispectrum triangles of both multipoles, 32^3, global sightline;
+    tolerance 1e-5 of the largest entry of the map."""
+    import polybin3d_amd as pb
+    n = 32
+    o = O.PolyBin3D(L, [n] * 3, Pk=powerlaw())
+    d = ng_field(o, 5).astype(np.float32)
+    d64 = d.astype(np.float64)
+    b = pb.PolyBin3D(L, [n] * 3, Pk=powerlaw())
+    V, N = o.volume, float(n ** 3)
+    D = o.to_fourier(d64)
+    # power spectrum
+    edges = np.linspace(0.01, 0.09, 9)
+    ps = pb.PSpec(b, edges, lmax=2)
+    ps.Pk_ideal(d)
+    F = ps.fish_ideal
+    w = o.invPk0_grid
+    Nk = len(edges) - 1
+    worst = 0.
+    for j in range(2 * Nk):
+        ell, kb = 2 * (j // Nk), j % Nk
+        Q = w * o.map_utils.fourier_filter(w * D, ell, edges[kb], edges[kb + 1])
+        ref = V / N * o.to_real(Q)
+        got = ps._pk_ideal_adjoint(d, torch.from_numpy(F[j, :].copy()).cuda()).cpu().numpy()
+        worst = max(worst, np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    print("[adjoint] Pk_ideal vs (V/N) C2R(Q): max error %.2e over %d bins" % (worst, 2 * Nk))
+    assert worst < 1e-5
+    # bispectrum
+    e3 = np.linspace(0.01, 0.07, 5)
+    bs = pb.BSpec(b, e3, lmax=2)
+    bs.Bk_ideal(d)
+    bo = O.BSpec(o, e3, lmax=2)
+    bo._compute_symmetry_factor()
+    g = bo._g_maps(bo.invPk0 * D)
+    Fb = bs.fish_ideal
+    sel = [i for i in range(bs.N_bins) if i % 3 == 0] + [i for i, t in enumerate(bs.all_bins) if t[0] == t[2]]
+    worst = 0.
+    for j in sorted(set(sel)):
+        b1, b2, b3, ell = bs.all_bins[j]
+        Q = bo._filt(o.to_fourier(g[0, b1] * g[0, b2]), ell, b3)
+        Q = Q + bo._filt(o.to_fourier(g[0, b2] * g[ell // 2, b3]), 0, b1)
+        Q = Q + bo._filt(o.to_fourier(g[0, b1] * g[ell // 2, b3]), 0, b2)
+        ref = V / N / bo.sym_factor[j] * o.to_real(bo.invPk0 * Q)
+        got = bs._bk_ideal_adjoint(d, torch.from_numpy(Fb[j, :].copy()).cuda()).cpu().numpy()
+        worst = max(worst, np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    print("[adjoint] Bk_ideal vs V/(N sym) C2R(invPk0 Q): max error %.2e over %d triangles" % (worst, len(set(sel))))
+    assert worst < 1e-5

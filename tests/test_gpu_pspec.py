@@ -504,3 +504,34 @@ def test_pk_unwindowed_shot_subtraction(hip):
     again = ps.Pk_unwindowed(d, subtract_shotnoise=False)
     plain = po.Pk_unwindowed(d, fish=fish)
     parity.check_solve("Pk_unwindowed without subtraction", again, plain, ['p0', 'p2'], fish, fish, n2, n1)
+
+
+@pytest.mark.parametrize("case", ["global_const", "local_mask"])
+def test_pk_shot_contribution_user_sinv(hip, case):
+    """PSpec.compute_shot_contribution with a user S^-1 (pspec.py:514-522): the callable filters
+    A^-1 a times the shot-noise mask (or the mask mean) and a itself; per seed against the oracle."""
+    import polybin3d_amd as pb
+    import torch
+    n = 32
+    edges = np.linspace(0.01, 0.09, 9)
+    kw = {} if case == "global_const" else dict(boxcenter=[10., 10., 10.], sightline='local')
+    mask = 0.7 * np.ones((n, n, n)) if case == "global_const" else shell_mask(n, L, [10., 10., 10.])
+    Pk = [np.linspace(0, 2, 500), 1e4 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    wk = 1. / (1. + (o.modk_grid / 0.03) ** 2)
+    wk_d = torch.from_numpy(wk.astype(np.float32)).cuda()
+
+    def sinv_o(m, input_type='real', output_type='fourier'):
+        f = (o.to_fourier(m) if input_type == 'real' else m) * wk
+        return f if output_type == 'fourier' else o.to_real(f)
+
+    def sinv_g(m, input_type='real', output_type='fourier'):
+        f = (b.to_fourier(m) if input_type == 'real' else m) * wk_d
+        return f if output_type == 'fourier' else b.to_real(f)
+    ps = pb.PSpec(b, edges, lmax=2, mask=mask, mask_shot=0.5 * mask, applySinv=sinv_g)
+    po = O.PSpec(o, edges, lmax=2, mask=mask, mask_shot=0.5 * mask, applySinv=sinv_o)
+    for seed in (0, 3):
+        s1, s2 = ps.compute_shot_contribution(seed), po.compute_shot_contribution(seed)
+        for li in range(ps.Nl):
+            sl = slice(li * ps.Nk, (li + 1) * ps.Nk)
+            close(s1[sl], s2[sl], scale=own_scale(s2[sl]), what="l=%d" % (2 * li))
