@@ -10,7 +10,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpb3d.so")
-SOURCES = ["pb3d_api.hip", "pb3d_fft.hip", "pb3d_maps.hip", "pb3d_bins.hip", "pb3d_triplet.hip", "pb3d_pairgemm.hip", "pb3d_fisher.hip"]
+SOURCES = ["pb3d_api.hip", "pb3d_fft.hip", "pb3d_fftgen.hip", "pb3d_maps.hip", "pb3d_bins.hip", "pb3d_triplet.hip", "pb3d_pairgemm.hip", "pb3d_fisher.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          # packed FP32 (v_pk_*_f32) disabled: hand-checked scalar FP32 only

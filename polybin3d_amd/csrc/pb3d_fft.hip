@@ -7,6 +7,7 @@
 // Replaces the reference's FFT dispatch to_fourier / to_real / _fft_setup
 // (PolyBin3D/base.py:176-229, 256-302); numpy conventions (forward unnormalised, inverse 1/N).
 #include "pb3d_ops.h"
+#include "pb3d_fftgen.h"
 #include <vector>
 #include <cmath>
 #include <atomic>
@@ -1145,6 +1146,11 @@ static void prof_end(bool on, ProfSample& smp, hipStream_t s) {
 }
 
 static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+// power-of-two fast path: n0, n1 in [2, 512], n2 in [4, 512]; other sizes take the mixed-radix passes
+static bool fast_grid(int n0, int n1, int n2) { return pow2_in(n0, 2, 512) && pow2_in(n1, 2, 512) && pow2_in(n2, 4, 512); }
+static bool generic_grid(int n0, int n1, int n2) {
+  return n0 >= 2 && n0 <= 1024 && n1 >= 2 && n1 <= 1024 && n2 >= 4 && n2 <= 2048 && n2 % 2 == 0;
+}
 
 }  // namespace pb3d
 
@@ -1154,6 +1160,8 @@ struct pb3d_plan {
   int L[3];         // n0, n1, n2/2
   float2* tw[3];    // device twiddles exp(-2 pi i m / L), m < L
   float2* twN;      // exp(-2 pi i k / n2), k = 0..n2/2
+  int generic;      // 1: mixed-radix passes (sizes outside the power-of-two fast path)
+  pb3d::GenAxis ax[3];  // radices of n0, n1, n2/2 (generic plans)
 };
 
 namespace pb3d {
@@ -1372,6 +1380,29 @@ static int npos(int rad, int L) { return rad < 0 ? L : (2 * rad + 1 < L ? 2 * ra
 static double col_bytes(const ColMap& cm, int L, int batch) {
   return 8.0 * batch * cm.ncols * (double)(npos(cm.rad_in, L) + npos(cm.rad_out, L));
 }
+// Mixed-radix column maps over the (j, l < lz) columns of the x pass and the (i, l < lz) columns of
+// the y pass; pitch 0 < pitch <= n2h selects the row pitch of the input / output array.
+static GenCols gen_xcols(const Geo& g, int lz, int pitch_in, int pitch_out, int rad_in, int rad_out, int qrad_in,
+                         int qrad_out) {
+  GenCols cm{};
+  cm.ncols = (long long)g.n1 * lz; cm.cpp = lz;
+  cm.plane = g.n2h; cm.S = (long long)g.n1 * g.n2h;
+  cm.in_plane = pitch_in; cm.in_S = (long long)g.n1 * pitch_in;
+  cm.out_plane = pitch_out; cm.out_S = (long long)g.n1 * pitch_out;
+  cm.rad_in = rad_in; cm.rad_out = rad_out; cm.qlen = g.n1; cm.qrad_in = qrad_in; cm.qrad_out = qrad_out;
+  return cm;
+}
+static GenCols gen_ycols(const Geo& g, int lz, int pitch_in, int pitch_out, int rad_in) {
+  GenCols cm{};
+  cm.ncols = (long long)g.n0 * lz; cm.cpp = lz;
+  cm.plane = (long long)g.n1 * g.n2h; cm.S = g.n2h;
+  cm.in_plane = (long long)g.n1 * pitch_in; cm.in_S = pitch_in;
+  cm.out_plane = (long long)g.n1 * pitch_out; cm.out_S = pitch_out;
+  cm.rad_in = rad_in; cm.rad_out = -1; cm.qlen = g.n0; cm.qrad_in = -1; cm.qrad_out = -1;
+  return cm;
+}
+static double gen_col_bytes(const GenCols& cm, int L, int batch) { return 16.0 * batch * (double)cm.ncols * L; }
+
 static int rop_operands(const pb3d_rop* r) {
   if (!r) return 1;
   return 1 + (r->b ? 1 : 0) + (r->c ? 1 : 0) + (r->d ? 1 : 0);
@@ -1459,15 +1490,18 @@ extern "C" int pb3d_plan_create(pb3d_plan** plan, const pb3d_grid* grid) {
   if (!plan || !grid) return fail(PB3D_ERR_ARG, "null plan/grid pointer");
   *plan = nullptr;
   const int n0 = grid->n[0], n1 = grid->n[1], n2 = grid->n[2];
-  if (!pow2_in(n0, 2, 512) || !pow2_in(n1, 2, 512) || !pow2_in(n2, 4, 512))
-    return fail(PB3D_ERR_UNSUPPORTED,
-                "grid sizes must be powers of two: n0,n1 in [2,512], n2 in [4,512]");
+  const bool fast = fast_grid(n0, n1, n2);
+  if (!fast && !generic_grid(n0, n1, n2))
+    return fail(PB3D_ERR_UNSUPPORTED, "grid sizes must be n0, n1 in [2, 1024] and even n2 in [4, 2048]");
   for (int i = 0; i < 3; ++i)
     if (!(grid->boxsize[i] > 0)) return fail(PB3D_ERR_ARG, "boxsize must be positive");
   pb3d_plan* p = new pb3d_plan();
   p->grid = *grid;
   p->geo = make_geo(*grid);
   p->L[0] = n0; p->L[1] = n1; p->L[2] = n2 / 2;
+  p->generic = fast ? 0 : 1;
+  for (int a = 0; a < 3; ++a)
+    if (!gen_factor(p->L[a], p->ax[a])) { pb3d_plan_destroy(p); return fail(PB3D_ERR_UNSUPPORTED, "axis length not factorable"); }
   for (int a = 0; a < 3; ++a) {
     p->tw[a] = upload_twiddles(p->L[a], p->L[a]);
     if (!p->tw[a]) { pb3d_plan_destroy(p); return fail(PB3D_ERR_ALLOC, "twiddle upload failed"); }
@@ -1511,6 +1545,23 @@ extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, voi
   const ColMap cy = ycols(g, pr, false, pitch, pitch), cx = xcols(g, pr, false, pitch, 0);
   const double br = (double)batch * (g.N * 4.0 * rop_operands(rop_) + (double)g.n0 * g.n1 * pr.lz * 8.0);
   ProfSample sm{};
+  const long long out_bs = (kop_ && kop_->kind == 5) ? kop_->packed_len : g.H;
+  if (p->generic) {
+    const int P = pitch ? pitch : g.n2h;
+    bool on = prof_begin(0, s, sm, br);
+    PB3D_CHECK_HIP(gen_r2c_rows(p->ax[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, pr.lz, P, s));
+    prof_end(on, sm, s);
+    const GenCols gy = gen_ycols(g, pr.lz, P, P, -1);
+    on = prof_begin(6, s, sm, gen_col_bytes(gy, g.n1, batch));
+    PB3D_CHECK_HIP(gen_cols(-1, 0, p->ax[1], work, work, gy, wbs, wbs, batch, p->tw[1], g, plain, s));
+    prof_end(on, sm, s);
+    const GenCols gx = gen_xcols(g, pr.lz, P, g.n2h, -1, pr.r0, -1, pr.r1);
+    on = prof_begin(4, s, sm, gen_col_bytes(gx, g.n0, batch));
+    PB3D_CHECK_HIP(gen_cols(-1, 2, p->ax[0], work, out, gx, wbs, out_bs, batch, p->tw[0], g, kop, s));
+    prof_end(on, sm, s);
+    g_fwd += batch;
+    return 0;
+  }
   bool on = prof_begin(0, s, sm, br);
   PB3D_CHECK_HIP(launch_r2c_rows(p->L[2], in, work, rows, g.n0 * g.n1, p->tw[2], p->twN, g, rop, pr.lz,
                                  pitch ? pitch : g.n2h, s));
@@ -1518,7 +1569,6 @@ extern "C" int pb3d_fft_r2c(const pb3d_plan* p, const float* in, void* out_, voi
   on = prof_begin(6, s, sm, col_bytes(cy, p->L[1], batch));
   PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], work, work, cy, wbs, wbs, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
-  const long long out_bs = (kop_ && kop_->kind == 5) ? kop_->packed_len : g.H;
   on = prof_begin(4, s, sm, col_bytes(cx, p->L[0], batch));
   PB3D_CHECK_HIP((launch_cols<-1, 2>(p->L[0], work, out, cx, wbs, out_bs, batch, p->tw[0], g, kop, s)));
   prof_end(on, sm, s);
@@ -1550,6 +1600,22 @@ extern "C" int pb3d_fft_c2r(const pb3d_plan* p, const void* in_, float* out, voi
   const double br = (double)batch * ((double)g.n0 * g.n1 * pr.lz * 8.0 +
                                      g.N * 4.0 * (rop_operands(rop_) + (rop_ && rop_->accumulate ? 1 : 0)));
   ProfSample sm{};
+  if (p->generic) {
+    const GenCols gx = gen_xcols(g, pr.lz, g.n2h, pitch, pr.r0, -1, pr.r1, -1);
+    bool on = prof_begin(3, s, sm, gen_col_bytes(gx, g.n0, batch));
+    PB3D_CHECK_HIP(gen_cols(+1, 1, p->ax[0], in, work, gx, in_bs, wbs, batch, p->tw[0], g, kop, s));
+    prof_end(on, sm, s);
+    const GenCols gy = gen_ycols(g, pr.lz, pitch, pitch, pr.r1);
+    on = prof_begin(2, s, sm, gen_col_bytes(gy, g.n1, batch));
+    PB3D_CHECK_HIP(gen_cols(+1, 0, p->ax[1], work, work, gy, wbs, wbs, batch, p->tw[1], g, plain, s));
+    prof_end(on, sm, s);
+    on = prof_begin(1, s, sm, br);
+    PB3D_CHECK_HIP(gen_c2r_rows(p->ax[2], work, out, (long long)batch * g.n0 * g.n1, g.n0 * g.n1, p->tw[2], p->twN,
+                                g, rop, (float)(1.0 / (double)g.N), pr.lz, pitch, s));
+    prof_end(on, sm, s);
+    g_rev += batch;
+    return 0;
+  }
   bool on = prof_begin(3, s, sm, col_bytes(cx, p->L[0], batch));
   PB3D_CHECK_HIP((launch_cols<+1, 1>(p->L[0], in, work, cx, in_bs, wbs, batch, p->tw[0], g, kop, s)));
   prof_end(on, sm, s);
@@ -1583,6 +1649,11 @@ extern "C" int64_t pb3d_icsum_partial_size(const pb3d_grid* grid, int batch, int
     return -1;
   }
   const Geo g = make_geo(*grid);
+  const long long rpm = (long long)g.n0 * g.n1;
+  if (!fast_grid(g.n0, g.n1, g.n2)) {  // mixed-radix pointing rows: one partial per block of rows
+    const long long rb = gen_rows_per_block(g.n2 / 2);
+    return (int64_t)batch * ((rpm + rb - 1) / rb) * nbins;
+  }
   long long rw = 64;
   switch (g.n2 / 2) {
     case 2: rw = PointCfg<2>::RW; break;
@@ -1595,7 +1666,6 @@ extern "C" int64_t pb3d_icsum_partial_size(const pb3d_grid* grid, int batch, int
     case 256: rw = PointCfg<256>::RW; break;
     default: rw = 1; break;
   }
-  const long long rpm = (long long)g.n0 * g.n1;
   return (int64_t)batch * ((rpm + rw - 1) / rw) * nbins;
 }
 
@@ -1646,6 +1716,44 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   const int pzi = compact_pitch(g, pin.lz), pzo = compact_pitch(g, pout.lz);
   const long long wsi = (long long)g.n0 * g.n1 * pzi, wso = (long long)g.n0 * g.n1 * pzo;
   float2* outw = work + (long long)nsub * batch * wsi;
+  if (p->generic) {
+    GenSubs gs{};
+    gs.n = nsub;
+    for (int k = 0; k < nsub; ++k) {
+      gs.ell[k] = subs[order[k]].ell; gs.m[k] = subs[order[k]].m; gs.coef[k] = subs[order[k]].coef;
+    }
+    for (int k = 0; k < nsub; ++k) {
+      const pb3d_submap& sb = subs[order[k]];
+      const GenCols gx = gen_xcols(g, pin.lz, g.n2h, pzi, pin.r0, -1, pin.r1, -1);
+      bool on = prof_begin(3, s, sm, gen_col_bytes(gx, g.n0, batch));
+      PB3D_CHECK_HIP(gen_cols(+1, 1, p->ax[0], (const float2*)sb.in, work + (long long)k * batch * wsi, gx,
+                              sb.kop.x_batch_stride, wsi, batch, p->tw[0], g, to_kop(&sb.kop), s));
+      prof_end(on, sm, s);
+    }
+    const GenCols gyi = gen_ycols(g, pin.lz, pzi, pzi, pin.r1);
+    bool on = prof_begin(2, s, sm, gen_col_bytes(gyi, g.n1, batch * nsub));
+    PB3D_CHECK_HIP(gen_cols(+1, 0, p->ax[1], work, work, gyi, wsi, wsi, batch * nsub, p->tw[1], g, plain, s));
+    prof_end(on, sm, s);
+    const double rows1 = (double)g.n0 * g.n1;
+    on = prof_begin(5, s, sm, (double)batch * (rows1 * 8.0 * (pin.lz * (double)nsub + pout.lz) + (mask ? g.N * 4.0 : 0.0)));
+    const GenIc gic{ic.weight, ic.bin, ic.nbins, ic.sums, ic.partial};
+    PB3D_CHECK_HIP(gen_pointing_rows(p->ax[2], work, wsi, pzi, outw, wso, pzo, (long long)g.n0 * g.n1, batch, gs, gic,
+                                     mask, scale, pin.lz, pout.lz, p->tw[2], p->twN, g, s));
+    prof_end(on, sm, s);
+    const GenCols gyo = gen_ycols(g, pout.lz, pzo, pzo, -1);
+    on = prof_begin(6, s, sm, gen_col_bytes(gyo, g.n1, batch));
+    PB3D_CHECK_HIP(gen_cols(-1, 0, p->ax[1], outw, outw, gyo, wso, wso, batch, p->tw[1], g, plain, s));
+    prof_end(on, sm, s);
+    const GenCols gxo = gen_xcols(g, pout.lz, pzo, g.n2h, -1, pout.r0, -1, pout.r1);
+    const long long out_bs = (kop_out && kop_out->kind == 5) ? kop_out->packed_len : g.H;
+    on = prof_begin(4, s, sm, gen_col_bytes(gxo, g.n0, batch));
+    PB3D_CHECK_HIP(gen_cols(-1, 2, p->ax[0], outw, (float2*)out_, gxo, wso, out_bs, batch, p->tw[0], g, to_kop(kop_out),
+                            s));
+    prof_end(on, sm, s);
+    g_rev += (long long)batch * nsub;
+    g_fwd += batch;
+    return 0;
+  }
   // 1. pruned x-column C2R pass of every sub-map into work[k*batch + b], over the common support
   //    (so that the y pass reads no entry that was not written)
   for (int k = 0; k < nsub; ++k) {

@@ -39,9 +39,12 @@ def test_struct_layouts_match_header():
 def test_host_side_argument_validation():
     lib = C.lib()
     p = ctypes.c_void_p()
-    for n in ([3, 4, 4], [4, 4, 6], [4, 4, 1024], [1, 4, 4]):
+    # sizes neither the power-of-two passes nor the mixed-radix passes take: odd n2, an axis of 1,
+    # n0 / n1 above 1024, n2 above 2048 or below 4
+    for n in ([4, 4, 5], [1, 4, 4], [4, 1, 4], [4, 4, 2], [1025, 4, 4], [4, 2000, 4], [4, 4, 2050]):
         g = C.make_grid(n, [1, 1, 1], [0, 0, 0], [0, 0, 1], 0)
         assert lib.pb3d_plan_create(ctypes.byref(p), ctypes.byref(g)) == -2
+        assert b"n0, n1 in [2, 1024]" in lib.pb3d_last_error()
     g = C.make_grid([8, 8, 8], [0, 1, 1], [0, 0, 0], [0, 0, 1], 0)
     assert lib.pb3d_plan_create(ctypes.byref(p), ctypes.byref(g)) == -1
     assert b"boxsize" in lib.pb3d_last_error()
