@@ -200,6 +200,9 @@ def main():
     ap.add_argument("--bk-kmax", type=float, default=0.17)
     ap.add_argument("--lmax", type=int, default=0)
     ap.add_argument("--precision", default="bf16x6", choices=["fp32", "bf16x3", "bf16x6"])
+    ap.add_argument("--rows-mode", type=int, default=1, choices=[0, 1],
+                    help="fused Q-map rows: 1 = harmonic sub-maps combined per row before the row transforms "
+                         "(default), 0 = one row transform per sub-map (both legs)")
     args = ap.parse_args()
     if args.realizations is None:
         args.realizations = 4 if args.workload == "mc_fisher" else 1
@@ -211,6 +214,7 @@ def main():
     import torch
     import polybin3d_amd as pb
     from polybin3d_amd import _capi as C
+    C.check(min(0, C.lib().pb3d_pointing_rows_mode(args.rows_mode)))
     if args.workload == "bk_ideal":
         return bk_ideal_workload(args, torch, pb)
     if args.workload == "pk_mc_fisher":
@@ -332,6 +336,7 @@ def main():
                       "bk": {"lmax": 2, "nbins": len(bk_bins) - 1, "kmax": float(bk_bins[-1]), "N_bins": bs.N_bins},
                       "mask": "shell 300-480 Mpc/h, 200 holes, 5 Mpc/h smoothing", "rng": "device",
                       "realizations_per_step": R, "pk_seeds_per_pass": R, "hip_graph": bool(args.graph),
+                      "pointing_rows_mode": args.rows_mode,
                       "bk_fisher_contraction": args.precision + " MFMA (split BF16, FP32 runs, FP64 store)",
                       "fisher_accumulation": "fp64", "setup_s": round(t_setup, 1)},
            "vs_baseline_is": "value / plain FP32 single-realization leg measured in this run (the metric's "

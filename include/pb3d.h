@@ -219,6 +219,13 @@ int pb3d_fft_pointing(const pb3d_plan* plan, int batch, int nsub, const pb3d_sub
  * pointing allocates full maps per step.) */
 int64_t pb3d_fft_pointing_work_size(const pb3d_grid* grid, int batch, int nsub, const pb3d_submap* subs,
                                     const pb3d_kop* kop_out);
+/* How the row kernel of pb3d_fft_pointing forms sum_g coef_g Y_g(x_hat) C2R(...)_g (no counterpart in
+ * the reference, which transforms every harmonic map): mode 1 (default) combines the sub-maps of one
+ * row in kz space into Lmax+1 rows when every ell is even, Lmax is 2 or 4 and that is fewer rows than
+ * sub-maps (n2 = 128, 256 or 512), so the row needs Lmax+1 inverse transforms (rows with
+ * x = y = 0 keep the per-sub-map form); mode 0 transforms every sub-map.  Process-wide; returns the
+ * previous mode, or -1 for a mode other than 0 or 1. */
+int pb3d_pointing_rows_mode(int mode);
 /* doubles of pb3d_icsum.partial for a pointing call of `batch` maps and `nbins` bins (host-only; -1 on
  * a bad argument). */
 int64_t pb3d_icsum_partial_size(const pb3d_grid* grid, int batch, int nbins);

@@ -33,6 +33,7 @@ EXPORTS = {
                                          c_void_p, c_void_p, c_void_p, c_void_p]),
     "pb3d_fft_pointing_work_size": (c_int64, [c_void_p, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p]),
     "pb3d_icsum_partial_size": (c_int64, [c_void_p, ctypes.c_int, ctypes.c_int]),
+    "pb3d_pointing_rows_mode": (ctypes.c_int, [ctypes.c_int]),
     "pb3d_kmap": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int, c_void_p, c_void_p]),
     "pb3d_kmap_packed": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, ctypes.c_int, c_void_p,
                                         c_void_p]),

@@ -164,6 +164,14 @@ __device__ __forceinline__ void pencil_fft(int t, Src0 src0, Lds lds, DstN dstN,
 #ifndef PB3D_POINT_WAVES
 #define PB3D_POINT_WAVES 4
 #endif
+// the same for the combined-row instances, which hold the Lmax+1 combined rows next to the
+// accumulators: 3 waves per SIMD (168 VGPRs) for Lmax = 2, 2 for Lmax = 4 (no spills)
+#ifndef PB3D_POINT_WAVES_COMB
+#define PB3D_POINT_WAVES_COMB 3
+#endif
+#ifndef PB3D_POINT_WAVES_COMB5
+#define PB3D_POINT_WAVES_COMB5 2
+#endif
 // Last Stockham stage with its outputs delivered to registers: for the last stage NB = L/R = NS,
 // so butterfly j = t + b*P < NS writes positions j + k*NS = t + P*(b + k*NS/P); the sink gets the
 // compile-time slot e = b + k*NS/P (slot e of thread t holds position t + e*P).
@@ -831,12 +839,17 @@ constexpr int kMaxMono = 15;  // monomials of a homogeneous polynomial of degree
 // that order); y0 = Y(0) at r = 0, where the reference's zero-vector convention applies.
 // The first nfact sub-maps are the ell = 2 ones: their weights are accumulated without the common
 // 1/|r|^2, which multiplies the sum once before the remaining sub-maps are added.
+// Combined form (nu > 0, every ell even, Lmax = nu - 1): w_g = cpoly_g(r) / |r|^Lmax with cpoly_g the
+// sub-map's polynomial times coef * |r|^(Lmax - ell) (degree Lmax, same index order).  Along a row
+// (rx, ry fixed) sum_g w_g z_g = |r|^-Lmax sum_u rz^u IFFT_z(sum_g D_ug S_g) with row constants D_ug,
+// so the rows need nu inverse transforms instead of n.
 struct SubW {
-  int n, nfact;
+  int n, nfact, nu;
   int ell[kMaxSub];
   float coef[kMaxSub];
   float y0[kMaxSub];
   float poly[kMaxSub][kMaxMono];
+  float cpoly[kMaxSub][kMaxMono];
 };
 
 // Per-bin weighted sums of the combined real map (pb3d_icsum), by value for the kernel: each wave
@@ -864,8 +877,30 @@ template <int M> struct PointCfg {
   static_assert(64 % P == 0, "lanes per row must divide the wave");
 };
 
-template <int M, bool IC>
-__global__ void __launch_bounds__(PointCfg<M>::NT, PB3D_POINT_WAVES)
+// Band-limited C2R pre-pass (lz_in <= M/2, so X[k] = 0 for k >= M/2) from registers: slot it of
+// lane (row r, tp) holds X[k], k = tp + P*it; writes Z'[k], Z'[M-k] of row r and Z'[M/2] = 0.
+template <int M, int P, int IV>
+__device__ __forceinline__ void prepass_regs(float2* buf, int LDT, int r, int tp, const float2 (&x)[IV],
+                                             const float2* twN) {
+#pragma unroll
+  for (int it = 0; it < IV; ++it) {
+    const int k = tp + P * it;
+    float2 a = x[it];
+    if (it == 0 && tp == 0) a.y = 0.f;  // Im of the DC entry is ignored
+    const float2 w = twN[k];
+    const float p = 1.f + w.y;
+    buf[k * LDT + r] = make_float2(fmaf(a.x, p, -a.y * w.x), fmaf(a.x, w.x, a.y * p));
+    if (!(it == 0 && tp == 0)) {
+      const float2 w2 = twN[M - k];
+      const float q = 1.f - w2.y;
+      buf[(M - k) * LDT + r] = make_float2(fmaf(a.x, q, -a.y * w2.x), -fmaf(a.x, w2.x, a.y * q));
+    }
+  }
+  if (tp == 0) buf[(M / 2) * LDT + r] = make_float2(0.f, 0.f);
+}
+
+template <int M, bool IC, int NU>
+__global__ void __launch_bounds__(PointCfg<M>::NT, NU == 0 ? PB3D_POINT_WAVES : NU == 3 ? PB3D_POINT_WAVES_COMB : PB3D_POINT_WAVES_COMB5)
 pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, long long slot_out, int pitch_out,
               long long rows_per_map, int batch, SubW sw, IcSum ic,
               const float* __restrict__ mask, float scale, int lz_in, int lz_out, const float2* __restrict__ tw,
@@ -908,7 +943,109 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
       acc[e][1] *= __builtin_amdgcn_rcpf(fmaxf(fmaf(rz1, rz1, rxy2), 1e-30f));
     }
   };
-  for (int g = 0; g < sw.n; ++g) {
+  bool combined = false;
+  if constexpr (NU > 0) {
+    // Combined rows: V_u = sum_g D_ug S_g in kz space (row constants D_ug = sum_a cpoly rx^a ry^b),
+    // then acc = IFFT(V_{NU-1}) rz^(NU-1) + ... + IFFT(V_0) by Horner, divided by |r|^(NU-1).  Waves
+    // holding a row with rx = ry = 0 (where r = 0 can occur) take the per-sub-map path.
+    constexpr int IV = (M / 2) / P;  // kz slots per lane: k = tp + P*it < M/2
+    constexpr int LM = NU - 1;
+    if (sw.nu == NU && lz_in >= 1 && lz_in <= M / 2 && __ballot(rxy2 < 1e-20f) == 0ull) {
+      combined = true;
+      const bool rv = row < rows_per_map;
+      // monomials rx^a ry^b, a + b = LM - u, in the index order of cpoly
+      float mono[(LM + 1) * (LM + 2) / 2];
+      {
+        float px[LM + 1], py[LM + 1];
+        px[0] = 1.f; py[0] = 1.f;
+#pragma unroll
+        for (int q = 1; q <= LM; ++q) { px[q] = px[q - 1] * rx; py[q] = py[q - 1] * ry; }
+        int idx = 0;
+#pragma unroll
+        for (int u = 0; u <= LM; ++u)
+#pragma unroll
+          for (int a = 0; a <= LM - u; ++a) mono[idx++] = px[a] * py[LM - u - a];
+      }
+      float2 V[NU][IV];
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int it = 0; it < IV; ++it) V[u][it] = make_float2(0.f, 0.f);
+      const int kl = lz_in - 1;
+      // every row of the wave present and lz_in = M/2: no clamped addresses or zero selects
+      const bool whole = lz_in == M / 2 && __ballot(!rv) == 0ull;
+      auto load = [&](int g, float2 (&s)[IV]) {
+        const float2* rowp = work + ((long long)g * batch + t) * slot_in + (row0 + (rv ? r : 0)) * pitch_in;
+        if (whole) {
+#pragma unroll
+          for (int it = 0; it < IV; ++it) s[it] = rowp[tp + P * it];
+          return;
+        }
+#pragma unroll
+        for (int it = 0; it < IV; ++it) {
+          const int k = tp + P * it;
+          s[it] = rowp[k < lz_in ? k : kl];
+        }
+#pragma unroll
+        for (int it = 0; it < IV; ++it) {
+          const int k = tp + P * it;
+          if (!(rv && k < lz_in)) s[it] = make_float2(0.f, 0.f);
+        }
+      };
+      for (int g = 0; g < sw.n; ++g) {
+        float2 s[IV];
+        load(g, s);
+        const float* pc = sw.cpoly[g];
+        int idx = 0;
+#pragma unroll
+        for (int u = 0; u <= LM; ++u) {
+          float d = 0.f;
+#pragma unroll
+          for (int a = 0; a <= LM - u; ++a, ++idx) d = fmaf(pc[idx], mono[idx], d);
+#pragma unroll
+          for (int it = 0; it < IV; ++it) {
+            V[u][it].x = fmaf(d, s[it].x, V[u][it].x);
+            V[u][it].y = fmaf(d, s[it].y, V[u][it].y);
+          }
+        }
+      }
+      const float rzb_neg = rzb - (float)geo.n2 * geo.dx2;
+      auto rz_of = [&](int e) { return fmaf((float)(2 * e * P), geo.dx2, 2 * e >= E ? rzb_neg : rzb); };
+#pragma unroll
+      for (int u = LM; u >= 0; --u) {
+        // opaque copies of the lane indices per transform: the twiddle reads derived from them are
+        // repeated per transform instead of being hoisted and held in registers across all of them
+        int tp_u = tp, r_u = r;
+        asm volatile("" : "+v"(tp_u), "+v"(r_u));
+        const RowT rw_u{buf, LDT, r_u};
+        wave_lds_sync();  // the previous transform's reads of the tile are issued
+        prepass_regs<M, P, IV>(buf, LDT, r_u, tp_u, V[u], stwN);
+        wave_lds_sync();
+        pencil_inverse_head<M, true>(tp_u, rw_u, stw);
+        if (u == LM) {
+          auto sink = [&](int e, int, float2 z) { acc[e][0] = z.x; acc[e][1] = z.y; };
+          pencil_inverse_tail<M>(tp_u, rw_u, sink, stw);
+        } else {
+          auto sink = [&](int e, int, float2 z) {
+            const float rz0 = rz_of(e);
+            acc[e][0] = fmaf(acc[e][0], rz0, z.x);
+            acc[e][1] = fmaf(acc[e][1], rz0 + geo.dx2, z.y);
+          };
+          pencil_inverse_tail<M>(tp_u, rw_u, sink, stw);
+        }
+      }
+      // divide by |r|^LM (r > 0 on every row of this wave)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float rz0 = rz_of(e), rz1 = rz0 + geo.dx2;
+        float i0 = __builtin_amdgcn_rcpf(fmaf(rz0, rz0, rxy2)), i1 = __builtin_amdgcn_rcpf(fmaf(rz1, rz1, rxy2));
+        if (LM == 4) { i0 *= i0; i1 *= i1; }
+        acc[e][0] *= i0;
+        acc[e][1] *= i1;
+      }
+    }
+  }
+  for (int g = 0; g < (combined ? 0 : sw.n); ++g) {
     const float2* src = work + ((long long)g * batch + t) * slot_in + row0 * pitch_in;
     // Opaque copies of the lane indices: index arithmetic derived from them is recomputed per
     // sub-map instead of being hoisted out of the loop and held in (spilled) registers.
@@ -1005,7 +1142,7 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
       }
     }
   }
-  if (sw.nfact == sw.n) rescale(rzb);
+  if (!combined && sw.nfact == sw.n) rescale(rzb);
   wave_lds_sync();  // all reads of the tile by the last inverse stage are issued
   // mask and scale in registers, then the forward row transform (first stage from registers)
   const float norm = scale / (float)geo.N;
@@ -1061,14 +1198,11 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   auto get = [&](int e) { return make_float2(acc[e][0], acc[e][1]); };
   pencil_forward_from_regs<M, true>(tp, get, rw, stw);
   wave_lds_sync();
-  float2* o = out + t * slot_out + row0 * pitch_out;
-  constexpr int ITS = (RW * (M + 1) + 63) / 64;
-#pragma unroll
-  for (int it = 0; it < ITS; ++it) {
-    const int f = lane + it * 64;
-    const int rr = f / (M + 1), k = f % (M + 1);
-    if (f >= RW * (M + 1) || row0 + rr >= rows_per_map || k >= lz_out) continue;
-    o[(long long)rr * pitch_out + k] = r2c_split<M>(buf, LDT, rr, k, stwN);
+  // kz < lz_out of this lane's row: positions tp, tp + P, ... (each store instruction writes P
+  // consecutive entries of every row of the wave)
+  if (row < rows_per_map) {
+    float2* o = out + t * slot_out + row * pitch_out;
+    for (int k = tp; k < lz_out; k += P) o[k] = r2c_split<M>(buf, LDT, r, k, stwN);
   }
 }
 
@@ -1227,37 +1361,90 @@ static hipError_t launch_c2r_rows(int M, const float2* in, float* out, long long
 #undef PB3D_ROWS
 }
 
+template <int MM, int NU>
+static hipError_t launch_prows(const float2* work, long long slot_in, int pitch_in, float2* out, long long slot_out,
+                               int pitch_out, long long rpm, int batch, const SubW& sw, const IcSum& ic,
+                               const float* mask, float scale, int lz_in, int lz_out, const float2* tw,
+                               const float2* twN, const Geo& geo, hipStream_t s) {
+  constexpr int RB = PointCfg<MM>::RB;
+  dim3 grid((unsigned)((rpm + RB - 1) / RB), batch);
+  if (ic.weight) {
+    IcSum icw = ic;
+    icw.nwm = (rpm + PointCfg<MM>::RW - 1) / PointCfg<MM>::RW;
+    pointing_rows<MM, true, NU><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, pitch_out,
+                                                                  rpm, batch, sw, icw, mask, scale, lz_in, lz_out, tw,
+                                                                  twN, geo);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const long long nf = (long long)batch * ic.nbins;
+    ic_finalize<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(icw.partial, icw.nwm, batch, ic.nbins, ic.sums);
+  } else {
+    pointing_rows<MM, false, NU><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, pitch_out,
+                                                                   rpm, batch, sw, ic, mask, scale, lz_in, lz_out, tw,
+                                                                   twN, geo);
+  }
+  return hipGetLastError();
+}
+
+// row lengths with a combined-row instance (sw.nu > 0 is set only for these)
+constexpr bool combined_rows_len(int M) { return M == 64 || M == 128 || M == 256; }
+
+template <int MM>
+static hipError_t launch_prows_m(const float2* work, long long slot_in, int pitch_in, float2* out, long long slot_out,
+                                 int pitch_out, long long rpm, int batch, const SubW& sw, const IcSum& ic,
+                                 const float* mask, float scale, int lz_in, int lz_out, const float2* tw,
+                                 const float2* twN, const Geo& geo, hipStream_t s) {
+  if constexpr (combined_rows_len(MM)) {
+    if (sw.nu == 3)
+      return launch_prows<MM, 3>(work, slot_in, pitch_in, out, slot_out, pitch_out, rpm, batch, sw, ic, mask, scale,
+                                 lz_in, lz_out, tw, twN, geo, s);
+    if (sw.nu == 5)
+      return launch_prows<MM, 5>(work, slot_in, pitch_in, out, slot_out, pitch_out, rpm, batch, sw, ic, mask, scale,
+                                 lz_in, lz_out, tw, twN, geo, s);
+  }
+  return launch_prows<MM, 0>(work, slot_in, pitch_in, out, slot_out, pitch_out, rpm, batch, sw, ic, mask, scale,
+                             lz_in, lz_out, tw, twN, geo, s);
+}
+
 static hipError_t launch_pointing_rows(int M, const float2* work, long long slot_in, int pitch_in, float2* out,
                                        long long slot_out, int pitch_out, long long rpm, int batch, const SubW& sw,
                                        const IcSum& ic,
                                        const float* mask, float scale, int lz_in, int lz_out, const float2* tw,
                                        const float2* twN, const Geo& geo, hipStream_t s) {
-#define PB3D_PROWS(MM)                                                                           \
-  case MM: {                                                                                     \
-    constexpr int RB = PointCfg<MM>::RB;                                                         \
-    dim3 grid((unsigned)((rpm + RB - 1) / RB), batch);                                           \
-    if (ic.weight) {                                                                             \
-      IcSum icw = ic;                                                                            \
-      icw.nwm = (rpm + PointCfg<MM>::RW - 1) / PointCfg<MM>::RW;                                 \
-      pointing_rows<MM, true><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, \
-                                                         pitch_out, rpm, batch, sw, icw, mask, scale, \
-                                                         lz_in, lz_out, tw, twN, geo);           \
-      hipError_t e = hipGetLastError();                                                          \
-      if (e != hipSuccess) return e;                                                             \
-      const long long nf = (long long)batch * ic.nbins;                                          \
-      ic_finalize<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(icw.partial, icw.nwm, batch, ic.nbins, ic.sums); \
-    } else                                                                                       \
-      pointing_rows<MM, false><<<grid, PointCfg<MM>::NT, 0, s>>>(work, slot_in, pitch_in, out, slot_out, \
-                                                         pitch_out, rpm, batch, sw, ic, mask, scale, \
-                                                         lz_in, lz_out, tw, twN, geo);           \
-    return hipGetLastError();                                                                    \
-  }
+#define PB3D_PROWS(MM)                                                                                     \
+  case MM:                                                                                                 \
+    return launch_prows_m<MM>(work, slot_in, pitch_in, out, slot_out, pitch_out, rpm, batch, sw, ic, mask, \
+                              scale, lz_in, lz_out, tw, twN, geo, s);
   switch (M) {
     PB3D_PROWS(2) PB3D_PROWS(4) PB3D_PROWS(8) PB3D_PROWS(16) PB3D_PROWS(32) PB3D_PROWS(64)
     PB3D_PROWS(128) PB3D_PROWS(256)
     default: return hipErrorInvalidValue;
   }
 #undef PB3D_PROWS
+}
+
+// c * p(x, y, z) * (x^2 + y^2 + z^2)^((L - ell) / 2) for p homogeneous of degree ell given in the
+// (k, a) index order (x^a y^(ell-k-a) z^k; p = 1 for ell = 0), written in the same order for degree L.
+static void raise_poly(int ell, const float* p, int L, float c, float* out) {
+  double A[5][5][5] = {};  // [x power][y power][z power]
+  int idx = 0;
+  for (int k = 0; k <= ell; ++k)
+    for (int a = 0; a <= ell - k; ++a, ++idx) A[a][ell - k - a][k] = ell > 0 ? (double)p[idx] : 1.0;
+  for (int d = ell; d < L; d += 2) {
+    double B[5][5][5] = {};
+    for (int a = 0; a <= d; ++a)
+      for (int b = 0; a + b <= d; ++b) {
+        const int k = d - a - b;
+        const double v = A[a][b][k];
+        B[a + 2][b][k] += v; B[a][b + 2][k] += v; B[a][b][k + 2] += v;
+      }
+    for (int a = 0; a < 5; ++a)
+      for (int b = 0; b < 5; ++b)
+        for (int k = 0; k < 5; ++k) A[a][b][k] = B[a][b][k];
+  }
+  idx = 0;
+  for (int k = 0; k <= L; ++k)
+    for (int a = 0; a <= L - k; ++a, ++idx) out[idx] = (float)(c * A[a][L - k - a][k]);
 }
 
 // Homogeneous-polynomial form of Y_{ell,m} (degree ell <= 4) on the unit sphere, fitted by least
@@ -1669,6 +1856,16 @@ extern "C" int64_t pb3d_icsum_partial_size(const pb3d_grid* grid, int batch, int
   return (int64_t)batch * ((rpm + rw - 1) / rw) * nbins;
 }
 
+static std::atomic<int> g_point_mode{1};
+
+extern "C" int pb3d_pointing_rows_mode(int mode) {
+  if (mode != 0 && mode != 1) {
+    fail(PB3D_ERR_ARG, "pointing_rows_mode: mode must be 0 or 1");
+    return -1;
+  }
+  return g_point_mode.exchange(mode);
+}
+
 extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const pb3d_submap* subs, const float* mask,
                                  float scale, const pb3d_kop* kop_out, const pb3d_icsum* ic_, void* out_, void* work_,
                                  void* stream) {
@@ -1705,6 +1902,17 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
     sw.y0[k] = sb.ell > 0 ? ylm_f(sb.ell, sb.m, 0.f, 0.f, 0.f) : 1.f;
     if (sb.ell > 0 && !ylm_poly(sb.ell, sb.m, sw.poly[k]))
       return fail(PB3D_ERR_ARG, "pointing: Y_lm polynomial fit failed");
+  }
+  // combined rows: every ell even, Lmax 2 or 4, fewer row transforms (Lmax + 1) than sub-maps
+  {
+    int lm = 0;
+    bool even = true;
+    for (int k = 0; k < nsub; ++k) { lm = sw.ell[k] > lm ? sw.ell[k] : lm; even = even && sw.ell[k] % 2 == 0; }
+    sw.nu = 0;
+    if (g_point_mode.load() == 1 && !p->generic && even && (lm == 2 || lm == 4) && lm + 1 < nsub && combined_rows_len(p->L[2])) {
+      for (int k = 0; k < nsub; ++k) raise_poly(sw.ell[k], sw.poly[k], lm, sw.coef[k], sw.cpoly[k]);
+      sw.nu = lm + 1;
+    }
   }
   Prune pin, pout;
   pointing_prunes(g, nsub, subs, kop_out, pin, pout);

@@ -155,18 +155,32 @@ def test_pruned_transforms(hip, n, kmax):
     assert rel(got[0], ref) < 1e-5 and rel(got[1], 2 * ref) < 1e-5
 
 
-# (shape, l of the Y_lm sub-maps): row lengths n2/2 = 16, 128, 32, 64, 256 and every weight form
-# (l = 1..4); (4, 4, 256) has fewer rows than one block holds; (16, 256, 64) runs the forward y pass
-# on plane-aligned tiles (33 z planes against 64-column tiles)
-POINT_CASES = [((32, 32, 32), 2), ((4, 4, 256), 2), ((16, 8, 64), 2), ((8, 16, 128), 2), ((8, 8, 512), 2),
-               ((32, 32, 32), 1), ((32, 32, 32), 3), ((32, 32, 32), 4), ((8, 16, 128), 4), ((16, 256, 64), 2)]
+# (shape, l of the Y_lm sub-maps, box centre): row lengths n2/2 = 16, 128, 32, 64, 256 and every weight
+# form (l = 1..4); (4, 4, 256) has fewer rows than one block holds; (16, 256, 64) runs the forward y pass
+# on plane-aligned tiles (33 z planes against 64-column tiles).  Even l on n2 = 128, 256, 512 runs the
+# combined rows (Lmax+1 row transforms) in mode 1.  A centre with x = y = 0 gives rows through the line
+# of sight (they keep the per-sub-map form); z = 5 * 1000/256 also puts r = 0 exactly on a voxel (the
+# reference refuses a zero centre with a local sightline).
+FAR = (400., 250., -300.)
+POINT_CASES = [((32, 32, 32), 2, FAR), ((4, 4, 256), 2, FAR), ((16, 8, 64), 2, FAR), ((8, 16, 128), 2, FAR),
+               ((8, 8, 512), 2, FAR), ((32, 32, 32), 1, FAR), ((32, 32, 32), 3, FAR), ((32, 32, 32), 4, FAR),
+               ((8, 16, 128), 4, FAR), ((16, 256, 64), 2, FAR), ((16, 8, 256), 2, FAR), ((8, 8, 256), 4, FAR),
+               ((16, 16, 256), 2, (0., 0., 19.53125)), ((16, 16, 256), 4, (0., 0., 37.)), ((8, 8, 512), 4, FAR)]
 
 
-@pytest.mark.parametrize("n,L", POINT_CASES, ids=lambda c: "x".join(map(str, c)) if isinstance(c, tuple) else "l%d" % c)
-def test_fused_pointing_chain(hip, n, L):
-    """pb3d_fft_pointing = R2C(mask * scale * sum_g coef_g Y_g(x) C2R(kop_g(in))) composed from numpy."""
+def _case_id(c):
+    if isinstance(c, tuple) and len(c) == 3 and isinstance(c[0], float):
+        return "far" if c == FAR else "c%g_%g_%g" % c
+    return "x".join(map(str, c)) if isinstance(c, tuple) else "l%d" % c
+
+
+@pytest.mark.parametrize("mode", [1, 0], ids=["combined", "per_submap"])
+@pytest.mark.parametrize("n,L,center", POINT_CASES, ids=_case_id)
+def test_fused_pointing_chain(hip, n, L, center, mode):
+    """pb3d_fft_pointing = R2C(mask * scale * sum_g coef_g Y_g(x) C2R(kop_g(in))) composed from numpy,
+    in both row modes (pb3d_pointing_rows_mode)."""
     from polybin3d_amd import _capi as C
-    center = [400., 250., -300.]
+    center = list(center)
     b = make_base(n, boxcenter=center, sightline='local')
     o = O.PolyBin3D(1000., list(n), boxcenter=center, sightline='local')
     rng = np.random.RandomState(6)
@@ -182,16 +196,31 @@ def test_fused_pointing_chain(hip, n, L):
     subs = [(C.kop(C.KOP_FILTER, ell=2, **common), Ad, 0, 0, 0.5)]
     subs += [(C.kop(C.KOP_YLM, ell=L, m=m, **common), Ad, L, m, 0.25) for m in range(2 * L + 1)]
     out = torch.empty((4,) + b.half_shape, dtype=torch.complex64, device='cuda')
-    b.pointing(subs, 4, out, mask=Md, scale=1.5, kop_out=None)
+    prev = C.lib().pb3d_pointing_rows_mode(mode)
+    try:
+        b.pointing(subs, 4, out, mask=Md, scale=1.5, kop_out=None)
+        got = out.cpu().numpy()
+    finally:
+        C.lib().pb3d_pointing_rows_mode(prev)
     Yr = O.real_harmonics(np.asarray(o.r_grids), L, True)[L]
     Yk = O.real_harmonics(np.asarray(o.k_grids), L, True)[L]
-    got = out.cpu().numpy()
     for q in range(4):
         r = 0.5 * o.to_real(o.map_utils.fourier_filter(A, 2, edges[q], edges[q + 1]))
         for m in range(2 * L + 1):
             r += 0.25 * Yr[m] * o.to_real(o.map_utils.prod_fourier_filter(A, Yk[m], edges[q], edges[q + 1]))
         ref = o.to_fourier(1.5 * mask * r)
-        assert rel(got[q], ref) < 2e-5
+        err = rel(got[q], ref)
+        print("pointing %s l=%d centre=%s mode=%d bin %d: rel L2 %.2e" % (n, L, center, mode, q, err))
+        assert err < 2e-5
+
+
+def test_pointing_rows_mode_rejects_bad_values(hip):
+    from polybin3d_amd import _capi as C
+    assert C.lib().pb3d_pointing_rows_mode(2) == -1
+    assert C.lib().pb3d_pointing_rows_mode(-1) == -1
+    prev = C.lib().pb3d_pointing_rows_mode(1)
+    assert prev in (0, 1)
+    C.lib().pb3d_pointing_rows_mode(prev)
 
 
 @pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512], ids=lambda n: "L%d" % n)
