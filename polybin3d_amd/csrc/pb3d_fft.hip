@@ -520,12 +520,18 @@ template <> struct RowCfg<4>  { static constexpr int T = 128; };
 template <> struct RowCfg<8>  { static constexpr int T = 64; };
 template <> struct RowCfg<16> { static constexpr int T = 64; };
 
-// accessor for row r of a transposed tile: element p at l[p * LDT + r]
-struct RowT {
+// accessor for row r of a transposed tile: element p at l[p * LDT + r + PADQ * (p / 16)] (PADQ > 0:
+// every 16 positions shifted by PADQ entries, so that a radix-16 stage's writes to positions 16j + k
+// of different j fall on different LDS banks)
+template <int PADQ>
+__device__ __forceinline__ int tile_at(int p, int LDT, int r) { return p * LDT + r + PADQ * (p >> 4); }
+template <int PADQ = 0>
+struct RowTp {
   float2* l; int LDT; int r;
-  __device__ float2 operator()(int p) const { return l[p * LDT + r]; }
-  __device__ void operator()(int p, float2 v) const { l[p * LDT + r] = v; }
+  __device__ float2 operator()(int p) const { return l[tile_at<PADQ>(p, LDT, r)]; }
+  __device__ void operator()(int p, float2 v) const { l[tile_at<PADQ>(p, LDT, r)] = v; }
 };
+using RowT = RowTp<0>;
 
 // LDS pitch of the transposed tile: T rows (+1 to spread the flat staging accesses)
 template <int M> struct RowTile {
@@ -544,10 +550,10 @@ __device__ __forceinline__ void stage_twiddles(float2* stw, float2* stwN, const 
 }
 
 // X[k] = E[k] + W^k O[k], E = (Z[k]+conj Z[M-k])/2, O = (Z[k]-conj Z[M-k])/(2i)   (row r of a tile)
-template <int M>
+template <int M, int PADQ = 0>
 __device__ __forceinline__ float2 r2c_split(const float2* b, int LDT, int r, int k, const float2* twN) {
-  const float2 zk = b[(k == M ? 0 : k) * LDT + r];
-  const float2 zm = cconj(b[(k == 0 ? 0 : M - k) * LDT + r]);
+  const float2 zk = b[tile_at<PADQ>(k == M ? 0 : k, LDT, r)];
+  const float2 zm = cconj(b[tile_at<PADQ>(k == 0 ? 0 : M - k, LDT, r)]);
   const float2 e = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
   const float2 d = make_float2(0.5f * (zk.x - zm.x), 0.5f * (zk.y - zm.y));
   const float2 od = make_float2(d.y, -d.x);
@@ -608,7 +614,7 @@ __device__ __forceinline__ void load_prepass(float2* buf, int LDT, const float2*
 // half X[M-k] is a known zero apart from k = M/2, so each pair needs one load and one complex
 // multiply per output; loads are unpredicated at clamped addresses), lz_in > M/2+1 (both halves
 // read, general pair formula).  Row rr starts at src + rr * pitch (pitch >= lz_in).
-template <int M, int T, int NT>
+template <int M, int T, int NT, int PADQ = 0>
 __device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const float2* __restrict__ src, int pitch,
                                                   long long rows_left, int lz_in, int tid, const float2* twN) {
   constexpr int K = M / 2 + 1;
@@ -639,11 +645,11 @@ __device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const fl
       const float p = 1.f + w.y;
       float2 z1 = make_float2(fmaf(a.x, p, -a.y * w.x), fmaf(a.x, w.x, a.y * p));
       if (2 * k == M) z1 = make_float2(2.f * a.x, -2.f * a.y);
-      buf[k * LDT + rr] = z1;
+      buf[tile_at<PADQ>(k, LDT, rr)] = z1;
       if (k != 0 && 2 * k != M) {
         const float2 w2 = twN[M - k];
         const float q = 1.f - w2.y;
-        buf[(M - k) * LDT + rr] = make_float2(fmaf(a.x, q, -a.y * w2.x), -fmaf(a.x, w2.x, a.y * q));
+        buf[tile_at<PADQ>(M - k, LDT, rr)] = make_float2(fmaf(a.x, q, -a.y * w2.x), -fmaf(a.x, w2.x, a.y * q));
       }
     }
     return;
@@ -672,13 +678,13 @@ __device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const fl
       const float2 sm = cadd(a, cconj(c));
       const float2 d = csub(a, cconj(c));
       const float2 wd = cmul(cconj(twN[k]), d);
-      buf[k * LDT + rr] = make_float2(sm.x - wd.y, sm.y + wd.x);
+      buf[tile_at<PADQ>(k, LDT, rr)] = make_float2(sm.x - wd.y, sm.y + wd.x);
     }
     if (k != 0 && 2 * k != M) {
       const float2 sm = cadd(c, cconj(a));
       const float2 d = csub(c, cconj(a));
       const float2 wd = cmul(cconj(twN[M - k]), d);
-      buf[(M - k) * LDT + rr] = make_float2(sm.x - wd.y, sm.y + wd.x);
+      buf[tile_at<PADQ>(M - k, LDT, rr)] = make_float2(sm.x - wd.y, sm.y + wd.x);
     }
   }
 }
@@ -871,15 +877,17 @@ template <int M> struct PointCfg {
   static constexpr int WAVES = 4;
   static constexpr int NT = 64 * WAVES;
   static constexpr int RB = RW * WAVES;       // rows per block
-  static constexpr int LDT = RW + 1;          // tile pitch: element p of row r at p * LDT + r
-  static constexpr int SIZE = (M + 1) * LDT;  // one wave's tile
+  static constexpr int LDT = RW + 1;          // tile pitch: element p of row r at p * LDT + r (+ pad)
+  static constexpr int PADQ = 0;              // bank padding per 16 positions (see RowTp): none
+                                               // (4 measured no fewer conflict cycles at M = 128)
+  static constexpr int SIZE = (M + 1) * LDT + PADQ * (M / 16 + 1);  // one wave's tile
   static constexpr int E = M / P;             // values per lane
   static_assert(64 % P == 0, "lanes per row must divide the wave");
 };
 
 // Band-limited C2R pre-pass (lz_in <= M/2, so X[k] = 0 for k >= M/2) from registers: slot it of
 // lane (row r, tp) holds X[k], k = tp + P*it; writes Z'[k], Z'[M-k] of row r and Z'[M/2] = 0.
-template <int M, int P, int IV>
+template <int M, int P, int IV, int PADQ>
 __device__ __forceinline__ void prepass_regs(float2* buf, int LDT, int r, int tp, const float2 (&x)[IV],
                                              const float2* twN) {
 #pragma unroll
@@ -889,14 +897,32 @@ __device__ __forceinline__ void prepass_regs(float2* buf, int LDT, int r, int tp
     if (it == 0 && tp == 0) a.y = 0.f;  // Im of the DC entry is ignored
     const float2 w = twN[k];
     const float p = 1.f + w.y;
-    buf[k * LDT + r] = make_float2(fmaf(a.x, p, -a.y * w.x), fmaf(a.x, w.x, a.y * p));
+    buf[tile_at<PADQ>(k, LDT, r)] = make_float2(fmaf(a.x, p, -a.y * w.x), fmaf(a.x, w.x, a.y * p));
     if (!(it == 0 && tp == 0)) {
       const float2 w2 = twN[M - k];
       const float q = 1.f - w2.y;
-      buf[(M - k) * LDT + r] = make_float2(fmaf(a.x, q, -a.y * w2.x), -fmaf(a.x, w2.x, a.y * q));
+      buf[tile_at<PADQ>(M - k, LDT, r)] = make_float2(fmaf(a.x, q, -a.y * w2.x), -fmaf(a.x, w2.x, a.y * q));
     }
   }
-  if (tp == 0) buf[(M / 2) * LDT + r] = make_float2(0.f, 0.f);
+  if (tp == 0) buf[tile_at<PADQ>(M / 2, LDT, r)] = make_float2(0.f, 0.f);
+}
+
+// Block order of the fused pointing rows, whose blocks of different maps read the same mask rows:
+// linear block b is dispatched to XCD b % 8 (round robin), so row blocks are dealt out in groups of 8
+// with all maps of one row block on one XCD, whose L2 then serves the mask rows to every map.  (map
+// t, row block rb) from b, a bijection onto [0, batch) x [0, nrb) whatever the dispatch order.
+__device__ __forceinline__ void xcd_rows_block(long long b, long long nrb, long long batch, long long& t,
+                                               long long& rb) {
+  const long long full = (nrb / 8) * 8 * batch;
+  if (b < full) {
+    const long long j = b / 8;
+    t = j % batch;
+    rb = (j / batch) * 8 + b % 8;
+  } else {
+    const long long rem = nrb % 8, q = b - full;
+    t = q / rem;
+    rb = (nrb / 8) * 8 + q % rem;
+  }
 }
 
 template <int M, bool IC, int NU>
@@ -906,7 +932,7 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
               const float* __restrict__ mask, float scale, int lz_in, int lz_out, const float2* __restrict__ tw,
               const float2* __restrict__ twN, Geo geo) {
   using PC = PointCfg<M>;
-  constexpr int RW = PC::RW, LDT = PC::LDT, E = PC::E, P = PC::P;
+  constexpr int RW = PC::RW, LDT = PC::LDT, E = PC::E, P = PC::P, PADQ = PC::PADQ;
   __shared__ float2 tiles[PC::WAVES * PC::SIZE];
   __shared__ float2 stw[M], stwN[M + 1];
   __shared__ float icl[IC ? PC::WAVES : 1][IC ? kMaxIcBins : 1];
@@ -915,11 +941,12 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   __syncthreads();  // twiddles staged: the only block barrier
   const int wave = tid / 64, lane = tid % 64;
   float2* buf = tiles + wave * PC::SIZE;
-  const long long t = blockIdx.y;
-  const long long row0 = (long long)blockIdx.x * PC::RB + wave * RW;  // this wave's first row
+  long long t, rb;
+  xcd_rows_block(blockIdx.x, (rows_per_map + PC::RB - 1) / PC::RB, batch, t, rb);
+  const long long row0 = rb * PC::RB + wave * RW;  // this wave's first row
   if (row0 >= rows_per_map) return;  // no rows for this wave (no block barrier follows)
   const int r = lane % RW, tp = lane / RW;
-  RowT rw{buf, LDT, r};
+  RowTp<PADQ> rw{buf, LDT, r};
   // this lane's voxels after the inverse transforms: positions m = tp + e*P of row r
   const long long row = row0 + r;
   const long long rowc = row < rows_per_map ? row : rows_per_map - 1;
@@ -1014,12 +1041,15 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
 #pragma unroll
       for (int u = LM; u >= 0; --u) {
         // opaque copies of the lane indices per transform: the twiddle reads derived from them are
-        // repeated per transform instead of being hoisted and held in registers across all of them
+        // repeated per transform instead of being hoisted and held in registers across all of them;
+        // the masks restore their ranges, so index products stay full-rate 24-bit multiplies
         int tp_u = tp, r_u = r;
         asm volatile("" : "+v"(tp_u), "+v"(r_u));
-        const RowT rw_u{buf, LDT, r_u};
+        tp_u &= P - 1;
+        r_u &= RW - 1;
+        const RowTp<PADQ> rw_u{buf, LDT, r_u};
         wave_lds_sync();  // the previous transform's reads of the tile are issued
-        prepass_regs<M, P, IV>(buf, LDT, r_u, tp_u, V[u], stwN);
+        prepass_regs<M, P, IV, PADQ>(buf, LDT, r_u, tp_u, V[u], stwN);
         wave_lds_sync();
         pencil_inverse_head<M, true>(tp_u, rw_u, stw);
         if (u == LM) {
@@ -1048,13 +1078,16 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   for (int g = 0; g < (combined ? 0 : sw.n); ++g) {
     const float2* src = work + ((long long)g * batch + t) * slot_in + row0 * pitch_in;
     // Opaque copies of the lane indices: index arithmetic derived from them is recomputed per
-    // sub-map instead of being hoisted out of the loop and held in (spilled) registers.
+    // sub-map instead of being hoisted out of the loop and held in (spilled) registers; the masks
+    // restore their ranges, so index products stay full-rate 24-bit multiplies.
     int lane_g = lane, tp_g = tp;
     float rzb_g = rzb;
     asm volatile("" : "+v"(lane_g), "+v"(tp_g), "+v"(rzb_g));
+    lane_g &= 63;
+    tp_g &= P - 1;
     if (g == sw.nfact && g > 0) rescale(rzb_g);
     wave_lds_sync();  // the previous sub-map's reads of the tile are issued
-    load_prepass_rows<M, RW, 64>(buf, LDT, src, pitch_in, rows_per_map - row0, lz_in, lane_g, stwN);
+    load_prepass_rows<M, RW, 64, PADQ>(buf, LDT, src, pitch_in, rows_per_map - row0, lz_in, lane_g, stwN);
     wave_lds_sync();
     const int ell = sw.ell[g];
     const float cf = sw.coef[g];
@@ -1202,7 +1235,7 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   // consecutive entries of every row of the wave)
   if (row < rows_per_map) {
     float2* o = out + t * slot_out + row * pitch_out;
-    for (int k = tp; k < lz_out; k += P) o[k] = r2c_split<M>(buf, LDT, r, k, stwN);
+    for (int k = tp; k < lz_out; k += P) o[k] = r2c_split<M, PADQ>(buf, LDT, r, k, stwN);
   }
 }
 
@@ -1367,7 +1400,9 @@ static hipError_t launch_prows(const float2* work, long long slot_in, int pitch_
                                const float* mask, float scale, int lz_in, int lz_out, const float2* tw,
                                const float2* twN, const Geo& geo, hipStream_t s) {
   constexpr int RB = PointCfg<MM>::RB;
-  dim3 grid((unsigned)((rpm + RB - 1) / RB), batch);
+  const long long nblk = (rpm + RB - 1) / RB * batch;  // one linear grid, see xcd_rows_block
+  if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+  dim3 grid((unsigned)nblk);
   if (ic.weight) {
     IcSum icw = ic;
     icw.nwm = (rpm + PointCfg<MM>::RW - 1) / PointCfg<MM>::RW;
