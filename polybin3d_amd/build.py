@@ -25,6 +25,10 @@ def build(verbose=True, jobs=None, out=None, defines=()):
     objdir = os.path.join(HERE, "_build") if out is None else os.path.join(HERE, "_build", "variant_" + os.path.basename(out).replace(".", "_"))
     os.makedirs(objdir, exist_ok=True)
     t0 = time.time()
+    # hipcc reads a source once per offload pass (device, then host): a source edited during the
+    # build would give a library whose host launch stubs and device kernels disagree
+    watched = [os.path.join(CSRC, f) for f in os.listdir(CSRC)] + [os.path.join(HERE, "..", "include", "pb3d.h")]
+    stamp = {f: os.stat(f).st_mtime_ns for f in watched}
     procs = []
     objs = []
     for src in SOURCES:
@@ -43,6 +47,9 @@ def build(verbose=True, jobs=None, out=None, defines=()):
             sys.stderr.write(out.decode(errors="replace")[-2000:])
     if failed:
         raise RuntimeError("libpb3d build failed")
+    changed = [f for f in watched if os.stat(f).st_mtime_ns != stamp[f]]
+    if changed:
+        raise RuntimeError("sources changed during the build (%s): build again" % ", ".join(map(os.path.basename, changed)))
     tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)

@@ -11,6 +11,7 @@
 #include <vector>
 #include <cmath>
 #include <atomic>
+#include <type_traits>
 
 namespace pb3d {
 
@@ -939,7 +940,7 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   const int tid = threadIdx.x;
   stage_twiddles<M>(stw, stwN, tw, twN, tid, PC::NT);
   __syncthreads();  // twiddles staged: the only block barrier
-  const int wave = tid / 64, lane = tid % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64;  // wave index: scalar
   float2* buf = tiles + wave * PC::SIZE;
   long long t, rb;
   xcd_rows_block(blockIdx.x, (rows_per_map + PC::RB - 1) / PC::RB, batch, t, rb);
@@ -949,16 +950,17 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   RowTp<PADQ> rw{buf, LDT, r};
   // this lane's voxels after the inverse transforms: positions m = tp + e*P of row r
   const long long row = row0 + r;
-  const long long rowc = row < rows_per_map ? row : rows_per_map - 1;
-  const int i0 = (int)(rowc / geo.n1), i1 = (int)(rowc % geo.n1);
+  // grid indices of this lane's row (rows past the end take the last row): the wave's first row is
+  // divided once (scalar), the lane offset (< RW) carried into i0
+  const int dr = (int)((row < rows_per_map ? row : rows_per_map - 1) - row0);
+  int i0 = (int)(row0 / geo.n1), i1 = (int)(row0 % geo.n1) + dr;
+  while (i1 >= geo.n1) { i1 -= geo.n1; ++i0; }
   const float rx = freq_index(i0, geo.n0) * geo.dx0 + geo.c0;
   const float ry = freq_index(i1, geo.n1) * geo.dx1 + geo.c1;
   const float rxy2 = rx * rx + ry * ry;
   const float rzb = (float)(2 * tp) * geo.dx2 + geo.c2;  // z coordinate of position 2*tp
   static_assert((M / 2) % P == 0, "negative-frequency slots must be whole");
-  float acc[E][2];
-#pragma unroll
-  for (int e = 0; e < E; ++e) { acc[e][0] = 0.f; acc[e][1] = 0.f; }
+  float acc[E][2];  // set by the first transform (combined rows) or zeroed for the per-sub-map sums
   // acc *= 1/|r|^2 (the ell = 2 weights accumulated so far carry only their numerators); at r = 0
   // those numerators vanish exactly, as Y_2m(0) = 0 does
   auto rescale = [&](float rzb_v) {
@@ -994,10 +996,6 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
           for (int a = 0; a <= LM - u; ++a) mono[idx++] = px[a] * py[LM - u - a];
       }
       float2 V[NU][IV];
-#pragma unroll
-      for (int u = 0; u < NU; ++u)
-#pragma unroll
-        for (int it = 0; it < IV; ++it) V[u][it] = make_float2(0.f, 0.f);
       const int kl = lz_in - 1;
       // every row of the wave present and lz_in = M/2: no clamped addresses or zero selects
       const bool whole = lz_in == M / 2 && __ballot(!rv) == 0ull;
@@ -1019,7 +1017,8 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
           if (!(rv && k < lz_in)) s[it] = make_float2(0.f, 0.f);
         }
       };
-      for (int g = 0; g < sw.n; ++g) {
+      // V_u += D_ug S_g (the first sub-map sets V)
+      auto combine = [&](int g, auto first) {
         float2 s[IV];
         load(g, s);
         const float* pc = sw.cpoly[g];
@@ -1031,11 +1030,17 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
           for (int a = 0; a <= LM - u; ++a, ++idx) d = fmaf(pc[idx], mono[idx], d);
 #pragma unroll
           for (int it = 0; it < IV; ++it) {
-            V[u][it].x = fmaf(d, s[it].x, V[u][it].x);
-            V[u][it].y = fmaf(d, s[it].y, V[u][it].y);
+            if constexpr (decltype(first)::value) {
+              V[u][it] = make_float2(d * s[it].x, d * s[it].y);
+            } else {
+              V[u][it].x = fmaf(d, s[it].x, V[u][it].x);
+              V[u][it].y = fmaf(d, s[it].y, V[u][it].y);
+            }
           }
         }
-      }
+      };
+      combine(0, std::true_type{});
+      for (int g = 1; g < sw.n; ++g) combine(g, std::false_type{});
       const float rzb_neg = rzb - (float)geo.n2 * geo.dx2;
       auto rz_of = [&](int e) { return fmaf((float)(2 * e * P), geo.dx2, 2 * e >= E ? rzb_neg : rzb); };
 #pragma unroll
@@ -1074,6 +1079,10 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
         acc[e][1] *= i1;
       }
     }
+  }
+  if (!combined) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) { acc[e][0] = 0.f; acc[e][1] = 0.f; }
   }
   for (int g = 0; g < (combined ? 0 : sw.n); ++g) {
     const float2* src = work + ((long long)g * batch + t) * slot_in + row0 * pitch_in;
