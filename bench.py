@@ -34,11 +34,11 @@ HBM_PEAK_GBS = 8000.0
 # kernel they were taken on (profiles/hbm_traffic_pk_bk_mc_fisher_256.md); other kernels report null.
 # The fused rows' algorithmic bytes count the mask once per map; its rows are shared by every map of a
 # launch and the XCD-aware block order lets L2 serve them, so the measured traffic is lower.
-MEASURED_TRAFFIC = {"pointing_rows<128>": 7.62e9}
+MEASURED_TRAFFIC = {"pointing_rows<128>": 5.29e9}
 # wave64 VALU instructions per algorithmic byte, rocprofv3 --pmc SQ_INSTS_VALU summed over the launches
 # of one realization of this workload divided by their algorithmic bytes (the sampler's, period 1)
 # (profiles/pointing_rows_sq_counters_pk_bk_mc_fisher_256.md)
-MEASURED_VALU_PER_BYTE = {"pointing_rows<128>": 0.1098}
+MEASURED_VALU_PER_BYTE = {"pointing_rows<128>": 0.1139}
 SIMDS = 256 * 4          # MI355X: 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9         # peak engine clock
 KIND_NAMES = {0: "r2c_rows", 1: "c2r_rows", 2: "fft_cols (plain C2R)", 3: "fft_cols (C2R first pass, fused prologue)",

@@ -72,13 +72,11 @@ class PolyBin3D:
                  nthreads=None, sightline='global', real_fft=True, device=None, rng='numpy', verb=False):
         if backend != 'hip':
             raise Exception("polybin3d_amd implements backend='hip' only")
-        if not real_fft:
-            raise Exception("backend='hip' supports real-to-complex FFTs only (real_fft=True)")
         self.backend = backend
         self.nthreads = nthreads if nthreads is not None else os.cpu_count()
         self.sightline = sightline
         self.pixel_window = pixel_window
-        self.real_fft = True
+        self.real_fft = bool(real_fft)
         self.rng = rng
         self.device = torch.device(device if device is not None else 'cuda')
         if self.device.type != 'cuda':
@@ -95,8 +93,13 @@ class PolyBin3D:
             assert len(self.boxcenter) == 3, "Need to specify x, y, z boxcenter"
         self.gridsize = np.asarray([gridsize] * 3 if np.ndim(gridsize) == 0 else gridsize, dtype=int)
         assert len(self.gridsize) == 3, "Need to specify n_x, n_y, n_z gridsize"
-        if self.gridsize[2] % 2 != 0:
+        if self.real_fft and self.gridsize[2] % 2 != 0:
             raise Exception("Last dimension of grid must be even for real-to-complex FFTs.")
+        if not self.real_fft and self.gridsize[2] % 2 != 0:
+            # the reference's full-grid frequencies fftshift(arange(-n//2, n//2)) give an odd axis a
+            # frequency set that is not symmetric under k -> -k, so Hermitian-paired modes can land in
+            # different bins; the half-spectrum transforms here cannot reproduce that
+            raise NotImplementedError("backend='hip' with real_fft=False needs an even last grid dimension")
         assert self.sightline in ['local', 'global'], "Sight-line must be 'local' or 'global'"
         if self.sightline == 'global':
             assert np.abs(self.boxcenter).sum() == 0., "Global sightlines should only be used for simulation data!"
@@ -120,6 +123,7 @@ class PolyBin3D:
         self.k_arrs = [np.fft.fftshift(np.arange(-n[i] // 2, n[i] // 2)) * self.kF[i] for i in range(2)]
         self.k_arrs += [np.arange(0, n[2] // 2 + 1) * self.kF[2]]
         self.half_shape = (int(n[0]), int(n[1]), int(n[2] // 2 + 1))
+        self.full_shape = (int(n[0]), int(n[1]), int(n[2]))
         self.H = int(np.prod(self.half_shape))
         self.N = int(np.prod(n))
         self._modk = None
@@ -135,7 +139,7 @@ class PolyBin3D:
         C.check(C.lib().pb3d_plan_create(C.ctypes.byref(plan), C.ctypes.byref(self.cgrid)))
         self._plan = plan
         # fiducial spectrum on the grid (base.py:122-131), FP64 on host then FP32 on device
-        modk = self.modk_grid
+        modk = self.modk_half
         Pk0 = scipy.interpolate.interp1d(self.Pfid[0], self.Pfid[1], bounds_error=False, fill_value=0.)(modk)
         invPk0 = np.zeros(modk.shape)
         good = (Pk0 != 0) & np.isfinite(Pk0)
@@ -169,28 +173,86 @@ class PolyBin3D:
             pass
 
     # ------------------------------------------------------------------ host-side grids
+    # The transforms work on the half spectrum kz = 0 .. n2/2 whatever real_fft says; with
+    # real_fft=False the public k-space grids and Fourier maps take the reference's full-grid form
+    # (base.py:86-94: every axis fftshift(arange(-n/2, n/2)) * kF, no degeneracy factor) and are
+    # converted at the boundary.
     @property
-    def k_grids(self):
+    def k_grids_half(self):
         return np.meshgrid(*self.k_arrs, indexing='ij')
 
     @property
-    def modk_grid(self):
-        """|k| in FP64 exactly as the reference computes it (base.py:91-92)."""
+    def modk_half(self):
+        """|k| on the half grid in FP64 exactly as the reference computes it (base.py:91-92)."""
         if self._modk is None:
-            kg = self.k_grids
+            kg = self.k_grids_half
             self._modk = np.sqrt(kg[0] ** 2 + kg[1] ** 2 + kg[2] ** 2)
         return self._modk
 
     @property
-    def muk_grid(self):
+    def muk_half(self):
         if self._muk is None:
-            kg = self.k_grids
-            modk = self.modk_grid
-            mu = np.zeros(modk.shape)
-            nz = modk != 0
-            mu[nz] = np.sum(self.sight_vector[:, None] * np.asarray(kg)[:, nz], axis=0) / modk[nz]
-            self._muk = mu
+            self._muk = self._mu_of(self.k_grids_half, self.modk_half)
         return self._muk
+
+    def _mu_of(self, kg, modk):
+        mu = np.zeros(modk.shape)
+        nz = modk != 0
+        mu[nz] = np.sum(self.sight_vector[:, None] * np.asarray(kg)[:, nz], axis=0) / modk[nz]
+        return mu
+
+    @property
+    def k_grids(self):
+        if self.real_fft:
+            return self.k_grids_half
+        n = self.gridsize
+        return np.meshgrid(*[np.fft.fftshift(np.arange(-n[i] // 2, n[i] // 2)) * self.kF[i] for i in range(3)],
+                           indexing='ij')
+
+    @property
+    def modk_grid(self):
+        if self.real_fft:
+            return self.modk_half
+        kg = self.k_grids
+        return np.sqrt(kg[0] ** 2 + kg[1] ** 2 + kg[2] ** 2)
+
+    @property
+    def muk_grid(self):
+        if self.real_fft:
+            return self.muk_half
+        kg = self.k_grids
+        return self._mu_of(kg, np.sqrt(kg[0] ** 2 + kg[1] ** 2 + kg[2] ** 2))
+
+    @property
+    def degeneracy_factor(self):
+        """1 + (kz > 0) on the half grid (base.py:93-96); None with real_fft=False."""
+        if not self.real_fft:
+            return None
+        return np.asarray(1. + (self.k_grids_half[2] > 0), dtype=np.float64)
+
+    def to_full(self, x):
+        """Half-spectrum map(s) [..., n0, n1, n2/2+1] -> the Hermitian full grid [..., n0, n1, n2]."""
+        n0, n1, n2 = self.full_shape
+        h = n2 // 2 + 1
+        out = torch.empty(tuple(x.shape[:-3]) + self.full_shape, dtype=x.dtype, device=x.device)
+        out[..., :h] = x
+        # X[i, j, k] = conj X[-i, -j, n2 - k] for k > n2/2
+        src = x[..., 1:n2 - h + 1].conj()
+        src = torch.roll(torch.flip(src, dims=(-3, -2, -1)), shifts=(1, 1), dims=(-3, -2))
+        out[..., h:] = src
+        return out
+
+    def from_full(self, x):
+        """Full-grid map(s) -> the half spectrum of their Hermitian part, (X(k) + conj X(-k)) / 2 for
+        kz = 0 .. n2/2: the map whose C2R is Re IFFT(X)."""
+        n2 = self.full_shape[2]
+        h = n2 // 2 + 1
+        neg = torch.roll(torch.flip(x, dims=(-3, -2, -1)), shifts=(1, 1, 1), dims=(-3, -2, -1))
+        return (0.5 * (x[..., :h] + neg[..., :h].conj())).contiguous()
+
+    def public_fourier(self, x):
+        """A Fourier map as the caller sees it: the half spectrum, or the full grid with real_fft=False."""
+        return x if self.real_fft else self.to_full(x)
 
     @property
     def r_grids(self):
@@ -214,7 +276,7 @@ class PolyBin3D:
         nb = len(edges) - 1
         if not np.all(np.diff(edges) > 0):
             raise ValueError("bin edges must be strictly increasing")
-        idx = np.searchsorted(edges, self.modk_grid, side='right') - 1
+        idx = np.searchsorted(edges, self.modk_half, side='right') - 1
         idx[(idx < 0) | (idx >= nb)] = -1
         return idx
 
@@ -229,7 +291,7 @@ class PolyBin3D:
                 raise ValueError("between 1 and 254 bins supported")
             if not np.all(np.diff(edges) > 0):
                 raise ValueError("bin edges must be strictly increasing")
-            idx = np.searchsorted(edges, self.modk_grid, side='right') - 1
+            idx = np.searchsorted(edges, self.modk_half, side='right') - 1
             idx[(idx < 0) | (idx >= nb)] = C.PB3D_NO_BIN
             self._bin_tables[key] = torch.from_numpy(idx.astype(np.uint8)).to(self.device)
             # the in-bin modes (ascending linear index), for reductions that visit only them
@@ -331,6 +393,8 @@ class PolyBin3D:
         if not torch.is_tensor(x):
             raise TypeError("expected a numpy array or torch tensor")
         x = x.to(device=self.device, dtype=torch.complex64).contiguous()
+        if not self.real_fft and tuple(x.shape[-3:]) == self.full_shape:
+            return self.from_full(x)
         if tuple(x.shape[-3:]) != self.half_shape:
             raise ValueError("Fourier map has shape %s, half grid is %s" % (tuple(x.shape), self.half_shape))
         return x
@@ -379,12 +443,13 @@ class PolyBin3D:
         return out
 
     def to_fourier(self, _input_rmap):
-        """Unnormalised R2C (base.py:176-200); float32 maps on the GPU."""
+        """Unnormalised R2C (base.py:176-200); float32 maps on the GPU.  With real_fft=False the full
+        complex grid (numpy fftn of the real map)."""
         x = self.as_real(_input_rmap)
-        return self.r2c(x)
+        return self.public_fourier(self.r2c(x))
 
     def to_real(self, _input_kmap):
-        """C2R with 1/N (base.py:202-229)."""
+        """C2R with 1/N (base.py:202-229); a full grid (real_fft=False) gives Re IFFT, as the reference."""
         k = self.as_fourier(_input_kmap)
         return self.c2r(k)
 
@@ -459,9 +524,9 @@ class PolyBin3D:
             return self.Pk0_grid_host
         assert len(np.asarray(Pk_input).shape) == 2, "Pk should contain k and P_0, (and optionally P_2, P_4) columns"
         assert len(Pk_input) in [2, 3, 4], "Pk should contain k and P_0, (and optionally P_2, P_4) columns"
-        modk = self.modk_grid
+        modk = self.modk_half
         P = scipy.interpolate.interp1d(Pk_input[0], Pk_input[1], bounds_error=False, fill_value=0.)(modk)
-        mu = self.muk_grid
+        mu = self.muk_half
         if len(Pk_input) > 2:
             P = P + 0.5 * (3 * mu ** 2 - 1) * scipy.interpolate.interp1d(
                 Pk_input[0], Pk_input[2], bounds_error=False, fill_value=0.)(modk)
@@ -472,7 +537,13 @@ class PolyBin3D:
 
     def generate_data(self, seed=None, Pk_input=[], output_type='real', include_pixel_window=True):
         """Gaussian random field (base.py:305-353).  rng='numpy': the reference's legacy
-        RandomState(seed).randn stream, drawn on the host and symmetrised on the GPU."""
+        RandomState(seed).randn stream, drawn on the host and symmetrised on the GPU.  Fourier output:
+        the half spectrum, or the full Hermitian grid with real_fft=False."""
+        out = self.grf_half(seed, Pk_input, output_type, include_pixel_window)
+        return self.public_fourier(out) if output_type == 'fourier' else out
+
+    def grf_half(self, seed=None, Pk_input=[], output_type='real', include_pixel_window=True):
+        """generate_data with a half-spectrum Fourier output (what the estimators use internally)."""
         assert output_type in ['fourier', 'real'], "Valid output types are 'fourier' and 'real' only!"
         key = ('sqrtPw', tuple(map(tuple, np.asarray(Pk_input))) if len(Pk_input) else None,
                bool(include_pixel_window and self.pixel_window != 'none'))
@@ -514,7 +585,7 @@ class PolyBin3D:
             return input_data
         if input_type == 'fourier':
             return self.to_real(input_data)
-        return self.to_fourier(input_data)
+        return self.to_fourier(input_data)  # the caller's Fourier form (full grid with real_fft=False)
 
     def ainv_weight(self, invPk0_grid=None):
         """invPk0 / W^2 as a device FP32 half grid, or None when it is identically one
@@ -548,7 +619,7 @@ class PolyBin3D:
         else:
             f = self.as_fourier(input_data)
             f = self.kmap(f, op) if op is not None else f.clone()
-        return self.c2r(f) if output_type == 'real' else f
+        return self.c2r(f) if output_type == 'real' else self.public_fourier(f)
 
     def apply_fourier_harmonics(self, a_map_real, Ylm_real, Ylm_fourier, out=None):
         """sum_m FFT[a(x) Y_lm(x)] Y_lm(k) (base.py:402-411), one fused R2C per m."""
@@ -577,12 +648,12 @@ class PolyBin3D:
         global {0: P_0 + L_2(mu) P_2 + L_4(mu) P_4}, local {0: P_0, 2: P_2, 4: P_4} (columns present
         in Pk_cov = [k, P_0 (, P_2, P_4)]), interpolated in FP64 with zero outside the table."""
         Pk_cov = np.asarray(Pk_cov, dtype=np.float64)
-        modk = self.modk_grid
+        modk = self.modk_half
         interp = lambda col: scipy.interpolate.interp1d(Pk_cov[0], Pk_cov[col], bounds_error=False,  # noqa: E731
                                                         fill_value=0.)(modk)
         if self.sightline == 'global':
             P = interp(1)
-            mu = self.muk_grid
+            mu = self.muk_half
             if len(Pk_cov) > 2:
                 P = P + 0.5 * (3 * mu ** 2 - 1) * interp(2)
             if len(Pk_cov) > 3:
@@ -610,6 +681,10 @@ class PolyBin3D:
                 t = torch.from_numpy(np.ascontiguousarray(g, dtype=np.float32)).to(self.device)
                 cache.append((g, t))
                 del cache[:-4]
+        if not self.real_fft and tuple(t.shape) == self.full_shape:
+            # a spectrum built on the full-grid modk / muk: its half-spectrum part (the kz = -n2/2 plane
+            # has the |k| and mu^2 of kz = +n2/2)
+            t = t[..., :self.half_shape[2]].contiguous()
         if tuple(t.shape) != self.half_shape:
             raise ValueError("a gridded spectrum must have the half-grid shape %s" % (self.half_shape,))
         return t

@@ -127,7 +127,7 @@ class BSpec:
             else:
                 assert len(Pk_fid) == 2, "Pk should contain k and P_0(k) columns"
                 Pfid = np.asarray(Pk_fid)
-            P = scipy.interpolate.interp1d(Pfid[0], Pfid[1], bounds_error=False, fill_value=0.)(b.modk_grid)
+            P = scipy.interpolate.interp1d(Pfid[0], Pfid[1], bounds_error=False, fill_value=0.)(b.modk_half)
             inv = np.zeros(P.shape)
             inv[P != 0] = 1. / P[P != 0]
         self.invPk0_host = inv
@@ -489,7 +489,7 @@ class BSpec:
         """g_{b,l} maps of one simulation (bspec.py:281-326)."""
         b = self.base
         if b.pixel_window != 'none':
-            sf = b.to_fourier(sim) if input_type == 'real' else b.as_fourier(sim)
+            sf = b.r2c(b.as_real(sim)) if input_type == 'real' else b.as_fourier(sim)
             pw = b.pixel_window_grid
             inv = torch.where(pw != 0, 1. / pw, torch.zeros_like(pw))
             sf = b.kmap(sf, C.kop(C.KOP_WEIGHT, w=inv.contiguous()))
@@ -515,7 +515,7 @@ class BSpec:
 
         def load_sim_data(ii):
             if self.const_mask:
-                sf = b.generate_data(int(1e5) + ii, Pk_input=Pk_input, output_type='fourier')
+                sf = b.grf_half(int(1e5) + ii, Pk_input=Pk_input, output_type='fourier')
                 sf = b.kmap(sf, C.kop(C.KOP_PLAIN, scale=self.mask_mean))
                 return self._process_sim(sf, input_type='fourier')
             sr = b.generate_data(int(1e5) + ii, Pk_input=Pk_input, output_type='real', include_pixel_window=False)
@@ -588,7 +588,7 @@ class BSpec:
         Legendre factor is the real-space sum_m Y_lm(x) IFFT[Theta_b3 Y_lm(k) FT_0(b1,b2)]
         (apply_real_harmonics, bspec.py:629-636), added as 2l+1 weighted sub-maps.  The number of maps
         per chain comes from the autotuner (knob bk_qmap_chunk), by default as many as fit 12 GiB of
-        work maps, at most 64."""
+        work maps, at most 64 (32 for chains of several sub-maps)."""
         ab = self.all_bins[self._qperm]  # rows of QA follow this order (sorted by l, then b3)
         t0 = 0
         while t0 < self.N_bins:
@@ -609,7 +609,9 @@ class BSpec:
         G = 2 * ell + 2 if ylm else 1
         per_map = int(b.gridsize[0]) * int(b.gridsize[1]) * min(b.half_shape[2], self._sup[2] + 1)
         fit = lambda budget: int(max(1, (budget << 30) // ((G + 1) * 8 * per_map)))  # noqa: E731
-        default = int(max(1, min(64, fit(12))))
+        # 64 maps per chain for single-sub-map chains; 32 when the harmonic sub-maps multiply the work
+        # maps (measured 2 % faster than 54 per chain for the local l = 2 chains at 256^3)
+        default = int(max(1, min(64 if G == 1 else 32, fit(12))))
         ab = self.all_bins[self._qperm]
         first = int(np.nonzero(ab[:, 3] == ell)[0][0])
         nl = int(np.sum(ab[:, 3] == ell))
@@ -703,8 +705,8 @@ class BSpec:
         if verb:
             print("Computing shot-noise g_{b,l}(r) maps")
         PkA = [b.Pfid[0], 1. / b.Pfid[1]]
-        a1 = b.generate_data(seed=seed + int(1e7), Pk_input=PkA, output_type='fourier')
-        a2 = b.generate_data(seed=seed + int(2e7), Pk_input=PkA, output_type='fourier')
+        a1 = b.grf_half(seed=seed + int(1e7), Pk_input=PkA, output_type='fourier')
+        a2 = b.grf_half(seed=seed + int(2e7), Pk_input=PkA, output_type='fourier')
         A1 = b.applyAinv(a1, invPk0_grid=b.Pk0_grid_host, input_type='fourier', output_type='real')
         A2 = b.applyAinv(a2, invPk0_grid=b.Pk0_grid_host, input_type='fourier', output_type='real')
         Sinv = lambda m, it: b.as_fourier(self.applySinv(m, input_type=it, output_type='fourier'))  # noqa: E731
@@ -819,8 +821,8 @@ class BSpec:
             self._QA = torch.empty((NB, M), dtype=torch.complex64, device=b.device)
         QA = self._QA
         # GRF pair (bspec.py:648-656)
-        a1 = b.generate_data(seed=seed + int(1e7), output_type='fourier')
-        a2 = b.generate_data(seed=seed + int(2e7), output_type='fourier')
+        a1 = b.grf_half(seed=seed + int(1e7), output_type='fourier')
+        a2 = b.grf_half(seed=seed + int(2e7), output_type='fourier')
         # A^-1 side: weighted maps, g maps, FT tables, Q maps (bspec.py:659-666)
         w = b.ainv_weight()
         w1 = b.kmap(a1, C.kop(C.KOP_WEIGHT, w=w)) if w is not None else a1

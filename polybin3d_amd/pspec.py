@@ -146,7 +146,7 @@ class PSpec:
             return b.r2c(b.as_real(data), kop=op)
         if b.pixel_window != 'none':
             df = b.r2c(b.as_real(data), kop=C.kop(C.KOP_WEIGHT, w=self._inv_pw()))
-            return self.applySinv(df, input_type='fourier', output_type='fourier')
+            return b.as_fourier(self.applySinv(df, input_type='fourier', output_type='fourier'))
         return b.as_fourier(self.applySinv(b.as_real(data), input_type='real', output_type='fourier'))
 
     def _inv_pw(self):
@@ -252,7 +252,7 @@ class PSpec:
             else:
                 assert len(Pk_fid) == 2, "Pk should contain k and P_0(k) columns"
                 Pfid = np.asarray(Pk_fid)
-            P = scipy.interpolate.interp1d(Pfid[0], Pfid[1], bounds_error=False, fill_value=0.)(b.modk_grid)
+            P = scipy.interpolate.interp1d(Pfid[0], Pfid[1], bounds_error=False, fill_value=0.)(b.modk_half)
             inv = np.zeros(P.shape)
             good = (P != 0) & np.isfinite(P)
             inv[good] = 1. / P[good]
@@ -340,7 +340,7 @@ class PSpec:
         if not hasattr(self, '_bins_dev'):
             self._bins_dev = torch.arange(Nk, dtype=torch.int32, device=b.device)
         # pspec.py:266-268
-        a = b.generate_data(seed=seed + int(1e7), output_type='fourier') if a_map_fourier is None else a_map_fourier
+        a = b.grf_half(seed=seed + int(1e7), output_type='fourier') if a_map_fourier is None else b.as_fourier(a_map_fourier)
         # A^-1 a (pspec.py:328)
         w = b.ainv_weight()
         Ainv = b.kmap(a, C.kop(C.KOP_WEIGHT, w=w)) if w is not None else a
@@ -413,7 +413,7 @@ class PSpec:
         if R not in self._bins_rep:
             self._bins_rep[R] = torch.arange(Nk, dtype=torch.int32, device=b.device).repeat(R)
         bins = self._bins_rep[R]
-        a = torch.stack([b.generate_data(seed=s + int(1e7), output_type='fourier') for s in seeds])
+        a = torch.stack([b.grf_half(seed=s + int(1e7), output_type='fourier') for s in seeds])
         w = b.ainv_weight()
         Ainv = b.kmap(a, C.kop(C.KOP_WEIGHT, w=w, x_batch_stride=H), batch=R) if w is not None else a
         if self.const_mask:
@@ -471,7 +471,7 @@ class PSpec:
         Nk, Nl = self.Nk, self.Nl
         F = np.zeros((self.N_bins, self.N_bins))
         Bm = np.zeros((self.N_bins, theory['N_bins'])) if theory is not None else None
-        a = b.generate_data(seed=seed + int(1e7), output_type='fourier')
+        a = b.grf_half(seed=seed + int(1e7), output_type='fourier')
         if not self.const_mask:
             a_real = b.c2r(a)
         def cov_filter(m, it):
@@ -698,13 +698,13 @@ class PSpec:
         PkA = [b.Pfid[0], 1. / b.Pfid[1]]
         if getattr(self, '_invPkA', None) is None:
             # 1/P_A on the grid, interpolated once (the weight is cached by the array's identity)
-            modk = b.modk_grid
+            modk = b.modk_half
             PkA_grid = scipy.interpolate.interp1d(PkA[0], PkA[1], bounds_error=False,
                                                   fill_value=(PkA[1][0], PkA[1][-1]))(modk)
             invPkA = np.zeros_like(PkA_grid)
             invPkA[modk != 0] = 1. / PkA_grid[modk != 0]
             self._invPkA = invPkA
-        a = b.generate_data(seed=seed + int(1e7), Pk_input=PkA, output_type='fourier')
+        a = b.grf_half(seed=seed + int(1e7), Pk_input=PkA, output_type='fourier')
         wA = b.ainv_weight(self._invPkA)
         AiA = b.kmap(a, C.kop(C.KOP_WEIGHT, w=wA))
         if self.user_Sinv:
