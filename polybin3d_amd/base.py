@@ -63,8 +63,8 @@ class PolyBin3D:
     """Base class (PolyBin3D/base.py:6-144) with backend='hip'.
 
     Inputs as in the reference: boxsize, gridsize, Pk=[k, P0(k)] (None = unit power), boxcenter,
-    pixel_window, backend (only 'hip'), nthreads (unused on the GPU), sightline, real_fft (must be
-    True).  Extra: device (default 'cuda'), rng ('numpy' reproduces the reference's legacy
+    pixel_window, backend (only 'hip'), nthreads (unused on the GPU), sightline, real_fft (False: the
+    reference's complex-grid conventions at the boundary; even n2 only).  Extra: device (default 'cuda'), rng ('numpy' reproduces the reference's legacy
     RandomState stream; 'device' draws on the GPU with the same distribution).
     """
 
@@ -230,29 +230,60 @@ class PolyBin3D:
             return None
         return np.asarray(1. + (self.k_grids_half[2] > 0), dtype=np.float64)
 
-    def to_full(self, x):
-        """Half-spectrum map(s) [..., n0, n1, n2/2+1] -> the Hermitian full grid [..., n0, n1, n2]."""
+    def to_full(self, x, parity=1):
+        """Half-spectrum map(s) [..., n0, n1, n2/2+1] -> the full grid [..., n0, n1, n2] with
+        X(-k) = parity * conj X(k): Hermitian (parity 1, the transform of a real map) or anti-Hermitian
+        (parity -1)."""
         n0, n1, n2 = self.full_shape
         h = n2 // 2 + 1
         out = torch.empty(tuple(x.shape[:-3]) + self.full_shape, dtype=x.dtype, device=x.device)
         out[..., :h] = x
-        # X[i, j, k] = conj X[-i, -j, n2 - k] for k > n2/2
+        # X[i, j, k] = parity * conj X[-i, -j, n2 - k] for k > n2/2
         src = x[..., 1:n2 - h + 1].conj()
         src = torch.roll(torch.flip(src, dims=(-3, -2, -1)), shifts=(1, 1), dims=(-3, -2))
-        out[..., h:] = src
+        out[..., h:] = src if parity > 0 else -src
         return out
 
+    def full_parts(self, x):
+        """Full-grid map(s) -> the half spectra (kz = 0 .. n2/2) of their Hermitian and anti-Hermitian
+        parts, (X(k) + conj X(-k)) / 2 and (X(k) - conj X(-k)) / 2."""
+        h = self.full_shape[2] // 2 + 1
+        neg = torch.roll(torch.flip(x, dims=(-3, -2, -1)), shifts=(1, 1, 1), dims=(-3, -2, -1))[..., :h].conj()
+        return (0.5 * (x[..., :h] + neg)).contiguous(), (0.5 * (x[..., :h] - neg)).contiguous()
+
     def from_full(self, x):
-        """Full-grid map(s) -> the half spectrum of their Hermitian part, (X(k) + conj X(-k)) / 2 for
-        kz = 0 .. n2/2: the map whose C2R is Re IFFT(X)."""
-        n2 = self.full_shape[2]
-        h = n2 // 2 + 1
-        neg = torch.roll(torch.flip(x, dims=(-3, -2, -1)), shifts=(1, 1, 1), dims=(-3, -2, -1))
-        return (0.5 * (x[..., :h] + neg[..., :h].conj())).contiguous()
+        """Full-grid map(s) -> the half spectrum of their Hermitian part: the map whose C2R is
+        Re IFFT(X)."""
+        return self.full_parts(x)[0]
+
+    def fourier_linear(self, x, f, parity):
+        """f (a k-space map on half spectra that multiplies mode k by a factor g(k) with
+        g(-k) = parity * conj g(k)) applied to a Fourier map in the caller's form.  A full grid
+        (real_fft=False) is split into its Hermitian and anti-Hermitian parts, each mapped on the half
+        spectrum and completed with its own symmetry, so that the result is f on every full-grid mode."""
+        x = x.to(device=self.device, dtype=torch.complex64) if torch.is_tensor(x) else x
+        if self.real_fft or not (tuple(np.shape(x)[-3:]) == self.full_shape):
+            return self.public_fourier(f(self.as_fourier(x)))
+        if not torch.is_tensor(x):
+            x = torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex64)).to(self.device)
+        hp, ap = self.full_parts(x.contiguous())
+        return self.to_full(f(hp), parity) + self.to_full(f(ap), -parity)
 
     def public_fourier(self, x):
         """A Fourier map as the caller sees it: the half spectrum, or the full grid with real_fft=False."""
         return x if self.real_fft else self.to_full(x)
+
+    def wrap_user_filter(self, f):
+        """A user S^-1 (applySinv / applySinv_transpose) as the reference calls it: with real_fft=False its
+        Fourier inputs are full grids (its outputs go through as_fourier, which takes either form)."""
+        if f is None or self.real_fft:
+            return f
+
+        def g(m, input_type='real', output_type='real'):
+            if input_type == 'fourier' and torch.is_tensor(m) and tuple(m.shape[-3:]) == self.half_shape:
+                m = self.to_full(m)
+            return f(m, input_type=input_type, output_type=output_type)
+        return g
 
     @property
     def r_grids(self):
@@ -599,6 +630,8 @@ class PolyBin3D:
             if arr is invPk0_grid:
                 return t
         inv = self.invPk0_grid_host if invPk0_grid is None else np.asarray(invPk0_grid)
+        if not self.real_fft and inv.shape == self.full_shape:
+            inv = inv[..., :self.half_shape[2]]  # a weight built on the full-grid |k|, even under k -> -k
         w = inv.copy()
         if self.pixel_window != 'none':
             pw = self.pixel_window_grid_host
@@ -616,6 +649,9 @@ class PolyBin3D:
         op = C.kop(C.KOP_WEIGHT, w=w) if w is not None else None
         if input_type == 'real':
             f = self.r2c(self.as_real(input_data), kop=op)
+        elif output_type == 'fourier':
+            # the weight is even under k -> -k; a full grid keeps its anti-Hermitian part too
+            return self.fourier_linear(input_data, lambda x: self.kmap(x, op) if op is not None else x.clone(), 1)
         else:
             f = self.as_fourier(input_data)
             f = self.kmap(f, op) if op is not None else f.clone()
@@ -691,6 +727,12 @@ class PolyBin3D:
 
     def apply_xi(self, input_map_fourier, input_map_real=None, Ylm_real=None, Ylm_fourier=None, Pk_grid={},
                  output_type='real'):
+        """apply_xi_half with the caller's Fourier form (the full grid with real_fft=False)."""
+        out = self.apply_xi_half(input_map_fourier, input_map_real, Ylm_real, Ylm_fourier, Pk_grid, output_type)
+        return self.public_fourier(out) if output_type == 'fourier' else out
+
+    def apply_xi_half(self, input_map_fourier, input_map_real=None, Ylm_real=None, Ylm_fourier=None, Pk_grid={},
+                      output_type='real'):
         """xi(x, y) applied symmetrically for a gridded spectrum (base.py:413-437), Pk_grid a dict of half
         grids (numpy, as the reference's callers build them, or device tensors, e.g. from pk_grid):
         P_0 a + sum_l [P_l sum_m Y_lm(k) FFT[Y_lm a_real] + sum_m Y_lm(x) IFFT[Y_lm(k) P_l a]] / 2 on
@@ -756,28 +798,57 @@ class IntegrationUtils:
     def __init__(self, base):
         self.base = base
 
-    def cross_integrate(self, spec1, spec2, k_edges, l):
+    def _parts(self, spec):
+        """(Hermitian, anti-Hermitian) half spectra of a full grid (real_fft=False), else (spec, None)."""
         b = self.base
-        return b.bin_reduce(b.as_fourier(spec1), k_edges, [l], y=b.as_fourier(spec2))[0, 0].cpu().numpy()
+        if not b.real_fft and tuple(np.shape(spec)[-3:]) == b.full_shape:
+            x = spec if torch.is_tensor(spec) else torch.from_numpy(np.ascontiguousarray(spec, dtype=np.complex64))
+            return b.full_parts(x.to(device=b.device, dtype=torch.complex64).contiguous())
+        return b.as_fourier(spec), None
+
+    def _pairs(self, spec1, spec2, l):
+        """Half-spectrum pairs whose degk-weighted sums of Re[x conj y] L_l add up to the full-grid sum of
+        Re[s1 conj s2] L_l: with H, A the Hermitian and anti-Hermitian parts, Re[H1 conj H2] and
+        Re[A1 conj A2] are even under k -> -k and the cross terms odd, so an even L_l keeps the former
+        and an odd L_l the latter (the half spectrum of a real_fft=True map is its own H)."""
+        h1, a1 = self._parts(spec1)
+        h2, a2 = self._parts(spec2)
+        if a1 is None and a2 is None:
+            return [(h1, h2)]
+        pairs = [(h1, h2), (a1, a2)] if l % 2 == 0 else [(h1, a2), (a1, h2)]
+        return [(x, y) for x, y in pairs if x is not None and y is not None]
+
+    def cross_integrate(self, spec1, spec2, k_edges, l):
+        """sum_{k in bin} degk Re[s1 conj s2] L_l(mu) (utils.pyx:51-78)."""
+        b = self.base
+        out = np.zeros(len(k_edges) - 1)
+        for x, y in self._pairs(spec1, spec2, l):
+            out = out + b.bin_reduce(x, k_edges, [l], y=y)[0, 0].cpu().numpy()
+        return out
 
     def integrate(self, spec, k_edges, l):
+        """sum_{k in bin} degk |s|^2 L_l(mu) (utils.pyx:80-107)."""
         b = self.base
-        return b.bin_reduce(b.as_fourier(spec), k_edges, [l])[0, 0].cpu().numpy()
+        if b.real_fft or tuple(np.shape(spec)[-3:]) != b.full_shape:
+            return b.bin_reduce(b.as_fourier(spec), k_edges, [l])[0, 0].cpu().numpy()
+        return self.cross_integrate(spec, spec, k_edges, l)
 
     def real_integrate_double(self, spec, l1, l2, k_edges):
         b = self.base
         s = spec if torch.is_tensor(spec) else torch.from_numpy(np.asarray(spec, dtype=np.float32))
         s = s.to(device=b.device, dtype=torch.float32).contiguous()
+        if not b.real_fft and tuple(s.shape[-3:]) == b.full_shape:
+            s = s[..., :b.half_shape[2]].contiguous()  # a real spectrum symmetric under k -> -k
         return b.bin_reduce(s, k_edges, [l1], ells2=[l2], real=True)[0, 0].cpu().numpy()
 
     def integrate_row(self, input_map, input_array, kmin, kmax):
         """out[j] = sum_{kmin<=k<kmax} degk Re[conj(m) Q_j] for every map Q_j (utils.pyx:162-179)."""
         b = self.base
-        m = b.as_fourier(input_map)
-        Q = b.as_fourier(input_array)
-        nq = Q.shape[0]
-        r = b.bin_reduce(Q, [kmin, kmax], [0], y=m, nq=nq, y_qstride=0)
-        return r[:, 0, 0].cpu().numpy()
+        out = 0.
+        for Q, m in self._pairs(input_array, input_map, 0):
+            nq = Q.shape[0]
+            out = out + b.bin_reduce(Q, [kmin, kmax], [0], y=m, nq=nq, y_qstride=0)[:, 0, 0].cpu().numpy()
+        return out
 
 
 class MapUtils:
@@ -788,19 +859,24 @@ class MapUtils:
 
     def fourier_filter(self, kmap, l, kmin, kmax):
         b = self.base
-        t = b.bin_table([kmin, kmax])
-        return b.kmap(b.as_fourier(kmap), C.kop(C.KOP_FILTER, bin_table=t, ell=l))
+        op = C.kop(C.KOP_FILTER, bin_table=b.bin_table([kmin, kmax]), ell=l)
+        return b.fourier_linear(kmap, lambda x: b.kmap(x, op), (-1) ** l)
 
     def fourier_filter_ll(self, kmap, l1, l2, kmin, kmax):
         b = self.base
-        t = b.bin_table([kmin, kmax])
-        return b.kmap(b.as_fourier(kmap), C.kop(C.KOP_FILTER, bin_table=t, ell=l1, ell2=l2))
+        op = C.kop(C.KOP_FILTER, bin_table=b.bin_table([kmin, kmax]), ell=l1, ell2=l2)
+        return b.fourier_linear(kmap, lambda x: b.kmap(x, op), (-1) ** (l1 + l2))
 
     def prod_fourier(self, map1, map2):
+        """map1 times a real k-space weight that is even under k -> -k (as the reference's callers'
+        |k|- and mu^2-dependent grids are)."""
         b = self.base
         w = map2 if torch.is_tensor(map2) else torch.from_numpy(np.asarray(map2, dtype=np.float32))
         w = w.to(device=b.device, dtype=torch.float32).contiguous()
-        return b.kmap(b.as_fourier(map1), C.kop(C.KOP_WEIGHT, w=w))
+        if not b.real_fft and tuple(w.shape[-3:]) == b.full_shape:
+            w = w[..., :b.half_shape[2]].contiguous()
+        op = C.kop(C.KOP_WEIGHT, w=w)
+        return b.fourier_linear(map1, lambda x: b.kmap(x, op), 1)
 
     def prod_real(self, map1, map2):
         return self.base.as_real(map1) * self.base.as_real(map2)

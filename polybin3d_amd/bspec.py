@@ -113,10 +113,11 @@ class BSpec:
         if applySinv is None:
             self.applySinv = b.applySinv_trivial
         else:
-            self.applySinv = applySinv
+            self.applySinv = b.wrap_user_filter(applySinv)
             if not callable(applySinv_transpose):
                 applySinv_transpose = applySinv
-        self.applySinv_transpose = b.applySinv_trivial if applySinv_transpose is None else applySinv_transpose
+        self.applySinv_transpose = (b.applySinv_trivial if applySinv_transpose is None
+                                    else b.wrap_user_filter(applySinv_transpose))
         # fiducial weighting (bspec.py:138-156); Pk_fid=None means unit weights with invPk0(0)=0
         if isinstance(Pk_fid, str) and Pk_fid == 'default':
             inv = b.invPk0_grid_host

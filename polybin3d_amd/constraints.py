@@ -131,12 +131,12 @@ def covariance_filter(spec, m, input_type, Pk_grid, noise_mask):
     b = spec.base
     if spec.const_mask:
         Sid = b.as_fourier(spec.applySinv_transpose(m, input_type=input_type, output_type='fourier'))
-        CSid = b.apply_xi(Sid, None, None, None, Pk_grid, 'fourier')
+        CSid = b.apply_xi_half(Sid, None, None, None, Pk_grid, 'fourier')
         CSid.mul_(spec.mask_mean ** 2).add_(Sid, alpha=spec.mask_mean)
         return b.as_fourier(spec.applySinv(CSid, input_type='fourier', output_type='fourier'))
     Sid = b.as_real(spec.applySinv_transpose(m, input_type=input_type, output_type='real'))
     PdSid_real = spec._P.apply(Sid, transpose=True)
-    CPdSid = b.apply_xi(b.r2c(PdSid_real), PdSid_real, None, None, Pk_grid, 'real')
+    CPdSid = b.apply_xi_half(b.r2c(PdSid_real), PdSid_real, None, None, Pk_grid, 'real')
     Cov = spec._P.apply(CPdSid)
     Cov.addcmul_(Sid, noise_mask)
     return b.as_fourier(spec.applySinv(Cov, input_type='real', output_type='fourier'))

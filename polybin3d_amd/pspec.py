@@ -105,10 +105,11 @@ class PSpec:
         if applySinv is None:
             self.applySinv = base.applySinv_trivial
         else:
-            self.applySinv = applySinv
+            self.applySinv = base.wrap_user_filter(applySinv)
             if not callable(applySinv_transpose):
                 applySinv_transpose = applySinv
-        self.applySinv_transpose = base.applySinv_trivial if applySinv_transpose is None else applySinv_transpose
+        self.applySinv_transpose = (base.applySinv_trivial if applySinv_transpose is None
+                                    else base.wrap_user_filter(applySinv_transpose))
         if base.sightline == 'local' and self.lmax > 0:
             self.Ylm_real = harmonics_tags(lmax, odd_l, 'real')
             self.Ylm_fourier = harmonics_tags(lmax, odd_l, 'fourier')

@@ -16,14 +16,21 @@ import numpy as np
 TOL = 1e-5
 
 
+def _as_array(x):
+    """float64, or complex128 for complex data (so that a comparison keeps the imaginary parts)."""
+    x = np.asarray(x)
+    return x.astype(np.complex128 if np.iscomplexobj(x) else np.float64)
+
+
 def scaled_error(got, ref, scale):
-    got, ref = np.asarray(got, dtype=np.float64), np.asarray(ref, dtype=np.float64)
+    got, ref = _as_array(got), _as_array(ref)
     return float(np.max(np.abs(got - ref) / np.maximum(np.asarray(scale, dtype=np.float64), 1e-300)))
 
 
 def check(what, got, ref, scale=None, tol=TOL):
-    """Assert max |got - ref| / scale < tol (scale = |ref| by default) and print the error."""
-    ref = np.asarray(ref, dtype=np.float64)
+    """Assert max |got - ref| / scale < tol (scale = |ref| by default; complex data compared by the
+    modulus of the difference) and print the error."""
+    ref = _as_array(ref)
     s = np.abs(ref) if scale is None else scale
     err = scaled_error(got, ref, s)
     print("[parity] %s: max scaled error %.2e (tol %.1e)" % (what, err, tol), flush=True)
