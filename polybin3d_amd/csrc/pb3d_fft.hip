@@ -157,6 +157,22 @@ __device__ __forceinline__ void pencil_fft(int t, Src0 src0, Lds lds, DstN dstN,
 #ifndef PB3D_COL_THREADS
 #define PB3D_COL_THREADS 1024
 #endif
+// Streaming stores of whole-map pass outputs (each far larger than L2 and the MALL; the next pass
+// reads them from HBM): 1 = non-temporal stores.  Used by the plain and first column passes, the
+// fused rows and the FT-table rows, whose stores cover whole rows; not by the scattered stores of the
+// packed epilogue, which non-temporal stores slow down.
+#ifndef PB3D_NT_STORE
+#define PB3D_NT_STORE 1
+#endif
+__device__ __forceinline__ void stream_store(float2* p, float2 v) {
+#if PB3D_NT_STORE
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  f2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+#else
+  *p = v;
+#endif
+}
 // plane-aligned y-pass tiles (see launch_cols)
 #ifndef PB3D_COL_ALIGN
 #define PB3D_COL_ALIGN 1
@@ -436,7 +452,7 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
       if (kop.accumulate) v = cadd(v, dst[od]);
       dst[od] = v;
     } else {
-      dst[bout + p * cm.out_S] = v;
+      stream_store(dst + bout + p * cm.out_S, v);
     }
   };
   struct LdsRW {
@@ -780,7 +796,7 @@ r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows
     const int f = tid + it * NT;
     const int rr = f / (M + 1), k = f % (M + 1);
     if (f >= T * (M + 1) || row0 + rr >= nrows || k >= lz_out) continue;
-    o[(long long)rr * pitch + k] = r2c_split<M>(buf, LDT, rr, k, stwN);
+    stream_store(o + (long long)rr * pitch + k, r2c_split<M>(buf, LDT, rr, k, stwN));
   }
 }
 
@@ -1244,7 +1260,7 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   // consecutive entries of every row of the wave)
   if (row < rows_per_map) {
     float2* o = out + t * slot_out + row * pitch_out;
-    for (int k = tp; k < lz_out; k += P) o[k] = r2c_split<M, PADQ>(buf, LDT, r, k, stwN);
+    for (int k = tp; k < lz_out; k += P) stream_store(o + k, r2c_split<M, PADQ>(buf, LDT, r, k, stwN));
   }
 }
 
