@@ -158,9 +158,10 @@ __device__ __forceinline__ void pencil_fft(int t, Src0 src0, Lds lds, DstN dstN,
 #define PB3D_COL_THREADS 1024
 #endif
 // Streaming stores of whole-map pass outputs (each far larger than L2 and the MALL; the next pass
-// reads them from HBM): 1 = non-temporal stores.  Used by the plain and first column passes, the
-// fused rows and the FT-table rows, whose stores cover whole rows; not by the scattered stores of the
-// packed epilogue, which non-temporal stores slow down.
+// reads them from HBM): 1 = non-temporal stores.  Used by the plain and first column passes and the
+// FT-table rows, whose stores cover whole rows.  Not by the scattered stores of the packed epilogue,
+// which non-temporal stores slow down, nor by the fused rows, whose rows of lz_out entries start off
+// the 64-byte boundaries: there the non-temporal stores wrote 1.3 times the bytes for the same time.
 #ifndef PB3D_NT_STORE
 #define PB3D_NT_STORE 1
 #endif
@@ -1260,7 +1261,7 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   // consecutive entries of every row of the wave)
   if (row < rows_per_map) {
     float2* o = out + t * slot_out + row * pitch_out;
-    for (int k = tp; k < lz_out; k += P) stream_store(o + k, r2c_split<M, PADQ>(buf, LDT, r, k, stwN));
+    for (int k = tp; k < lz_out; k += P) o[k] = r2c_split<M, PADQ>(buf, LDT, r, k, stwN);
   }
 }
 
