@@ -229,3 +229,35 @@ def test_full_grid_odd_l(hip):
     close(n1, n2, scale=np.abs(n2[:ps.Nk]).max())
     F1, F2 = ps.compute_fisher_contribution(1), po.compute_fisher_contribution(1)
     close(F1, F2, scale=fisher_scale(F2))
+
+
+def test_full_grid_bk_user_sinv(hip):
+    """BSpec MC Fisher with a user S^-1 written for full grids (the generic path: the callable runs on
+    every Q map, bspec.py:499-505) and real_fft=False, against the oracle (real_fft=True) with the same
+    filter on the half spectrum."""
+    import polybin3d_amd as pb
+    n = 16
+    edges = np.linspace(0.007, 0.05, 4)
+    kw = dict(boxcenter=[10., 10., 10.], sightline='local')
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b = pb.PolyBin3D(L, [n] * 3, Pk=Pk, real_fft=False, **kw)
+    o = O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    mask = 0.4 + np.linspace(0, 1, n)[None, :, None] * np.ones((n, n, n))
+    wk_o = 1. / (1. + (o.modk_grid / 0.02) ** 2)
+    wk_full = torch.from_numpy((1. / (1. + (b.modk_grid / 0.02) ** 2)).astype(np.float32)).cuda()
+    seen = set()
+
+    def sinv_o(m, input_type='real', output_type='fourier'):
+        f = (o.to_fourier(m) if input_type == 'real' else m) * wk_o
+        return f if output_type == 'fourier' else o.to_real(f)
+
+    def sinv_g(m, input_type='real', output_type='fourier'):
+        if input_type == 'fourier':
+            seen.add(tuple(m.shape[-3:]))
+        f = (b.to_fourier(m) if input_type == 'real' else m) * wk_full
+        return f if output_type == 'fourier' else b.to_real(f)
+    bs = pb.BSpec(b, edges, lmax=2, mask=mask, applySinv=sinv_g)
+    bo = O.BSpec(o, edges, lmax=2, mask=mask, applySinv=sinv_o)
+    F1, F2 = bs.compute_fisher_contribution(3), bo.compute_fisher_contribution(3)
+    close(F1, F2, scale=fisher_scale(F2))
+    assert seen <= {(n, n, n)}
