@@ -3,17 +3,21 @@
 The metric compares two legs measured in the same run.  The timed (reported) leg: one step = R
 Monte Carlo realizations (--realizations, default 4) of both Fisher matrices on one GPU, the R Pk
 seeds sharing every pass and the Bk Fisher contraction on the matrix cores in split BF16
-(pb3d_fisher_contract, accumulating into the FP64 sum); the plain leg: one realization at a time with
-the Bk contraction as an FP32 GEMM (hipBLASLt).  vs_baseline = timed / plain.  Each realization:
+(pb3d_fisher_contract, accumulating into the FP64 sum), the Bk Q-map chains dealt to --streams HIP
+streams (default 4); the plain leg: one realization at a time on one stream with the Bk contraction as
+an FP32 GEMM (hipBLASLt).  vs_baseline = timed / plain.  Each realization:
   * PSpec MC Fisher: 256^3, box 1000 Mpc/h, local (Yamamoto) sightline, lmax=4, 30 bins 0.01-0.30 h/Mpc;
   * BSpec MC Fisher: same grid, lmax=2, 40 bins 0.01-0.40 h/Mpc (12700 triangle bins);
   * mask: shell 300 <= r <= 480 Mpc/h minus 200 random holes (r = 10-30 Mpc/h), smoothed 5 Mpc/h;
   * GRFs drawn on the GPU (power law P(k) = 2e4 (k/0.1)^-1.5), Fisher rows accumulated in FP64.
-Prints one JSON line with the throughput, the roofline of the dominant kernel (sampled HIP-event
-timing over the timed region) and a CPU baseline (the FP64 oracle's unit operations timed on this
-host and scaled by the reference's exact per-realization operation counts).
+Prints one JSON line with the throughput, the roofline of the dominant kernel (HIP events around
+every launch of one single-stream realization of the timed path, run right after the timed steps:
+under the multi-stream schedule launches overlap, so their durations would not time one kernel) and a
+CPU baseline (the FP64 oracle's unit operations timed on this host and scaled by the reference's exact
+per-realization operation counts).
 
-usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 256] [--realizations R] [--plain-steps P] [--no-cpu]
+usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 256] [--realizations R] [--plain-steps P]
+                      [--streams S] [--no-cpu]
 """
 import argparse
 import ctypes
@@ -177,9 +181,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    # odd sampling period: launches of one kind come in patterns of even length (sub-maps, y passes of
-    # both directions), which an even period would sample at one phase only
-    ap.add_argument("--prof-period", type=int, default=7)
+    # sampling period of the launch timer over the roofline leg (1: every launch of that realization;
+    # a larger period should be odd: launches of one kind come in patterns of even length)
+    ap.add_argument("--prof-period", type=int, default=1)
     ap.add_argument("--workload", default="mc_fisher", choices=["mc_fisher", "bk_ideal", "pk_mc_fisher", "extras"],
                     help="mc_fisher: the Pk+Bk MC Fisher metric (default); bk_ideal: Bk_ideal evaluations "
                          "and the MFMA triplet contraction (--n, --bk-bins, --lmax, --precision); pk_mc_fisher: "
@@ -202,6 +206,9 @@ def main():
     ap.add_argument("--bk-kmax", type=float, default=0.17)
     ap.add_argument("--lmax", type=int, default=0)
     ap.add_argument("--precision", default="bf16x6", choices=["fp32", "bf16x3", "bf16x6"])
+    ap.add_argument("--streams", type=int, default=4,
+                    help="mc_fisher: HIP streams the timed leg deals the Bk Q-map chains to (BSpec chain_streams; "
+                         "the plain and roofline legs use one)")
     ap.add_argument("--rows-mode", type=int, default=1, choices=[0, 1],
                     help="fused Q-map rows: 1 = harmonic sub-maps combined per row before the row transforms "
                          "(default), 0 = one row transform per sub-map (both legs)")
@@ -233,7 +240,7 @@ def main():
     base = pb.PolyBin3D(L, [n] * 3, Pk=powerlaw_pk(2 * kny * 1.8), boxcenter=center, sightline='local', rng='device')
     mask = shell_mask(n, L, center)
     ps = pb.PSpec(base, pk_bins, lmax=4, mask=mask)
-    bs = pb.BSpec(base, bk_bins, lmax=2, mask=mask, precision=args.precision)
+    bs = pb.BSpec(base, bk_bins, lmax=2, mask=mask, precision=args.precision, chain_streams=args.streams)
     bs._compute_symmetry_factor()
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
@@ -243,11 +250,20 @@ def main():
     def plain_step(seed):
         """The plain FP32 single-realization path: one Pk seed, then one Bk seed whose Fisher
         contraction is an FP32 GEMM (hipBLASLt), each Fisher formed on its own and then accumulated."""
-        prec = bs.precision
-        bs.precision = 'fp32'
+        prec, cs = bs.precision, bs.chain_streams
+        bs.precision, bs.chain_streams = 'fp32', 1
         Fp.add_(ps._fisher_contribution_dev(seed))
         Fb.add_(bs._fisher_contribution_dev(seed))
-        bs.precision = prec
+        bs.precision, bs.chain_streams = prec, cs
+
+    def roofline_step(seed):
+        """One realization of the timed path with every chain on the current stream, so that each
+        sampled launch runs alone and its duration times that kernel."""
+        cs = bs.chain_streams
+        bs.chain_streams = 1
+        Fp.add_(ps._fisher_contribution_dev(seed))
+        bs._fisher_contribution_dev(seed, out=Fb)
+        bs.chain_streams = cs
 
     graphs = None
 
@@ -288,7 +304,6 @@ def main():
         batched_step(10 ** 6 + w * R)
         print("warmup step %d done" % w, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    C.check(C.lib().pb3d_prof_enable(args.prof_period))
     t0 = time.perf_counter()
     for s in range(args.steps):
         batched_step(s * R)
@@ -296,6 +311,13 @@ def main():
         print("timed step %d done (%.2f s)" % (s, time.perf_counter() - t0), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    # leg 3: the launch sampler (HIP events on each launch's stream) over one single-stream realization
+    # of the timed path: with the chains dealt to several streams, launches of different chains overlap
+    # and a launch's duration no longer times one kernel
+    C.check(C.lib().pb3d_prof_enable(args.prof_period))
+    roofline_step(3 * 10 ** 7)
+    torch.cuda.synchronize()
+    print("roofline leg done", file=sys.stderr, flush=True)
     kinds = {}
     for kind in sorted(KIND_NAMES):
         cnt, ms, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
@@ -323,6 +345,7 @@ def main():
                 "bound_from": "rocprofv3 SQ_INSTS_VALU and FETCH_SIZE/WRITE_SIZE of %s on this workload "
                               "(profiles/) with this run's launch durations" % kname,
                 "kernel": kname, "pass": KIND_NAMES[dom],
+                "measured_on": "one single-stream realization of the timed path, right after the timed steps",
                 "avg_launch_ms": round(ms / cnt, 4), "bytes_per_launch": by / cnt,
                 "sampled_launches": cnt, "share_of_sampled_fft_time": round(ms / sum(v[1] for v in kinds.values()), 3),
                 "passes": {KIND_NAMES[k]: {"sampled_launches": v[0], "avg_launch_ms": round(v[1] / v[0], 4),
@@ -338,7 +361,7 @@ def main():
                       "bk": {"lmax": 2, "nbins": len(bk_bins) - 1, "kmax": float(bk_bins[-1]), "N_bins": bs.N_bins},
                       "mask": "shell 300-480 Mpc/h, 200 holes, 5 Mpc/h smoothing", "rng": "device",
                       "realizations_per_step": R, "pk_seeds_per_pass": R, "hip_graph": bool(args.graph),
-                      "pointing_rows_mode": args.rows_mode,
+                      "pointing_rows_mode": args.rows_mode, "bk_chain_streams": args.streams,
                       "bk_fisher_contraction": args.precision + " MFMA (split BF16, FP32 runs, FP64 store)",
                       "fisher_accumulation": "fp64", "setup_s": round(t_setup, 1)},
            "vs_baseline_is": "value / plain FP32 single-realization leg measured in this run (the metric's "

@@ -354,11 +354,14 @@ class PolyBin3D:
         return tuple(int(s) for s in size) + base
 
     def work(self, name, nbytes):
-        """Reusable device scratch (stream-ordered use only)."""
-        b = self._bufs.get(name)
+        """Reusable device scratch (stream-ordered use only).  Work that runs on a side stream sets
+        _work_lane to that stream's index, so that it gets buffers of its own."""
+        lane = getattr(self, '_work_lane', 0)
+        key = name if not lane else (name, lane)
+        b = self._bufs.get(key)
         if b is None or b.numel() < nbytes:
             b = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=self.device)
-            self._bufs[name] = b
+            self._bufs[key] = b
         return b
 
     def stream(self):

@@ -45,11 +45,16 @@ def triangle_bins(k_bins, k_bins_squeeze, lmax, include_partial_triangles):
 
 
 class BSpec:
-    """Bispectrum estimators (PolyBin3D/bspec.py:12-1119)."""
+    """Bispectrum estimators (PolyBin3D/bspec.py:12-1119).
+
+    Extra keywords: precision ('bf16x6', 'bf16x3' or 'fp32': the triplet and Fisher contractions) and
+    chain_streams (HIP streams over which the MC Fisher's Q-map chains are dealt; 1 = all on the
+    current stream).  Neither changes the result beyond FP32 rounding order."""
 
     def __init__(self, base, k_bins, lmax=0, mask=None, applySinv=None, k_bins_squeeze=None,
                  include_partial_triangles=False, add_GIC=False, mask_IC=None, add_RIC=False, radial_bins_RIC=[],
-                 Pk_fid=None, mask_shot=None, applySinv_transpose=None, verb=False, precision='bf16x6'):
+                 Pk_fid=None, mask_shot=None, applySinv_transpose=None, verb=False, precision='bf16x6',
+                 chain_streams=4):
         b = self.base = base
         self.k_bins = np.asarray(k_bins, dtype=np.float64)
         self.lmax = lmax
@@ -58,6 +63,7 @@ class BSpec:
         self.include_partial_triangles = include_partial_triangles
         self.Pk_fid = Pk_fid
         self.precision = precision
+        self.chain_streams = max(1, int(chain_streams))
         assert self.lmax % 2 == 0, "l-max must be even!"
         assert np.max(self.k_bins) < np.min(b.kNy), "k_max must be less than k_Nyquist!"
         assert np.min(self.k_bins) >= np.max(b.kF), "k_min must be at least the k_fundamental!"
@@ -591,6 +597,8 @@ class BSpec:
         per chain comes from the autotuner (knob bk_qmap_chunk), by default as many as fit 12 GiB of
         work maps, at most 64 (32 for chains of several sub-maps)."""
         ab = self.all_bins[self._qperm]  # rows of QA follow this order (sorted by l, then b3)
+        if self.chain_streams > 1 and not torch.cuda.is_current_stream_capturing():
+            return self._q_ainv_streams(FT, pair_tab, QA, ab, self.chain_streams)
         t0 = 0
         while t0 < self.N_bins:
             ell = int(ab[t0, 3])
@@ -600,6 +608,42 @@ class BSpec:
                 t1 += 1
             self._q_chunk(FT, pair_tab, QA, t0, t1)
             t0 = t1
+
+    def _q_ainv_streams(self, FT, pair_tab, QA, ab, nstreams):
+        """_q_ainv with its chains dealt round-robin to nstreams HIP streams: the current one and side
+        streams at the default priority, each with work maps of its own (base.work lanes).  The chains
+        are independent (disjoint QA rows), so a VALU-bound row pass of one chain runs beside the
+        memory-bound column passes of another.  The first chain runs alone on the current stream, so
+        that whatever a chain builds once (tables, integral-constraint maps) exists before the side
+        streams start; the current stream waits for every side stream at the end."""
+        b = self.base
+        chunks, t0 = [], 0
+        while t0 < self.N_bins:
+            ell = int(ab[t0, 3])
+            CH = self._qmap_chunk(FT, pair_tab, QA, ell)  # tuning, if any, runs here on the current stream
+            t1 = t0
+            while t1 < self.N_bins and t1 - t0 < CH and ab[t1, 3] == ell:
+                t1 += 1
+            chunks.append((t0, t1))
+            t0 = t1
+        self._q_chunk(FT, pair_tab, QA, *chunks[0])
+        cur = torch.cuda.current_stream(b.device)
+        side = getattr(self, '_side_streams', None)
+        if side is None or len(side) != nstreams - 1 or side[0].device != b.device:
+            side = self._side_streams = [torch.cuda.Stream(device=b.device) for _ in range(nstreams - 1)]
+        streams = [cur] + side
+        for s in side:
+            s.wait_stream(cur)
+        try:
+            for i, (t0, t1) in enumerate(chunks[1:]):
+                lane = (i + 1) % nstreams
+                b._work_lane = lane
+                with torch.cuda.stream(streams[lane]):
+                    self._q_chunk(FT, pair_tab, QA, t0, t1)
+        finally:
+            b._work_lane = 0
+            for s in side:
+                cur.wait_stream(s)
 
     def _qmap_chunk(self, FT, pair_tab, QA, ell):
         """Q maps per fused chain for multipole ell: the tuned choice (timed per map on the first maps

@@ -34,6 +34,18 @@ class FisherGraph:
             raise ValueError("FisherGraph: out must be an FP64 [N_bins, N_bins] tensor on the spec's device")
         self.spec, self.out, self.R = spec, out, max(1, int(seeds_per_replay))
         self.seed = torch.zeros(1, dtype=torch.int64, device=b.device)
+        # the warm-up pass sizes the work buffers that the capture then uses: both run every chain on one
+        # stream (a capture stays on the capturing stream)
+        self._streams = getattr(spec, 'chain_streams', None)
+        if self._streams is not None:
+            spec.chain_streams = 1
+        try:
+            self._capture(spec, b, out)
+        finally:
+            if self._streams is not None:
+                spec.chain_streams = self._streams
+
+    def _capture(self, spec, b, out):
         scratch = torch.zeros_like(out)
         with self._device_seed():
             self._pass(scratch)
