@@ -269,16 +269,16 @@ def main():
 
     def batched_step(s0):
         """R realizations: the R Pk seeds share every pass (batched), the Bk seeds run back to back with
-        the split-BF16 MFMA contraction accumulating straight into the FP64 Fisher sum; with --graph each
-        of the two passes is one HIP-graph replay."""
+        the split-BF16 MFMA contraction accumulating straight into the FP64 Fisher sum (each on a stream
+        of its own, beside the next seed's maps and tables); with --graph each of the two passes is one
+        HIP-graph replay."""
         if graphs is not None:
             graphs[0].replay(s0)
             graphs[1].replay(s0)
             return
         seeds = list(range(s0, s0 + R))
         Fp.add_(ps._fisher_contributions_dev(seeds) if R > 1 else ps._fisher_contribution_dev(seeds[0]))
-        for sd in seeds:
-            bs._fisher_contribution_dev(sd, out=Fb)
+        bs._fisher_contributions_dev(seeds, Fb)
 
     # leg 1: the plain FP32 single-realization path (the metric's comparison point)
     plain = None

@@ -841,9 +841,37 @@ class BSpec:
         assert len(Pk_cov) in [2, 3, 4], "Pk should contain k and P_0 (and optionally P_2, P_4) columns"
         return self._fisher_contribution_dev(seed, cov=(b.pk_grid(Pk_cov), noise)).cpu().numpy()
 
-    def _fisher_contribution_dev(self, seed, cov=None, out=None):
+    def _contraction_stream(self):
+        s = getattr(self, '_cstream', None)
+        if s is None or s.device != self.base.device:
+            s = self._cstream = torch.cuda.Stream(device=self.base.device)  # default priority
+        return s
+
+    def _fisher_contributions_dev(self, seeds, out):
+        """out += the Fisher contributions of `seeds` (FP64 device tensor).  With the MFMA contraction,
+        each realization's contraction runs on a stream of its own, beside the next realization's GRFs,
+        g maps and FT tables; that realization's Q maps (which overwrite the Q-map buffer) wait for it.
+        The current stream waits for the last contraction before returning."""
+        b = self.base
+        seeds = list(seeds)
+        if self.precision == 'fp32' or len(seeds) < 2 or torch.cuda.is_current_stream_capturing():
+            for sd in seeds:
+                self._fisher_contribution_dev(sd, out=out)
+            return out
+        cur = torch.cuda.current_stream(b.device)
+        defer = {'stream': self._contraction_stream(), 'done': None}
+        try:
+            for sd in seeds:
+                self._fisher_contribution_dev(sd, out=out, defer=defer)
+        finally:
+            cur.wait_stream(defer['stream'])
+        return out
+
+    def _fisher_contribution_dev(self, seed, cov=None, out=None, defer=None):
         """One realization's Fisher contribution as an FP64 device tensor [N_bins, N_bins] (added to
-        `out` when given)."""
+        `out` when given).  defer (from _fisher_contributions_dev): run the MFMA contraction on
+        defer['stream'] and leave its completion event in defer['done']; the Q maps of the next
+        realization wait for that event."""
         b = self.base
         if not hasattr(self, 'sym_factor'):
             self._compute_symmetry_factor()
@@ -877,6 +905,8 @@ class BSpec:
         self._ft_tables(G1, G2, self._ainv_pairs, FT)
         del G1, G2
         filt = None if cov is None else (lambda m, it: covariance_filter(self, m, it, cov[0], cov[1]))  # noqa: E731
+        if defer is not None and defer['done'] is not None:
+            torch.cuda.current_stream(b.device).wait_event(defer['done'])  # the previous contraction reads QA
         if filt is not None or self.user_Sinv:
             self._q_ainv_generic(FT, self._ainv_tab, QA, filt=filt)
         else:
@@ -900,7 +930,22 @@ class BSpec:
         # contraction: F[i, j] = sum_{k in b2(i)} degk Re[conj(L_i) QA_j]  (integrate_row, bspec.py:597)
         if self.precision != 'fp32':
             F = torch.zeros((NB, NB), dtype=torch.float64, device=b.device) if out is None else out
-            self._contract_mfma(LT, QA, F)
+            if defer is None:
+                self._contract_mfma(LT, QA, F)
+                return F
+            cur, cs = torch.cuda.current_stream(b.device), defer['stream']
+            cs.wait_stream(cur)
+            lane = getattr(b, '_work_lane', 0)
+            b._work_lane = -1  # the contraction's gather buffer, apart from the chains' work maps
+            try:
+                with torch.cuda.stream(cs):
+                    self._contract_mfma(LT, QA, F)
+            finally:
+                b._work_lane = lane
+            LT.record_stream(cs)
+            done = torch.cuda.Event()
+            done.record(cs)
+            defer['done'] = done
             return F
         F = torch.zeros((NB, NB), dtype=torch.float64, device=b.device)
         QAr = torch.view_as_real(QA)
@@ -992,12 +1037,11 @@ class BSpec:
 
     def compute_fisher(self, N_it, verb=False):
         """Mean over N_it realizations, FP64 on the GPU, then inverted (bspec.py:678-693)."""
-        fish = None
-        for seed in range(N_it):
-            if verb and seed % 5 == 0:
-                print("Computing Fisher contribution %d of %d" % (seed + 1, N_it))
-            f = self._fisher_contribution_dev(seed)
-            fish = f / N_it if fish is None else fish + f / N_it
+        if verb:
+            print("Computing %d Fisher contributions" % N_it)
+        fish = torch.zeros((self.N_bins, self.N_bins), dtype=torch.float64, device=self.base.device)
+        self._fisher_contributions_dev(range(N_it), fish)
+        fish /= N_it
         self.fish = fish.cpu().numpy()
         self._inv_fish_dev = torch.linalg.inv(fish)
         self.inv_fish = self._inv_fish_dev.cpu().numpy()

@@ -46,6 +46,30 @@ def test_bk_fisher_realization_bitwise(hip, ic):
     assert np.array_equal(F1, F2)
 
 
+@pytest.mark.parametrize("streams", [1, 4])
+def test_bk_fisher_streams_and_deferred_contractions_bitwise(hip, streams):
+    """The Q-map chains dealt to several streams, and the contractions of consecutive realizations
+    deferred to a stream of their own (each next realization's Q maps waiting for the previous
+    contraction), give the bits of the serial one-stream sum."""
+    import polybin3d_amd as pb
+    import torch
+    n, c = 32, [10., 10., 10.]
+    b = pb.PolyBin3D(L, [n] * 3, boxcenter=c, sightline='local', rng='device')
+    mask = shell(n, c)
+    edges = np.linspace(0.01, 0.09, 9)  # 8 bins: several chains per multipole
+    serial = pb.BSpec(b, edges, lmax=2, mask=mask, chain_streams=1)
+    ref = torch.zeros((serial.N_bins,) * 2, dtype=torch.float64, device='cuda')
+    for sd in (4, 9, 2):
+        serial._fisher_contribution_dev(sd, out=ref)
+    bs = pb.BSpec(b, edges, lmax=2, mask=mask, chain_streams=streams)
+    assert bs.N_bins > 2 * 64
+    got = torch.zeros_like(ref)
+    bs._fisher_contributions_dev([4, 9, 2], got)
+    torch.cuda.synchronize()
+    assert ref.abs().max().item() > 0
+    assert torch.equal(got, ref)
+
+
 def test_bin_sums_bitwise(hip):
     import polybin3d_amd as pb
     import torch
