@@ -174,6 +174,10 @@ __device__ __forceinline__ void stream_store(float2* p, float2 v) {
   *p = v;
 #endif
 }
+// row passes over several batch entries: XCD-aware block order (see r2c_rows)
+#ifndef PB3D_ROW_XCD
+#define PB3D_ROW_XCD 1
+#endif
 // plane-aligned y-pass tiles (see launch_cols)
 #ifndef PB3D_COL_ALIGN
 #define PB3D_COL_ALIGN 1
@@ -707,6 +711,23 @@ __device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const fl
   }
 }
 
+// linear block b is dispatched to XCD b % 8 (round robin), so row blocks are dealt out in groups of 8
+// with all maps of one row block on one XCD, whose L2 then serves the mask rows to every map.  (map
+// t, row block rb) from b, a bijection onto [0, batch) x [0, nrb) whatever the dispatch order.
+__device__ __forceinline__ void xcd_rows_block(long long b, long long nrb, long long batch, long long& t,
+                                               long long& rb) {
+  const long long full = (nrb / 8) * 8 * batch;
+  if (b < full) {
+    const long long j = b / 8;
+    t = j % batch;
+    rb = (j / batch) * 8 + b % 8;
+  } else {
+    const long long rem = nrb % 8, q = b - full;
+    t = q / rem;
+    rb = (nrb / 8) * 8 + q % rem;
+  }
+}
+
 template <int M>
 __global__ void __launch_bounds__(RowTile<M>::NT)
 r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows, int rows_per_batch,
@@ -717,7 +738,14 @@ r2c_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows
   __shared__ float2 stw[M], stwN[M + 1];
   const int tid = threadIdx.x;
   stage_twiddles<M>(stw, stwN, tw, twN, tid, NT);
-  const long long row0 = (long long)blockIdx.x * T;
+  long long row0 = (long long)blockIdx.x * T;
+  if (PB3D_ROW_XCD && rows_per_batch % T == 0 && nrows > rows_per_batch) {
+    // every batch entry of one row block on one XCD, whose L2 then serves the rows of an operand the
+    // entries share (stride 0: the fixed map A of the FT-table rows) to all of them
+    long long t, rb;
+    xcd_rows_block(blockIdx.x, rows_per_batch / T, nrows / rows_per_batch, t, rb);
+    row0 = t * rows_per_batch + rb * T;
+  }
   // 1. coalesced load of T rows (T*M float2) with the voxel prologue (T*M is a multiple of NT)
   static_assert((T * M) % NT == 0, "row tile");
   constexpr int IT = (T * M) / NT;
@@ -926,22 +954,6 @@ __device__ __forceinline__ void prepass_regs(float2* buf, int LDT, int r, int tp
 }
 
 // Block order of the fused pointing rows, whose blocks of different maps read the same mask rows:
-// linear block b is dispatched to XCD b % 8 (round robin), so row blocks are dealt out in groups of 8
-// with all maps of one row block on one XCD, whose L2 then serves the mask rows to every map.  (map
-// t, row block rb) from b, a bijection onto [0, batch) x [0, nrb) whatever the dispatch order.
-__device__ __forceinline__ void xcd_rows_block(long long b, long long nrb, long long batch, long long& t,
-                                               long long& rb) {
-  const long long full = (nrb / 8) * 8 * batch;
-  if (b < full) {
-    const long long j = b / 8;
-    t = j % batch;
-    rb = (j / batch) * 8 + b % 8;
-  } else {
-    const long long rem = nrb % 8, q = b - full;
-    t = q / rem;
-    rb = (nrb / 8) * 8 + q % rem;
-  }
-}
 
 template <int M, bool IC, int NU>
 __global__ void __launch_bounds__(PointCfg<M>::NT, NU == 0 ? PB3D_POINT_WAVES : NU == 3 ? PB3D_POINT_WAVES_COMB : PB3D_POINT_WAVES_COMB5)
