@@ -362,6 +362,7 @@ def main():
                       "mask": "shell 300-480 Mpc/h, 200 holes, 5 Mpc/h smoothing", "rng": "device",
                       "realizations_per_step": R, "pk_seeds_per_pass": R, "hip_graph": bool(args.graph),
                       "pointing_rows_mode": args.rows_mode, "bk_chain_streams": args.streams,
+                      "bk_chain_streams_used": getattr(bs, 'streams_used', 1),
                       "bk_fisher_contraction": args.precision + " MFMA (split BF16, FP32 runs, FP64 store)",
                       "fisher_accumulation": "fp64", "setup_s": round(t_setup, 1)},
            "vs_baseline_is": "value / plain FP32 single-realization leg measured in this run (the metric's "

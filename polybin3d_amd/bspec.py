@@ -627,6 +627,11 @@ class BSpec:
             chunks.append((t0, t1))
             t0 = t1
         self._q_chunk(FT, pair_tab, QA, *chunks[0])
+        nstreams = self.streams_used = self._streams_that_fit(ab, chunks, nstreams)
+        if nstreams < 2:
+            for t0, t1 in chunks[1:]:
+                self._q_chunk(FT, pair_tab, QA, t0, t1)
+            return
         cur = torch.cuda.current_stream(b.device)
         side = getattr(self, '_side_streams', None)
         if side is None or len(side) != nstreams - 1 or side[0].device != b.device:
@@ -644,6 +649,22 @@ class BSpec:
             b._work_lane = 0
             for s in side:
                 cur.wait_stream(s)
+
+    def _streams_that_fit(self, ab, chunks, nstreams):
+        """At most nstreams, fewer when the device cannot hold another stream's work maps: each side
+        stream keeps the work maps of its largest chain, (G + 1) maps of n0 n1 lz complex entries per
+        Q map of the chain (G sub-maps and the output), and 2 GiB stay free."""
+        b = self.base
+        local = b.sightline == 'local'
+        per_map = int(b.gridsize[0]) * int(b.gridsize[1]) * min(b.half_shape[2], self._sup[2] + 1)
+        need = max(((2 * int(ab[t0, 3]) + 2 if local and ab[t0, 3] > 0 else 1) + 1) * 8 * per_map * (t1 - t0)
+                   for t0, t1 in chunks)
+        held = sum(1 for lane in range(1, nstreams)
+                   if (lambda t: t is not None and t.numel() >= need)(b._bufs.get(('fft_point', lane))))
+        free, _ = torch.cuda.mem_get_info(b.device)
+        avail = free + torch.cuda.memory_reserved(b.device) - torch.cuda.memory_allocated(b.device)
+        fit = held + int(max(0, avail - (2 << 30)) // need)
+        return max(1, min(nstreams, 1 + fit))
 
     def _qmap_chunk(self, FT, pair_tab, QA, ell):
         """Q maps per fused chain for multipole ell: the tuned choice (timed per map on the first maps
