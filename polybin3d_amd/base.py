@@ -829,6 +829,18 @@ class IntegrationUtils:
             out = out + b.bin_reduce(x, k_edges, [l], y=y)[0, 0].cpu().numpy()
         return out
 
+    def cross_integrate_wcoeff(self, coeff, spec1, spec2, k_edges):
+        """sum_{k in bin} degk Re[coeff_bin s1 conj s2], one complex coefficient per bin
+        (utils.pyx:139-157): Re(coeff) sum Re[s1 conj s2] - Im(coeff) sum Im[s1 conj s2], the second sum
+        being sum Re[s1 conj(i s2)]."""
+        b = self.base
+        c = np.asarray(coeff, dtype=np.complex128)
+        s2 = spec2 if torch.is_tensor(spec2) else torch.from_numpy(np.ascontiguousarray(spec2, dtype=np.complex64))
+        s2 = s2.to(device=b.device, dtype=torch.complex64)
+        re = self.cross_integrate(spec1, s2, k_edges, 0)
+        im = self.cross_integrate(spec1, (1j * s2).contiguous(), k_edges, 0)
+        return c.real * re - c.imag * im
+
     def integrate(self, spec, k_edges, l):
         """sum_{k in bin} degk |s|^2 L_l(mu) (utils.pyx:80-107)."""
         b = self.base
@@ -870,16 +882,59 @@ class MapUtils:
         op = C.kop(C.KOP_FILTER, bin_table=b.bin_table([kmin, kmax]), ell=l1, ell2=l2)
         return b.fourier_linear(kmap, lambda x: b.kmap(x, op), (-1) ** (l1 + l2))
 
-    def prod_fourier(self, map1, map2):
-        """map1 times a real k-space weight that is even under k -> -k (as the reference's callers'
-        |k|- and mu^2-dependent grids are)."""
+    def _weight(self, map2):
+        """A real k-space weight as a device FP32 half grid (a full grid with real_fft=False gives its
+        half: the callers' weights are even under k -> -k)."""
         b = self.base
         w = map2 if torch.is_tensor(map2) else torch.from_numpy(np.asarray(map2, dtype=np.float32))
         w = w.to(device=b.device, dtype=torch.float32).contiguous()
         if not b.real_fft and tuple(w.shape[-3:]) == b.full_shape:
             w = w[..., :b.half_shape[2]].contiguous()
-        op = C.kop(C.KOP_WEIGHT, w=w)
+        return w
+
+    def prod_fourier(self, map1, map2):
+        """map1 times a real k-space weight that is even under k -> -k (as the reference's callers'
+        |k|- and mu^2-dependent grids are) (utils.pyx:213-228)."""
+        b = self.base
+        op = C.kop(C.KOP_WEIGHT, w=self._weight(map2))
         return b.fourier_linear(map1, lambda x: b.kmap(x, op), 1)
 
+    def div_fourier(self, map1, map2):
+        """map1 / map2 for a real k-space map2, zero where map2 is zero (utils.pyx:230-243): the
+        weight 1 / map2 is formed once on the device and applied in the mode op."""
+        b = self.base
+        w = self._weight(map2)
+        inv = torch.where(w != 0, 1. / torch.where(w != 0, w, torch.ones_like(w)), torch.zeros_like(w))
+        op = C.kop(C.KOP_WEIGHT, w=inv.contiguous())
+        return b.fourier_linear(map1, lambda x: b.kmap(x, op), 1)
+
+    def prod_fourier_filter(self, map1, map2, kmin, kmax):
+        """map1 map2 on the modes kmin <= |k| < kmax, zero elsewhere (utils.pyx:325-338)."""
+        b = self.base
+        wop = C.kop(C.KOP_WEIGHT, w=self._weight(map2))
+        fop = C.kop(C.KOP_FILTER, bin_table=b.bin_table([kmin, kmax]), ell=0)
+        return b.fourier_linear(map1, lambda x: b.kmap(b.kmap(x, wop), fop), 1)
+
     def prod_real(self, map1, map2):
+        """map1 map2 in real space (utils.pyx:248-260)."""
         return self.base.as_real(map1) * self.base.as_real(map2)
+
+    def div_real(self, map1, map2):
+        """map1 / map2 in real space, zero where map2 is zero (utils.pyx:265-278)."""
+        a, d = self.base.as_real(map1), self.base.as_real(map2)
+        return torch.where(d != 0, a / torch.where(d != 0, d, torch.ones_like(d)), torch.zeros_like(a))
+
+    def prod_real_diff(self, map1, map2, map3, map4):
+        """map1 map2 - map3 map4 in real space (utils.pyx:371-383)."""
+        r = self.base.as_real
+        return r(map1) * r(map2) - r(map3) * r(map4)
+
+    def sum_pair(self, map1, map2):
+        """sum_x map1 map2, accumulated in FP64 (utils.pyx:343-352)."""
+        r = self.base.as_real
+        return float(torch.sum(r(map1).double() * r(map2).double()).item())
+
+    def sum_triplet(self, map1, map2, map3):
+        """sum_x map1 map2 map3, accumulated in FP64 (utils.pyx:357-366)."""
+        r = self.base.as_real
+        return float(torch.sum(r(map1).double() * r(map2).double() * r(map3).double()).item())

@@ -121,6 +121,16 @@ def test_full_grid_map_and_integration_utils(hip):
         close(b.map_utils.prod_fourier(src, w), src * w, scale=np.abs(src * w).max())
         close(b.applyAinv(src, invPk0_grid=w, input_type='fourier', output_type='fourier'), src * w,
               scale=np.abs(src * w).max(), what="applyAinv")
+        close(b.map_utils.div_fourier(src, w), src / w, scale=np.abs(src / w).max(), what="div_fourier")
+        sel = (modk >= edges[1]) & (modk < edges[4])
+        ref = np.where(sel, src * w, 0.)
+        close(b.map_utils.prod_fourier_filter(src, w, edges[1], edges[4]), ref, scale=np.abs(ref).max(),
+              what="prod_fourier_filter")
+    coeff = np.array([1. + 2j, -0.5 + 1j, 3., -2j, 0.7 - 0.3j])
+    for s1, s2, what in ((herm, herm, "hh"), (herm, other, "ho"), (other, other, "oo")):
+        ref = coeff.real * shells(np.real(s1 * np.conj(s2)), 0) - coeff.imag * shells(np.imag(s1 * np.conj(s2)), 0)
+        close(b.integrator.cross_integrate_wcoeff(coeff, s1, s2, edges), ref,
+              scale=np.abs(coeff) * shells(np.abs(s1 * s2), 0).max(), what="wcoeff " + what)
 
 
 @pytest.mark.parametrize("case", ["global_l4", "local_l2"])

@@ -56,6 +56,44 @@ def test_bin_reduce_matches_oracle(hip):
           scale=np.abs(o.integrator.integrate(s2, edges, 0)[3]))
 
 
+def test_map_and_integration_utils_match_oracle(hip):
+    """The rest of the reference's MapUtils / IntegrationUtils contract (utils.pyx:139-157, 213-383):
+    div_fourier, prod_fourier_filter, div_real, prod_real_diff, sum_pair, sum_triplet and
+    cross_integrate_wcoeff against the oracle (or numpy for the real-space ones) on the same inputs."""
+    import polybin3d_amd as pb
+    n = (32, 32, 32)
+    b = pb.PolyBin3D(L, list(n), boxcenter=[300., 200., 800.], sightline='local')
+    o = O.PolyBin3D(L, list(n), boxcenter=[300., 200., 800.], sightline='local')
+    rng = np.random.RandomState(2)
+    hshape = o.modk_grid.shape
+    m1 = (rng.randn(*hshape) + 1j * rng.randn(*hshape)).astype(np.complex64)
+    w = np.where(rng.rand(*hshape) < 0.2, 0., 0.5 + rng.rand(*hshape)).astype(np.float32)
+    ref = o.map_utils.div_fourier(m1.astype(np.complex128), w.astype(np.float64))
+    close(b.map_utils.div_fourier(m1, w).cpu().numpy(), ref, scale=np.abs(ref).max())
+    assert np.all(b.map_utils.div_fourier(m1, w).cpu().numpy()[w == 0] == 0)
+    ref = o.map_utils.prod_fourier_filter(m1.astype(np.complex128), w.astype(np.float64), 0.02, 0.08)
+    close(b.map_utils.prod_fourier_filter(m1, w, 0.02, 0.08).cpu().numpy(), ref, scale=np.abs(ref).max())
+    x = [rng.randn(*n).astype(np.float32) for _ in range(4)]
+    x[1][rng.rand(*n) < 0.1] = 0.
+    ref = np.where(x[1] != 0, x[0] / np.where(x[1] != 0, x[1], 1.), 0.)
+    close(b.map_utils.div_real(x[0], x[1]).cpu().numpy(), ref, scale=np.abs(ref).max())
+    ref = x[0].astype(np.float64) * x[1] - x[2].astype(np.float64) * x[3]
+    close(b.map_utils.prod_real_diff(*x).cpu().numpy(), ref, scale=np.abs(ref).max())
+    sp = b.map_utils.sum_pair(x[0], x[2])
+    close(np.array([sp]), np.array([O.sum_pair(x[0].astype(np.float64), x[2])]), scale=np.sqrt(np.sum(x[0].astype(np.float64) ** 2 * x[2] ** 2)))
+    st = b.map_utils.sum_triplet(x[0], x[2], x[3])
+    close(np.array([st]), np.array([O.sum_triplet(x[0].astype(np.float64), x[2], x[3])]),
+          scale=np.sqrt(np.sum((x[0].astype(np.float64) * x[2] * x[3]) ** 2)))
+    s1 = np.fft.rfftn(x[0]).astype(np.complex64)
+    s2 = np.fft.rfftn(x[2]).astype(np.complex64)
+    edges = np.linspace(0.007, 0.1, 9)
+    coeff = rng.randn(8) + 1j * rng.randn(8)
+    ref = o.integrator.cross_integrate_wcoeff(coeff, s1.astype(np.complex128), s2.astype(np.complex128), edges)
+    scale = np.abs(coeff) * o.integrator.integrate(s1.astype(np.complex128), edges, 0) ** 0.5 * \
+        o.integrator.integrate(s2.astype(np.complex128), edges, 0) ** 0.5
+    close(b.integrator.cross_integrate_wcoeff(coeff, s1, s2, edges), ref, scale=scale)
+
+
 @pytest.mark.parametrize("case", ["64_global_l2", "64_local_l4", "128_global_l4"])
 def test_pk_ideal(hip, case):
     import polybin3d_amd as pb
