@@ -492,6 +492,28 @@ class PolyBin3D:
         (pruned) transforms: |fs(i)| * kF_i <= |k| < kmax."""
         return [int(min(np.floor(kmax / self.kF[i]), self.gridsize[i] // 2)) for i in range(3)]
 
+    def mask_rows(self, mask, weight=None):
+        """Row flags for pb3d_fft_pointing: uint8 per (x, y) row, 0 on the aligned groups of 64 rows
+        where the mask (and the integral-constraint weight) is zero on every voxel, whose outputs are
+        then zero rows written without transforms; None when every group has a nonzero voxel.  Built
+        once per mask (and weight) tensor and in-place version."""
+        cache = self._bufs.setdefault('mask_rows', [])
+        wv = None if weight is None else weight._version
+        for m, mv, w, wv_, f in cache:
+            if m is mask and mv == mask._version and w is weight and wv_ == wv:
+                return f
+        n0, n1, n2 = (int(v) for v in self.gridsize)
+        rows, group = n0 * n1, 64
+        on = (mask.reshape(rows, n2) != 0).any(dim=1)
+        if weight is not None:
+            on |= (weight.reshape(rows, n2) != 0).any(dim=1)
+        ng = (rows + group - 1) // group
+        grp = torch.nn.functional.pad(on, (0, ng * group - rows)).reshape(ng, group).any(dim=1)
+        f = None if bool(grp.all()) else grp.repeat_interleave(group)[:rows].to(torch.uint8).contiguous()
+        cache.append((mask, mask._version, weight, wv, f))
+        del cache[:-4]
+        return f
+
     def pointing(self, subs, batch, out, mask=None, scale=1.0, kop_out=None, ic=None):
         """out[b] = kop_out(R2C(mask * scale * sum_g coef_g Y_g(x) C2R(kop_g(in_g))[b])), fused
         (pb3d_fft_pointing).  subs: list of (kop, input or None for BKQ, ell, m, coef).  ic: optional
@@ -534,9 +556,10 @@ class PolyBin3D:
             part = self.work('icsum_partial', 8 * npart)
             icd = C.IcSum(weight.data_ptr(), C._ptr(rbin), int(nbins), 0, sums.data_ptr(), part.data_ptr())
             keep.append((weight, rbin, sums))
+        rows = None if mask is None else self.mask_rows(mask, None if ic is None else ic[0])
         C.check(C.lib().pb3d_fft_pointing(self._plan, batch, G, arr, C._ptr(mask), float(scale),
-                                          C.byref_or_none(kop_out), C.byref_or_none(icd), out.data_ptr(),
-                                          work.data_ptr(), self.stream()))
+                                          C.byref_or_none(kop_out), C.byref_or_none(icd), C._ptr(rows),
+                                          out.data_ptr(), work.data_ptr(), self.stream()))
         self.n_FFTs_reverse += batch * G
         self.n_FFTs_forward += batch
         return out

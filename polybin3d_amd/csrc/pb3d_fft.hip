@@ -308,6 +308,8 @@ template <int L, int MODE> struct ColTile {
 //   tpp > 0: plane-aligned tiles, tpp tiles per plane (tile b covers columns (b % tpp)*T + col of
 //   plane b / tpp), so that no tile straddles two planes, whose data lie far apart in the y pass.
 struct ColMap {
+  const unsigned char* row_on;  // MODE 0: (plane, position) rows not stored where 0 (NULL: all)
+  int row_n1;                   // positions per plane in row_on
   int ncols, cpp, split, gap, tpp;
   long long plane, S;          // mode coordinates
   long long in_plane, in_S;    // input addressing
@@ -416,10 +418,12 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
     valid = c < cm.ncols;
   }
   long long base = 0, bin = 0, bout = 0;
+  int qplane = 0;
   if (valid) {
     int q = c / cm.cpp;
     const int l = c % cm.cpp;
     q = q < cm.split ? q : q + cm.gap;
+    qplane = q;
     base = (long long)q * cm.plane + l;
     bin = (long long)q * cm.in_plane + l;
     bout = (long long)q * cm.out_plane + l;
@@ -457,6 +461,7 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
       if (kop.accumulate) v = cadd(v, dst[od]);
       dst[od] = v;
     } else {
+      if (MODE == 0 && cm.row_on && !cm.row_on[(long long)qplane * cm.row_n1 + p]) return;
       stream_store(dst + bout + p * cm.out_S, v);
     }
   };
@@ -896,6 +901,7 @@ constexpr int kMaxMono = 15;  // monomials of a homogeneous polynomial of degree
 // (rx, ry fixed) sum_g w_g z_g = |r|^-Lmax sum_u rz^u IFFT_z(sum_g D_ug S_g) with row constants D_ug,
 // so the rows need nu inverse transforms instead of n.
 struct SubW {
+  const unsigned char* row_on;  // per (x, y) row: 0 = the mask (and IC weight) is zero on its group
   int n, nfact, nu;
   int ell[kMaxSub];
   float coef[kMaxSub];
@@ -976,6 +982,21 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   const long long row0 = rb * PC::RB + wave * RW;  // this wave's first row
   if (row0 >= rows_per_map) return;  // no rows for this wave (no block barrier follows)
   const int r = lane % RW, tp = lane / RW;
+  if (sw.row_on) {
+    // rows whose mask (and IC weight) is zero everywhere come out as zero rows: a wave with no other
+    // row writes them and leaves (its IC partials are zero); the C2R y pass did not store them
+    const long long rr = row0 + r;
+    if (__ballot(rr < rows_per_map && sw.row_on[rr] != 0) == 0ull) {
+      if constexpr (IC) {
+        if (lane < ic.nbins) ic.partial[(t * ic.nwm + row0 / RW) * ic.nbins + lane] = 0.0;
+      }
+      if (rr < rows_per_map) {
+        float2* o = out + t * slot_out + rr * pitch_out;
+        for (int k = tp; k < lz_out; k += P) o[k] = make_float2(0.f, 0.f);
+      }
+      return;
+    }
+  }
   RowTp<PADQ> rw{buf, LDT, r};
   // this lane's voxels after the inverse transforms: positions m = tp + e*P of row r
   const long long row = row0 + r;
@@ -1940,8 +1961,8 @@ extern "C" int pb3d_pointing_rows_mode(int mode) {
 }
 
 extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const pb3d_submap* subs, const float* mask,
-                                 float scale, const pb3d_kop* kop_out, const pb3d_icsum* ic_, void* out_, void* work_,
-                                 void* stream) {
+                                 float scale, const pb3d_kop* kop_out, const pb3d_icsum* ic_,
+                                 const unsigned char* row_on, void* out_, void* work_, void* stream) {
   if (!p || !subs || !out_ || !work_ || batch < 1 || batch > 65535)
     return fail(PB3D_ERR_ARG, "pointing: null pointer or batch out of [1, 65535]");
   if (nsub < 1 || nsub > kMaxSub) return fail(PB3D_ERR_ARG, "pointing: nsub out of range [1, 16]");
@@ -1958,6 +1979,8 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   float2* work = (float2*)work_;
   SubW sw{};
   sw.n = nsub;
+  // row skipping needs a mask (rows are off only where it is zero) and the fast-path kernels
+  sw.row_on = (mask && !p->generic) ? row_on : nullptr;
   // sub-map order in the fused rows (and work slots): the ell = 2 sub-maps first
   int order[kMaxSub];
   int no = 0;
@@ -2046,8 +2069,10 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
                                        sb.kop.x_batch_stride, wsi, batch, p->tw[0], g, kk, s)));
     prof_end(on, sm, s);
   }
-  // 2. pruned y-column C2R pass over all sub-maps
-  const ColMap cyi = ycols(g, pin, true, pzi, pzi);
+  // 2. pruned y-column C2R pass over all sub-maps (rows the row kernel skips are not stored)
+  ColMap cyi = ycols(g, pin, true, pzi, pzi);
+  cyi.row_on = sw.row_on;
+  cyi.row_n1 = g.n1;
   bool on = prof_begin(2, s, sm, col_bytes(cyi, p->L[1], batch * nsub));
   PB3D_CHECK_HIP((launch_cols<+1, 0>(p->L[1], work, work, cyi, wsi, wsi, batch * nsub, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
