@@ -308,8 +308,9 @@ template <int L, int MODE> struct ColTile {
 //   tpp > 0: plane-aligned tiles, tpp tiles per plane (tile b covers columns (b % tpp)*T + col of
 //   plane b / tpp), so that no tile straddles two planes, whose data lie far apart in the y pass.
 struct ColMap {
-  const unsigned char* row_on;  // MODE 0: (plane, position) rows not stored where 0 (NULL: all)
-  int row_n1;                   // positions per plane in row_on
+  const unsigned char* row_on;     // MODE 0: (plane, position) rows not stored where 0 (NULL: all)
+  const unsigned char* row_zero;   // MODE 0: (plane, position) rows read as zeros where 0 (NULL: none)
+  int row_n1;                      // positions per plane in row_on / row_zero
   int ncols, cpp, split, gap, tpp;
   long long plane, S;          // mode coordinates
   long long in_plane, in_S;    // input addressing
@@ -439,6 +440,7 @@ fft_cols(const float2* in, float2* out, ColMap cm, long long in_bstride, long lo
     if (!valid || !in_radius(p, L, cm.rad_in)) return make_float2(0.f, 0.f);
     if (MODE == 1 && kop.kind == 4)  // BKQ assembles the map from packed rows; no input read
       return kop_value(kop, geo, bt, base + p * cm.S, p, mj, ml, make_float2(0.f, 0.f));
+    if (MODE == 0 && cm.row_zero && !cm.row_zero[(long long)qplane * cm.row_n1 + p]) return make_float2(0.f, 0.f);
     float2 v = src[bin + p * cm.in_S];
     if (MODE == 1) v = kop_value(kop, geo, bt, base + p * cm.S, p, mj, ml, v);
     return v;
@@ -983,16 +985,13 @@ pointing_rows(const float2* work, long long slot_in, int pitch_in, float2* out, 
   if (row0 >= rows_per_map) return;  // no rows for this wave (no block barrier follows)
   const int r = lane % RW, tp = lane / RW;
   if (sw.row_on) {
-    // rows whose mask (and IC weight) is zero everywhere come out as zero rows: a wave with no other
-    // row writes them and leaves (its IC partials are zero); the C2R y pass did not store them
+    // rows whose mask (and IC weight) is zero everywhere are zero rows of the output: a wave with no
+    // other row leaves at once (its IC partials are zero); the C2R y pass did not store these rows and
+    // the R2C y pass reads them as zeros, so they are not written either
     const long long rr = row0 + r;
     if (__ballot(rr < rows_per_map && sw.row_on[rr] != 0) == 0ull) {
       if constexpr (IC) {
         if (lane < ic.nbins) ic.partial[(t * ic.nwm + row0 / RW) * ic.nbins + lane] = 0.0;
-      }
-      if (rr < rows_per_map) {
-        float2* o = out + t * slot_out + rr * pitch_out;
-        for (int k = tp; k < lz_out; k += P) o[k] = make_float2(0.f, 0.f);
       }
       return;
     }
@@ -2083,7 +2082,10 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
                                       mask, scale, pin.lz, pout.lz, p->tw[2], p->twN, g, s));
   prof_end(on, sm, s);
   // 4. pruned forward y and x column passes, epilogue into out
-  const ColMap cyo = ycols(g, pout, false, pzo, pzo), cxo = xcols(g, pout, false, pzo, 0);
+  ColMap cyo = ycols(g, pout, false, pzo, pzo);
+  const ColMap cxo = xcols(g, pout, false, pzo, 0);
+  cyo.row_zero = sw.row_on;  // the rows skipped above are zero rows: not read again
+  cyo.row_n1 = g.n1;
   on = prof_begin(6, s, sm, col_bytes(cyo, p->L[1], batch));
   PB3D_CHECK_HIP((launch_cols<-1, 0>(p->L[1], outw, outw, cyo, wso, wso, batch, p->tw[1], g, plain, s)));
   prof_end(on, sm, s);
