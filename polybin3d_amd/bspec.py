@@ -659,8 +659,10 @@ class BSpec:
         per_map = int(b.gridsize[0]) * int(b.gridsize[1]) * min(b.half_shape[2], self._sup[2] + 1)
         need = max(((2 * int(ab[t0, 3]) + 2 if local and ab[t0, 3] > 0 else 1) + 1) * 8 * per_map * (t1 - t0)
                    for t0, t1 in chunks)
-        held = sum(1 for lane in range(1, nstreams)
-                   if (lambda t: t is not None and t.numel() >= need)(b._bufs.get(('fft_point', lane))))
+        held = 0  # side streams whose work maps already have that size
+        for lane in range(1, nstreams):
+            buf = b._bufs.get(('fft_point', lane))
+            held += int(buf is not None and buf.numel() >= need)
         free, _ = torch.cuda.mem_get_info(b.device)
         avail = free + torch.cuda.memory_reserved(b.device) - torch.cuda.memory_allocated(b.device)
         fit = held + int(max(0, avail - (2 << 30)) // need)
