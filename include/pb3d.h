@@ -212,8 +212,7 @@ int pb3d_fft_c2r(const pb3d_plan* plan, const void* in, float* out, void* work, 
  * only the z planes inside the band limits.  nsub <= 16.  row_on (NULL = every row): one byte per
  * (x, y) row of the grid, 0 where the mask (and the ic weight) is zero on the whole z row; the rows
  * must come in aligned groups of 64 that are all on or all off.  Rows that are off are neither
- * transformed nor stored between the passes, and come out as zero rows (a mask is required;
- * mixed-radix plans ignore row_on). */
+ * transformed nor stored between the passes, and come out as zero rows (a mask is required). */
 int pb3d_fft_pointing(const pb3d_plan* plan, int batch, int nsub, const pb3d_submap* subs,
                       const float* mask, float scale, const pb3d_kop* kop_out, const pb3d_icsum* ic,
                       const unsigned char* row_on, void* out, void* work, void* stream);

@@ -1978,8 +1978,8 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   float2* work = (float2*)work_;
   SubW sw{};
   sw.n = nsub;
-  // row skipping needs a mask (rows are off only where it is zero) and the fast-path kernels
-  sw.row_on = (mask && !p->generic) ? row_on : nullptr;
+  // row skipping needs a mask (rows are off only where it is zero)
+  sw.row_on = mask ? row_on : nullptr;
   // sub-map order in the fused rows (and work slots): the ell = 2 sub-maps first
   int order[kMaxSub];
   int no = 0;
@@ -2022,6 +2022,7 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
   if (p->generic) {
     GenSubs gs{};
     gs.n = nsub;
+    gs.row_on = mask ? row_on : nullptr;
     for (int k = 0; k < nsub; ++k) {
       gs.ell[k] = subs[order[k]].ell; gs.m[k] = subs[order[k]].m; gs.coef[k] = subs[order[k]].coef;
     }
@@ -2033,7 +2034,9 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
                               sb.kop.x_batch_stride, wsi, batch, p->tw[0], g, to_kop(&sb.kop), s));
       prof_end(on, sm, s);
     }
-    const GenCols gyi = gen_ycols(g, pin.lz, pzi, pzi, pin.r1);
+    GenCols gyi = gen_ycols(g, pin.lz, pzi, pzi, pin.r1);
+    gyi.row_on = gs.row_on;  // rows the row kernel skips are not stored
+    gyi.row_n1 = g.n1;
     bool on = prof_begin(2, s, sm, gen_col_bytes(gyi, g.n1, batch * nsub));
     PB3D_CHECK_HIP(gen_cols(+1, 0, p->ax[1], work, work, gyi, wsi, wsi, batch * nsub, p->tw[1], g, plain, s));
     prof_end(on, sm, s);
@@ -2043,7 +2046,9 @@ extern "C" int pb3d_fft_pointing(const pb3d_plan* p, int batch, int nsub, const 
     PB3D_CHECK_HIP(gen_pointing_rows(p->ax[2], work, wsi, pzi, outw, wso, pzo, (long long)g.n0 * g.n1, batch, gs, gic,
                                      mask, scale, pin.lz, pout.lz, p->tw[2], p->twN, g, s));
     prof_end(on, sm, s);
-    const GenCols gyo = gen_ycols(g, pout.lz, pzo, pzo, -1);
+    GenCols gyo = gen_ycols(g, pout.lz, pzo, pzo, -1);
+    gyo.row_zero = gs.row_on;  // and read as zeros
+    gyo.row_n1 = g.n1;
     on = prof_begin(6, s, sm, gen_col_bytes(gyo, g.n1, batch));
     PB3D_CHECK_HIP(gen_cols(-1, 0, p->ax[1], outw, outw, gyo, wso, wso, batch, p->tw[1], g, plain, s));
     prof_end(on, sm, s);

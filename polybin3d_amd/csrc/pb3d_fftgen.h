@@ -33,6 +33,9 @@ bool gen_factor(int L, GenAxis& ax);
 // zeros (not read); outputs with p outside rad_out or q outside qrad_out are not written
 // (rad < 0: whole axis).  qlen is the length of the q axis.
 struct GenCols {
+  const unsigned char* row_on;    // MODE 0: (q, p) rows not stored where 0 (NULL: all)
+  const unsigned char* row_zero;  // MODE 0: (q, p) rows read as zeros where 0 (NULL: none)
+  int row_n1;                     // p positions per q in the row flags
   long long ncols;
   int cpp;
   long long plane, S, in_plane, in_S, out_plane, out_S;
@@ -41,6 +44,7 @@ struct GenCols {
 
 // Pointing sub-maps: weight coef * Y_{ell,m}(r_hat) (coef alone for ell = 0).
 struct GenSubs {
+  const unsigned char* row_on;  // per row: 0 = the mask (and IC weight) is zero on it (NULL: all on)
   int n;
   int ell[kGenMaxSub], m[kGenMaxSub];
   float coef[kGenMaxSub];

@@ -117,7 +117,8 @@ gen_cols_kernel(const float2* in, float2* out, GenCols cm, long long in_bs, long
     float2 v = make_float2(0.f, 0.f);
     if (c < cm.ncols) {
       const int q = (int)(c / cm.cpp), l = (int)(c % cm.cpp);
-      if (gen_sup(p, L, cm.rad_in) && gen_sup(q, cm.qlen, cm.qrad_in)) {
+      const bool zrow = MODE == 0 && cm.row_zero && !cm.row_zero[(long long)q * cm.row_n1 + p];
+      if (!zrow && gen_sup(p, L, cm.rad_in) && gen_sup(q, cm.qlen, cm.qrad_in)) {
         if (MODE != 1 || kop.kind != 4) v = src[(long long)q * cm.in_plane + l + (long long)p * cm.in_S];
         if (MODE == 1) v = kop_value(kop, geo, bt, (long long)q * cm.plane + l + (long long)p * cm.S, p, q, l, v);
       }
@@ -148,6 +149,7 @@ gen_cols_kernel(const float2* in, float2* out, GenCols cm, long long in_bs, long
       v = cscale(kop_value(kop, geo, bt, md, p, q, l, v), kop.scale);
       if (kop.accumulate) v = cadd(v, dst[od]);
     }
+    if (MODE == 0 && cm.row_on && !cm.row_on[(long long)q * cm.row_n1 + p]) continue;
     dst[od] = v;
   }
 }
@@ -251,16 +253,28 @@ gen_pointing_kernel(const float2* __restrict__ work, long long slot_in, int pitc
   float* acc = reinterpret_cast<float*>(b + lpb * M);  // lpb * 2M voxels
   for (int i = tid; i < M; i += kGenThreads) T[i] = tw[i];
   for (int i = tid; i <= M; i += kGenThreads) W[i] = twN[i];
-  for (int i = tid; i < lpb * 2 * M; i += kGenThreads) acc[i] = 0.f;
   const long long t = blockIdx.y;
   const long long row0 = (long long)blockIdx.x * lpb;
+  if (gs.row_on) {
+    // rows whose mask (and IC weight) is zero everywhere: a block with no other row leaves at once
+    // (its IC partials are zero); the inverse y pass did not store them and the forward one reads
+    // them as zeros.  In a block that runs, their inputs are taken as zeros below.
+    bool any = false;
+    for (int r = tid; r < lpb; r += kGenThreads) any = any || (row0 + r < rpm && gs.row_on[row0 + r] != 0);
+    if (!__syncthreads_or(any)) {
+      if (ic.weight && tid < ic.nbins) ic.partial[(t * gridDim.x + blockIdx.x) * ic.nbins + tid] = 0.;
+      return;
+    }
+  }
+  for (int i = tid; i < lpb * 2 * M; i += kGenThreads) acc[i] = 0.f;
   for (int g = 0; g < gs.n; ++g) {
     __syncthreads();  // tables staged; the previous sub-map's reads of a and b are done
     const float2* src = work + ((long long)g * batch + t) * slot_in;
     for (int f = tid; f < lpb * M; f += kGenThreads) {
       const int r = f / M, k = f - r * M;
       const long long row = row0 + r;
-      a[f] = row < rpm ? gen_merge(src + row * pitch_in, k, M, lz_in, W) : make_float2(0.f, 0.f);
+      const bool on = row < rpm && (!gs.row_on || gs.row_on[row] != 0);
+      a[f] = on ? gen_merge(src + row * pitch_in, k, M, lz_in, W) : make_float2(0.f, 0.f);
     }
     __syncthreads();
     const float2* Z = line_dft<+1>(a, b, T, ax, lpb);
