@@ -511,7 +511,7 @@ class PolyBin3D:
         grp = torch.nn.functional.pad(on, (0, ng * group - rows)).reshape(ng, group).any(dim=1)
         f = None if bool(grp.all()) else grp.repeat_interleave(group)[:rows].to(torch.uint8).contiguous()
         cache.append((mask, mask._version, weight, wv, f))
-        del cache[:-4]
+        del cache[:-16]  # every (mask, weight) pair of a few specs on this base
         return f
 
     def pointing(self, subs, batch, out, mask=None, scale=1.0, kop_out=None, ic=None):
