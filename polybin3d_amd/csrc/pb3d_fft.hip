@@ -719,8 +719,9 @@ __device__ __forceinline__ void load_prepass_rows(float2* buf, int LDT, const fl
 }
 
 // linear block b is dispatched to XCD b % 8 (round robin), so row blocks are dealt out in groups of 8
-// with all maps of one row block on one XCD, whose L2 then serves the mask rows to every map.  (map
-// t, row block rb) from b, a bijection onto [0, batch) x [0, nrb) whatever the dispatch order.
+// with all maps of one row block on one XCD, whose L2 then serves the rows every map reads (the mask
+// rows of the fused rows, the fixed operand of the FT-table rows) to all of them.  (map t, row block
+// rb) from b, a bijection onto [0, batch) x [0, nrb) whatever the dispatch order.
 __device__ __forceinline__ void xcd_rows_block(long long b, long long nrb, long long batch, long long& t,
                                                long long& rb) {
   const long long full = (nrb / 8) * 8 * batch;
