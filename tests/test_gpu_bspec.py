@@ -369,3 +369,51 @@ def test_masked_rows_skipped(hip, shape):
     assert np.array_equal(P1, P2)
     po = O.PSpec(o, np.linspace(0.01, 0.07, 7), lmax=2, mask=mask, add_GIC=True, mask_IC=mask)
     close(P1, po.compute_fisher_contribution(1), scale=fisher_scale(P1))
+
+
+@pytest.mark.parametrize("case", ["global_mask_l4", "local_mask_l4"])
+def test_bk_lmax4(hip, case):
+    """lmax = 4, the largest the reference takes (bspec.py:84): Bk_ideal (b0, b2, b4) and a masked MC
+    Fisher realization against the oracle.  Global: the l = 4 Legendre weight on the third leg, 32^3.
+    Local: the l = 4 Q-map chains carry ten sub-maps, combined into five row transforms on the
+    16 x 16 x 128 grid (rows of 128 points) in a 500 x 500 x 1000 box."""
+    import polybin3d_amd as pb
+    local = case.startswith("local")
+    shape, box = ([16, 16, 128], [500., 500., 1000.]) if local else ([32, 32, 32], [L] * 3)
+    c = [10., 20., 30.] if local else [0., 0., 0.]
+    kw = dict(boxcenter=c, sightline='local') if local else {}
+    x = [np.fft.fftshift(np.arange(-shape[i] // 2, shape[i] // 2)) * box[i] / shape[i] + c[i] for i in range(3)]
+    X = np.meshgrid(*x, indexing='ij')
+    r = np.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
+    mask = 0.2 + ((r >= 150.) & (r <= 480.)).astype(np.float64)
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(box, shape, Pk=Pk, **kw), O.PolyBin3D(box, shape, Pk=Pk, **kw)
+    edges = np.linspace(0.013, 0.07, 4)
+    rng = np.random.RandomState(11)
+    d = rng.randn(*shape)
+    d = d + 0.3 * (d ** 2 - 1)
+    bi, bio = pb.BSpec(b, edges, lmax=4), O.BSpec(o, edges, lmax=4)
+    got, ref = bi.Bk_ideal(d), bio.Bk_ideal(d)
+    assert sorted(got) == sorted(ref) == ['b0', 'b2', 'b4']
+    # Bk_ideal = inv(F_ideal) num, with F_ideal coupling the multipoles: the numerator per
+    # multipole, F_ideal on its diagonal, and the solve held to the error bound that the measured
+    # numerator and Fisher errors give through |inv(F_ideal)|.  The numerator's entries are sums
+    # over the grid of FP32 products, whose rounding is about 1e-7 of the multipole's largest
+    # entry; where harmonic weights of both signs cancel (the local l = 4 entry of bins (0, 1, 2) is
+    # 2e-3 of its multipole's maximum) that is above 1e-5 of the entry, so each entry is held to
+    # 1e-5 of max(|entry|, 0.1 x the multipole's maximum).
+    num, num_ref = np.asarray(bi.Bk_numerator_ideal(d)), np.asarray(bio.Bk_numerator_ideal(d))
+    n3 = len(num_ref) // 3
+    for j in range(3):
+        blk = slice(j * n3, (j + 1) * n3)
+        err, a = np.abs(num[blk] - num_ref[blk]), np.abs(num_ref[blk])
+        i = int(np.argmax(err / a))
+        print("numerator l=%d: worst entry %d at %.1e of the maximum, error %.2e of it, %.2e of the maximum"
+              % (2 * j, i, a[i] / a.max(), err[i] / a[i], err[i] / a.max()))
+        close(num[blk], num_ref[blk], scale=own_scale(num_ref[blk], floor=0.1), what="numerator l=%d" % (2 * j))
+    close(bi.fish_ideal, bio.fish_ideal, scale=fisher_scale(bio.fish_ideal))
+    parity.check_solve("Bk_ideal lmax 4 %s" % case, got, ref, ['b0', 'b2', 'b4'], bio.fish_ideal, bi.fish_ideal,
+                       num_ref, num)
+    bs, bo = pb.BSpec(b, edges, lmax=4, mask=mask), O.BSpec(o, edges, lmax=4, mask=mask)
+    F1, F2 = bs.compute_fisher_contribution(2), bo.compute_fisher_contribution(2)
+    close(F1, F2, scale=fisher_scale(F2))
