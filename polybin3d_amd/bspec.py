@@ -652,13 +652,18 @@ class BSpec:
 
     def _streams_that_fit(self, ab, chunks, nstreams):
         """At most nstreams, fewer when the device cannot hold another stream's work maps: each side
-        stream keeps the work maps of its largest chain, (G + 1) maps of n0 n1 lz complex entries per
-        Q map of the chain (G sub-maps and the output), and 2 GiB stay free."""
+        stream keeps the work maps of its largest chain as pb3d_fft_pointing sizes them (per Q map,
+        G sub-maps of n0 n1 lz_in complex entries at the chain's input band limit and the output's
+        n0 n1 lz_out at k_max), and 2 GiB stay free."""
         b = self.base
         local = b.sightline == 'local'
-        per_map = int(b.gridsize[0]) * int(b.gridsize[1]) * min(b.half_shape[2], self._sup[2] + 1)
-        need = max(((2 * int(ab[t0, 3]) + 2 if local and ab[t0, 3] > 0 else 1) + 1) * 8 * per_map * (t1 - t0)
-                   for t0, t1 in chunks)
+        n01, nh = int(b.gridsize[0]) * int(b.gridsize[1]), int(b.half_shape[2])
+        lz_out = min(nh, self._sup[2] + 1)
+        need = 0
+        for t0, t1 in chunks:
+            G = 2 * int(ab[t0, 3]) + 2 if local and ab[t0, 3] > 0 else 1
+            lz_in = min(nh, b.support(self.k_bins_squeeze[int(ab[t0:t1, 2].max()) + 1])[2] + 1)
+            need = max(need, 8 * n01 * (t1 - t0) * (G * lz_in + lz_out))
         held = 0  # side streams whose work maps already have that size
         for lane in range(1, nstreams):
             buf = b._bufs.get(('fft_point', lane))

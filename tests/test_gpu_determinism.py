@@ -81,3 +81,41 @@ def test_bin_sums_bitwise(hip):
     r1 = b.bin_reduce(x, edges, [0, 2, 4], nq=4).cpu().numpy()
     r2 = b.bin_reduce(x, edges, [0, 2, 4], nq=4).cpu().numpy()
     assert np.array_equal(r1, r2)
+
+
+def test_bk_fisher_streams_fall_back_when_memory_is_short(hip, monkeypatch):
+    """With no device memory beyond the 2 GiB kept free, the side streams whose work maps already
+    hold the largest chain keep running; without them, and with room for one more stream's work
+    maps, the Q-map chains go to two streams instead of four (streams_used).  The Fisher contribution
+    keeps the bits of the one-stream path throughout."""
+    import polybin3d_amd as pb
+    import torch
+    n, c = 32, [10., 10., 10.]
+    b = pb.PolyBin3D(L, [n] * 3, boxcenter=c, sightline='local', rng='device')
+    mask = shell(n, c)
+    edges = np.linspace(0.01, 0.09, 9)
+    ref = pb.BSpec(b, edges, lmax=2, mask=mask, chain_streams=1)._fisher_contribution_dev(4).clone()
+    bs = pb.BSpec(b, edges, lmax=2, mask=mask, chain_streams=4)
+    first = bs._fisher_contribution_dev(4).clone()
+    assert bs.streams_used == 4 and torch.equal(first, ref)
+    lanes = [('fft_point', lane) for lane in (1, 2, 3)]
+    need = max(b._bufs[k].numel() for k in lanes + ['fft_point'])  # the largest chain's work maps
+    held = sum(b._bufs[k].numel() >= need for k in lanes)
+    real, extra = torch.cuda.mem_get_info, [0]
+
+    def short(dev=None):
+        free, total = real(dev)
+        slack = torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev)
+        return (2 << 30) + extra[0] - slack, total
+    monkeypatch.setattr(torch.cuda, "mem_get_info", short)
+    got = bs._fisher_contribution_dev(4).clone()
+    print("streams used with no spare memory: %d (side streams holding the largest chain: %d)"
+          % (bs.streams_used, held))
+    assert bs.streams_used == 1 + held and torch.equal(got, ref)
+    for k in lanes:
+        del b._bufs[k]
+    extra[0] = need + need // 2
+    got = bs._fisher_contribution_dev(4)
+    torch.cuda.synchronize()
+    print("streams used with memory for one more:", bs.streams_used)
+    assert bs.streams_used == 2 and torch.equal(got, ref)

@@ -48,8 +48,12 @@ def test_mask_rows_groups_of_64():
 def test_chain_streams_that_fit(monkeypatch, free_gib, held, expect):
     n = (256, 256, 256)
     b = fake_base(n)
-    spec = types.SimpleNamespace(base=b, _sup=[63, 63, 63])
-    # two chains: l = 0 (one sub-map, 64 maps) and l = 2 (six sub-maps, 32 maps), local sightline
+    b.kF = np.full(3, 2 * np.pi / 1000.)
+    b.support = types.MethodType(PolyBin3D.support, b)
+    # bin edges to 0.4: input and output band limits of 63 frequency indices (64 kz planes)
+    spec = types.SimpleNamespace(base=b, _sup=[63, 63, 63], k_bins_squeeze=np.array([0.01, 0.4]))
+    # two chains: l = 0 (one sub-map, 64 maps) and l = 2 (six sub-maps, 32 maps), local sightline;
+    # the l = 2 chain's work maps are the larger: 32 maps x (6 sub-maps + the output) x 256^2 x 64
     ab = np.zeros((96, 4), dtype=np.int64)
     ab[64:, 3] = 2
     chunks = [(0, 64), (64, 96)]
