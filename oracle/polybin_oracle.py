@@ -1089,8 +1089,10 @@ class BSpec:
                     g[l // 2, q] = b.to_real(self._filt(leg, 0, q))
         return g
 
-    def _compute_bk_numerator(self, data, include_linear_term=True, filtering='ideal'):
-        """bspec.py:895-983."""
+    def _compute_bk_numerator(self, data, include_linear_term=True, filtering='ideal', return_scale=False):
+        """bspec.py:895-983.  return_scale (for tests): also the same sums over |g| maps, the
+        magnitude that FP32 rounding acts on (the cubic and linear terms cancel by design, and each
+        grid sum can cancel between voxels)."""
         b = self.base
         if include_linear_term and not hasattr(self, 'preload'):
             raise Exception("Need to generate or specify bias simulations!")
@@ -1110,7 +1112,15 @@ class BSpec:
         if include_linear_term:
             for ii in range(self.N_it):
                 B1 += -1. / self.N_it * assemble_b1(g, self.g_bl_maps[ii], self.all_bins)
-        return (B3 + B1) * b.volume / b.gridsize.prod() / self.sym_factor
+        norm = b.volume / b.gridsize.prod() / self.sym_factor
+        if not return_scale:
+            return (B3 + B1) * norm
+        ga = np.abs(g)
+        mag = assemble_b3(ga, self.all_bins)
+        if include_linear_term:
+            for ii in range(self.N_it):
+                mag += 1. / self.N_it * assemble_b1(ga, np.abs(self.g_bl_maps[ii]), self.all_bins)
+        return (B3 + B1) * norm, mag * np.abs(norm)
 
     def Bk_numerator_ideal(self, data):
         return self._compute_bk_numerator(data, include_linear_term=False, filtering='ideal')

@@ -95,24 +95,40 @@ def test_bk_mc_fisher_per_seed(hip, case):
         close(F1, F2, scale=fisher_scale(F2))
 
 
-def test_bk_unwindowed_with_linear_term(hip):
+@pytest.mark.parametrize("case", ["global_l2", "local_l4"])
+def test_bk_unwindowed_with_linear_term(hip, case):
+    """Bk_unwindowed with the linear term (generated sims), its MC Fisher and numerator against the
+    oracle; local l = 4 on 16^3 runs the l = 4 Q-map chains one sub-map at a time (rows too short to
+    combine).  The numerator is the cubic term minus the linear term, which cancel by design, and
+    each term is a grid sum whose voxels cancel (an entry of the local l = 4 case is 4e-5 of the
+    largest).  FP32 rounding acts on the same sums over |g| maps, so each entry is held to 1e-5 of
+    max(|entry|, 0.1 x its |g| sum).  Bk_unwindowed = inv(F) num goes through the solve bound (the
+    same F on both sides, so only the measured numerator error propagates)."""
     import polybin3d_amd as pb
     n = 16
     edges = np.linspace(0.007, 0.05, 4)
-    mask = shell_mask(n, L, [0., 0., 0.])
+    local = case.startswith("local")
+    c = [10., 10., 10.] if local else [0., 0., 0.]
+    kw, lmax = (dict(boxcenter=c, sightline='local'), 4) if local else ({}, 2)
+    mask = shell_mask(n, L, c)
     Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
-    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk), O.PolyBin3D(L, [n] * 3, Pk=Pk)
-    bs, bo = pb.BSpec(b, edges, lmax=2, mask=mask), O.BSpec(o, edges, lmax=2, mask=mask)
+    b, o = pb.PolyBin3D(L, [n] * 3, Pk=Pk, **kw), O.PolyBin3D(L, [n] * 3, Pk=Pk, **kw)
+    bs, bo = pb.BSpec(b, edges, lmax=lmax, mask=mask), O.BSpec(o, edges, lmax=lmax, mask=mask)
     bs.generate_sims(3)
     bo.generate_sims(3)
     fish = bo.compute_fisher(2)
     close(bs.compute_fisher(2), fish, scale=fisher_scale(fish))
     d = ng_field((n, n, n), 4) * mask
-    n1, n2 = bs.Bk_numerator(d), bo.Bk_numerator(d)
-    close(n1, n2, scale=np.maximum(np.abs(n2), 1e-3 * np.abs(n2).max()))
+    n1 = np.asarray(bs.Bk_numerator(d))
+    n2, mag = bo._compute_bk_numerator(d, include_linear_term=True, filtering='Sinv', return_scale=True)
+    err = np.abs(n1 - n2)
+    i = int(np.argmax(err / mag))
+    print("numerator: worst entry %d at %.1e of the maximum, %.1e of its |g| sum; error %.2e of that sum"
+          % (i, abs(n2[i]) / np.abs(n2).max(), abs(n2[i]) / mag[i], err[i] / mag[i]))
+    close(n1, n2, scale=np.maximum(np.abs(n2), 0.1 * mag))
     got, ref = bs.Bk_unwindowed(d, fish=fish), bo.Bk_unwindowed(d, fish=fish)
-    for k in ref:
-        close(got[k], ref[k], scale=own_scale(ref[k]), what=k)
+    parity.check_solve("Bk_unwindowed %s" % case, got, ref, ['b%d' % l for l in range(0, lmax + 1, 2)], fish, fish,
+                       n2, n1)
 
 
 @pytest.mark.parametrize("case", ["global_mask_l2", "local_mask_l2"])

@@ -455,3 +455,18 @@ def test_bk_fisher_subblock_equals_full(kind):
     err = np.max(np.abs(sub - ref)) / np.max(np.abs(ref))
     print("Bk Fisher sub-block (%s): %.2e" % (kind, err))
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["global_mask", "local_mask"])
+def test_bk_numerator_magnitude_scale(kind):
+    """The oracle's numerator with return_scale (the sums over |g| maps that GPU tests scale by)
+    returns the plain call's values, and each |g| sum bounds its entry (triangle inequality), also
+    with the linear term."""
+    bs, _ = _bk_case(kind)
+    bs.generate_sims(2)
+    d = np.random.RandomState(8).randn(*bs.base.gridsize)
+    for lin in (False, True):
+        plain = bs._compute_bk_numerator(d, include_linear_term=lin, filtering='Sinv')
+        val, mag = bs._compute_bk_numerator(d, include_linear_term=lin, filtering='Sinv', return_scale=True)
+        assert np.array_equal(plain, val)
+        assert np.all(mag >= np.abs(val) * (1 - 1e-12)) and np.all(mag > 0)
