@@ -433,3 +433,30 @@ def test_bk_lmax4(hip, case):
     bs, bo = pb.BSpec(b, edges, lmax=4, mask=mask), O.BSpec(o, edges, lmax=4, mask=mask)
     F1, F2 = bs.compute_fisher_contribution(2), bo.compute_fisher_contribution(2)
     close(F1, F2, scale=fisher_scale(F2))
+
+
+def test_integral_constraints_on_combined_rows(hip):
+    """Integral constraints where the row pass combines the harmonic sub-maps (rows of 128 points, a
+    16 x 16 x 128 grid in a 500 x 500 x 1000 box, local sightline): a Bk MC Fisher seed with the
+    radial constraint at lmax 2 (three combined row transforms) and a Pk MC Fisher seed with the global
+    constraint at lmax 4 (five), against the oracle."""
+    import polybin3d_amd as pb
+    shape, box, c = [16, 16, 128], [500., 500., 1000.], [10., 20., 30.]
+    kw = dict(boxcenter=c, sightline='local')
+    x = [np.fft.fftshift(np.arange(-shape[i] // 2, shape[i] // 2)) * box[i] / shape[i] + c[i] for i in range(3)]
+    X = np.meshgrid(*x, indexing='ij')
+    r = np.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
+    mask = 0.2 + ((r >= 150.) & (r <= 480.)).astype(np.float64)
+    mask_IC = mask * (1.2 + 0.1 * np.cos(np.arange(shape[2]) * 0.05))[None, None, :]
+    Pk = [np.linspace(0, 2, 500), 3e3 / (1 + (np.linspace(0, 2, 500) / 0.05) ** 2)]
+    b, o = pb.PolyBin3D(box, shape, Pk=Pk, **kw), O.PolyBin3D(box, shape, Pk=Pk, **kw)
+    ric = dict(add_RIC=True, mask_IC=mask_IC, radial_bins_RIC=[100., 250., 400., 600.])
+    e3 = np.linspace(0.013, 0.07, 4)
+    F1 = pb.BSpec(b, e3, lmax=2, mask=mask, **ric).compute_fisher_contribution(1)
+    F2 = O.BSpec(o, e3, lmax=2, mask=mask, **ric).compute_fisher_contribution(1)
+    close(F1, F2, scale=fisher_scale(F2), what="Bk RIC lmax 2")
+    gic = dict(add_GIC=True, mask_IC=mask_IC)
+    e2 = np.linspace(0.013, 0.09, 6)
+    P1 = pb.PSpec(b, e2, lmax=4, mask=mask, **gic).compute_fisher_contribution(2)
+    P2 = O.PSpec(o, e2, lmax=4, mask=mask, **gic).compute_fisher_contribution(2)
+    close(P1, P2, scale=fisher_scale(P2), what="Pk GIC lmax 4")
